@@ -1,0 +1,33 @@
+"""Shared test configuration: the `gpu` marker and fixture helpers."""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+REPO = Path(__file__).resolve().parent.parent
+GOLDEN = REPO / "tests" / "golden"
+if str(REPO) not in sys.path:
+    sys.path.insert(0, str(REPO))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+def load_golden(name: str) -> dict:
+    with np.load(GOLDEN / f"{name}.npz", allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def potential_table():
+    from oracle import oracle as O
+    with np.load(GOLDEN / "potential_table.npz", allow_pickle=False) as z:
+        return O.Table(z["field"], z["potential"], z["potential_derivative"])
+
+
+@pytest.fixture(scope="session")
+def table():
+    return potential_table()

@@ -1,0 +1,115 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ from the reference program.
+
+Runs only in a container that has the reference sources at /root/reference: each
+case compiles the reference for its compile-time configuration with
+oracle/build_ref.sh (binaries land in oracle/_ref/, never committed), runs it in a
+scratch directory on one of the reference's own presets (params.analytic.txt,
+params.numerical.txt) with the overrides listed below, and stores the numeric
+results/*.dat files as arrays in a compressed .npz next to this script.  The
+fixtures are data only: the parameters used, and the reference's outputs.
+
+It also stores the tabulated-potential input tables (inputs/*.dat of the
+reference) as tests/golden/potential_table.npz, since tests on machines without
+the reference need them.
+
+usage: python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parent.parent
+REF = Path(os.environ.get("IE_REFERENCE_DIR", "/root/reference"))
+
+sys.path.insert(0, str(REPO))
+from oracle.oracle import read_blocks  # noqa: E402
+
+# name: (N, numerical_potential, preset, overrides)
+CASES = {
+    "hilltop_n16": (16, 0, "params.analytic.txt", {"output_box3D": 1, "output_box2D": 1}),
+    "tabulated_n16": (16, 1, "params.numerical.txt", {"output_box3D": 1, "output_box2D": 1}),
+    "tabulated_n16_rk4": (16, 1, "params.numerical.txt",
+                          {"output_box2D": 1, "inflation_integrator": "rk4", "deltaN_integrator": "rk4",
+                           "af": 4, "Nend": 0.5}),
+    "tabulated_n16_rk45": (16, 1, "params.numerical.txt",
+                           {"output_box2D": 1, "inflation_integrator": "rk45", "deltaN_integrator": "rk45",
+                            "af": 4, "Nend": 0.5}),
+    "hilltop_n32": (32, 0, "params.analytic.txt", {"output_box2D": 1}),
+    "tabulated_n32": (32, 1, "params.numerical.txt", {"output_box2D": 1}),
+}
+
+# files whose first block only is kept (large text dumps)
+FIRST_BLOCK_ONLY = {"box"}
+
+
+def params_text(preset: str, overrides: dict) -> str:
+    lines = []
+    for raw in (REF / preset).read_text().splitlines():
+        key = raw.split("#", 1)[0].split("=", 1)[0].strip()
+        if key and key in overrides:
+            continue
+        lines.append(raw)
+    for k, v in overrides.items():
+        lines.append(f"{k} = {v}")
+    return "\n".join(lines) + "\n"
+
+
+def build(N: int, numerical: int) -> Path:
+    out = subprocess.run([str(REPO / "oracle" / "build_ref.sh"), str(N), str(numerical), "1", "1", "0"],
+                         check=True, capture_output=True, text=True).stdout.strip().splitlines()[-1]
+    return Path(out)
+
+
+def run_case(name: str) -> None:
+    N, numerical, preset, overrides = CASES[name]
+    binary = build(N, numerical)
+    text = params_text(preset, overrides)
+    with tempfile.TemporaryDirectory(prefix=f"ie_golden_{name}_") as tmp:
+        tmp = Path(tmp)
+        (tmp / "params.txt").write_text(text)
+        os.symlink(REF / "inputs", tmp / "inputs")
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([str(binary)], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL)
+        arrays = {"params_text": np.array(text), "N": np.array(N), "numerical_potential": np.array(numerical)}
+        for f in sorted((tmp / "results").glob("*.dat")):
+            stem = f.stem
+            if stem == "info":
+                continue
+            blocks = read_blocks(f)
+            if stem in FIRST_BLOCK_ONLY:
+                blocks = blocks[:1]
+            if stem in ("means", "variance", "velocity", "sf", "energy", "conservation", "spectratimes",
+                        "histogramtimes", "histogramtimesN"):
+                arrays[stem] = np.concatenate(blocks, axis=0)
+            else:
+                arrays[stem] = np.stack(blocks, axis=0)
+    np.savez_compressed(HERE / f"{name}.npz", **arrays)
+    print(f"{name}: {len(arrays) - 3} files")
+
+
+def write_table() -> None:
+    x = np.loadtxt(REF / "inputs" / "field_values.dat")
+    v = np.loadtxt(REF / "inputs" / "potential.dat")
+    dv = np.loadtxt(REF / "inputs" / "potential_derivative.dat")
+    np.savez_compressed(HERE / "potential_table.npz", field=x, potential=v, potential_derivative=dv)
+
+
+def main(argv: list[str]) -> None:
+    if not (REF / "src" / "parameters.h").exists():
+        sys.exit(f"reference not found under {REF}")
+    write_table()
+    for name in argv or list(CASES):
+        run_case(name)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

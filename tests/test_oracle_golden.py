@@ -1,0 +1,43 @@
+"""Pin the oracle to the reference: fixtures written by the compiled reference
+(tests/golden/make_golden.py) must be reproduced exactly, file by file.
+
+The reference prints %e / %f / %.17g text; both sides are parsed back to doubles,
+so equality of the arrays is equality of the printed text.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, potential_table
+from oracle import oracle as O
+from oracle.oracle import read_blocks
+
+CASES = ["hilltop_n16", "tabulated_n16", "tabulated_n16_rk4", "tabulated_n16_rk45",
+         "hilltop_n32", "tabulated_n32"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_reproduces_reference(case, tmp_path):
+    g = load_golden(case)
+    N = int(g["N"])
+    numerical = int(g["numerical_potential"])
+    params = O.parse_params_text(str(g["params_text"]))
+    table = potential_table() if numerical else None
+    cfg = O.make_config(N, numerical, params, table)
+    assert O.run_program(cfg, tmp_path) == 0
+    checked = 0
+    for key, ref in g.items():
+        if key in ("params_text", "N", "numerical_potential"):
+            continue
+        blocks = read_blocks(tmp_path / f"{key}.dat")
+        if key == "box":
+            blocks = blocks[:1]
+        if ref.ndim == 2 and key not in ("box",):
+            got = np.concatenate(blocks, axis=0)
+        else:
+            got = np.stack(blocks, axis=0)
+        assert got.shape == ref.shape, key
+        assert np.array_equal(got, ref), f"{case}/{key}: max |diff| {np.max(np.abs(got - ref))}"
+        checked += 1
+    assert checked >= 18
