@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark: lattice site-updates/s of the leapfrog step at 256^3 on one MI355X.
+
+Workload (one "step" = one leapfrog step of run_evolution_loop over the whole
+lattice: Friedmann reduction, kick, drift; src/evolution.cpp:937-945):
+  256^3 lattice, analytic hilltop preset (params.analytic.txt: V0 = 3.338e-13,
+  ns = 0.97, phi0 = 0.0935, dphi0 = 0.000796, L = 10, dt = 0.001, seed 8),
+  Bunch-Davies initial conditions from the engine's own initializer.
+  value: split storage (FP32 fluctuation about the FP64 homogeneous mode, every
+  right-hand side in FP64), SIGW off.  Also reported: the FP64-storage path at
+  the same size and the split path with the six SIGW tensors on.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "inflation-easy_amd"))
+sys.path.insert(0, str(REPO))
+
+METRIC = "lattice site-updates/s at 256³ (split-FP32 vs FP64 GPU); p50/p99 run latency"
+HBM_PEAK_GBS = 8000.0
+PRESET = dict(V0=3.338e-13, ns=0.97, L=10.0, dt=0.001)
+PHI0, DPHI0, SEED = 0.0935, 0.000796, 8
+
+
+def make_run(N: int, precision: int, sigw: int):
+    import ie_engine as E
+    cfg = E.make_config(N, precision=precision, sigw=sigw, **PRESET)
+    run = E.Run(cfg)
+    run.initialize(SEED, PHI0, DPHI0)
+    run.evolve_fields(0.5 * PRESET["dt"])   # leapfrog stagger (src/evolution.cpp:935)
+    return run
+
+
+def time_steps(run, steps: int, warmup: int) -> tuple[float, dict]:
+    if warmup:
+        run.leapfrog_steps(warmup)
+    t0 = time.perf_counter()
+    bg = run.leapfrog_steps(steps)
+    t1 = time.perf_counter()
+    return (t1 - t0), bg
+
+
+def bench_case(N: int, precision: int, sigw: int, steps: int, warmup: int, profile_steps: int):
+    run = make_run(N, precision, sigw)
+    secs, bg = time_steps(run, steps, warmup)
+    if bg["steps"] != steps + warmup + 0 or bg["done"]:
+        raise RuntimeError(f"unexpected loop state {bg}")
+    prof = run.profile_steps(profile_steps) if profile_steps else None
+    run.close()
+    return secs, prof
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/traffic.json)."""
+    p = REPO / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get(name)
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(N: int = 256, target_steps_af: float = 1.02) -> dict | None:
+    """The reference program (built from its sources by oracle/build_ref.sh, OpenMP)
+    timed on this host: loop time = wall(af = target) - wall(af = 1, no step), the
+    initial conditions and outputs being identical in both runs."""
+    binary = REPO / "oracle" / "_ref" / f"ie_ref_n{N}_p0_g0_d0_o1"
+    if not binary.exists():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    base = (f"seed = {SEED}\ndt = {PRESET['dt']}\nV0 = {PRESET['V0']}\nns = {PRESET['ns']}\n"
+            f"initial_field = {PHI0}\ninitial_derivative = {DPHI0}\nL = {PRESET['L']}\n"
+            "output_freq = 100000000\noutput_infrequent_freq = 100000000\noutput_spectra = 0\n"
+            "output_histogram = 0\noutput_energy = 0\nscreen_updates = 0\n")
+
+    def one(af: float):
+        d = Path(tempfile.mkdtemp(prefix="ie_cpu_"))
+        try:
+            (d / "params.txt").write_text(base + f"af = {af}\n")
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            t0 = time.perf_counter()
+            subprocess.run([str(binary)], cwd=d, env=env, check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=600)
+            wall = time.perf_counter() - t0
+            text = (d / "results" / "output.txt").read_text()
+            found = re.findall(r"numsteps (\d+)", text)
+            return wall, int(found[-1]) if found else 0
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+
+    w0, n0 = one(1.0)
+    w1, n1 = one(target_steps_af)
+    steps = n1 - n0
+    if steps <= 0 or w1 <= w0:
+        return None
+    rate = N ** 3 * steps / (w1 - w0)
+    return {"value": rate, "unit": "site-updates/s", "cores": threads, "kind": "reference",
+            "sample": f"{N}^3 hilltop, FP64, {steps} leapfrog steps (af={target_steps_af}) of the reference "
+                      f"OpenMP build; loop time = wall(af={target_steps_af}) - wall(af=1), "
+                      f"{w1 - w0:.1f} s of CPU loop time"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["split", "fp64", "sigw"], default=None,
+                    help="time a single case (for profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+              "(ensembles shard by run, see DESIGN.md).", file=sys.stderr)
+        return 2
+    import ie_engine as E
+    if E.lib().ie_device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 3
+    N, K, W = args.N, args.steps, args.warmup
+    sites = N ** 3
+    cases = {"split": (E.IE_SPLIT, 0), "fp64": (E.IE_FP64, 0), "sigw": (E.IE_SPLIT, 1)}
+    todo = [args.only] if args.only else ["split", "fp64", "sigw"]
+    res = {}
+    for name in todo:
+        prec, sigw = cases[name]
+        secs, prof = bench_case(N, prec, sigw, K, W, args.profile_steps)
+        res[name] = {"secs": secs, "sups": sites * K / secs, "ms_per_step": 1e3 * secs / K, "profile": prof}
+        print(f"# {name}: {res[name]['sups']:.4e} site-updates/s, {res[name]['ms_per_step']:.4f} ms/step, "
+              f"profile {prof}", file=sys.stderr, flush=True)
+    head = res.get("split") or next(iter(res.values()))
+    out = {
+        "metric": METRIC,
+        "value": head["sups"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{N}^3 hilltop lattice, leapfrog, split storage (FP32 fluctuation about FP64 "
+                               f"homogeneous mode, FP64 arithmetic), SIGW off; params.analytic.txt preset, seed 8",
+                   "N": N, "potential": "hilltop", "integrator": "leapfrog", "storage": "split-fp32"},
+    }
+    if "fp64" in res:
+        out["fp64_value"] = res["fp64"]["sups"]
+        out["fp64_ms_per_step"] = res["fp64"]["ms_per_step"]
+        if "split" in res:
+            out["split_over_fp64"] = res["split"]["sups"] / res["fp64"]["sups"]
+    if "sigw" in res:
+        out["sigw_split_value"] = res["sigw"]["sups"]
+        out["sigw_ms_per_step"] = res["sigw"]["ms_per_step"]
+    prof = head["profile"]
+    if prof:
+        # dominant kernel: fused kick + drift; algorithmic bytes per site: read dphi and
+        # dpi, write dpi and the next dphi (4 B each in split storage, 8 B in FP64)
+        per_site = 16 if head is res.get("split") else 32
+        alg = per_site * sites
+        ms = prof["kick_drift_ms"]
+        achieved = alg / (ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("kick_drift_split_256"),
+                           "kernel": "k_kick_drift", "kernel_ms": ms, "algorithmic_bytes": alg,
+                           "reduce_ms": prof["reduce_ms"], "step_ms_events": prof["step_ms"]}
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(N)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

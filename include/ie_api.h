@@ -1,0 +1,206 @@
+/*
+ * ie_api.h -- C ABI of the gfx950 lattice engine for InflationEasy.
+ *
+ * The reference (caravangelo/inflation-easy) has no FFI: its hot path is a set of
+ * free functions over global state (src/main.h:138-205, src/ffteasy.hpp:195-204,
+ * file-static back-ends in src/output.cpp).  Each entry point below is the
+ * handle-based replacement of one of those functions; the comment names the
+ * reference function (file:line) whose arguments, meaning and error behaviour it
+ * keeps.  State lives on the GPU and is owned by the handle; host pointers are
+ * borrowed for the duration of a call; every call returns an ie_status and leaves
+ * a message retrievable with ie_last_error().  A handle is thread-compatible (one
+ * host thread at a time), not thread-safe.
+ */
+#ifndef IE_API_H
+#define IE_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ie_status {
+    IE_OK = 0,
+    IE_ERR_ARG = 1,          /* invalid argument or unsupported configuration */
+    IE_ERR_COURANT = 2,      /* dt > dx/sqrt(3) (src/initialize.cpp:94-97) */
+    IE_ERR_TABLE_RANGE = 3,  /* field outside the tabulated potential (src/potential.cpp:19-27) */
+    IE_ERR_NONFINITE = 4,    /* <phi> not finite (src/output.cpp:251-258) */
+    IE_ERR_RK45 = 5,         /* RK45 failed to converge (src/evolution.cpp:974-979) */
+    IE_ERR_OOM = 6,          /* device allocation failed */
+    IE_ERR_HIP = 7,          /* HIP runtime error */
+    IE_ERR_STATE = 8         /* call not valid in the current phase */
+} ie_status;
+
+enum { IE_FP64 = 0, IE_SPLIT = 1 };                       /* field storage */
+enum { IE_LEAPFROG = 0, IE_RK4 = 1, IE_RK45 = 2 };         /* src/parameters.h:109-113 */
+enum { IE_GENERIC = 0, IE_MONOTONIC = 1, IE_ANTIMONOTONIC = -1 }; /* src/parameters.h:60-66 */
+enum { IE_FIELD_F = 0, IE_FIELD_FD = 1, IE_FIELD_DELTAN = 2 };
+
+/* One run: the reference's compile-time toggles (src/parameters.h:22-70) become
+ * per-run fields, plus its run-time parameters (src/runtime_parameters.cpp:17-112)
+ * that the hot path reads.  N must be a power of two, 8 <= N <= 1024. */
+typedef struct ie_config {
+    int N;
+    int precision;             /* IE_FP64 or IE_SPLIT */
+    int numerical_potential;   /* 1 tabulated, 0 analytic hilltop */
+    int calculate_sigw;
+    int monotonic;             /* deltaN stopping criterion */
+    int integrator;            /* inflation integrator */
+    int deltaN_integrator;
+    double rescale_s, af, V0, rescale_B, ns, L, dt;
+    double rk45_abs_tol, rk45_rel_tol, rk45_min_dt, rk45_max_dt, rk45_safety;
+    double dN, Nend;
+    int use_phiref_manual;
+    double phiref_manual_value;
+    int tab_n;                 /* tabulated potential, descending field grid */
+    const double *tab_x, *tab_v, *tab_dv;
+} ie_config;
+
+/* Background scalars (src/main.cpp:324-326) plus loop bookkeeping. */
+typedef struct ie_background {
+    double t, a, ad, ad2, astep, Ne, phiref, hubble_init;
+    double dt_last;            /* step of the last kick or drift (dt astep^{s-1} in the loop) */
+    int64_t steps;             /* leapfrog steps taken by ie_leapfrog_steps */
+    int32_t done;              /* a > af reached */
+    int32_t error;             /* ie_status raised inside a kernel (table range) */
+} ie_background;
+
+typedef struct ie_run ie_run;
+
+/* --- lifecycle -------------------------------------------------------------- */
+/* Allocate device state for one run (replaces the global arrays of
+ * src/main.cpp:15-37 and the checks of initialize(), src/initialize.cpp:93-114). */
+ie_status ie_create(const ie_config *cfg, ie_run **out);
+ie_status ie_destroy(ie_run *run);
+const char *ie_last_error(const ie_run *run);
+/* Bytes of device memory the run holds (admission by HBM footprint). */
+size_t ie_footprint_bytes(const ie_config *cfg);
+
+/* --- state transfer --------------------------------------------------------- */
+/* Upload real-space f, fd (N^3 doubles each), as initializef() leaves them
+ * (src/initialize.cpp:118-214).  In IE_SPLIT mode the FP64 volume averages become
+ * the homogeneous mode and the rest is stored in FP32. */
+ie_status ie_upload_fields(ie_run *run, const double *f, const double *fd);
+/* Generate real-space f, fd from FFTEASY-layout mode arrays (the k-space fill of
+ * initializef, src/initialize.cpp:142-191) with the inverse transform on the GPU
+ * (src/initialize.cpp:194-200): fk, fdk hold N^3 doubles (kz < N/2, interleaved
+ * complex), fny, fdny the N x 2N Nyquist planes; phi0, dphi0 are added after. */
+ie_status ie_upload_modes(ie_run *run, const double *fk, const double *fny,
+                          const double *fdk, const double *fdny, double phi0, double dphi0);
+/* Download a field as FP64 (N^3); IE_FIELD_F reconstitutes phibar + dphi in split mode. */
+ie_status ie_download_field(ie_run *run, int which, double *out);
+/* Tensors: comp 0..5 h_ij, 6..11 hdot_ij (N^3 floats each). */
+ie_status ie_upload_tensor(ie_run *run, int comp, const float *h);
+ie_status ie_download_tensor(ie_run *run, int comp, float *out);
+ie_status ie_get_background(ie_run *run, ie_background *out);
+ie_status ie_set_background(ie_run *run, const ie_background *in);
+
+/* --- initial conditions ----------------------------------------------------- */
+/* Inputs of initializef() (src/initialize.cpp:118-214) that are not in ie_config. */
+typedef struct ie_ic_params {
+    int seed;                  /* Park-Miller seed (src/initialize.cpp:18-31) */
+    double initial_field, initial_derivative;
+    double high_cutoff_index, low_cutoff_index;   /* mode cutoff (src/initialize.cpp:69-72) */
+} ie_ic_params;
+/* Host-side k-space fill of initializef (src/initialize.cpp:118-191): the
+ * Park-Miller draws in the reference's order, Rayleigh amplitudes, Bunch-Davies
+ * momenta, Hermitian mirror planes, DC removed.  Writes FFTEASY-layout arrays
+ * (fk, fdk: N^3 doubles; fny, fdny: N x 2N) and *hubble_init, the homogeneous
+ * H0 the momenta use (src/initialize.cpp:135).  Needs no GPU. */
+ie_status ie_fill_modes(const ie_config *cfg, const ie_ic_params *ic, double *fk, double *fny, double *fdk,
+                        double *fdny, double *hubble_init);
+/* initialize() + initializef() (src/initialize.cpp:93-214) on a run: the
+ * k-space fill above, the inverse transform on the GPU, phi0 and dphi0 added,
+ * then t = 0, a = 1 and ad = H0 from the lattice energies.  *out receives the
+ * background. */
+ie_status ie_initialize(ie_run *run, const ie_ic_params *ic, ie_background *out);
+/* potential(v) and its derivative in units of B^2 (src/potential.cpp:91-112). */
+ie_status ie_potential_value(const ie_config *cfg, double v, double *V, double *dV);
+
+/* --- lattice reductions ----------------------------------------------------- */
+/* gradient_energy (src/evolution.cpp:130-136) */
+ie_status ie_gradient_energy(ie_run *run, double *out);
+/* kin_energy (src/evolution.cpp:139-145) */
+ie_status ie_kin_energy(ie_run *run, double *out);
+/* potential_energy (src/potential.cpp:151-171) */
+ie_status ie_potential_energy(ie_run *run, double *out);
+/* meansvars sums (src/output.cpp:235-240): out = {sum f, sum fd, sum f^2} */
+ie_status ie_field_sums(ie_run *run, double out[3]);
+/* min and max of f or deltaN (src/output.cpp:1095-1101) and the histogram with
+ * nbins bins over [min, max] (src/output.cpp:1104-1123): counts has nbins entries. */
+ie_status ie_histogram(ie_run *run, int which, int nbins, double *bmin, double *bmax, double *counts);
+
+/* --- time stepping ---------------------------------------------------------- */
+/* evolve_derivs(d): Friedmann half-kicks around the field (and tensor) kick
+ * (src/evolution.cpp:150-257). */
+ie_status ie_evolve_derivs(ie_run *run, double d);
+/* evolve_fields(d): drift of f (and h), t and a (src/evolution.cpp:260-284). */
+ie_status ie_evolve_fields(ie_run *run, double d);
+/* Up to max_steps iterations of the leapfrog loop body of run_evolution_loop
+ * (src/evolution.cpp:937-945) with the loop test a <= af evaluated on the device:
+ * no host synchronisation between steps.  *out receives the background after. */
+ie_status ie_leapfrog_steps(ie_run *run, int max_steps, ie_background *out);
+/* The same loop as ie_leapfrog_steps, with HIP events recorded on the run's
+ * stream around every launch; *out receives the mean device time per launch of
+ * each kernel of the step and per whole step (milliseconds).  No reference
+ * counterpart: the reference has wall-clock timing only (src/output.cpp:1451-1461). */
+typedef struct ie_profile {
+    double step_ms;            /* whole step, first to last event */
+    double reduce_ms;          /* Friedmann reduction (gradient/potential sums) */
+    double kick_drift_ms;      /* scalar kick + drift */
+    double tensor_ms;          /* tensor kick + drift (0 without SIGW) */
+    int64_t steps;
+} ie_profile;
+ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
+/* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904); RK45
+ * returns *accepted and the proposed next step. */
+ie_status ie_rk4_step(ie_run *run, double h);
+ie_status ie_rk45_step(ie_run *run, double h, double hmax, int *accepted, double *h_next);
+
+/* --- spectra ---------------------------------------------------------------- */
+/* Isotropic power spectrum of f or deltaN in the reference shells
+ * (write_isotropic_spectrum_from_fft_r2c, src/output.cpp:63-151): numbins =
+ * int(1.73205*(N/2))+1 entries of p, counts and vals = norm1 * <|F|^2>.  The
+ * forward FFT is out of place; the field is not modified. */
+int ie_spectrum_numbins(int N);
+ie_status ie_spectrum(ie_run *run, int which, double norm1, double *p, int *counts, double *vals);
+/* TT-projected tensor spectrum (write_gw_spectrum_impl, src/output.cpp:466-625):
+ * hdot = 0 for spectraGW, 1 for spectraGWdot (to_phys = B a^{s-1}).
+ * numbins = int(sqrt(3)*(N/2))+1; bins >= floor(0.8 numbins) are empty. */
+ie_status ie_gw_spectrum(ie_run *run, int hdot, double *p, int *counts, double *vals);
+
+/* --- deltaN ----------------------------------------------------------------- */
+/* initializeN (src/initialize.cpp:219-232): fd -> dphi/dN, deltaN = 0. */
+ie_status ie_deltaN_init(ie_run *run);
+/* evolve_fieldsN (src/evolution.cpp:1015-1037) with the current phiref. */
+ie_status ie_evolve_fieldsN(ie_run *run, double d);
+/* get_phiref (src/evolution.cpp:1271-1288): ordered search, first wins. */
+ie_status ie_get_phiref(ie_run *run, double *out);
+/* nsteps iterations of evolve_derivsN + evolve_fieldsN (src/evolution.cpp:1322-1327)
+ * kept in registers on the device: one read and one write of the state. */
+ie_status ie_deltaN_steps(ie_run *run, double dN, int64_t nsteps);
+/* The zeta map of saveN (src/output.cpp:1527-1542): mean of deltaN over sites
+ * with deltaN <= cutoff, accumulated in float in lattice order and divided by N^3
+ * in float as the reference does, then subtracted from every site.  *nmean
+ * receives that mean. */
+ie_status ie_deltaN_zeta(ie_run *run, double cutoff, float *nmean);
+/* Sites with deltaN > cutoff - nmean (not finished) set to 0, after the
+ * histogram (src/output.cpp:1549-1553); uses the mean of the last ie_deltaN_zeta. */
+ie_status ie_deltaN_zero_unfinished(ie_run *run, double cutoff);
+/* subtract the FP64 mean of deltaN (build_deltaN_linear, src/output.cpp:156-164) */
+ie_status ie_deltaN_center(ie_run *run);
+
+/* --- modes ------------------------------------------------------------------ */
+/* get_modes (src/output.cpp:674-738): B * mean lattice momentum per truncated shell. */
+ie_status ie_modes(int N, double L, double rescale_B, double *out);
+
+/* --- library ---------------------------------------------------------------- */
+const char *ie_version(void);
+int ie_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,913 @@
+// ie_api.hip -- C ABI of the gfx950 lattice engine (declared in include/ie_api.h).
+//
+// A run handle owns the device memory of one lattice and one HIP stream.  Every
+// entry point validates its arguments, enqueues work on the run's stream and
+// returns once its host-visible outputs are complete; errors are returned as
+// ie_status with a message kept in the handle (the reference prints and exits,
+// e.g. src/potential.cpp:19-27; the host driver maps the status back to that).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "ie_fft.h"
+#include "ie_internal.h"
+
+using namespace ie;
+
+struct ie_run {
+    ie_config cfg;
+    int N = 0, logN = 0;
+    size_t gs = 0;
+    double dx = 0.0;
+    hipStream_t stream = nullptr;
+    Fields F{};
+    Pot P{};
+    DevBg *bg = nullptr;         // device
+    DevBg *bg_host = nullptr;    // pinned mirror
+    double *partials = nullptr;  // reduction partials
+    double *table = nullptr;     // x, V, V' (device)
+    double *scratch = nullptr;   // N^3 doubles: split staging, then the deltaN map
+    FftPlan *fft = nullptr;
+    double *work = nullptr;      // 4096 doubles: small reductions
+    float *nmean = nullptr;      // saveN mean (device)
+    unsigned long long *hist = nullptr;
+    int hist_bins = 0;
+    double *dn_f = nullptr, *dn_fd = nullptr;  // FP64 deltaN state (split runs)
+    int in_deltaN = 0;
+    std::vector<double> htab_x, htab_v, htab_dv;   // host copy of the table
+    std::vector<void *> allocs;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+ie_status fail(ie_run *r, ie_status st, const std::string &msg)
+{
+    if (r) r->err = msg;
+    else g_create_error = msg;
+    return st;
+}
+
+#define IE_HIP(r, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail((r), IE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+ie_status dev_alloc(ie_run *r, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(r, IE_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    r->allocs.push_back(*p);
+    return IE_OK;
+}
+
+ie_status sync(ie_run *r)
+{
+    IE_HIP(r, hipStreamSynchronize(r->stream));
+    return IE_OK;
+}
+
+ie_status pull_bg(ie_run *r)
+{
+    IE_HIP(r, hipMemcpyAsync(r->bg_host, r->bg, sizeof(DevBg), hipMemcpyDeviceToHost, r->stream));
+    return sync(r);
+}
+
+ie_status push_bg(ie_run *r)
+{
+    IE_HIP(r, hipMemcpyAsync(r->bg, r->bg_host, sizeof(DevBg), hipMemcpyHostToDevice, r->stream));
+    return sync(r);
+}
+
+ie_status check_kernel_error(ie_run *r)
+{
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    if (r->bg_host->error) {
+        r->bg_host->error = 0;
+        push_bg(r);
+        return fail(r, IE_ERR_TABLE_RANGE, "field value outside the numerical potential range");
+    }
+    return IE_OK;
+}
+
+__global__ void k_loop_begin(DevBg *bg)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) bg->done = !(bg->a <= bg->af) ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ie_version(void) { return "inflation-easy-amd 0.1 (gfx950)"; }
+
+int ie_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char *ie_last_error(const ie_run *run) { return run ? run->err.c_str() : g_create_error.c_str(); }
+
+size_t ie_footprint_bytes(const ie_config *c)
+{
+    if (!c || !is_pow2(c->N)) return 0;
+    const size_t gs = (size_t)c->N * c->N * c->N;
+    const size_t es = c->precision == IE_SPLIT ? 4 : 8;
+    size_t b = 3 * gs * es;                        // f ping-pong + fd
+    if (c->calculate_sigw) b += 18 * gs * 4;       // h ping-pong + hd
+    b += gs * 8;                                   // scratch (downloads, deltaN)
+    b += ie::fft_workspace_bytes(c->N);            // spectra
+    b += 2048 * 8 * 8 + sizeof(DevBg) + 3 * (size_t)std::max(c->tab_n, 0) * 8;
+    return b;
+}
+
+ie_status ie_create(const ie_config *cfg, ie_run **out)
+{
+    if (!cfg || !out) return fail(nullptr, IE_ERR_ARG, "null argument");
+    *out = nullptr;
+    const ie_config &c = *cfg;
+    if (!is_pow2(c.N) || c.N < 8 || c.N > 1024)
+        return fail(nullptr, IE_ERR_ARG, "N must be a power of two in [8, 1024]");
+    if (c.precision != IE_FP64 && c.precision != IE_SPLIT) return fail(nullptr, IE_ERR_ARG, "bad precision");
+    if (c.numerical_potential && (c.tab_n < 2 || !c.tab_x || !c.tab_v || !c.tab_dv))
+        return fail(nullptr, IE_ERR_ARG, "Numerical potential table must contain at least two points.");
+    if (!(c.L > 0.0) || !(c.dt > 0.0)) return fail(nullptr, IE_ERR_ARG, "L and dt must be positive");
+    const double dx = c.L / (double)c.N;
+    if (c.dt > dx / sqrt(3.0)) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "Time step too large. Courant condition violated: dt/dx = %f", c.dt / dx);
+        return fail(nullptr, IE_ERR_COURANT, buf);
+    }
+    if (c.numerical_potential)
+        for (int q = 1; q < c.tab_n; ++q)
+            if (!(c.tab_x[q] < c.tab_x[q - 1]))
+                return fail(nullptr, IE_ERR_ARG, "tabulated field grid must be strictly descending");
+
+    ie_run *r = new ie_run();
+    r->cfg = c;
+    r->cfg.tab_x = r->cfg.tab_v = r->cfg.tab_dv = nullptr;
+    if (c.numerical_potential) {
+        r->htab_x.assign(c.tab_x, c.tab_x + c.tab_n);
+        r->htab_v.assign(c.tab_v, c.tab_v + c.tab_n);
+        r->htab_dv.assign(c.tab_dv, c.tab_dv + c.tab_n);
+        r->cfg.tab_x = r->htab_x.data();
+        r->cfg.tab_v = r->htab_v.data();
+        r->cfg.tab_dv = r->htab_dv.data();
+    }
+    r->N = c.N;
+    while ((1 << r->logN) < c.N) ++r->logN;
+    r->gs = (size_t)c.N * c.N * c.N;
+    r->dx = dx;
+    ie_status st = IE_OK;
+    auto bail = [&](ie_status s) {
+        ie_destroy(r);
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete r;
+        return fail(nullptr, IE_ERR_HIP, "hipStreamCreate failed");
+    }
+    const size_t es = c.precision == IE_SPLIT ? sizeof(float) : sizeof(double);
+    r->F.N = c.N;
+    r->F.logN = r->logN;
+    r->F.sigw = c.calculate_sigw ? 1 : 0;
+    for (int p = 0; p < 2; ++p)
+        if ((st = dev_alloc(r, &r->F.f[p], r->gs * es)) != IE_OK) return bail(st);
+    if ((st = dev_alloc(r, &r->F.fd, r->gs * es)) != IE_OK) return bail(st);
+    if (r->F.sigw) {
+        for (int q = 0; q < 6; ++q) {
+            for (int p = 0; p < 2; ++p)
+                if ((st = dev_alloc(r, (void **)&r->F.h[p][q], r->gs * 4)) != IE_OK) return bail(st);
+            if ((st = dev_alloc(r, (void **)&r->F.hd[q], r->gs * 4)) != IE_OK) return bail(st);
+        }
+    }
+    if ((st = dev_alloc(r, (void **)&r->scratch, r->gs * 8)) != IE_OK) return bail(st);
+    if ((st = dev_alloc(r, (void **)&r->partials, 2048 * 8 * 8)) != IE_OK) return bail(st);
+    if ((st = dev_alloc(r, (void **)&r->bg, sizeof(DevBg))) != IE_OK) return bail(st);
+    if ((st = dev_alloc(r, (void **)&r->work, 4096 * 8)) != IE_OK) return bail(st);
+    if ((st = dev_alloc(r, (void **)&r->nmean, 256)) != IE_OK) return bail(st);
+    if (hipHostMalloc((void **)&r->bg_host, sizeof(DevBg), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(nullptr, IE_ERR_OOM, "hipHostMalloc failed"));
+    r->F.dn = r->scratch;
+    for (int p = 0; p < 2; ++p) {
+        if (hipMemsetAsync(r->F.f[p], 0, r->gs * es, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
+    }
+    if (hipMemsetAsync(r->F.fd, 0, r->gs * es, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
+    if (r->F.sigw)
+        for (int q = 0; q < 6; ++q) {
+            for (int p = 0; p < 2; ++p)
+                if (hipMemsetAsync(r->F.h[p][q], 0, r->gs * 4, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
+            if (hipMemsetAsync(r->F.hd[q], 0, r->gs * 4, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
+        }
+    r->P.numerical = c.numerical_potential ? 1 : 0;
+    r->P.V0 = c.V0;
+    r->P.ns = c.ns;
+    r->P.B2 = c.rescale_B * c.rescale_B;
+    r->P.n = 0;
+    if (c.numerical_potential) {
+        const size_t n = (size_t)c.tab_n;
+        if ((st = dev_alloc(r, (void **)&r->table, 3 * n * 8)) != IE_OK) return bail(st);
+        if (hipMemcpy(r->table, c.tab_x, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(r->table + n, c.tab_v, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(r->table + 2 * n, c.tab_dv, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(nullptr, IE_ERR_HIP, "table upload failed"));
+        r->P.n = c.tab_n;
+        r->P.x = r->table;
+        r->P.v = r->table + n;
+        r->P.dv = r->table + 2 * n;
+    }
+    DevBg &b = *r->bg_host;
+    memset(&b, 0, sizeof(b));
+    b.a = 1.0;
+    b.astep = 1.0;
+    b.dt = c.dt;
+    b.af = c.af;
+    b.rescale_s = c.rescale_s;
+    b.dx = dx;
+    if (push_bg(r) != IE_OK) return bail(IE_ERR_HIP);
+    r->fft = fft_create(c.N);
+    if (!r->fft) return bail(fail(nullptr, IE_ERR_OOM, "FFT workspace allocation failed"));
+    *out = r;
+    return IE_OK;
+}
+
+ie_status ie_destroy(ie_run *r)
+{
+    if (!r) return IE_OK;
+    if (r->stream) hipStreamSynchronize(r->stream);
+    if (r->fft) fft_destroy(r->fft);
+    for (void *p : r->allocs) hipFree(p);
+    if (r->bg_host) hipHostFree(r->bg_host);
+    if (r->stream) hipStreamDestroy(r->stream);
+    delete r;
+    return IE_OK;
+}
+
+ie_status ie_upload_fields(ie_run *r, const double *f, const double *fd)
+{
+    if (!r || !f || !fd) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    r->bg_host->parity = 0;
+    if ((st = push_bg(r)) != IE_OK) return st;
+    const size_t bytes = r->gs * 8;
+    if (r->cfg.precision == IE_FP64) {
+        IE_HIP(r, hipMemcpyAsync(r->F.f[0], f, bytes, hipMemcpyHostToDevice, r->stream));
+        IE_HIP(r, hipMemcpyAsync(r->F.fd, fd, bytes, hipMemcpyHostToDevice, r->stream));
+        return sync(r);
+    }
+    // split: stage both fields in FP64 (scratch + the unused second f buffer is too
+    // small in FP32), compute the homogeneous mode on the device, store fluctuations
+    double *tmp = nullptr;
+    IE_HIP(r, hipMallocAsync((void **)&tmp, bytes, r->stream));
+    IE_HIP(r, hipMemcpyAsync(r->scratch, f, bytes, hipMemcpyHostToDevice, r->stream));
+    IE_HIP(r, hipMemcpyAsync(tmp, fd, bytes, hipMemcpyHostToDevice, r->stream));
+    IE_HIP(r, launch_split_upload(r->scratch, tmp, r->F, r->bg, r->partials, r->stream));
+    IE_HIP(r, hipFreeAsync(tmp, r->stream));
+    return sync(r);
+}
+
+ie_status ie_download_field(ie_run *r, int which, double *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    if (which == IE_FIELD_DELTAN) {
+        IE_HIP(r, hipMemcpyAsync(out, r->F.dn, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
+        return sync(r);
+    }
+    if (which != IE_FIELD_F && which != IE_FIELD_FD) return fail(r, IE_ERR_ARG, "bad field");
+    if (r->in_deltaN) {
+        ie_status st = pull_bg(r);
+        if (st != IE_OK) return st;
+        const int split = r->cfg.precision == IE_SPLIT;
+        const double *src = which == IE_FIELD_F ? (split ? r->dn_f : (const double *)r->F.f[r->bg_host->parity])
+                                                : (split ? r->dn_fd : (const double *)r->F.fd);
+        IE_HIP(r, hipMemcpyAsync(out, src, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
+        return sync(r);
+    }
+    double *tmp = nullptr;
+    IE_HIP(r, hipMallocAsync((void **)&tmp, r->gs * 8, r->stream));
+    IE_HIP(r, launch_reconstitute(r->F, r->cfg.precision == IE_SPLIT, which, r->bg, tmp, r->stream));
+    IE_HIP(r, hipMemcpyAsync(out, tmp, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
+    IE_HIP(r, hipFreeAsync(tmp, r->stream));
+    return sync(r);
+}
+
+ie_status ie_upload_tensor(ie_run *r, int comp, const float *h)
+{
+    if (!r || !h || comp < 0 || comp > 11) return fail(r, IE_ERR_ARG, "bad argument");
+    if (!r->F.sigw) return fail(r, IE_ERR_STATE, "run has no tensors (calculate_sigw = 0)");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    float *dst = comp < 6 ? r->F.h[r->bg_host->parity][comp] : r->F.hd[comp - 6];
+    IE_HIP(r, hipMemcpyAsync(dst, h, r->gs * 4, hipMemcpyHostToDevice, r->stream));
+    return sync(r);
+}
+
+ie_status ie_download_tensor(ie_run *r, int comp, float *out)
+{
+    if (!r || !out || comp < 0 || comp > 11) return fail(r, IE_ERR_ARG, "bad argument");
+    if (!r->F.sigw) return fail(r, IE_ERR_STATE, "run has no tensors (calculate_sigw = 0)");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const float *src = comp < 6 ? r->F.h[r->bg_host->parity][comp] : r->F.hd[comp - 6];
+    IE_HIP(r, hipMemcpyAsync(out, src, r->gs * 4, hipMemcpyDeviceToHost, r->stream));
+    return sync(r);
+}
+
+ie_status ie_get_background(ie_run *r, ie_background *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const DevBg &b = *r->bg_host;
+    out->t = b.t;
+    out->a = b.a;
+    out->ad = b.ad;
+    out->ad2 = b.ad2;
+    out->astep = b.astep;
+    out->Ne = b.Ne;
+    out->phiref = b.phiref;
+    out->hubble_init = b.hubble_init;
+    out->dt_last = b.d;
+    out->steps = b.steps;
+    out->done = b.done;
+    out->error = b.error ? IE_ERR_TABLE_RANGE : IE_OK;
+    return IE_OK;
+}
+
+ie_status ie_set_background(ie_run *r, const ie_background *in)
+{
+    if (!r || !in) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    b.t = in->t;
+    b.a = in->a;
+    b.ad = in->ad;
+    b.ad2 = in->ad2;
+    b.astep = in->astep;
+    b.Ne = in->Ne;
+    b.phiref = in->phiref;
+    b.hubble_init = in->hubble_init;
+    b.steps = in->steps;
+    b.done = in->done;
+    b.error = 0;
+    return push_bg(r);
+}
+
+static ie_status sums(ie_run *r)
+{
+    IE_HIP(r, launch_field_sums(r->F, r->cfg.precision == IE_SPLIT, r->P, r->bg, r->partials, r->stream));
+    return check_kernel_error(r);
+}
+
+ie_status ie_gradient_energy(ie_run *r, double *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = sums(r);
+    if (st != IE_OK) return st;
+    const DevBg &b = *r->bg_host;
+    const double norm = (1.0 / (b.a * r->dx)) * (1.0 / (b.a * r->dx));
+    *out = 0.5 * b.red[4] * norm / (double)r->gs;
+    return IE_OK;
+}
+
+ie_status ie_kin_energy(ie_run *r, double *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = sums(r);
+    if (st != IE_OK) return st;
+    const DevBg &b = *r->bg_host;
+    double s2 = b.red[3];
+    if (r->cfg.precision == IE_SPLIT) {
+        // sum (pibar + dpi)^2 = N^3 pibar^2 + 2 pibar sum dpi + sum dpi^2
+        s2 = (double)r->gs * b.pibar * b.pibar + 2.0 * b.pibar * b.red[1] + b.red[3];
+    }
+    s2 /= (double)r->gs;
+    *out = 0.5 * pow(b.a, 2.0 * r->cfg.rescale_s - 2.0) * s2;
+    return IE_OK;
+}
+
+ie_status ie_potential_energy(ie_run *r, double *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = sums(r);
+    if (st != IE_OK) return st;
+    *out = r->bg_host->red[5] / (double)r->gs;
+    return IE_OK;
+}
+
+ie_status ie_field_sums(ie_run *r, double out[3])
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = sums(r);
+    if (st != IE_OK) return st;
+    const DevBg &b = *r->bg_host;
+    if (r->cfg.precision == IE_SPLIT) {
+        const double n = (double)r->gs;
+        out[0] = n * b.phibar + b.red[0];
+        out[1] = n * b.pibar + b.red[1];
+        out[2] = n * b.phibar * b.phibar + 2.0 * b.phibar * b.red[0] + b.red[2];
+    } else {
+        out[0] = b.red[0];
+        out[1] = b.red[1];
+        out[2] = b.red[2];
+    }
+    return IE_OK;
+}
+
+ie_status ie_evolve_derivs(ie_run *r, double d)
+{
+    if (!r || !(d != 0.0)) return fail(r, IE_ERR_ARG, "bad step");
+    const int split = r->cfg.precision == IE_SPLIT;
+    IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, d, 0, r->stream));
+    IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICK, d, 0, r->stream));
+    return check_kernel_error(r);
+}
+
+ie_status ie_evolve_fields(ie_run *r, double d)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null argument");
+    if (d == 0.0) return IE_OK;
+    const int split = r->cfg.precision == IE_SPLIT;
+    IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, DRIFT, d, 0, r->stream));
+    return sync(r);
+}
+
+ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
+{
+    if (!r || max_steps < 0) return fail(r, IE_ERR_ARG, "bad argument");
+    const int split = r->cfg.precision == IE_SPLIT;
+    k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+    IE_HIP(r, hipGetLastError());
+    for (int n = 0; n < max_steps; ++n) {
+        IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+        IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+    }
+    ie_status st = check_kernel_error(r);
+    if (st != IE_OK) return st;
+    if (out) return ie_get_background(r, out);
+    return IE_OK;
+}
+
+ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
+{
+    if (!r || nsteps < 1 || !out) return fail(r, IE_ERR_ARG, "bad argument");
+    const int split = r->cfg.precision == IE_SPLIT;
+    const int nev = r->F.sigw ? 4 : 3;
+    std::vector<hipEvent_t> ev((size_t)nsteps * nev);
+    for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
+    k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+    for (int n = 0; n < nsteps; ++n) {
+        hipEvent_t *e = &ev[(size_t)n * nev];
+        IE_HIP(r, hipEventRecord(e[0], r->stream));
+        IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+        IE_HIP(r, hipEventRecord(e[1], r->stream));
+        IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream, 0));
+        IE_HIP(r, hipEventRecord(e[2], r->stream));
+        if (r->F.sigw) {
+            IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream, 1));
+            IE_HIP(r, hipEventRecord(e[3], r->stream));
+        }
+    }
+    ie_status st = sync(r);
+    if (st != IE_OK) return st;
+    double red = 0, kd = 0, ten = 0, step = 0;
+    for (int n = 0; n < nsteps; ++n) {
+        hipEvent_t *e = &ev[(size_t)n * nev];
+        float ms = 0.f;
+        IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
+        red += ms;
+        IE_HIP(r, hipEventElapsedTime(&ms, e[1], e[2]));
+        kd += ms;
+        if (r->F.sigw) {
+            IE_HIP(r, hipEventElapsedTime(&ms, e[2], e[3]));
+            ten += ms;
+        }
+        IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[nev - 1]));
+        step += ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    out->reduce_ms = red / nsteps;
+    out->kick_drift_ms = kd / nsteps;
+    out->tensor_ms = ten / nsteps;
+    out->step_ms = step / nsteps;
+    out->steps = r->bg_host->steps;
+    return IE_OK;
+}
+
+int ie_spectrum_numbins(int N) { return (int)(1.73205 * (N / 2)) + 1; }
+
+ie_status ie_modes(int N, double L, double B, double *out)
+{
+    if (!is_pow2(N) || !out) return fail(nullptr, IE_ERR_ARG, "bad argument");
+    // get_modes (src/output.cpp:674-738): closed form per shell, host side
+    const double pi = 2.0 * asin(1.0);
+    const int nb = (int)(sqrt(3.0) * (N / 2)) + 1;
+    std::vector<int> np(nb, 0);
+    std::vector<double> pp(nb, 0.0);
+    auto shell = [&](int px, int py, int pz, int w) {
+        const double pm = sqrt((double)px * px + (double)py * py + (double)pz * pz);
+        const double sx = sin((double)px * pi / N), sy = sin((double)py * pi / N), sz = sin((double)pz * pi / N);
+        const double ph = sqrt(4.0 * (((double)N / L) * ((double)N / L)) * (sx * sx + sy * sy + sz * sz));
+        np[(int)pm] += w;
+        pp[(int)pm] += w == 2 ? 2.0 * ph : ph;
+    };
+    for (int i = 0; i < N; ++i) {
+        const int px = i <= N / 2 ? i : i - N;
+        for (int j = 0; j < N; ++j) {
+            const int py = j <= N / 2 ? j : j - N;
+            for (int k = 1; k < N / 2; ++k) shell(px, py, k, 2);   // weight-2 interior modes
+            shell(px, py, 0, 1);                                  // kz = 0
+            shell(px, py, N / 2, 1);                              // kz = N/2
+        }
+    }
+    for (int b = 0; b < nb; ++b) out[b] = B * (np[b] > 0 ? pp[b] / np[b] : pp[b]);
+    return IE_OK;
+}
+
+/* ------------------------------------------------------------- spectra */
+
+ie_status ie_spectrum(ie_run *r, int which, double norm1, double *p, int *counts, double *vals)
+{
+    if (!r || !p || !counts || !vals) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int split = r->cfg.precision == IE_SPLIT;
+    double dc = 0.0;
+    if (which == IE_FIELD_DELTAN) {
+        IE_HIP(r, fft_forward_double(r->fft, r->F.dn, 0, r->stream));
+    } else if (which == IE_FIELD_F || which == IE_FIELD_FD) {
+        const int par = r->bg_host->parity;
+        if (r->in_deltaN) {
+            const double *src = which == IE_FIELD_F ? (split ? r->dn_f : (const double *)r->F.f[par])
+                                                    : (split ? r->dn_fd : (const double *)r->F.fd);
+            IE_HIP(r, fft_forward_double(r->fft, src, 0, r->stream));
+        } else if (split) {
+            const float *src = (const float *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd);
+            dc = (double)r->gs * (which == IE_FIELD_F ? r->bg_host->phibar : r->bg_host->pibar);
+            IE_HIP(r, fft_forward_float(r->fft, src, dc, 0, r->stream));
+        } else {
+            const double *src = (const double *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd);
+            IE_HIP(r, fft_forward_double(r->fft, src, 0, r->stream));
+        }
+    } else {
+        return fail(r, IE_ERR_ARG, "bad field");
+    }
+    const int nb = fft_numbins(r->N);
+    std::vector<double> sums(nb);
+    IE_HIP(r, fft_bin_scalar_dc(r->fft, 0, dc, sums.data(), r->stream));
+    fft_counts(r->N, counts);
+    const double pi = 2.0 * asin(1.0), dp = 2.0 * pi / r->cfg.L;
+    for (int b = 0; b < nb; ++b) {
+        p[b] = dp * b;
+        double v = sums[b];
+        if (counts[b] > 0) v /= (double)counts[b];
+        vals[b] = norm1 * v;
+    }
+    return IE_OK;
+}
+
+ie_status ie_gw_spectrum(ie_run *r, int hdot, double *p, int *counts, double *vals)
+{
+    if (!r || !p || !counts || !vals) return fail(r, IE_ERR_ARG, "null argument");
+    if (!r->F.sigw) return fail(r, IE_ERR_STATE, "run has no tensors (calculate_sigw = 0)");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int par = r->bg_host->parity;
+    for (int c = 0; c < 6; ++c)
+        IE_HIP(r, fft_forward_float(r->fft, hdot ? r->F.hd[c] : r->F.h[par][c], 0.0, c, r->stream));
+    const double to_phys = hdot ? r->cfg.rescale_B * pow(r->bg_host->a, r->cfg.rescale_s - 1.0) : 1.0;
+    const int nb = fft_gw_numbins(r->N);
+    std::vector<double> sums(nb);
+    IE_HIP(r, fft_bin_tt(r->fft, r->dx, to_phys, sums.data(), r->stream));
+    fft_gw_counts(r->N, counts);
+    // output units follow src/output.cpp:475,618-621 (float p and norm1)
+    const float dp = 2.f * (float)(2.0 * asin(1.0)) / (float)r->cfg.L;
+    const float norm1 = 0.5f * pow(r->cfg.L / r->cfg.rescale_B, 3) / pow((float)r->N, 6);
+    for (int b = 0; b < nb; ++b) {
+        p[b] = (double)(dp * b);
+        double v = sums[b];
+        if (counts[b] > 0) v /= (double)counts[b];
+        vals[b] = (double)norm1 * v;
+    }
+    return IE_OK;
+}
+
+ie_status ie_upload_modes(ie_run *r, const double *fk, const double *fny, const double *fdk, const double *fdny,
+                          double phi0, double dphi0)
+{
+    if (!r || !fk || !fny || !fdk || !fdny) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    r->bg_host->parity = 0;
+    if ((st = push_bg(r)) != IE_OK) return st;
+    if (r->cfg.precision == IE_FP64) {
+        IE_HIP(r, fft_inverse_ffteasy(r->fft, fk, fny, phi0, (double *)r->F.f[0], r->stream));
+        IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, (double *)r->F.fd, r->stream));
+        return sync(r);
+    }
+    double *tmp = nullptr;
+    IE_HIP(r, hipMallocAsync((void **)&tmp, r->gs * 8, r->stream));
+    IE_HIP(r, fft_inverse_ffteasy(r->fft, fk, fny, phi0, r->scratch, r->stream));
+    IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, tmp, r->stream));
+    IE_HIP(r, launch_split_upload(r->scratch, tmp, r->F, r->bg, r->partials, r->stream));
+    IE_HIP(r, hipFreeAsync(tmp, r->stream));
+    return sync(r);
+}
+
+ie_status ie_initialize(ie_run *r, const ie_ic_params *ic, ie_background *out)
+{
+    if (!r || !ic) return fail(r, IE_ERR_ARG, "null argument");
+    const size_t ny = (size_t)r->N * 2 * r->N;
+    std::vector<double> fk(r->gs), fdk(r->gs), fny(ny), fdny(ny);
+    double H = 0.0;
+    ie_status st = ie_fill_modes(&r->cfg, ic, fk.data(), fny.data(), fdk.data(), fdny.data(), &H);
+    if (st == IE_ERR_COURANT) {
+        char buf[160];
+        snprintf(buf, sizeof(buf), "Time step too large. Courant condition violated: dt/dx = %f", r->cfg.dt / r->dx);
+        return fail(r, st, buf);
+    }
+    if (st == IE_ERR_TABLE_RANGE) return fail(r, st, "initial_field outside the numerical potential range");
+    if (st != IE_OK) return fail(r, st, "mode fill failed");
+    DevBg &b = *r->bg_host;
+    if ((st = pull_bg(r)) != IE_OK) return st;
+    b.t = 0.0;
+    b.a = 1.0;
+    b.ad = H;
+    b.ad2 = 0.0;
+    b.astep = 1.0;
+    b.Ne = 0.0;
+    b.phiref = 0.0;
+    b.steps = 0;
+    b.done = 0;
+    b.error = 0;
+    if ((st = push_bg(r)) != IE_OK) return st;
+    st = ie_upload_modes(r, fk.data(), fny.data(), fdk.data(), fdny.data(), ic->initial_field,
+                         ic->initial_derivative);
+    if (st != IE_OK) return st;
+    // H0 from the lattice energies (src/initialize.cpp:202-212)
+    if ((st = sums(r)) != IE_OK) return st;
+    double S2 = b.red[3];
+    if (r->cfg.precision == IE_SPLIT) S2 = (double)r->gs * b.pibar * b.pibar + 2.0 * b.pibar * b.red[1] + b.red[3];
+    const double kin = 0.5 * S2 / (double)r->gs;
+    const double pe = b.red[5] / (double)r->gs;
+    const double ge = 0.5 * b.red[4] * ((1.0 / r->dx) * (1.0 / r->dx)) / (double)r->gs;
+    b.hubble_init = sqrt((kin + pe + ge) / 3.0);
+    b.ad = b.hubble_init;
+    if ((st = push_bg(r)) != IE_OK) return st;
+    if (out) return ie_get_background(r, out);
+    return IE_OK;
+}
+
+/* ------------------------------------------------------------- deltaN */
+
+static ie_status dn_state(ie_run *r, double **f, double **fd)
+{
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    if (r->cfg.precision == IE_SPLIT) {
+        *f = r->dn_f;
+        *fd = r->dn_fd;
+    } else {
+        *f = (double *)r->F.f[r->bg_host->parity];
+        *fd = (double *)r->F.fd;
+    }
+    return IE_OK;
+}
+
+static int dn_error(ie_run *r)
+{
+    int e = 0;
+    (void)hipMemcpyAsync(&e, &r->bg->error, sizeof(int), hipMemcpyDeviceToHost, r->stream);
+    (void)hipStreamSynchronize(r->stream);
+    return e;
+}
+
+static double ref_potential(const ie_run *r, double v, bool *ok)
+{
+    // potential(phiref) (src/potential.cpp:91-98) on the host
+    const ie_config &c = r->cfg;
+    *ok = true;
+    const double B2 = c.rescale_B * c.rescale_B;
+    if (!c.numerical_potential) return c.V0 * (1.0 - (1.0 - c.ns) / 4.0 * (v * v)) / B2;
+    std::vector<double> x(c.tab_n);
+    if (hipMemcpy(x.data(), r->table, c.tab_n * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+        *ok = false;
+        return 0.0;
+    }
+    if (v > x[0] || v < x[c.tab_n - 1]) {
+        *ok = false;
+        return 0.0;
+    }
+    int lo = 0, hi = c.tab_n - 1;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (x[mid] >= v) lo = mid;
+        else hi = mid;
+    }
+    double y[2];
+    hipMemcpy(y, r->table + c.tab_n + hi - 1, 16, hipMemcpyDeviceToHost);
+    return (y[1] + (v - x[hi]) * (y[0] - y[1]) / (x[hi - 1] - x[hi])) / B2;
+}
+
+ie_status ie_deltaN_init(ie_run *r)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    if (r->cfg.precision == IE_SPLIT) {
+        if (!r->dn_f) {
+            if ((st = dev_alloc(r, (void **)&r->dn_f, r->gs * 8)) != IE_OK) return st;
+            if ((st = dev_alloc(r, (void **)&r->dn_fd, r->gs * 8)) != IE_OK) return st;
+        }
+        IE_HIP(r, launch_reconstitute(r->F, 1, 0, r->bg, r->dn_f, r->stream));
+        IE_HIP(r, launch_reconstitute(r->F, 1, 1, r->bg, r->dn_fd, r->stream));
+    }
+    r->in_deltaN = 1;
+    double *f, *fd;
+    if ((st = dn_state(r, &f, &fd)) != IE_OK) return st;
+    // a^{s-1} on the host with the C library's pow, as the reference
+    const double apow = pow(r->bg_host->a, r->cfg.rescale_s - 1.0);
+    IE_HIP(r, dn_init(r->F, f, fd, r->F.dn, r->P, apow, &r->bg->error, r->stream));
+    if (dn_error(r)) return check_kernel_error(r);
+    return sync(r);
+}
+
+ie_status ie_evolve_fieldsN(ie_run *r, double d)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null argument");
+    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    double *f, *fd;
+    ie_status st = dn_state(r, &f, &fd);
+    if (st != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    bool ok = true;
+    const double Vref = r->cfg.monotonic == IE_GENERIC ? ref_potential(r, b.phiref, &ok) : 0.0;
+    if (!ok) return fail(r, IE_ERR_TABLE_RANGE, "phiref outside the numerical potential range");
+    IE_HIP(r, dn_fieldsN(r->gs, f, fd, r->F.dn, r->P, r->cfg.monotonic, b.phiref, Vref, d, r->stream));
+    b.t += d;
+    b.a += d * b.ad;
+    return push_bg(r);
+}
+
+ie_status ie_get_phiref(ie_run *r, double *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    double *f, *fd;
+    ie_status st = dn_state(r, &f, &fd);
+    if (st != IE_OK) return st;
+    IE_HIP(r, dn_phiref(r->gs, f, r->P, r->cfg.monotonic, r->work, r->work + 4000, r->stream));
+    IE_HIP(r, hipMemcpyAsync(out, r->work + 4000, 8, hipMemcpyDeviceToHost, r->stream));
+    return sync(r);
+}
+
+ie_status ie_deltaN_steps(ie_run *r, double dN, int64_t nsteps)
+{
+    if (!r || nsteps < 0) return fail(r, IE_ERR_ARG, "bad argument");
+    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    double *f, *fd;
+    ie_status st = dn_state(r, &f, &fd);
+    if (st != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    bool ok = true;
+    const double Vref = r->cfg.monotonic == IE_GENERIC ? ref_potential(r, b.phiref, &ok) : 0.0;
+    if (!ok) return fail(r, IE_ERR_TABLE_RANGE, "phiref outside the numerical potential range");
+    IE_HIP(r, dn_steps(r->gs, f, fd, r->F.dn, r->P, r->cfg.monotonic, b.phiref, Vref, dN, (long long)nsteps,
+                       &r->bg->error, r->stream));
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    // t and a advance with every step (src/evolution.cpp:1017,1036)
+    for (int64_t q = 0; q < nsteps; ++q) {
+        r->bg_host->t += dN;
+        r->bg_host->a += dN * r->bg_host->ad;
+    }
+    return push_bg(r);
+}
+
+ie_status ie_deltaN_zeta(ie_run *r, double cutoff, float *nmean)
+{
+    if (!r || !nmean) return fail(r, IE_ERR_ARG, "null argument");
+    IE_HIP(r, dn_float_mean(r->F.dn, r->gs, cutoff, r->nmean, r->stream));
+    IE_HIP(r, dn_zeta(r->F.dn, r->gs, r->nmean, cutoff, 0, r->stream));
+    IE_HIP(r, hipMemcpyAsync(nmean, r->nmean, 4, hipMemcpyDeviceToHost, r->stream));
+    return sync(r);
+}
+
+ie_status ie_deltaN_zero_unfinished(ie_run *r, double cutoff)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null argument");
+    IE_HIP(r, dn_zeta(r->F.dn, r->gs, r->nmean, cutoff, 1, r->stream));
+    return sync(r);
+}
+
+ie_status ie_deltaN_center(ie_run *r)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null argument");
+    IE_HIP(r, field_center(r->F.dn, r->gs, r->work, r->work + 4000, r->stream));
+    return sync(r);
+}
+
+ie_status ie_histogram(ie_run *r, int which, int nbins, double *bmin, double *bmax, double *counts)
+{
+    if (!r || !bmin || !bmax || !counts || nbins < 1) return fail(r, IE_ERR_ARG, "bad argument");
+    const double *g = nullptr;
+    ie_status st;
+    if (which == IE_FIELD_DELTAN) g = r->F.dn;
+    else if (which == IE_FIELD_F) {
+        if (r->in_deltaN) {
+            double *f, *fd;
+            if ((st = dn_state(r, &f, &fd)) != IE_OK) return st;
+            g = f;
+        } else {
+            if (r->cfg.precision == IE_SPLIT) {
+                IE_HIP(r, launch_reconstitute(r->F, 1, 0, r->bg, r->scratch, r->stream));
+                g = r->scratch;
+            } else {
+                if ((st = pull_bg(r)) != IE_OK) return st;
+                g = (const double *)r->F.f[r->bg_host->parity];
+            }
+        }
+    } else return fail(r, IE_ERR_ARG, "bad field");
+    int nb = 0;
+    IE_HIP(r, field_minmax(g, r->gs, r->work, r->stream, &nb));
+    std::vector<double> mm(2 * nb);
+    IE_HIP(r, hipMemcpyAsync(mm.data(), r->work, 2 * nb * 8, hipMemcpyDeviceToHost, r->stream));
+    if ((st = sync(r)) != IE_OK) return st;
+    double lo = mm[0], hi = mm[nb];
+    for (int q = 0; q < nb; ++q) {
+        lo = mm[q] < lo ? mm[q] : lo;
+        hi = mm[nb + q] > hi ? mm[nb + q] : hi;
+    }
+    double df = (hi - lo) / (double)nbins;
+    if (!std::isfinite(df) || df <= 0.0) df = 1.0;
+    if (r->hist_bins < nbins) {
+        if (r->hist) (void)hipFree(r->hist);
+        IE_HIP(r, hipMalloc((void **)&r->hist, (size_t)nbins * 8));
+        r->hist_bins = nbins;
+    }
+    IE_HIP(r, hipMemsetAsync(r->hist, 0, (size_t)nbins * 8, r->stream));
+    IE_HIP(r, field_histogram(g, r->gs, lo, hi, df, nbins, r->hist, r->stream));
+    std::vector<unsigned long long> c(nbins);
+    IE_HIP(r, hipMemcpyAsync(c.data(), r->hist, (size_t)nbins * 8, hipMemcpyDeviceToHost, r->stream));
+    if ((st = sync(r)) != IE_OK) return st;
+    for (int q = 0; q < nbins; ++q) counts[q] = (double)c[q];
+    *bmin = lo;
+    *bmax = hi;
+    return IE_OK;
+}
+
+ie_status ie_rk4_step(ie_run *r, double h)
+{
+    (void)h;
+    return fail(r, IE_ERR_STATE, "RK4 inflation stepping is not available on the GPU path yet");
+}
+
+ie_status ie_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *h_next)
+{
+    (void)h;
+    (void)hmax;
+    (void)accepted;
+    (void)h_next;
+    return fail(r, IE_ERR_STATE, "RK45 inflation stepping is not available on the GPU path yet");
+}
+
+}  // extern "C"
+
+// accessors for the other translation units of the library
+namespace ie {
+ie_status run_internal(ie_run *r, Fields **F, Pot **P, DevBg **bg, DevBg **bgh, double **partials,
+                       double **scratch, FftPlan **fft, hipStream_t *s)
+{
+    *F = &r->F;
+    *P = &r->P;
+    *bg = r->bg;
+    *bgh = r->bg_host;
+    *partials = r->partials;
+    *scratch = r->scratch;
+    *fft = r->fft;
+    *s = r->stream;
+    return IE_OK;
+}
+ie_status run_fail(ie_run *r, ie_status st, const std::string &msg) { return fail(r, st, msg); }
+const ie_config &run_config(const ie_run *r) { return r->cfg; }
+ie_status run_pull_bg(ie_run *r) { return pull_bg(r); }
+ie_status run_push_bg(ie_run *r) { return push_bg(r); }
+}  // namespace ie

@@ -1,0 +1,449 @@
+// ie_deltaN.hip -- separate-universe deltaN map on gfx950.
+//
+// The deltaN phase of the reference (run_deltaN_loop, src/evolution.cpp:1291-1375)
+// integrates an independent ODE per site in e-fold time:
+//   pi' = -(3 - pi^2/2)(pi + V'/V),  phi' = pi,  deltaN' = 1   while the site is
+// short of the phiref surface (src/evolution.cpp:1004-1037).  There is no
+// neighbour coupling, so k_deltaN_steps keeps (phi, pi, deltaN) of a site in
+// registers for all steps: one read and one write of the state per launch instead
+// of two lattice sweeps per step.  A site that reaches the surface is frozen: its
+// phi and deltaN never change again in the reference either; its pi, which the
+// reference keeps integrating but never reads again, is left as it was.
+//
+// The site arithmetic keeps the reference's operation order with FP contraction
+// off, so the map equals the reference's bit for bit given the same input state.
+// The tabulated potential is staged in LDS once per workgroup and searched with
+// a per-site hint (the register analogue of lstart, src/potential.cpp:40-42); the
+// bracket found is the unique l with x[l-1] >= phi > x[l], as in the reference.
+#include <math.h>
+
+#include "ie_internal.h"
+
+namespace ie {
+
+namespace {
+
+constexpr int kMaxTable = 2048;   // table rows staged in LDS (3 x 2048 doubles = 48 KB)
+
+struct DnParams {
+    int mono;          // 1 monotonic, -1 antimonotonic, 0 generic
+    double phiref, Vref;
+    Pot P;
+};
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double hill_V(const Pot &P, double v)
+{
+    return P.V0 * (1.0 - (1.0 - P.ns) / 4.0 * (v * v)) / P.B2;
+}
+__device__ __forceinline__ double hill_dV(const Pot &P, double v)
+{
+    return -P.V0 * (1.0 - P.ns) / 2.0 * v / P.B2;
+}
+
+// Hinted bracket walk in an LDS table: returns l with x[l-1] >= v > x[l].
+__device__ __forceinline__ int walk(const double *x, int n, double v, int l)
+{
+    if (l < 1) l = 1;
+    if (l > n - 1) l = n - 1;
+    while (l < n - 1 && x[l] >= v) ++l;
+    while (l > 1 && x[l - 1] < v) --l;
+    return l;
+}
+
+__device__ __forceinline__ double interp(const double *x, const double *y, double v, int l)
+{
+    const double x0 = x[l - 1], x1 = x[l], y0 = y[l - 1], y1 = y[l];
+    return y1 + (v - x1) * (y0 - y1) / (x0 - x1);
+}
+
+__device__ __forceinline__ bool active(const DnParams &D, double v, const double *x, const double *tv, int n, int *hint)
+{
+    if (D.mono > 0) return fabs(v) > fabs(D.phiref);
+    if (D.mono < 0) return fabs(v) < fabs(D.phiref);
+    double V;
+    if (D.P.numerical) {
+        *hint = walk(x, n, v, *hint);
+        V = interp(x, tv, v, *hint) / D.P.B2;
+    } else {
+        V = hill_V(D.P, v);
+    }
+    return V > D.Vref;
+}
+
+__device__ __forceinline__ bool in_table(const double *x, int n, double v) { return !(v > x[0] || v < x[n - 1]); }
+
+__global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f, double *__restrict__ fd,
+                                                          double *__restrict__ dn, size_t gs, DnParams D,
+                                                          double dN, long long nsteps, int *error)
+{
+    __shared__ double tx[kMaxTable], tv[kMaxTable], tdv[kMaxTable];
+    const int n = D.P.numerical ? D.P.n : 0;
+    const bool lds = n > 0 && n <= kMaxTable;
+    for (int e = threadIdx.x; lds && e < n; e += blockDim.x) {
+        tx[e] = D.P.x[e];
+        tv[e] = D.P.v[e];
+        tdv[e] = D.P.dv[e];
+    }
+    __syncthreads();
+    const double *x = lds ? tx : D.P.x, *yv = lds ? tv : D.P.v, *ydv = lds ? tdv : D.P.dv;
+    int err = 0;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        double phi = f[id], pi = fd[id], N = dn[id];
+        int hint = n / 2;
+        if (D.P.numerical && in_table(x, n, phi)) {
+            int lo = 0, hi = n - 1;   // initial bracket by bisection
+            while (hi - lo > 1) {
+                const int mid = lo + (hi - lo) / 2;
+                if (x[mid] >= phi) lo = mid;
+                else hi = mid;
+            }
+            hint = hi;
+        }
+        int ghint = hint;
+        bool act = active(D, phi, x, yv, n, &ghint);
+        for (long long s = 0; s < nsteps && act; ++s) {
+            // evolve_derivsN (src/evolution.cpp:1010) with pot_ratio (src/potential.cpp:177-192)
+            double ratio;
+            if (D.P.numerical) {
+                if (!in_table(x, n, phi)) {
+                    err = 1;
+                    break;
+                }
+                hint = walk(x, n, phi, hint);
+                ratio = interp(x, ydv, phi, hint) / interp(x, yv, phi, hint);
+            } else {
+                ratio = hill_dV(D.P, phi) / hill_V(D.P, phi);
+            }
+            pi += dN * (-(3.0 - 0.5 * (pi * pi)) * (pi + ratio));
+            // evolve_fieldsN (src/evolution.cpp:1022-1034)
+            N += dN;
+            phi += dN * pi;
+            act = active(D, phi, x, yv, n, &ghint);
+        }
+        f[id] = phi;
+        fd[id] = pi;
+        dn[id] = N;
+    }
+    if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(error, 1);
+}
+
+// evolve_fieldsN(d) alone (src/evolution.cpp:1015-1037).
+__global__ __launch_bounds__(kBlock) void k_fieldsN(double *__restrict__ f, const double *__restrict__ fd,
+                                                     double *__restrict__ dn, size_t gs, DnParams D, double d)
+{
+    const int n = D.P.numerical ? D.P.n : 0;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        int hint = n / 2;
+        const double v = f[id];
+        if (active(D, v, D.P.x, D.P.v, n, &hint)) {
+            dn[id] += d;
+            f[id] = v + d * fd[id];
+        }
+    }
+}
+
+// initializeN (src/initialize.cpp:219-232): fd -> fd a^{s-1} / sqrt(V/3), dn = 0.
+__global__ __launch_bounds__(kBlock) void k_initN(const double *__restrict__ f, double *__restrict__ fd,
+                                                   double *__restrict__ dn, size_t gs, Pot P, double apow,
+                                                   int *error)
+{
+    int err = 0;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        const double v = f[id];
+        double V;
+        if (P.numerical) {
+            if (v > P.x[0] || v < P.x[P.n - 1]) {
+                err = 1;
+                V = 1.0;
+            } else {
+                int lo = 0, hi = P.n - 1;
+                while (hi - lo > 1) {
+                    const int mid = lo + (hi - lo) / 2;
+                    if (P.x[mid] >= v) lo = mid;
+                    else hi = mid;
+                }
+                V = interp(P.x, P.v, v, hi) / P.B2;
+            }
+        } else {
+            V = hill_V(P, v);
+        }
+        const double H = sqrt(V / 3.0);
+        fd[id] = fd[id] * apow / H;
+        dn[id] = 0.0;
+    }
+    if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(error, 1);
+}
+
+#pragma clang fp contract(fast)
+
+// Ordered search of get_phiref (src/evolution.cpp:1271-1288): key per site
+// (|f| for monotonic/antimonotonic, V(f) otherwise), best key with the lowest
+// index wins, which is the value the serial strict-improvement scan returns.
+struct Best {
+    double key;
+    unsigned long long idx;
+};
+
+__device__ __forceinline__ bool better(int mono, double a, unsigned long long ia, double b, unsigned long long ib)
+{
+    if (mono < 0) return a > b || (a == b && ia < ib);
+    return a < b || (a == b && ia < ib);
+}
+
+__global__ __launch_bounds__(kBlock) void k_phiref(const double *__restrict__ f, size_t gs, DnParams D,
+                                                    double *__restrict__ pkey, unsigned long long *__restrict__ pidx)
+{
+    __shared__ double sk[kBlock];
+    __shared__ unsigned long long si[kBlock];
+    double bk = 0.0;
+    unsigned long long bi = ~0ull;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        const double v = f[id];
+        double key;
+        if (D.mono != 0) key = fabs(v);
+        else if (D.P.numerical) {
+            int lo = 0, hi = D.P.n - 1;
+            while (hi - lo > 1) {
+                const int mid = lo + (hi - lo) / 2;
+                if (D.P.x[mid] >= v) lo = mid;
+                else hi = mid;
+            }
+            key = interp(D.P.x, D.P.v, v, hi) / D.P.B2;
+        } else key = hill_V(D.P, v);
+        if (bi == ~0ull || better(D.mono, key, id, bk, bi)) {
+            bk = key;
+            bi = id;
+        }
+    }
+    sk[threadIdx.x] = bk;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const int o = threadIdx.x + off;
+            if (si[o] != ~0ull && (si[threadIdx.x] == ~0ull || better(D.mono, sk[o], si[o], sk[threadIdx.x], si[threadIdx.x]))) {
+                sk[threadIdx.x] = sk[o];
+                si[threadIdx.x] = si[o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pkey[blockIdx.x] = sk[0];
+        pidx[blockIdx.x] = si[0];
+    }
+}
+
+__global__ void k_phiref_final(const double *__restrict__ f, const double *__restrict__ pkey,
+                               const unsigned long long *__restrict__ pidx, int nb, int mono, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double bk = 0.0;
+    unsigned long long bi = ~0ull;
+    for (int b = 0; b < nb; ++b) {
+        if (pidx[b] == ~0ull) continue;
+        if (bi == ~0ull || better(mono, pkey[b], pidx[b], bk, bi)) {
+            bk = pkey[b];
+            bi = pidx[b];
+        }
+    }
+    *out = f[bi];
+}
+
+// saveN's mean (src/output.cpp:1527-1537): a float accumulator walked in lattice
+// order over sites with deltaN <= cutoff, then divided by N^3 in float.  The
+// serial order is part of the result, so one wave streams the map through LDS
+// and lane 0 accumulates in order.
+__global__ __launch_bounds__(64) void k_float_mean(const double *__restrict__ dn, size_t gs, double cutoff,
+                                                    float *out)
+{
+    __shared__ double chunk[1024];
+    float acc = 0.f;
+    for (size_t base = 0; base < gs; base += 1024) {
+        const size_t len = gs - base < 1024 ? gs - base : 1024;
+        for (size_t e = threadIdx.x; e < len; e += 64) chunk[e] = dn[base + e];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (size_t e = 0; e < len; ++e)
+                if (chunk[e] <= cutoff) acc = (float)((double)acc + chunk[e]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = acc / (float)gs;
+}
+
+// zero = 0: deltaN -= Nmean (src/output.cpp:1539-1542); zero = 1: sites above
+// cutoff - Nmean set to 0 (src/output.cpp:1549-1553).
+__global__ __launch_bounds__(kBlock) void k_zeta(double *__restrict__ dn, size_t gs, const float *nmean,
+                                                  double cutoff, int zero)
+{
+    const double m = (double)*nmean;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        if (zero) {
+            if (dn[id] > (cutoff - m)) dn[id] = 0;
+        } else {
+            dn[id] -= m;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub(double *__restrict__ g, size_t gs, const double *mean_src, double inv)
+{
+    const double m = mean_src[0] * inv;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x)
+        g[id] -= m;
+}
+
+// Plain FP64 sum of a field (fixed order: per-block tree, then block order).
+__global__ __launch_bounds__(kBlock) void k_sum(const double *__restrict__ g, size_t gs, double *partials)
+{
+    __shared__ double smem[kWaves];
+    double acc[1] = {0.0};
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x)
+        acc[0] += g[id];
+    block_sum<1>(acc, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+__global__ void k_sum_final(const double *partials, int nb, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partials[b];
+    *out = s;
+}
+
+// min / max (src/output.cpp:1095-1101) and histogram (src/output.cpp:1104-1123).
+__global__ __launch_bounds__(kBlock) void k_minmax(const double *__restrict__ g, size_t gs, double *pmin, double *pmax)
+{
+    __shared__ double smn[kBlock], smx[kBlock];
+    double mn = g[0], mx = g[0];
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        mn = g[id] < mn ? g[id] : mn;
+        mx = g[id] > mx ? g[id] : mx;
+    }
+    smn[threadIdx.x] = mn;
+    smx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + off]);
+            smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pmin[blockIdx.x] = smn[0];
+        pmax[blockIdx.x] = smx[0];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_histogram(const double *__restrict__ g, size_t gs, double bmin,
+                                                       double bmax, double df, int nbins,
+                                                       unsigned long long *counts)
+{
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        const double v = g[id];
+        int b = (int)((v - bmin) / df);
+        if (v == bmax) b = nbins - 1;
+        if (b >= 0 && b < nbins) atomicAdd(&counts[b], 1ull);
+    }
+}
+
+int grid_for(size_t gs)
+{
+    size_t b = (gs + kBlock - 1) / kBlock;
+    if (b > 4096) b = 4096;
+    return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+hipError_t dn_init(const Fields &F, const double *f, double *fd, double *dn, const Pot &P, double apow, int *error,
+                   hipStream_t s)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    k_initN<<<grid_for(gs), kBlock, 0, s>>>(f, fd, dn, gs, P, apow, error);
+    return hipGetLastError();
+}
+
+static DnParams params(const Pot &P, int mono, double phiref, double Vref)
+{
+    DnParams D;
+    D.mono = mono;
+    D.phiref = phiref;
+    D.Vref = Vref;
+    D.P = P;
+    return D;
+}
+
+hipError_t dn_fieldsN(size_t gs, double *f, const double *fd, double *dn, const Pot &P, int mono, double phiref,
+                      double Vref, double d, hipStream_t s)
+{
+    k_fieldsN<<<grid_for(gs), kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), d);
+    return hipGetLastError();
+}
+
+hipError_t dn_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, int mono, double phiref, double Vref,
+                    double dN, long long nsteps, int *error, hipStream_t s)
+{
+    size_t b = (gs + kBlock - 1) / kBlock;
+    k_deltaN_steps<<<(int)b, kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), dN, nsteps, error);
+    return hipGetLastError();
+}
+
+hipError_t dn_phiref(size_t gs, const double *f, const Pot &P, int mono, double *work, double *out, hipStream_t s)
+{
+    const int nb = grid_for(gs) > 1024 ? 1024 : grid_for(gs);
+    double *pkey = work;
+    unsigned long long *pidx = (unsigned long long *)(work + nb);
+    k_phiref<<<nb, kBlock, 0, s>>>(f, gs, params(P, mono, 0.0, 0.0), pkey, pidx);
+    k_phiref_final<<<1, 64, 0, s>>>(f, pkey, pidx, nb, mono, out);
+    return hipGetLastError();
+}
+
+hipError_t dn_float_mean(const double *dn, size_t gs, double cutoff, float *out, hipStream_t s)
+{
+    k_float_mean<<<1, 64, 0, s>>>(dn, gs, cutoff, out);
+    return hipGetLastError();
+}
+
+hipError_t dn_zeta(double *dn, size_t gs, const float *nmean, double cutoff, int zero, hipStream_t s)
+{
+    k_zeta<<<grid_for(gs), kBlock, 0, s>>>(dn, gs, nmean, cutoff, zero);
+    return hipGetLastError();
+}
+
+hipError_t field_sum(const double *g, size_t gs, double *work, double *out, hipStream_t s)
+{
+    const int nb = grid_for(gs) > 1024 ? 1024 : grid_for(gs);
+    k_sum<<<nb, kBlock, 0, s>>>(g, gs, work);
+    k_sum_final<<<1, 64, 0, s>>>(work, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t field_center(double *g, size_t gs, double *work, double *sum_out, hipStream_t s)
+{
+    hipError_t e = field_sum(g, gs, work, sum_out, s);
+    if (e != hipSuccess) return e;
+    k_sub<<<grid_for(gs), kBlock, 0, s>>>(g, gs, sum_out, 1.0 / (double)gs);
+    return hipGetLastError();
+}
+
+hipError_t field_minmax(const double *g, size_t gs, double *work, hipStream_t s, int *nb_out)
+{
+    const int nb = grid_for(gs) > 1024 ? 1024 : grid_for(gs);
+    k_minmax<<<nb, kBlock, 0, s>>>(g, gs, work, work + nb);
+    *nb_out = nb;
+    return hipGetLastError();
+}
+
+hipError_t field_histogram(const double *g, size_t gs, double bmin, double bmax, double df, int nbins,
+                           unsigned long long *counts, hipStream_t s)
+{
+    k_histogram<<<grid_for(gs), kBlock, 0, s>>>(g, gs, bmin, bmax, df, nbins, counts);
+    return hipGetLastError();
+}
+
+}  // namespace ie

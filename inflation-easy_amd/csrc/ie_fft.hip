@@ -1,0 +1,481 @@
+// ie_fft.hip -- 3-D real FFT, isotropic shell binning and TT projection on gfx950.
+//
+// Replaces the serial FFTEASY transform (src/ffteasy.hpp:21-190) and the binning
+// back-ends of src/output.cpp:63-151 and :466-625.  Differences of design:
+//   - out of place: the field is read once and never modified, so the inverse
+//     "restore" transforms of the reference (src/output.cpp:404,616,1167) are gone;
+//   - half-spectrum layout H[(i N + j) K + kz], K = N/2 + 1, complex FP64, with
+//     the kz = N/2 plane in place rather than in a separate Nyquist array;
+//   - each pass loads a tile of L lines into LDS (consecutive rows for z, L
+//     consecutive kz columns for y and x, so strided passes stay coalesced) and
+//     runs log2(N) radix-2 butterfly stages there; twiddles are exact
+//     (host libm), not a rotation recurrence;
+//   - all spectra are computed in FP64 (the reference uses FP32 for h_ij, which
+//     limits GW parity to the reference's own float rounding);
+//   - shell sums are accumulated per wave in LDS, per workgroup in a partials
+//     array and summed in fixed order; mode counts per shell are closed-form.
+// The forward kernel is e^{+2 pi i k n/N} like FFTEASY's (|F|^2 does not depend
+// on the sign); the inverse used for initial conditions is FFTEASY's inverse.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ie_fft.h"
+
+namespace ie {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLdsComplex = 2048;   // complex doubles of LDS per FFT workgroup (32 KB)
+
+__device__ __forceinline__ int bitrev(int t, int logn) { return (int)(__brev((unsigned)t) >> (32 - logn)); }
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// In-place radix-2 DIT transform of nl lines of n points in LDS (line l at
+// x[l n], input in bit-reversed order).  tw[k] = e^{+2 pi i k / n}.
+__device__ void lds_fft(double2 *x, int n, int logn, int nl, const double2 *__restrict__ tw, int sign)
+{
+    const int halfn = n >> 1;
+    for (int span = 2, ls = 1; span <= n; span <<= 1, ++ls) {
+        const int half = span >> 1, twstride = n >> ls;
+        __syncthreads();
+        for (int p = threadIdx.x; p < nl * halfn; p += blockDim.x) {
+            const int l = p >> (logn - 1), q = p & (halfn - 1);
+            const int grp = q >> (ls - 1), pos = q & (half - 1);
+            const int a = l * n + grp * span + pos, b = a + half;
+            double2 w = tw[pos * twstride];
+            if (sign < 0) w.y = -w.y;
+            const double2 u = x[a], t = cmul(w, x[b]);
+            x[a] = make_double2(u.x + t.x, u.y + t.y);
+            x[b] = make_double2(u.x - t.x, u.y - t.y);
+        }
+    }
+    __syncthreads();
+}
+
+// z pass, real to half-complex: L rows per workgroup.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fft_z_r2c(const T *__restrict__ in, double2 *__restrict__ out, int N,
+                                                         int logN, int L, const double2 *__restrict__ tw)
+{
+    extern __shared__ double2 lds[];
+    const int K = N / 2 + 1;
+    const size_t row0 = (size_t)blockIdx.x * L;
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        const int l = e >> logN, t = e & (N - 1);
+        lds[l * N + bitrev(t, logN)] = make_double2((double)in[(row0 + l) * N + t], 0.0);
+    }
+    lds_fft(lds, N, logN, L, tw, +1);
+    for (int e = threadIdx.x; e < L * K; e += blockDim.x) {
+        const int l = e / K, kz = e - l * K;
+        out[(row0 + l) * K + kz] = lds[l * N + kz];
+    }
+}
+
+// y (dim = 1) or x (dim = 0) pass over the half spectrum, in place: a workgroup
+// holds L consecutive kz columns of one outer index.
+__global__ __launch_bounds__(kThreads) void k_fft_cols(double2 *__restrict__ H, int N, int logN, int dim, int L,
+                                                        const double2 *__restrict__ tw, int sign)
+{
+    extern __shared__ double2 lds[];
+    const int K = N / 2 + 1;
+    const int o = blockIdx.y, kz0 = blockIdx.x * L;
+    const size_t stride = dim == 1 ? (size_t)K : (size_t)N * K;
+    const size_t base = dim == 1 ? (size_t)o * N * K : (size_t)o * K;
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        const int t = e / L, l = e - t * L, kz = kz0 + l;
+        lds[l * N + bitrev(t, logN)] = kz < K ? H[base + t * stride + kz] : make_double2(0.0, 0.0);
+    }
+    lds_fft(lds, N, logN, L, tw, sign);
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        const int t = e / L, l = e - t * L, kz = kz0 + l;
+        if (kz < K) H[base + t * stride + kz] = lds[l * N + t];
+    }
+}
+
+// z pass, half-complex to real with Hermitian extension; out = scale Re + add.
+__global__ __launch_bounds__(kThreads) void k_fft_z_c2r(const double2 *__restrict__ H, double *__restrict__ out,
+                                                         int N, int logN, int L, const double2 *__restrict__ tw,
+                                                         double scale, double add)
+{
+    extern __shared__ double2 lds[];
+    const int K = N / 2 + 1;
+    const size_t row0 = (size_t)blockIdx.x * L;
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        const int l = e >> logN, t = e & (N - 1);
+        double2 v;
+        if (t < K) v = H[(row0 + l) * K + t];
+        else {
+            v = H[(row0 + l) * K + (N - t)];
+            v.y = -v.y;
+        }
+        lds[l * N + bitrev(t, logN)] = v;
+    }
+    lds_fft(lds, N, logN, L, tw, -1);
+    for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+        const int l = e >> logN, t = e & (N - 1);
+        out[(row0 + l) * N + t] = lds[l * N + t].x * scale + add;
+    }
+}
+
+__device__ __forceinline__ void mode_coords(size_t m, int N, int K, int *px, int *py, int *kz)
+{
+    const size_t row = m / K;
+    *kz = (int)(m - row * K);
+    const int j = (int)(row % N), i = (int)(row / N);
+    *px = i <= N / 2 ? i : i - N;
+    *py = j <= N / 2 ? j : j - N;
+}
+
+// Shell sums of w |F|^2, w = 2 inside (0 < kz < N/2), 1 on kz = 0, N/2
+// (src/output.cpp:85-141); bins by lround(|n|).
+__global__ __launch_bounds__(kThreads) void k_bin_scalar(const double2 *__restrict__ H, int N, int nb,
+                                                          double dc_add, double *__restrict__ partials)
+{
+    extern __shared__ double wbins[];   // [kThreads/64][nb]
+    const int K = N / 2 + 1, wave = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < (kThreads / 64) * nb; e += blockDim.x) wbins[e] = 0.0;
+    __syncthreads();
+    const size_t modes = (size_t)N * N * K;
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < modes; m += (size_t)gridDim.x * blockDim.x) {
+        int px, py, kz;
+        mode_coords(m, N, K, &px, &py, &kz);
+        const int bin = (int)lround(sqrt((double)px * px + (double)py * py + (double)kz * kz));
+        if (bin >= nb) continue;
+        double2 F = H[m];
+        if (m == 0) F.x += dc_add;
+        const double p2 = F.x * F.x + F.y * F.y;
+        const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+        atomicAdd(&wbins[wave * nb + bin], w * p2);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) s += wbins[w * nb + b];
+        partials[(size_t)blockIdx.x * nb + b] = s;
+    }
+}
+
+// TT projection Lambda_{ab,lm} h_lm = (P h P)_ab - 1/2 P_ab tr(P h) with
+// P = 1 - k^ k^, k = (2/dx) sin(pi n/N) (src/output.cpp:493-543); shell sums of
+// w sum_ab |X_ab|^2 for shells below floor(0.8 numbins).
+struct SixSpectra {
+    const double2 *h[6];
+};
+
+__global__ __launch_bounds__(kThreads) void k_bin_tt(SixSpectra S, int N, int nb, int imax, double kpre,
+                                                      double to_phys, double *__restrict__ partials)
+{
+    extern __shared__ double wbins[];
+    const int K = N / 2 + 1, wave = threadIdx.x >> 6;
+    for (int e = threadIdx.x; e < (kThreads / 64) * nb; e += blockDim.x) wbins[e] = 0.0;
+    __syncthreads();
+    const size_t modes = (size_t)N * N * K;
+    const double pin = 3.14159265358979323846 / N;
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < modes; m += (size_t)gridDim.x * blockDim.x) {
+        int px, py, kz;
+        mode_coords(m, N, K, &px, &py, &kz);
+        if (px == 0 && py == 0 && kz == 0) continue;
+        const int bin = (int)lround(sqrt((double)px * px + (double)py * py + (double)kz * kz));
+        if (bin >= imax) continue;
+        double k3[3] = {kpre * sin(pin * px), kpre * sin(pin * py), kpre * sin(pin * kz)};
+        const double inv = 1.0 / sqrt(k3[0] * k3[0] + k3[1] * k3[1] + k3[2] * k3[2]);
+        double P[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) P[a][b] = (a == b ? 1.0 : 0.0) - (k3[a] * inv) * (k3[b] * inv);
+        double Cr[3][3], Ci[3][3];
+        const int comp[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const double2 v = S.h[comp[a][b]][m];
+                Cr[a][b] = v.x * to_phys;
+                Ci[a][b] = v.y * to_phys;
+            }
+        double PCr[3][3], PCi[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                double sr = 0.0, si = 0.0;
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    sr += P[a][l] * Cr[l][b];
+                    si += P[a][l] * Ci[l][b];
+                }
+                PCr[a][b] = sr;
+                PCi[a][b] = si;
+            }
+        const double Tr = PCr[0][0] + PCr[1][1] + PCr[2][2], Ti = PCi[0][0] + PCi[1][1] + PCi[2][2];
+        double fp2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                double ar = 0.0, ai = 0.0;
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    ar += PCr[a][l] * P[l][b];
+                    ai += PCi[a][l] * P[l][b];
+                }
+                const double xr = ar - 0.5 * P[a][b] * Tr, xi = ai - 0.5 * P[a][b] * Ti;
+                fp2 += xr * xr + xi * xi;
+            }
+        const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+        atomicAdd(&wbins[wave * nb + bin], w * fp2);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) s += wbins[w * nb + b];
+        partials[(size_t)blockIdx.x * nb + b] = s;
+    }
+}
+
+// Fixed-order sum of the per-workgroup shell partials.
+__global__ void k_sum_bins(const double *__restrict__ partials, int nblocks, int nb, double *__restrict__ out)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double s = 0.0;
+    for (int q = 0; q < nblocks; ++q) s += partials[(size_t)q * nb + b];
+    out[b] = s;
+}
+
+int ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+}  // namespace
+
+struct FftPlan {
+    int N = 0, logN = 0, K = 0;
+    size_t modes = 0;
+    double2 *H[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double2 *tw = nullptr;
+    double *partials = nullptr;
+    double *sums = nullptr;
+    int bin_blocks = 0;
+};
+
+int fft_numbins(int N) { return (int)(1.73205 * (N / 2)) + 1; }
+int fft_gw_numbins(int N) { return (int)(sqrt(3.0) * (N / 2)) + 1; }
+
+static int bin_blocks_for(size_t modes)
+{
+    size_t b = (modes + kThreads * 8 - 1) / (kThreads * 8);
+    if (b > 1024) b = 1024;
+    return (int)(b < 1 ? 1 : b);
+}
+
+size_t fft_workspace_bytes(int N)
+{
+    const size_t K = (size_t)N / 2 + 1;
+    const size_t modes = (size_t)N * N * K;
+    const int nb = fft_gw_numbins(N);
+    return modes * sizeof(double2) + (size_t)N * sizeof(double2) +
+           (size_t)bin_blocks_for(modes) * nb * 8 + (size_t)nb * 8;
+}
+
+FftPlan *fft_create(int N)
+{
+    FftPlan *p = new FftPlan();
+    p->N = N;
+    p->logN = ilog2(N);
+    p->K = N / 2 + 1;
+    p->modes = (size_t)N * N * p->K;
+    p->bin_blocks = bin_blocks_for(p->modes);
+    const int nb = fft_gw_numbins(N);
+    bool ok = hipMalloc(&p->H[0], p->modes * sizeof(double2)) == hipSuccess &&
+              hipMalloc(&p->tw, (size_t)N * sizeof(double2)) == hipSuccess &&
+              hipMalloc(&p->partials, (size_t)p->bin_blocks * nb * 8) == hipSuccess &&
+              hipMalloc(&p->sums, (size_t)nb * 8) == hipSuccess;
+    if (!ok) {
+        fft_destroy(p);
+        return nullptr;
+    }
+    std::vector<double2> tw(N);
+    const double two_pi = 4.0 * asin(1.0);
+    for (int k = 0; k < N; ++k) tw[k] = make_double2(cos(two_pi * k / N), sin(two_pi * k / N));
+    if (hipMemcpy(p->tw, tw.data(), N * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        fft_destroy(p);
+        return nullptr;
+    }
+    return p;
+}
+
+void fft_destroy(FftPlan *p)
+{
+    if (!p) return;
+    for (int s = 0; s < 6; ++s)
+        if (p->H[s]) hipFree(p->H[s]);
+    if (p->tw) hipFree(p->tw);
+    if (p->partials) hipFree(p->partials);
+    if (p->sums) hipFree(p->sums);
+    delete p;
+}
+
+static hipError_t ensure_slot(FftPlan *p, int slot)
+{
+    if (slot < 0 || slot > 5) return hipErrorInvalidValue;
+    if (p->H[slot]) return hipSuccess;
+    return hipMalloc(&p->H[slot], p->modes * sizeof(double2));
+}
+
+// Rows of the z pass per workgroup: fills kLdsComplex, never more than the N^2
+// rows there are (both powers of two, so L divides N^2).
+static int rows_per_block(int N)
+{
+    int L = kLdsComplex / N;
+    if (L > N * N) L = N * N;
+    return L < 1 ? 1 : L;
+}
+
+static hipError_t columns(FftPlan *p, double2 *H, int sign, hipStream_t s)
+{
+    const int N = p->N;
+    int L = kLdsComplex / N;
+    if (L > 8) L = 8;
+    if (L < 1) L = 1;
+    const dim3 grid((p->K + L - 1) / L, N);
+    const size_t lds = (size_t)L * N * sizeof(double2);
+    k_fft_cols<<<grid, kThreads, lds, s>>>(H, N, p->logN, 1, L, p->tw, sign);
+    k_fft_cols<<<grid, kThreads, lds, s>>>(H, N, p->logN, 0, L, p->tw, sign);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t forward(FftPlan *p, const T *in, int slot, hipStream_t s)
+{
+    hipError_t e = ensure_slot(p, slot);
+    if (e != hipSuccess) return e;
+    const int N = p->N, L = rows_per_block(N);
+    const int rows = N * N;
+    k_fft_z_r2c<T><<<rows / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(in, p->H[slot], N, p->logN, L,
+                                                                                p->tw);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return columns(p, p->H[slot], +1, s);
+}
+
+hipError_t fft_forward_double(FftPlan *p, const double *in, int slot, hipStream_t s)
+{
+    return forward<double>(p, in, slot, s);
+}
+
+hipError_t fft_forward_float(FftPlan *p, const float *in, double dc_add, int slot, hipStream_t s)
+{
+    (void)dc_add;  // applied at binning time (only the k = 0 mode changes)
+    return forward<float>(p, in, slot, s);
+}
+
+static hipError_t finish_bins(FftPlan *p, int nb, double *sums_host, hipStream_t s)
+{
+    k_sum_bins<<<(nb + 255) / 256, 256, 0, s>>>(p->partials, p->bin_blocks, nb, p->sums);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(sums_host, p->sums, (size_t)nb * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t fft_bin_scalar_dc(FftPlan *p, int slot, double dc_add, double *sums_host, hipStream_t s)
+{
+    const int nb = fft_numbins(p->N);
+    k_bin_scalar<<<p->bin_blocks, kThreads, (size_t)(kThreads / 64) * nb * 8, s>>>(p->H[slot], p->N, nb, dc_add,
+                                                                                   p->partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return finish_bins(p, nb, sums_host, s);
+}
+
+hipError_t fft_bin_scalar(FftPlan *p, int slot, double *sums_host, hipStream_t s)
+{
+    return fft_bin_scalar_dc(p, slot, 0.0, sums_host, s);
+}
+
+hipError_t fft_bin_tt(FftPlan *p, double dx, double to_phys, double *sums_host, hipStream_t s)
+{
+    const int nb = fft_gw_numbins(p->N);
+    const int imax = (int)floor(0.80 * nb);
+    SixSpectra S;
+    for (int c = 0; c < 6; ++c) {
+        if (!p->H[c]) return hipErrorInvalidValue;
+        S.h[c] = p->H[c];
+    }
+    k_bin_tt<<<p->bin_blocks, kThreads, (size_t)(kThreads / 64) * nb * 8, s>>>(S, p->N, nb, imax, 2.0 / dx,
+                                                                               to_phys, p->partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return finish_bins(p, nb, sums_host, s);
+}
+
+hipError_t fft_inverse_ffteasy(FftPlan *p, const double *fk, const double *fny, double add, double *out,
+                               hipStream_t s)
+{
+    const int N = p->N, K = p->K;
+    std::vector<double2> h((size_t)N * N * K);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            const size_t row = (size_t)i * N + j;
+            for (int kz = 0; kz < N / 2; ++kz)
+                h[row * K + kz] = make_double2(fk[row * N + 2 * kz], fk[row * N + 2 * kz + 1]);
+            h[row * K + N / 2] = make_double2(fny[(size_t)i * 2 * N + 2 * j], fny[(size_t)i * 2 * N + 2 * j + 1]);
+        }
+    hipError_t e = hipMemcpyAsync(p->H[0], h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    if ((e = columns(p, p->H[0], -1, s)) != hipSuccess) return e;
+    const int L = rows_per_block(N);
+    const double scale = 1.0 / ((double)N * N * N);
+    k_fft_z_c2r<<<N * N / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(p->H[0], out, N, p->logN, L, p->tw,
+                                                                             scale, add);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+void fft_counts(int N, int *counts)
+{
+    const int nb = fft_numbins(N);
+    for (int b = 0; b < nb; ++b) counts[b] = 0;
+    for (int i = 0; i < N; ++i) {
+        const int px = i <= N / 2 ? i : i - N;
+        for (int j = 0; j < N; ++j) {
+            const int py = j <= N / 2 ? j : j - N;
+            for (int kz = 0; kz <= N / 2; ++kz) {
+                const int bin = (int)lround(sqrt((double)px * px + (double)py * py + (double)kz * kz));
+                if (bin >= nb) continue;
+                counts[bin] += (kz == 0 || kz == N / 2) ? 1 : 2;
+            }
+        }
+    }
+}
+
+void fft_gw_counts(int N, int *counts)
+{
+    const int nb = fft_gw_numbins(N), imax = (int)floor(0.80 * nb);
+    for (int b = 0; b < nb; ++b) counts[b] = 0;
+    for (int i = 0; i < N; ++i) {
+        const int px = i <= N / 2 ? i : i - N;
+        for (int j = 0; j < N; ++j) {
+            const int py = j <= N / 2 ? j : j - N;
+            for (int kz = 0; kz <= N / 2; ++kz) {
+                if (px == 0 && py == 0 && kz == 0) continue;
+                const int bin = (int)lround(sqrt((double)px * px + (double)py * py + (double)kz * kz));
+                if (bin >= imax) continue;
+                counts[bin] += (kz == 0 || kz == N / 2) ? 1 : 2;
+            }
+        }
+    }
+}
+
+}  // namespace ie

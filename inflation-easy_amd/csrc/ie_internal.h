@@ -1,0 +1,213 @@
+// ie_internal.h -- device state, shared device helpers and launcher declarations
+// of the gfx950 lattice engine.  Not part of the public ABI (include/ie_api.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "ie_api.h"
+
+namespace ie {
+
+constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxRed = 8;       // reduction slots per block
+
+// Background of one run, resident on the device.  Kernels of a step read it at
+// their start; the last workgroup of a kernel (arrival counter) updates it, so a
+// whole step sequence runs without the host.
+struct DevBg {
+    double t, a, ad, ad2, astep, Ne, phiref, hubble_init;
+    double d;                                   // step of the current kick/drift
+    double laplnorm, friction, potnorm, srcamp;
+    double phibar, pibar;                       // homogeneous mode (split storage)
+    double vpmean;                              // <V'(phi)> over the lattice (split)
+    double red[kMaxRed];                        // results of the last reduction
+    double dt, af, rescale_s, dx, inv_gs;       // constants of the run
+    long long steps;
+    int done, error, parity, pad0;
+    unsigned int counter;
+    unsigned int pad;
+};
+
+// Potential parameters (src/potential.cpp:77-83 analytic hilltop; tabulated
+// interpolation :11-68).  Values are in units of B^2 = rescale_B^2.
+struct Pot {
+    int numerical;
+    int n;
+    double V0, ns, B2;
+    const double *x, *v, *dv;   // device table, descending x
+};
+
+// Device pointers of one lattice's fields.  Storage of f, fd is double (FP64) or
+// float (split: fluctuation about the homogeneous mode in DevBg).
+struct Fields {
+    void *f[2];
+    void *fd;
+    float *h[2][6];
+    float *hd[6];
+    double *dn;
+    int N, logN;
+    int sigw;
+};
+
+// ---------------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Deterministic sum of NV values over the workgroup; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *smem)
+{
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) smem[q * kWaves + wave] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < kWaves; ++w) s += smem[q * kWaves + w];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Arrival counter: returns true in every thread of the last workgroup to arrive.
+// Preceded by a release fence so the partials of all workgroups are visible.
+__device__ __forceinline__ bool last_block(unsigned int *counter, unsigned int nblocks)
+{
+    __shared__ int is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // release this workgroup's partials (device scope, all XCDs)
+        const unsigned int prev = atomicAdd(counter, 1u);
+        is_last = (prev == nblocks - 1);
+        if (is_last) *counter = 0;
+    }
+    __syncthreads();
+    if (is_last) __threadfence();  // acquire in every thread before reading partials
+    return is_last != 0;
+}
+
+// Fixed-order sum of `nblocks` partials of NV values by the last workgroup.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *partials, int nblocks, double (&out)[NV],
+                                             double *smem)
+{
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            acc[q] += partials[(size_t)b * NV + q];
+    }
+    block_sum<NV>(acc, smem);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = acc[q];
+}
+
+// ----------------------------------------------------------------- potential
+
+__device__ __forceinline__ int pot_bracket(const Pot &P, double v, int *err)
+{
+    if (v > P.x[0] || v < P.x[P.n - 1]) {
+        *err = 1;
+        return 1;
+    }
+    int lo = 0, hi = P.n - 1;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (P.x[mid] >= v) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+__device__ __forceinline__ double pot_interp(const Pot &P, double v, int l, const double *y)
+{
+    const double x0 = P.x[l - 1], x1 = P.x[l], y0 = y[l - 1], y1 = y[l];
+    return y1 + (v - x1) * (y0 - y1) / (x0 - x1);
+}
+
+// V/B^2 and V'/B^2 at v; *err set when v leaves the table.
+__device__ __forceinline__ void pot_eval(const Pot &P, double v, double *V, double *dV, int *err)
+{
+    if (P.numerical) {
+        const int l = pot_bracket(P, v, err);
+        *V = pot_interp(P, v, l, P.v) / P.B2;
+        *dV = pot_interp(P, v, l, P.dv) / P.B2;
+    } else {
+        *V = P.V0 * (1.0 - (1.0 - P.ns) / 4.0 * (v * v)) / P.B2;
+        *dV = -P.V0 * (1.0 - P.ns) / 2.0 * v / P.B2;
+    }
+}
+
+__device__ __forceinline__ double pot_dV(const Pot &P, double v, int *err)
+{
+    if (P.numerical) {
+        const int l = pot_bracket(P, v, err);
+        return pot_interp(P, v, l, P.dv) / P.B2;
+    }
+    return -P.V0 * (1.0 - P.ns) / 2.0 * v / P.B2;
+}
+
+// ---------------------------------------------------------------- launchers
+// (ie_lattice.hip)
+
+enum KickMode { KICK = 1, DRIFT = 2, KICKDRIFT = 3 };
+
+// Reduction of the Friedmann sums over the current f with the pre-kick scalar
+// update of evolve_derivs (src/evolution.cpp:153-167) in the last workgroup.
+// loop = 0: explicit step d_arg; loop = 1: loop step d = dt astep^{s-1}, and the
+// whole step is skipped once done.
+hipError_t launch_friedmann_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials,
+                                   double d_arg, int loop, hipStream_t s);
+// Kick / drift / fused kick+drift of f, fd (and h, hd when sigw).  loop = 1: a
+// step of the device-resident loop (skipped once done, step from DevBg);
+// loop = 0: explicit call with step d_arg (drift) or the step of the last reduce.
+// part: -1 scalar and tensor launches, 0 scalar only, 1 tensor only.
+hipError_t launch_kick_drift(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg,
+                             int loop, hipStream_t s, int part = -1);
+// Sums {sum f, sum fd, sum f^2} (split: about the homogeneous mode), kinetic,
+// gradient and potential sums into bg->red[0..5].
+hipError_t launch_field_sums(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials,
+                             hipStream_t s);
+hipError_t launch_split_upload(const double *f, const double *fd, Fields &F, DevBg *bg, double *partials,
+                               hipStream_t s);
+hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevBg *bg, double *out,
+                               hipStream_t s);
+int reduce_blocks(size_t gs);
+
+// deltaN phase and field utilities (ie_deltaN.hip).  `work` is a device buffer of
+// at least 4096 doubles.
+hipError_t dn_init(const Fields &F, const double *f, double *fd, double *dn, const Pot &P, double apow, int *error,
+                   hipStream_t s);
+hipError_t dn_fieldsN(size_t gs, double *f, const double *fd, double *dn, const Pot &P, int mono, double phiref,
+                      double Vref, double d, hipStream_t s);
+hipError_t dn_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, int mono, double phiref, double Vref,
+                    double dN, long long nsteps, int *error, hipStream_t s);
+hipError_t dn_phiref(size_t gs, const double *f, const Pot &P, int mono, double *work, double *out, hipStream_t s);
+hipError_t dn_float_mean(const double *dn, size_t gs, double cutoff, float *out, hipStream_t s);
+hipError_t dn_zeta(double *dn, size_t gs, const float *nmean, double cutoff, int zero, hipStream_t s);
+hipError_t field_sum(const double *g, size_t gs, double *work, double *out, hipStream_t s);
+hipError_t field_center(double *g, size_t gs, double *work, double *sum_out, hipStream_t s);
+hipError_t field_minmax(const double *g, size_t gs, double *work, hipStream_t s, int *nb_out);
+hipError_t field_histogram(const double *g, size_t gs, double bmin, double bmax, double df, int nbins,
+                           unsigned long long *counts, hipStream_t s);
+
+}  // namespace ie

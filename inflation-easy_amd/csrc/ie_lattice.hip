@@ -1,0 +1,424 @@
+// ie_lattice.hip -- leapfrog step of the inflaton lattice on gfx950.
+//
+// One step of run_evolution_loop (src/evolution.cpp:937-945) is three launches on
+// the run's stream, with no host involvement:
+//   1. k_friedmann_reduce: sum_x -f lapl f and sum_x V(f) (gradient_energy and
+//      potential_energy, src/evolution.cpp:130-136, src/potential.cpp:151-171),
+//      and in its last workgroup the implicit Friedmann half-kick of ad
+//      (src/evolution.cpp:153-167);
+//   2. k_kick_drift: fd += d (laplnorm lapl f - friction fd - potnorm V'(f)) and
+//      f_next = f + d fd (src/evolution.cpp:170-181, 265-270), f ping-ponged
+//      between two buffers so neighbours read the old f;
+//   3. (SIGW) k_tensor: the same for the six h_ij with source 2 d_i phi d_j phi
+//      (src/evolution.cpp:185-253, 274-280);
+// the last workgroup of the final launch does the second half-kick of ad, the
+// drift of t and a, flips the buffer parity and evaluates the loop test a <= af.
+// Every launch starts by reading DevBg::done, so steps queued past af are no-ops.
+//
+// Memory: fields are z-fastest (idx = (i N + j) N + k, src/main.h:34-38); a wave
+// covers 64 consecutive k, so every access is a coalesced 256/512-byte row
+// segment.  The grid is a fixed number of workgroups that stride over the
+// lattice; workgroup b works on the contiguous slab of XCD b % 8, keeping the
+// i +- 1 planes of the stencil inside one XCD's L2.
+//
+// Storage: FP64 (f, fd double) or split (float fluctuation about the FP64
+// homogeneous mode phibar, pibar in DevBg).  Every right-hand side is evaluated
+// in FP64; only the stored fluctuation is rounded to FP32.
+#include "ie_internal.h"
+
+namespace ie {
+
+namespace {
+
+constexpr int kXcd = 8;
+
+struct Walk {
+    size_t begin, stride, end;
+};
+
+// Site range of this workgroup: XCD-major contiguous slabs, strided inside.
+__device__ __forceinline__ Walk site_walk(size_t gs)
+{
+    const unsigned G = gridDim.x;
+    Walk w;
+    if (G % kXcd == 0 && gs % ((size_t)kXcd * kBlock) == 0) {
+        const unsigned xcd = blockIdx.x % kXcd, lb = blockIdx.x / kXcd, per = G / kXcd;
+        const size_t slab = gs / kXcd;
+        w.begin = (size_t)xcd * slab + (size_t)lb * kBlock + threadIdx.x;
+        w.stride = (size_t)per * kBlock;
+        w.end = (size_t)(xcd + 1) * slab;
+    } else {
+        w.begin = (size_t)blockIdx.x * kBlock + threadIdx.x;
+        w.stride = (size_t)G * kBlock;
+        w.end = gs;
+    }
+    return w;
+}
+
+struct Nb {
+    size_t ip, im, jp, jm, kp, km;
+};
+
+__device__ __forceinline__ Nb neighbours(size_t id, int N, int logN)
+{
+    const size_t m = (size_t)N - 1;
+    const size_t k = id & m, j = (id >> logN) & m, i = id >> (2 * logN);
+    const size_t rowN = (size_t)N, planeN = (size_t)N * N;
+    const size_t base_j = id - j * rowN, base_i = id - i * planeN;
+    Nb n;
+    n.kp = id - k + ((k + 1) & m);
+    n.km = id - k + ((k - 1) & m);
+    n.jp = base_j + ((j + 1) & m) * rowN;
+    n.jm = base_j + ((j - 1) & m) * rowN;
+    n.ip = base_i + ((i + 1) & m) * planeN;
+    n.im = base_i + ((i - 1) & m) * planeN;
+    return n;
+}
+
+// 7-point Laplacian in FP64 in the reference's summation order
+// (src/evolution.cpp:34-51): k+, k-, j+, j-, i+, i-, then -6 f.
+template <typename T>
+__device__ __forceinline__ double lapl64(const T *__restrict__ g, size_t id, const Nb &n)
+{
+    return (double)g[n.kp] + (double)g[n.km] + (double)g[n.jp] + (double)g[n.jm] + (double)g[n.ip] +
+           (double)g[n.im] - 6.0 * (double)g[id];
+}
+
+// FP32 Laplacian of a tensor component (lapl<float>, same order).
+__device__ __forceinline__ float lapl32(const float *__restrict__ g, size_t id, const Nb &n)
+{
+    return g[n.kp] + g[n.km] + g[n.jp] + g[n.jm] + g[n.ip] + g[n.im] - 6.0f * g[id];
+}
+
+__device__ __forceinline__ void flag_error(DevBg *bg, int err)
+{
+    if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(&bg->error, 1);
+}
+
+// Scalar updates that close a kick, a drift or a whole step (thread 0 of the
+// last workgroup).  Mirrors src/evolution.cpp:256,262,283 and :942-944.
+__device__ void finalize_scalars(DevBg *bg, int mode, int split)
+{
+    const double d = bg->d;
+    if (mode & KICK) {
+        bg->ad += 0.5 * d * bg->ad2;
+        if (split) bg->pibar += d * (-bg->friction * bg->pibar - bg->potnorm * bg->vpmean);
+    }
+    if (mode & DRIFT) {
+        bg->t += d;
+        if (split) bg->phibar += d * bg->pibar;
+        bg->a += d * bg->ad;
+    }
+    if (mode == KICKDRIFT) {  // only the device-resident loop uses the fused step
+        bg->parity ^= 1;
+        bg->steps += 1;
+        bg->astep = bg->a;
+        if (!(bg->a <= bg->af)) bg->done = 1;
+    }
+}
+
+// ----------------------------------------------------------------- kernels
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, DevBg *bg,
+                                                              double *partials, double d_arg, int loop)
+{
+    __shared__ double smem[3 * kWaves];
+    if (loop && bg->done) return;
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const T *__restrict__ f = (const T *)F.f[bg->parity];
+    const double phibar = SPLIT ? bg->phibar : 0.0;
+    double acc[3] = {0.0, 0.0, 0.0};
+    int err = 0;
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) {
+        const Nb n = neighbours(id, F.N, F.logN);
+        const double c = (double)f[id];
+        const double lap = lapl64(f, id, n);
+        double V, dV;
+        pot_eval(P, SPLIT ? phibar + c : c, &V, &dV, &err);
+        acc[0] -= c * lap;
+        acc[1] += V;
+        acc[2] += dV;
+    }
+    flag_error(bg, err);
+    block_sum<3>(acc, smem);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) partials[(size_t)blockIdx.x * 3 + q] = acc[q];
+    if (!last_block(&bg->counter, gridDim.x)) return;
+    double tot[3];
+    sum_partials<3>(partials, gridDim.x, tot, smem);
+    if (threadIdx.x != 0) return;
+    bg->red[0] = tot[0];
+    bg->red[1] = tot[1];
+    bg->red[2] = tot[2];
+    const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
+    const double gsd = (double)gs;
+    const double d = loop ? bg->dt * pow(bg->astep, s - 1.0) : d_arg;
+    const double G = 0.5 * tot[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
+    const double Pm = tot[1] / gsd;
+    const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
+    bg->d = d;
+    bg->ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
+               (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a +
+                           d * d * sfev1 * pow(a, sfev2) * (2.0 * G / 3.0 + Pm)))) / d;
+    bg->ad = ad + 0.5 * d * bg->ad2;
+    bg->friction = (2.0 + s) * bg->ad / a;
+    bg->potnorm = pow(a, 2.0 - 2.0 * s);
+    bg->laplnorm = pow(a, -2.0 * s) / (dx * dx);
+    bg->srcamp = 2.0 * pow(a, -2.0 * s);
+    bg->vpmean = tot[2] / gsd;
+}
+
+template <typename T, bool SPLIT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_kick_drift(Fields F, Pot P, DevBg *bg, double d_arg,
+                                                        int loop, int finalize)
+{
+    if (loop && bg->done) return;
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int par = bg->parity;
+    const T *__restrict__ f = (const T *)F.f[par];
+    T *__restrict__ fout = (T *)F.f[MODE == KICKDRIFT ? par ^ 1 : par];
+    T *__restrict__ fd = (T *)F.fd;
+    // explicit drift takes d from the caller; kicks use the step set by the reduce
+    const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
+    const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
+    const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
+    int err = 0;
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) {
+        double v = (double)fd[id];
+        if (MODE & KICK) {
+            const Nb n = neighbours(id, F.N, F.logN);
+            const double c = (double)f[id];
+            const double lap = lapl64(f, id, n);
+            double dV = pot_dV(P, SPLIT ? phibar + c : c, &err);
+            if (SPLIT) dV -= vpmean;
+            v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
+            fd[id] = (T)v;
+            v = (double)(T)v;
+        }
+        if (MODE & DRIFT) fout[id] = (T)((double)f[id] + d * v);
+    }
+    flag_error(bg, err);
+    if (!finalize) return;
+    if (!last_block(&bg->counter, gridDim.x)) return;
+    if (threadIdx.x == 0) {
+        if (MODE == DRIFT) bg->d = d;
+        finalize_scalars(bg, MODE, SPLIT);
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop, int split)
+{
+    if (loop && bg->done) return;
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int par = bg->parity;
+    const T *__restrict__ f = (const T *)F.f[par];
+    const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
+    const double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    const double half_over_dx = 0.5 / bg->dx;
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) {
+        const Nb n = neighbours(id, F.N, F.logN);
+        double src[6];
+        if (MODE & KICK) {
+            // central gradient of phi (dfdx, src/evolution.cpp:55-77)
+            const double gx = ((double)f[n.ip] - (double)f[n.im]) * half_over_dx;
+            const double gy = ((double)f[n.jp] - (double)f[n.jm]) * half_over_dx;
+            const double gz = ((double)f[n.kp] - (double)f[n.km]) * half_over_dx;
+            src[0] = gx * gx; src[1] = gy * gy; src[2] = gz * gz;
+            src[3] = gx * gy; src[4] = gx * gz; src[5] = gy * gz;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float *__restrict__ h = F.h[par][c];
+            float *__restrict__ hout = F.h[MODE == KICKDRIFT ? par ^ 1 : par][c];
+            float *__restrict__ hd = F.hd[c];
+            float hv = hd[id];
+            if (MODE & KICK) {
+                const double lap_h = (double)lapl32(h, id, n);
+                const double rhs = d * (laplnorm * lap_h - friction * (double)hv + srcamp * src[c]);
+                hv = hv + (float)rhs;
+                hd[id] = hv;
+            }
+            if (MODE & DRIFT) hout[id] = h[id] + (float)(d * (double)hv);
+        }
+    }
+    if (!last_block(&bg->counter, gridDim.x)) return;
+    if (threadIdx.x == 0) {
+        if (MODE == DRIFT) bg->d = d;
+        finalize_scalars(bg, MODE, split);
+    }
+}
+
+// Volume sums for outputs and for the split upload.
+//   red[0] sum f (split: sum dphi), red[1] sum fd (split: sum dpi),
+//   red[2] sum f^2 (split: sum dphi^2), red[3] sum fd^2 (split: sum dpi^2),
+//   red[4] sum -f lapl f (split: -dphi lapl dphi), red[5] sum V(f).
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_field_sums(Fields F, Pot P, DevBg *bg, double *partials)
+{
+    __shared__ double smem[6 * kWaves];
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const T *__restrict__ f = (const T *)F.f[bg->parity];
+    const T *__restrict__ fd = (const T *)F.fd;
+    const double phibar = SPLIT ? bg->phibar : 0.0;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int err = 0;
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) {
+        const Nb n = neighbours(id, F.N, F.logN);
+        const double c = (double)f[id], v = (double)fd[id];
+        acc[0] += c;
+        acc[1] += v;
+        acc[2] += c * c;
+        acc[3] += v * v;
+        acc[4] -= c * lapl64(f, id, n);
+        double V, dV;
+        pot_eval(P, SPLIT ? phibar + c : c, &V, &dV, &err);
+        acc[5] += V;
+    }
+    flag_error(bg, err);
+    block_sum<6>(acc, smem);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) partials[(size_t)blockIdx.x * 6 + q] = acc[q];
+    if (!last_block(&bg->counter, gridDim.x)) return;
+    double tot[6];
+    sum_partials<6>(partials, gridDim.x, tot, smem);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) bg->red[q] = tot[q];
+}
+
+// Split upload, pass 1: FP64 means of f and fd (the homogeneous mode).
+__global__ __launch_bounds__(kBlock) void k_mean2(const double *f, const double *fd, size_t gs, DevBg *bg,
+                                                   double *partials)
+{
+    __shared__ double smem[2 * kWaves];
+    double acc[2] = {0.0, 0.0};
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) {
+        acc[0] += f[id];
+        acc[1] += fd[id];
+    }
+    block_sum<2>(acc, smem);
+    if (threadIdx.x == 0) {
+        partials[(size_t)blockIdx.x * 2] = acc[0];
+        partials[(size_t)blockIdx.x * 2 + 1] = acc[1];
+    }
+    if (!last_block(&bg->counter, gridDim.x)) return;
+    double tot[2];
+    sum_partials<2>(partials, gridDim.x, tot, smem);
+    if (threadIdx.x == 0) {
+        bg->phibar = tot[0] / (double)gs;
+        bg->pibar = tot[1] / (double)gs;
+    }
+}
+
+// Split upload, pass 2: store the fluctuations in FP32.
+__global__ __launch_bounds__(kBlock) void k_split_store(const double *f, const double *fd, size_t gs,
+                                                        const DevBg *bg, float *df, float *dfd)
+{
+    const double pb = bg->phibar, qb = bg->pibar;
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) {
+        df[id] = (float)(f[id] - pb);
+        dfd[id] = (float)(fd[id] - qb);
+    }
+}
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_reconstitute(Fields F, const DevBg *bg, int which, double *out)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const T *__restrict__ src = which == 0 ? (const T *)F.f[bg->parity] : (const T *)F.fd;
+    const double base = SPLIT ? (which == 0 ? bg->phibar : bg->pibar) : 0.0;
+    const Walk w = site_walk(gs);
+    for (size_t id = w.begin; id < w.end; id += w.stride) out[id] = SPLIT ? base + (double)src[id] : (double)src[id];
+}
+
+}  // namespace
+
+int reduce_blocks(size_t gs)
+{
+    size_t b = gs / kBlock;
+    if (b > 2048) b = 2048;
+    if (b >= kXcd) b -= b % kXcd;
+    return (int)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_friedmann_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials,
+                                   double d_arg, int loop, hipStream_t s)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int G = reduce_blocks(gs);
+    if (split) k_friedmann_reduce<float, true><<<G, kBlock, 0, s>>>(F, P, bg, partials, d_arg, loop);
+    else k_friedmann_reduce<double, false><<<G, kBlock, 0, s>>>(F, P, bg, partials, d_arg, loop);
+    return hipGetLastError();
+}
+
+template <typename T, bool SPLIT>
+static void kd_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_arg, int loop, int fin,
+                        int G, hipStream_t s)
+{
+    if (mode == KICK) k_kick_drift<T, SPLIT, KICK><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop, fin);
+    else if (mode == DRIFT) k_kick_drift<T, SPLIT, DRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop, fin);
+    else k_kick_drift<T, SPLIT, KICKDRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop, fin);
+}
+
+template <typename T>
+static void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loop, int split, int G,
+                            hipStream_t s)
+{
+    if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, split);
+    else if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, split);
+    else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, split);
+}
+
+hipError_t launch_kick_drift(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg,
+                             int loop, hipStream_t s, int part)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int G = reduce_blocks(gs);
+    const int fin_scalar = F.sigw ? 0 : 1;
+    if (part != 1) {
+        if (split) kd_dispatch<float, true>(F, P, bg, mode, d_arg, loop, fin_scalar, G, s);
+        else kd_dispatch<double, false>(F, P, bg, mode, d_arg, loop, fin_scalar, G, s);
+    }
+    if (F.sigw && part != 0) {
+        if (split) tensor_dispatch<float>(F, bg, mode, d_arg, loop, split, G, s);
+        else tensor_dispatch<double>(F, bg, mode, d_arg, loop, split, G, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_field_sums(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int G = reduce_blocks(gs);
+    if (split) k_field_sums<float, true><<<G, kBlock, 0, s>>>(F, P, bg, partials);
+    else k_field_sums<double, false><<<G, kBlock, 0, s>>>(F, P, bg, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_upload(const double *f, const double *fd, Fields &F, DevBg *bg, double *partials,
+                               hipStream_t s)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int G = reduce_blocks(gs);
+    k_mean2<<<G, kBlock, 0, s>>>(f, fd, gs, bg, partials);
+    k_split_store<<<G, kBlock, 0, s>>>(f, fd, gs, bg, (float *)F.f[0], (float *)F.fd);
+    return hipGetLastError();
+}
+
+hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevBg *bg, double *out, hipStream_t s)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    const int G = reduce_blocks(gs);
+    if (split) k_reconstitute<float, true><<<G, kBlock, 0, s>>>(F, bg, which, out);
+    else k_reconstitute<double, false><<<G, kBlock, 0, s>>>(F, bg, which, out);
+    return hipGetLastError();
+}
+
+}  // namespace ie

@@ -1114,9 +1114,10 @@ static void set_mode(ieo_state *st, double p2, double p2lat, double m2, double *
     if (real == 1) { field[1] = 0.0; deriv[1] = 0.0; }
 }
 
-/* initialize() + initializef() + initializeGW() (src/initialize.cpp:93-242).
- * Returns 0, or 1 for a Courant violation. */
-int ieo_initialize(ieo_state *st)
+/* initialize() and the k-space part of initializef() (src/initialize.cpp:93-191):
+ * f, fd hold the FFTEASY-layout mode arrays afterwards, fny, fdny the Nyquist
+ * planes.  Returns 0, or 1 for a Courant violation. */
+int ieo_fill_modes(ieo_state *st)
 {
     const ieo_config *c = &st->cfg;
     const int N = st->N;
@@ -1179,6 +1180,16 @@ int ieo_initialize(ieo_state *st)
     }
 #undef NY
     f[0] = 0.0; f[1] = 0.0; fd[0] = 0.0; fd[1] = 0.0;
+    return 0;
+}
+
+int ieo_initialize(ieo_state *st)
+{
+    const ieo_config *c = &st->cfg;
+    const int N = st->N;
+    double *f = st->f, *fd = st->fd, *fny = st->fny, *fdny = st->fdny;
+    const int rc = ieo_fill_modes(st);
+    if (rc) return rc;
 
     ieo_fftrn_d(f, fny, N, -1);
     ieo_fftrn_d(fd, fdny, N, -1);

@@ -89,6 +89,9 @@ double ieo_potential_derivative_value(const ieo_state *st, double v);
 /* Initial conditions: initialize(), initializef(), initializeGW()
  * (src/initialize.cpp:40-49,93-242).  Returns 0 or an error code. */
 int ieo_initialize(ieo_state *st);
+/* The k-space part only (src/initialize.cpp:93-191): afterwards f, fd hold the
+ * FFTEASY-layout mode arrays and fields 3, 4 the Nyquist planes. */
+int ieo_fill_modes(ieo_state *st);
 
 /* Energies (src/evolution.cpp:130-145, src/potential.cpp:151-171) */
 double ieo_gradient_energy(ieo_state *st);

@@ -107,6 +107,7 @@ def lib() -> C.CDLL:
             "ieo_potential": (C.c_double, [P, C.c_double]),
             "ieo_potential_derivative_value": (C.c_double, [P, C.c_double]),
             "ieo_initialize": (C.c_int, [P]),
+            "ieo_fill_modes": (C.c_int, [P]),
             "ieo_gradient_energy": (C.c_double, [P]),
             "ieo_kin_energy": (C.c_double, [P]),
             "ieo_potential_energy": (C.c_double, [P]),

@@ -31,3 +31,18 @@ def potential_table():
 @pytest.fixture(scope="session")
 def table():
     return potential_table()
+
+ENGINE_DIR = REPO / "inflation-easy_amd"
+if str(ENGINE_DIR) not in sys.path:
+    sys.path.insert(0, str(ENGINE_DIR))
+
+
+def golden_config(case: str):
+    """Oracle config and parsed params of a golden case."""
+    from oracle import oracle as O
+    g = load_golden(case)
+    N = int(g["N"])
+    numerical = int(g["numerical_potential"])
+    params = O.parse_params_text(str(g["params_text"]))
+    table = potential_table() if numerical else None
+    return O.make_config(N, numerical, params, table), params, table

@@ -1,0 +1,176 @@
+"""GPU parity of the lattice time step, energies and spectra against the oracle.
+
+Both sides start from the same initial conditions (the oracle's, uploaded through
+the C ABI) and take the same steps.  Tolerances:
+  - FP64 storage: background a, ad to 1e-13 relative; fields compared as the
+    fluctuation f - <f>, to 1e-9 of its rms (sum order and FMA contraction differ
+    from the serial reference; both are ulp-level per step);
+  - energies and spectra of identical fields: 1e-12 relative per bin;
+  - GW spectra: 2e-4 relative per bin (the oracle, like the reference, sums shells
+    in float; the engine in FP64);
+  - split storage (FP32 fluctuation about an FP64 homogeneous mode) against FP64:
+    P(k) per bin to 1e-4 relative after 200 steps.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import ie_engine as E
+from conftest import golden_config, potential_table
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_state(case: str, **over):
+    cfg, params, table = golden_config(case)
+    for k, v in over.items():
+        setattr(cfg, k, v)
+    st = O.State(cfg, table)
+    st.initialize()
+    return st, params, table
+
+
+def engine_for(st: O.State, table, precision=E.IE_FP64, sigw=None):
+    c = st.cfg
+    cfg = E.make_config(c.N, precision=precision, numerical=c.numerical_potential,
+                        sigw=c.calculate_sigw if sigw is None else sigw, table=table,
+                        V0=c.V0, ns=c.ns, L=c.L, dt=c.dt, af=c.af, rescale_s=c.rescale_s, rescale_B=c.rescale_B,
+                        dN=c.dN, Nend=c.Nend)
+    run = E.Run(cfg, table)
+    run.upload_fields(st.field(0), st.field(1))
+    bg = st.background()
+    run.set_background(t=bg["t"], a=bg["a"], ad=bg["ad"], astep=bg["astep"], hubble_init=bg["hubble_init"])
+    return run
+
+
+def fluct_err(a, b):
+    da = a - a.mean()
+    return np.max(np.abs(da - (b - b.mean()))) / np.sqrt(np.mean(da ** 2))
+
+
+@pytest.mark.parametrize("case", ["hilltop_n32", "tabulated_n16"])
+def test_energies_match_oracle(case):
+    st, _, table = oracle_state(case)
+    run = engine_for(st, table)
+    g_o = O.lib().ieo_gradient_energy(st.ptr)
+    k_o = O.lib().ieo_kin_energy(st.ptr)
+    p_o = O.lib().ieo_potential_energy(st.ptr)
+    assert run.gradient_energy() == pytest.approx(g_o, rel=1e-12)
+    assert run.kin_energy() == pytest.approx(k_o, rel=1e-12)
+    assert run.potential_energy() == pytest.approx(p_o, rel=1e-13)
+    s = run.field_sums()
+    f = st.field(0)
+    assert s[0] == pytest.approx(f.sum(), rel=1e-13)
+    assert s[2] == pytest.approx((f * f).sum(), rel=1e-13)
+
+
+@pytest.mark.parametrize("case,nsteps", [("hilltop_n32", 200), ("tabulated_n16", 200)])
+def test_leapfrog_fp64_matches_oracle(case, nsteps):
+    st, _, table = oracle_state(case, calculate_sigw=0)
+    run = engine_for(st, table, sigw=0)
+    dt = st.cfg.dt
+    O.lib().ieo_evolve_fields(st.ptr, 0.5 * dt)
+    run.evolve_fields(0.5 * dt)
+    n_o = O.lib().ieo_leapfrog_steps(st.ptr, nsteps)
+    b = run.leapfrog_steps(nsteps)
+    bo = st.background()
+    assert b["steps"] == n_o == nsteps
+    for key in ("a", "ad", "t", "astep"):
+        assert b[key] == pytest.approx(bo[key], rel=1e-13), key
+    assert b["ad2"] == pytest.approx(bo["ad2"], rel=1e-9)
+    assert fluct_err(st.field(0), run.field(E.IE_FIELD_F)) < 1e-9
+    assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < 1e-9
+
+
+def test_explicit_kick_and_drift_match_oracle():
+    st, _, table = oracle_state("hilltop_n16", calculate_sigw=0)
+    run = engine_for(st, table, sigw=0)
+    for d in (0.0005, 0.0007):
+        O.lib().ieo_evolve_derivs(st.ptr, d)
+        run.evolve_derivs(d)
+        O.lib().ieo_evolve_fields(st.ptr, -0.5 * d)
+        run.evolve_fields(-0.5 * d)
+    bo, b = st.background(), run.background()
+    for key in ("a", "ad", "ad2", "t"):
+        assert b[key] == pytest.approx(bo[key], rel=1e-13), key
+    assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < 1e-10
+
+
+def test_loop_stops_at_af():
+    st, _, table = oracle_state("hilltop_n16", calculate_sigw=0, af=1.01)
+    run = engine_for(st, table, sigw=0)
+    run.evolve_fields(0.5 * st.cfg.dt)
+    O.lib().ieo_evolve_fields(st.ptr, 0.5 * st.cfg.dt)
+    n_o = O.lib().ieo_leapfrog_steps(st.ptr, 100000)
+    b = run.leapfrog_steps(n_o + 50)
+    assert b["done"] == 1 and b["steps"] == n_o
+    assert b["a"] == pytest.approx(st.background()["a"], rel=1e-13)
+
+
+@pytest.mark.parametrize("case", ["hilltop_n32", "tabulated_n16"])
+def test_spectrum_matches_oracle(case):
+    st, _, table = oracle_state(case)
+    run = engine_for(st, table)
+    norm1 = (st.cfg.L / st.cfg.rescale_B) ** 3 / st.cfg.N ** 6
+    p, c, v = run.spectrum(E.IE_FIELD_F, norm1)
+    po, co, vo = st.spectrum_f()
+    assert np.array_equal(c, co)
+    assert np.allclose(p, po, rtol=1e-15)
+    assert np.allclose(v, vo, rtol=1e-11, atol=0)
+
+
+def test_sigw_tensors_and_gw_spectra_match_oracle():
+    st, _, table = oracle_state("hilltop_n16")
+    run = engine_for(st, table, sigw=1)
+    dt = st.cfg.dt
+    O.lib().ieo_evolve_fields(st.ptr, 0.5 * dt)
+    run.evolve_fields(0.5 * dt)
+    O.lib().ieo_leapfrog_steps(st.ptr, 100)
+    run.leapfrog_steps(100)
+    for comp in (0, 3, 5, 6, 9, 11):
+        ho, hg = st.tensor(comp).astype(np.float64), run.tensor(comp).astype(np.float64)
+        scale = np.sqrt(np.mean(ho ** 2))
+        assert scale > 0
+        assert np.max(np.abs(ho - hg)) < 1e-4 * scale, comp
+    for hdot in (0, 1):
+        po, co, vo = st.gw_spectrum(hdot)
+        p, c, v = run.gw_spectrum(hdot)
+        assert np.array_equal(c, co)
+        assert np.array_equal(p, po)
+        live = vo > 0
+        assert live.sum() > 5
+        assert np.allclose(v[live], vo[live], rtol=2e-4)
+        assert np.all(v[~live] == 0)
+
+
+def test_upload_modes_matches_oracle_initial_conditions():
+    cfg, _, table = golden_config("hilltop_n32")
+    st = O.State(cfg, table)
+    assert O.lib().ieo_fill_modes(st.ptr) == 0
+    fk, fny, fdk, fdny = (st.field(q).copy() for q in (0, 3, 1, 4))
+    st2 = O.State(cfg, table)
+    st2.initialize()
+    run = E.Run(E.make_config(32, sigw=0, V0=cfg.V0, ns=cfg.ns, dt=cfg.dt), None)
+    run.upload_modes(fk, fny, fdk, fdny, cfg.initial_field, cfg.initial_derivative)
+    f, fd = run.field(E.IE_FIELD_F), run.field(E.IE_FIELD_FD)
+    assert fluct_err(st2.field(0), f) < 1e-12
+    assert fluct_err(st2.field(1), fd) < 1e-12
+
+
+def test_split_storage_matches_fp64_spectra():
+    st, _, table = oracle_state("hilltop_n32", calculate_sigw=0)
+    r64 = engine_for(st, table, precision=E.IE_FP64, sigw=0)
+    r32 = engine_for(st, table, precision=E.IE_SPLIT, sigw=0)
+    dt = st.cfg.dt
+    for r in (r64, r32):
+        r.evolve_fields(0.5 * dt)
+        r.leapfrog_steps(200)
+    b64, b32 = r64.background(), r32.background()
+    assert b32["a"] == pytest.approx(b64["a"], rel=1e-12)
+    norm1 = (st.cfg.L / st.cfg.rescale_B) ** 3 / 32 ** 6
+    _, _, v64 = r64.spectrum(E.IE_FIELD_F, norm1)
+    _, _, v32 = r32.spectrum(E.IE_FIELD_F, norm1)
+    assert np.allclose(v32[1:], v64[1:], rtol=1e-4)
+    assert v32[0] == pytest.approx(v64[0], rel=1e-12)
