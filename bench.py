@@ -102,8 +102,11 @@ def cpu_baseline(N: int = 256, target_steps_af: float = 1.02) -> dict | None:
         finally:
             shutil.rmtree(d, ignore_errors=True)
 
+    print(f"# cpu baseline: reference {binary.name}, {threads} threads, af=1.0", file=sys.stderr, flush=True)
     w0, n0 = one(1.0)
+    print(f"# cpu baseline: af=1.0 took {w0:.1f} s; af={target_steps_af}", file=sys.stderr, flush=True)
     w1, n1 = one(target_steps_af)
+    print(f"# cpu baseline: af={target_steps_af} took {w1:.1f} s, {n1 - n0} steps", file=sys.stderr, flush=True)
     steps = n1 - n0
     if steps <= 0 or w1 <= w0:
         return None

@@ -192,6 +192,61 @@ ie_status ie_deltaN_zero_unfinished(ie_run *run, double cutoff);
 /* subtract the FP64 mean of deltaN (build_deltaN_linear, src/output.cpp:156-164) */
 ie_status ie_deltaN_center(ie_run *run);
 
+/* --- batches ---------------------------------------------------------------- */
+/* Many runs advanced together: each step of the leapfrog loop
+ * (src/evolution.cpp:937-945) is one launch per kernel for all runs of the batch
+ * (any mix of N, potentials, SIGW on or off; one storage kind per batch), with
+ * each run's own background, loop test and partial sums, captured in a HIP graph
+ * and replayed.  No reference counterpart: the reference runs one lattice per
+ * process.  A run in a batch keeps working with every other entry point; the
+ * batch only holds pointers to the runs' state. */
+typedef struct ie_batch ie_batch;
+ie_status ie_batch_create(ie_run *const *runs, int nruns, ie_batch **out);
+/* nsteps loop steps of every run (runs with a > af are skipped on the device);
+ * graph_steps > 0 captures that many steps per HIP graph launch, 0 launches
+ * directly. */
+ie_status ie_batch_steps(ie_batch *batch, int nsteps, int graph_steps);
+ie_status ie_batch_destroy(ie_batch *batch);
+const char *ie_batch_last_error(const ie_batch *batch);
+
+/* --- serving ---------------------------------------------------------------- */
+/* A request loop over whole runs (initial conditions, inflation loop to af, then
+ * the deltaN map and P_zeta).  Requests wait in FIFO order until their device
+ * footprint (ie_footprint_bytes) fits the budget; admitted runs stay resident
+ * and are advanced together by batches (continuous batching: runs join and
+ * leave between quanta); finished runs are completed on worker threads while
+ * the others keep stepping.  No reference counterpart (the reference runs one
+ * lattice per process). */
+typedef struct ie_server ie_server;
+typedef struct ie_request {
+    ie_config cfg;
+    ie_ic_params ic;
+    int deltaN;                /* 1: deltaN map and P_zeta after inflation */
+    int fp64_check;            /* 1: also run an FP64 twin and report the P_zeta error */
+} ie_request;
+typedef struct ie_result {
+    int64_t id;
+    int32_t status;            /* ie_status of the run */
+    int32_t N;
+    double submit_s, admit_s, done_s;   /* server clock, seconds */
+    int64_t steps;             /* inflation steps taken */
+    double a_end;
+    int32_t nbins;             /* P_zeta shells written to pzeta (<= 512) */
+    double pzeta[512];         /* spectraN values (norm1 <|zeta_k|^2>) */
+    double fp64_max_rel_err;   /* max over shells of |P_split - P_fp64| / P_fp64, -1 if not checked */
+} ie_result;
+ie_status ie_server_create(size_t hbm_budget_bytes, int quantum_steps, int finish_threads, ie_server **out);
+/* Copy a request in (table pointers are copied too); *id identifies it. */
+ie_status ie_server_submit(ie_server *s, const ie_request *req, int64_t *id);
+/* One scheduling round: admit what fits, advance the resident runs by one
+ * quantum, hand finished runs to the finishing threads, collect results.
+ * *active and *pending receive the counts after the round. */
+ie_status ie_server_pump(ie_server *s, int *active, int *pending);
+/* Pop one completed result (returns IE_ERR_STATE when none is ready). */
+ie_status ie_server_pop_result(ie_server *s, ie_result *out);
+double ie_server_clock(const ie_server *s);
+ie_status ie_server_destroy(ie_server *s);
+
 /* --- modes ------------------------------------------------------------------ */
 /* get_modes (src/output.cpp:674-738): B * mean lattice momentum per truncated shell. */
 ie_status ie_modes(int N, double L, double rescale_B, double *out);

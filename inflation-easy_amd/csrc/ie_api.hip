@@ -874,6 +874,116 @@ ie_status ie_histogram(ie_run *r, int which, int nbins, double *bmin, double *bm
     return IE_OK;
 }
 
+/* ------------------------------------------------------------- batches */
+
+}  // extern "C"
+
+struct ie_batch {
+    std::vector<ie_run *> runs;
+    BatchDev *dev = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    std::string err;
+};
+
+static ie_status bfail(ie_batch *b, ie_status st, const std::string &m)
+{
+    if (b) b->err = m;
+    else g_create_error = m;
+    return st;
+}
+
+#define IE_BHIP(b, call)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return bfail((b), IE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" {
+
+const char *ie_batch_last_error(const ie_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+ie_status ie_batch_create(ie_run *const *runs, int nruns, ie_batch **out)
+{
+    if (!runs || nruns < 1 || !out) return bfail(nullptr, IE_ERR_ARG, "bad argument");
+    *out = nullptr;
+    const int split = runs[0]->cfg.precision == IE_SPLIT;
+    std::vector<BatchRun> br(nruns);
+    for (int q = 0; q < nruns; ++q) {
+        ie_run *r = runs[q];
+        if (!r) return bfail(nullptr, IE_ERR_ARG, "null run");
+        if ((r->cfg.precision == IE_SPLIT) != split) return bfail(nullptr, IE_ERR_ARG, "mixed storage in one batch");
+        if (r->in_deltaN) return bfail(nullptr, IE_ERR_STATE, "run is in its deltaN phase");
+        if (hipStreamSynchronize(r->stream) != hipSuccess) return bfail(nullptr, IE_ERR_HIP, "run stream");
+        br[q].F = r->F;
+        br[q].P = r->P;
+        br[q].bg = r->bg;
+        br[q].partials = r->partials;
+    }
+    ie_batch *b = new ie_batch();
+    b->runs.assign(runs, runs + nruns);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return bfail(nullptr, IE_ERR_HIP, "hipStreamCreate failed");
+    }
+    if (batch_build(br.data(), nruns, split, &b->dev) != hipSuccess) {
+        ie_batch_destroy(b);
+        return bfail(nullptr, IE_ERR_OOM, "batch descriptor upload failed");
+    }
+    *out = b;
+    return IE_OK;
+}
+
+ie_status ie_batch_steps(ie_batch *b, int nsteps, int graph_steps)
+{
+    if (!b || nsteps < 0 || graph_steps < 0) return bfail(b, IE_ERR_ARG, "bad argument");
+    IE_BHIP(b, batch_loop_begin(b->dev, b->stream));
+    int done = 0;
+    if (graph_steps > 0 && nsteps >= graph_steps) {
+        if (!b->exec || b->graph_steps != graph_steps) {
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+            IE_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int n = 0; n < graph_steps; ++n) {
+                hipError_t e = batch_step(b->dev, b->stream);
+                if (e != hipSuccess) {
+                    hipGraph_t g;
+                    (void)hipStreamEndCapture(b->stream, &g);
+                    return bfail(b, IE_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+                }
+            }
+            IE_BHIP(b, hipStreamEndCapture(b->stream, &b->graph));
+            IE_BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_steps = graph_steps;
+        }
+        for (; done + graph_steps <= nsteps; done += graph_steps) IE_BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    }
+    for (; done < nsteps; ++done) IE_BHIP(b, batch_step(b->dev, b->stream));
+    IE_BHIP(b, hipStreamSynchronize(b->stream));
+    for (ie_run *r : b->runs) {
+        if (pull_bg(r) != IE_OK) return bfail(b, IE_ERR_HIP, r->err);
+        if (r->bg_host->error) return bfail(b, IE_ERR_TABLE_RANGE, "field value outside the numerical potential range");
+    }
+    return IE_OK;
+}
+
+ie_status ie_batch_destroy(ie_batch *b)
+{
+    if (!b) return IE_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    batch_free(b->dev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return IE_OK;
+}
+
 ie_status ie_rk4_step(ie_run *r, double h)
 {
     (void)h;

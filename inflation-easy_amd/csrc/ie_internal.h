@@ -193,6 +193,19 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
                                hipStream_t s);
 int reduce_blocks(size_t gs);
 
+// Batched loop steps over many lattices of one storage kind (ie_lattice.hip).
+struct BatchRun {
+    Fields F;
+    Pot P;
+    DevBg *bg;
+    double *partials;
+};
+struct BatchDev;
+hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out);
+void batch_free(BatchDev *B);
+hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s);
+hipError_t batch_step(const BatchDev *B, hipStream_t s);
+
 // deltaN phase and field utilities (ie_deltaN.hip).  `work` is a device buffer of
 // at least 4096 doubles.
 hipError_t dn_init(const Fields &F, const double *f, double *fd, double *dn, const Pot &P, double apow, int *error,

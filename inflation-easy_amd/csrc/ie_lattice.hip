@@ -24,6 +24,8 @@
 // Storage: FP64 (f, fd double) or split (float fluctuation about the FP64
 // homogeneous mode phibar, pibar in DevBg).  Every right-hand side is evaluated
 // in FP64; only the stored fluctuation is rounded to FP32.
+#include <vector>
+
 #include "ie_internal.h"
 
 namespace ie {
@@ -36,24 +38,37 @@ struct Walk {
     size_t begin, stride, end;
 };
 
-// Site range of this workgroup: XCD-major contiguous slabs, strided inside.
-__device__ __forceinline__ Walk site_walk(size_t gs)
+// Site range of workgroup b of the G working on one lattice: XCD-major
+// contiguous slabs, strided inside.
+__device__ __forceinline__ Walk site_walk(size_t gs, unsigned G, unsigned b)
 {
-    const unsigned G = gridDim.x;
     Walk w;
     if (G % kXcd == 0 && gs % ((size_t)kXcd * kBlock) == 0) {
-        const unsigned xcd = blockIdx.x % kXcd, lb = blockIdx.x / kXcd, per = G / kXcd;
+        const unsigned xcd = b % kXcd, lb = b / kXcd, per = G / kXcd;
         const size_t slab = gs / kXcd;
         w.begin = (size_t)xcd * slab + (size_t)lb * kBlock + threadIdx.x;
         w.stride = (size_t)per * kBlock;
         w.end = (size_t)(xcd + 1) * slab;
     } else {
-        w.begin = (size_t)blockIdx.x * kBlock + threadIdx.x;
+        w.begin = (size_t)b * kBlock + threadIdx.x;
         w.stride = (size_t)G * kBlock;
         w.end = gs;
     }
     return w;
 }
+
+__device__ __forceinline__ Walk site_walk(size_t gs) { return site_walk(gs, gridDim.x, blockIdx.x); }
+
+// One lattice of a batched launch (ie_batch_steps): its state, its workgroup
+// count G and its first workgroup in the launch.
+struct RunDesc {
+    Fields F;
+    Pot P;
+    DevBg *bg;
+    double *partials;
+    unsigned G;
+    int split;
+};
 
 struct Nb {
     size_t ip, im, jp, jm, kp, km;
@@ -120,17 +135,16 @@ __device__ void finalize_scalars(DevBg *bg, int mode, int split)
 // ----------------------------------------------------------------- kernels
 
 template <typename T, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, DevBg *bg,
-                                                              double *partials, double d_arg, int loop)
+__device__ __forceinline__ void friedmann_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials,
+                                                      double d_arg, int loop, unsigned G, unsigned blk, double *smem)
 {
-    __shared__ double smem[3 * kWaves];
     if (loop && bg->done) return;
     const size_t gs = (size_t)F.N * F.N * F.N;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
     const double phibar = SPLIT ? bg->phibar : 0.0;
     double acc[3] = {0.0, 0.0, 0.0};
     int err = 0;
-    const Walk w = site_walk(gs);
+    const Walk w = site_walk(gs, G, blk);
     for (size_t id = w.begin; id < w.end; id += w.stride) {
         const Nb n = neighbours(id, F.N, F.logN);
         const double c = (double)f[id];
@@ -144,10 +158,10 @@ __global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, De
     flag_error(bg, err);
     block_sum<3>(acc, smem);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 3; ++q) partials[(size_t)blockIdx.x * 3 + q] = acc[q];
-    if (!last_block(&bg->counter, gridDim.x)) return;
+        for (int q = 0; q < 3; ++q) partials[(size_t)blk * 3 + q] = acc[q];
+    if (!last_block(&bg->counter, G)) return;
     double tot[3];
-    sum_partials<3>(partials, gridDim.x, tot, smem);
+    sum_partials<3>(partials, G, tot, smem);
     if (threadIdx.x != 0) return;
     bg->red[0] = tot[0];
     bg->red[1] = tot[1];
@@ -155,13 +169,13 @@ __global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, De
     const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
     const double gsd = (double)gs;
     const double d = loop ? bg->dt * pow(bg->astep, s - 1.0) : d_arg;
-    const double G = 0.5 * tot[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
+    const double Ge = 0.5 * tot[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
     const double Pm = tot[1] / gsd;
     const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
     bg->d = d;
     bg->ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
                (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a +
-                           d * d * sfev1 * pow(a, sfev2) * (2.0 * G / 3.0 + Pm)))) / d;
+                           d * d * sfev1 * pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) / d;
     bg->ad = ad + 0.5 * d * bg->ad2;
     bg->friction = (2.0 + s) * bg->ad / a;
     bg->potnorm = pow(a, 2.0 - 2.0 * s);
@@ -170,9 +184,27 @@ __global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, De
     bg->vpmean = tot[2] / gsd;
 }
 
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, DevBg *bg,
+                                                              double *partials, double d_arg, int loop)
+{
+    __shared__ double smem[3 * kWaves];
+    friedmann_reduce_body<T, SPLIT>(F, P, bg, partials, d_arg, loop, gridDim.x, blockIdx.x, smem);
+}
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void kb_friedmann_reduce(const RunDesc *__restrict__ runs,
+                                                               const uint2 *__restrict__ map)
+{
+    __shared__ double smem[3 * kWaves];
+    const uint2 m = map[blockIdx.x];
+    const RunDesc &R = runs[m.x];
+    friedmann_reduce_body<T, SPLIT>(R.F, R.P, R.bg, R.partials, 0.0, 1, R.G, m.y, smem);
+}
+
 template <typename T, bool SPLIT, int MODE>
-__global__ __launch_bounds__(kBlock) void k_kick_drift(Fields F, Pot P, DevBg *bg, double d_arg,
-                                                        int loop, int finalize)
+__device__ __forceinline__ void kick_drift_body(const Fields &F, const Pot &P, DevBg *bg, double d_arg, int loop,
+                                                int finalize, unsigned G, unsigned blk)
 {
     if (loop && bg->done) return;
     const size_t gs = (size_t)F.N * F.N * F.N;
@@ -185,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(Fields F, Pot P, DevBg *b
     const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
     const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
     int err = 0;
-    const Walk w = site_walk(gs);
+    const Walk w = site_walk(gs, G, blk);
     for (size_t id = w.begin; id < w.end; id += w.stride) {
         double v = (double)fd[id];
         if (MODE & KICK) {
@@ -202,15 +234,33 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(Fields F, Pot P, DevBg *b
     }
     flag_error(bg, err);
     if (!finalize) return;
-    if (!last_block(&bg->counter, gridDim.x)) return;
+    if (!last_block(&bg->counter, G)) return;
     if (threadIdx.x == 0) {
         if (MODE == DRIFT) bg->d = d;
         finalize_scalars(bg, MODE, SPLIT);
     }
 }
 
+template <typename T, bool SPLIT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_kick_drift(Fields F, Pot P, DevBg *bg, double d_arg,
+                                                        int loop, int finalize)
+{
+    kick_drift_body<T, SPLIT, MODE>(F, P, bg, d_arg, loop, finalize, gridDim.x, blockIdx.x);
+}
+
+// batched loop step: the scalar kernel closes the step of runs without tensors
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void kb_kick_drift(const RunDesc *__restrict__ runs,
+                                                         const uint2 *__restrict__ map)
+{
+    const uint2 m = map[blockIdx.x];
+    const RunDesc &R = runs[m.x];
+    kick_drift_body<T, SPLIT, KICKDRIFT>(R.F, R.P, R.bg, 0.0, 1, R.F.sigw ? 0 : 1, R.G, m.y);
+}
+
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop, int split)
+__device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d_arg, int loop, int split,
+                                            unsigned G, unsigned blk)
 {
     if (loop && bg->done) return;
     const size_t gs = (size_t)F.N * F.N * F.N;
@@ -219,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d
     const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
     const double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
     const double half_over_dx = 0.5 / bg->dx;
-    const Walk w = site_walk(gs);
+    const Walk w = site_walk(gs, G, blk);
     for (size_t id = w.begin; id < w.end; id += w.stride) {
         const Nb n = neighbours(id, F.N, F.logN);
         double src[6];
@@ -246,10 +296,33 @@ __global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d
             if (MODE & DRIFT) hout[id] = h[id] + (float)(d * (double)hv);
         }
     }
-    if (!last_block(&bg->counter, gridDim.x)) return;
+    if (!last_block(&bg->counter, G)) return;
     if (threadIdx.x == 0) {
         if (MODE == DRIFT) bg->d = d;
         finalize_scalars(bg, MODE, split);
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop, int split)
+{
+    tensor_body<T, MODE>(F, bg, d_arg, loop, split, gridDim.x, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kb_tensor(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
+{
+    const uint2 m = map[blockIdx.x];
+    const RunDesc &R = runs[m.x];
+    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.split, R.G, m.y);
+}
+
+__global__ void kb_loop_begin(const RunDesc *__restrict__ runs, int nruns)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nruns) {
+        DevBg *bg = runs[r].bg;
+        bg->done = !(bg->a <= bg->af) ? 1 : 0;
     }
 }
 
@@ -418,6 +491,91 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
     const int G = reduce_blocks(gs);
     if (split) k_reconstitute<float, true><<<G, kBlock, 0, s>>>(F, bg, which, out);
     else k_reconstitute<double, false><<<G, kBlock, 0, s>>>(F, bg, which, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ batches
+
+struct BatchDev {
+    RunDesc *runs = nullptr;
+    uint2 *map = nullptr, *tmap = nullptr;
+    int nruns = 0, nblocks = 0, ntblocks = 0, split = 0;
+};
+
+// Workgroups per lattice in a batched launch: a multiple of 8 (whole XCD slabs),
+// at most 256 so the last workgroup's fixed-order sum stays short.
+static unsigned batch_blocks(size_t gs)
+{
+    size_t b = gs / kBlock;
+    if (b > 256) b = 256;
+    b = (b + kXcd - 1) / kXcd * kXcd;
+    return (unsigned)(b < kXcd ? kXcd : b);
+}
+
+hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
+{
+    *out = nullptr;
+    std::vector<RunDesc> d(n);
+    std::vector<uint2> map, tmap;
+    for (int r = 0; r < n; ++r) {
+        const size_t gs = (size_t)runs[r].F.N * runs[r].F.N * runs[r].F.N;
+        d[r].F = runs[r].F;
+        d[r].P = runs[r].P;
+        d[r].bg = runs[r].bg;
+        d[r].partials = runs[r].partials;
+        d[r].G = batch_blocks(gs);
+        d[r].split = split;
+        for (unsigned b = 0; b < d[r].G; ++b) {
+            map.push_back(make_uint2((unsigned)r, b));
+            if (runs[r].F.sigw) tmap.push_back(make_uint2((unsigned)r, b));
+        }
+    }
+    BatchDev *B = new BatchDev();
+    B->nruns = n;
+    B->nblocks = (int)map.size();
+    B->ntblocks = (int)tmap.size();
+    B->split = split;
+    hipError_t e = hipMalloc(&B->runs, sizeof(RunDesc) * n);
+    if (e == hipSuccess) e = hipMalloc(&B->map, sizeof(uint2) * map.size());
+    if (e == hipSuccess && !tmap.empty()) e = hipMalloc(&B->tmap, sizeof(uint2) * tmap.size());
+    if (e == hipSuccess) e = hipMemcpy(B->runs, d.data(), sizeof(RunDesc) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->map, map.data(), sizeof(uint2) * map.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !tmap.empty())
+        e = hipMemcpy(B->tmap, tmap.data(), sizeof(uint2) * tmap.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        batch_free(B);
+        return e;
+    }
+    *out = B;
+    return hipSuccess;
+}
+
+void batch_free(BatchDev *B)
+{
+    if (!B) return;
+    if (B->runs) (void)hipFree(B->runs);
+    if (B->map) (void)hipFree(B->map);
+    if (B->tmap) (void)hipFree(B->tmap);
+    delete B;
+}
+
+hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
+{
+    kb_loop_begin<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
+    return hipGetLastError();
+}
+
+hipError_t batch_step(const BatchDev *B, hipStream_t s)
+{
+    if (B->split) {
+        kb_friedmann_reduce<float, true><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
+        kb_kick_drift<float, true><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
+        if (B->ntblocks) kb_tensor<float><<<B->ntblocks, kBlock, 0, s>>>(B->runs, B->tmap);
+    } else {
+        kb_friedmann_reduce<double, false><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
+        kb_kick_drift<double, false><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
+        if (B->ntblocks) kb_tensor<double><<<B->ntblocks, kBlock, 0, s>>>(B->runs, B->tmap);
+    }
     return hipGetLastError();
 }
 

@@ -126,6 +126,10 @@ def lib() -> C.CDLL:
             "ie_deltaN_zero_unfinished": (st, [P, C.c_double]),
             "ie_deltaN_center": (st, [P]),
             "ie_modes": (st, [C.c_int, C.c_double, C.c_double, dp]),
+            "ie_batch_create": (st, [C.POINTER(P), C.c_int, C.POINTER(P)]),
+            "ie_batch_steps": (st, [P, C.c_int, C.c_int]),
+            "ie_batch_destroy": (st, [P]),
+            "ie_batch_last_error": (C.c_char_p, [P]),
             "ie_version": (C.c_char_p, []),
             "ie_device_count": (C.c_int, []),
         }
@@ -348,6 +352,32 @@ class Run:
 
     def deltaN_center(self):
         self._check(lib().ie_deltaN_center(self.h))
+
+
+class Batch:
+    """Runs stepped together, one launch per kernel per step (ie_batch_*)."""
+
+    def __init__(self, runs: list[Run]):
+        self.runs = list(runs)
+        arr = (C.c_void_p * len(runs))(*[r.h.value for r in runs])
+        h = C.c_void_p()
+        st = lib().ie_batch_create(arr, len(runs), C.byref(h))
+        if st:
+            raise EngineError(st, lib().ie_batch_last_error(None).decode())
+        self.h = h
+
+    def steps(self, n: int, graph_steps: int = 0):
+        st = lib().ie_batch_steps(self.h, n, graph_steps)
+        if st:
+            raise EngineError(st, lib().ie_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ie_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def fill_modes(cfg: Config, seed: int, phi0: float, dphi0: float, high_cutoff: float = 0.0,
