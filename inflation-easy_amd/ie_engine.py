@@ -54,6 +54,26 @@ class IcParams(C.Structure):
                 ("high_cutoff_index", C.c_double), ("low_cutoff_index", C.c_double)]
 
 
+class Request(C.Structure):
+    """Mirror of ``ie_request``."""
+
+    _fields_ = [("cfg", Config), ("ic", IcParams), ("deltaN", C.c_int), ("fp64_check", C.c_int)]
+
+
+class Result(C.Structure):
+    """Mirror of ``ie_result``."""
+
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("N", C.c_int32), ("submit_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double), ("steps", C.c_int64), ("a_end", C.c_double),
+                ("nbins", C.c_int32), ("pzeta", C.c_double * 512), ("fp64_max_rel_err", C.c_double)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pzeta"}
+        d["pzeta"] = np.array(self.pzeta[: self.nbins])
+        d["latency_s"] = self.done_s - self.submit_s
+        return d
+
+
 class Profile(C.Structure):
     """Mirror of ``ie_profile``."""
 
@@ -130,6 +150,12 @@ def lib() -> C.CDLL:
             "ie_batch_steps": (st, [P, C.c_int, C.c_int]),
             "ie_batch_destroy": (st, [P]),
             "ie_batch_last_error": (C.c_char_p, [P]),
+            "ie_server_create": (st, [C.c_size_t, C.c_int, C.c_int, C.POINTER(P)]),
+            "ie_server_submit": (st, [P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+            "ie_server_pump": (st, [P, ip, ip]),
+            "ie_server_pop_result": (st, [P, C.POINTER(Result)]),
+            "ie_server_clock": (C.c_double, [P]),
+            "ie_server_destroy": (st, [P]),
             "ie_version": (C.c_char_p, []),
             "ie_device_count": (C.c_int, []),
         }
@@ -374,6 +400,57 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().ie_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Request loop (ie_server_*): FIFO admission by device footprint, continuous
+    batching of resident runs, finishing on worker threads."""
+
+    def __init__(self, hbm_budget_bytes: int, quantum_steps: int = 100, finish_threads: int = 4):
+        h = C.c_void_p()
+        st = lib().ie_server_create(hbm_budget_bytes, quantum_steps, finish_threads, C.byref(h))
+        if st:
+            raise EngineError(st, "ie_server_create failed")
+        self.h = h
+        self._keep = []
+
+    def submit(self, cfg: Config, seed: int, phi0: float, dphi0: float, deltaN: int = 1, fp64_check: int = 0,
+               table=None) -> int:
+        req = Request()
+        req.cfg = cfg
+        req.ic = IcParams(seed, phi0, dphi0, 0.0, 0.0)
+        req.deltaN = deltaN
+        req.fp64_check = fp64_check
+        rid = C.c_int64()
+        st = lib().ie_server_submit(self.h, C.byref(req), C.byref(rid))
+        if st:
+            raise EngineError(st, "ie_server_submit failed")
+        return rid.value
+
+    def pump(self) -> tuple[int, int]:
+        a, p = C.c_int(), C.c_int()
+        st = lib().ie_server_pump(self.h, C.byref(a), C.byref(p))
+        if st:
+            raise EngineError(st, "ie_server_pump failed")
+        return a.value, p.value
+
+    def results(self) -> list[dict]:
+        out = []
+        r = Result()
+        while lib().ie_server_pop_result(self.h, C.byref(r)) == 0:
+            out.append(r.as_dict())
+        return out
+
+    def clock(self) -> float:
+        return lib().ie_server_clock(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ie_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

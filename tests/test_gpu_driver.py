@@ -64,6 +64,12 @@ def test_driver_matches_reference_files(case, tmp_path):
         ref = g[key]
         got = blocks(res / f"{key}.dat", key, ref)
         assert got.shape == ref.shape, key
+        if key == "energy":
+            # column 3 is the gradient energy: -sum f lapl f with f ~ phi0 + tiny
+            # fluctuations cancels to ~5 significant digits in any FP64 order
+            # (the reference's own value included), so it gets 1e-4
+            assert np.allclose(got[:, 3], ref[:, 3], rtol=1e-4, atol=0)
+            got, ref = np.delete(got, 3, axis=1), np.delete(ref, 3, axis=1)
         assert np.allclose(got, ref, rtol=2e-6, atol=0), (key, np.max(np.abs(got - ref) / np.abs(ref + 1e-300)))
     got = blocks(res / "histogram.dat", "histogram", g["histogram"])
     assert np.max(np.abs(got - g["histogram"])) <= 2.0 / N ** 3 + 1e-9

@@ -5,7 +5,9 @@ the C ABI) and take the same steps.  Tolerances:
   - FP64 storage: background a, ad to 1e-13 relative; fields compared as the
     fluctuation f - <f>, to 1e-9 of its rms (sum order and FMA contraction differ
     from the serial reference; both are ulp-level per step);
-  - energies and spectra of identical fields: 1e-12 relative per bin;
+  - energies of identical fields: 1e-12 relative; spectra: 1e-8 per shell (the
+    k = 0 power of f = phi0 + fluctuation dominates and its FFT rounding leaks
+    into the other shells, in the reference as here);
   - GW spectra: 2e-4 relative per bin (the oracle, like the reference, sums shells
     in float; the engine in FP64);
   - split storage (FP32 fluctuation about an FP64 homogeneous mode) against FP64:
@@ -118,7 +120,11 @@ def test_spectrum_matches_oracle(case):
     po, co, vo = st.spectrum_f()
     assert np.array_equal(c, co)
     assert np.allclose(p, po, rtol=1e-15)
-    assert np.allclose(v, vo, rtol=1e-11, atol=0)
+    # FP64 FFTs of f = phi0 + fluctuation: the k = 0 power exceeds the others by
+    # up to 1e14, and its rounding leaks into every shell at ~1e-16 x that ratio
+    # in amplitude (in the reference's FFTEASY as here), so shells agree to 1e-8
+    assert v[0] == pytest.approx(vo[0], rel=1e-13)
+    assert np.allclose(v[1:], vo[1:], rtol=1e-8, atol=0)
 
 
 def test_sigw_tensors_and_gw_spectra_match_oracle():
@@ -155,8 +161,10 @@ def test_upload_modes_matches_oracle_initial_conditions():
     run = E.Run(E.make_config(32, sigw=0, V0=cfg.V0, ns=cfg.ns, dt=cfg.dt), None)
     run.upload_modes(fk, fny, fdk, fdny, cfg.initial_field, cfg.initial_derivative)
     f, fd = run.field(E.IE_FIELD_F), run.field(E.IE_FIELD_FD)
-    assert fluct_err(st2.field(0), f) < 1e-12
-    assert fluct_err(st2.field(1), fd) < 1e-12
+    # FFTEASY advances its twiddles by a rotation recurrence, the GPU transform
+    # uses exact twiddles: 1e-11 of the fluctuation rms apart
+    assert fluct_err(st2.field(0), f) < 1e-10
+    assert fluct_err(st2.field(1), fd) < 1e-10
 
 
 def test_split_storage_matches_fp64_spectra():
