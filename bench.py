@@ -74,7 +74,7 @@ def read_traffic(name: str):
         return None
 
 
-def cpu_baseline(N: int = 256, target_steps_af: float = 1.02) -> dict | None:
+def cpu_baseline(N: int = 256, target_steps_af: float = 1.25) -> dict | None:
     """The reference program (built from its sources by oracle/build_ref.sh, OpenMP)
     timed on this host: loop time = wall(af = target) - wall(af = 1, no step), the
     initial conditions and outputs being identical in both runs."""
@@ -117,6 +117,117 @@ def cpu_baseline(N: int = 256, target_steps_af: float = 1.02) -> dict | None:
                       f"{w1 - w0:.1f} s of CPU loop time"}
 
 
+def serve_stream(nreq: int, rate: float, seed: int = 20261203) -> dict:
+    """Serving stream: Poisson arrivals, N in {32, 64, 128,
+    256} with weights 0.4/0.3/0.2/0.1, half hilltop / half tabulated USR, 20% with
+    SIGW, 5% with an FP64 twin; each request is a whole run (ICs, inflation to
+    af = 2N, deltaN map, P_zeta)."""
+    import numpy as np
+
+    import ie_engine as E
+    sys.path.insert(0, str(REPO / "tests"))
+    with np.load(REPO / "tests" / "golden" / "potential_table.npz") as z:
+        class T:  # table kept alive for the request pointers
+            x = np.ascontiguousarray(z["field"])
+            v = np.ascontiguousarray(z["potential"])
+            dv = np.ascontiguousarray(z["potential_derivative"])
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    sizes = rng.choice([32, 64, 128, 256], size=nreq, p=[0.4, 0.3, 0.2, 0.1])
+    tab = rng.random(nreq) < 0.5
+    sigw = rng.random(nreq) < 0.2
+    check = rng.random(nreq) < 0.05
+    srv = E.Server(240 << 30, 100, 8)
+    submitted, results, n = {}, [], 0
+    t0 = srv.clock()
+    while len(results) < nreq:
+        now = srv.clock() - t0
+        while n < nreq and arrivals[n] <= now:
+            N = int(sizes[n])
+            if tab[n]:
+                cfg = E.make_config(N, precision=E.IE_SPLIT, numerical=1, sigw=int(sigw[n]), table=T,
+                                    V0=3e-9, dt=0.001, dN=1e-4, Nend=2.0)
+                rid = srv.submit(cfg, 1000 + n, 2.9181235049318586, -0.06727651095116181, 1, int(check[n]), T)
+            else:
+                cfg = E.make_config(N, precision=E.IE_SPLIT, sigw=int(sigw[n]), V0=3.338e-13, ns=0.97, dt=0.001,
+                                    dN=1e-7, Nend=0.001)
+                rid = srv.submit(cfg, 1000 + n, 0.0935, 0.000796, 1, int(check[n]))
+            submitted[rid] = n
+            n += 1
+        srv.pump()
+        for r in srv.results():
+            results.append(r)
+            print(f"# serve: done N={r['N']} status={r['status']} latency={r['latency_s']:.3f}s "
+                  f"({len(results)}/{nreq})", file=sys.stderr, flush=True)
+    makespan = max(r["done_s"] for r in results) - t0
+    srv.close()
+    lat = np.array([r["latency_s"] for r in results])
+    errs = [r["fp64_max_rel_err"] for r in results if r["fp64_max_rel_err"] >= 0]
+    return {"requests": nreq, "arrival_rate_per_s": rate, "runs_per_s": nreq / makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "failed": int(sum(r["status"] != 0 for r in results)),
+            "fp64_checks": len(errs), "fp64_max_rel_err_pzeta": max(errs) if errs else None,
+            "mix": "N 32/64/128/256 (0.4/0.3/0.2/0.1), 50% tabulated USR, 20% SIGW, 5% FP64 twin; "
+                   "each run: ICs, inflation to af=2N, deltaN, P_zeta"}
+
+
+def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203) -> dict:
+    """Batched ensemble: nruns hilltop lattices of N^3, seeds
+    1000 + r, V0 log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split
+    storage.  The same number of steps is timed once as one batch (one launch per
+    kernel per step, HIP graph replay) and once as sequential per-run launches."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import ie_engine as E
+    rng = np.random.default_rng(seed)
+    V0 = 10.0 ** rng.uniform(-11, -9, nruns)
+    ns = rng.uniform(0.95, 0.98, nruns)
+
+    def make(r):
+        cfg = E.make_config(N, precision=E.IE_SPLIT, V0=float(V0[r]), ns=float(ns[r]), dt=0.001, L=10.0)
+        run = E.Run(cfg)
+        run.initialize(1000 + r, PHI0, DPHI0)
+        run.evolve_fields(0.0005)
+        return run
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        runs = list(ex.map(make, range(nruns)))
+    batch = E.Batch(runs)
+    batch.steps(20, graph_steps=20)                # warm-up, graph instantiation
+    t0 = time.perf_counter()
+    batch.steps(steps, graph_steps=50)
+    tb = time.perf_counter() - t0
+    batch.close()
+    for r in runs[:4]:
+        r.leapfrog_steps(5)                          # warm-up of the per-run path
+    t0 = time.perf_counter()
+    for r in runs:
+        r.leapfrog_steps(steps)
+    ts = time.perf_counter() - t0
+    sites = nruns * N ** 3 * steps
+    for r in runs:
+        r.close()
+    return {"runs": nruns, "N": N, "steps": steps, "batched_site_updates_per_s": sites / tb,
+            "sequential_site_updates_per_s": sites / ts, "batched_over_sequential": ts / tb,
+            "batched_ms_per_step": 1e3 * tb / steps, "sequential_ms_per_step": 1e3 * ts / steps}
+
+
+def copy_calibration(nbytes: int = 1 << 30) -> None:
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings (run under the profiler)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 3}))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,7 +238,14 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["split", "fp64", "sigw"], default=None,
                     help="time a single case (for profiling runs)")
+    ap.add_argument("--ensemble-runs", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
               "(ensembles shard by run, see DESIGN.md).", file=sys.stderr)
@@ -185,6 +303,13 @@ def main() -> int:
                            "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("kick_drift_split_256"),
                            "kernel": "k_kick_drift", "kernel_ms": ms, "algorithmic_bytes": alg,
                            "reduce_ms": prof["reduce_ms"], "step_ms_events": prof["step_ms"]}
+    if args.ensemble_runs > 0 and not args.only:
+        out["ensemble"] = ensemble(args.ensemble_runs)
+        print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)
+    if args.serve_requests > 0 and not args.only:
+        out["serving"] = serve_stream(args.serve_requests, args.serve_rate)
+        out["p50_latency_s"] = out["serving"]["p50_latency_s"]
+        out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     if not args.no_cpu_baseline:
         cb = cpu_baseline(N)
         out["cpu_baseline"] = cb

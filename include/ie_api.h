@@ -154,8 +154,11 @@ typedef struct ie_profile {
     int64_t steps;
 } ie_profile;
 ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
-/* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904); RK45
- * returns *accepted and the proposed next step. */
+/* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904), FP64
+ * storage only (IE_ERR_STATE for split storage).  RK45 returns *accepted and the
+ * proposed next step clamped to [rk45_min_dt, hmax]; a rejected step leaves the
+ * state as it was.  Stage buffers (4 or 7 stages of the state) are allocated on
+ * the first call. */
 ie_status ie_rk4_step(ie_run *run, double h);
 ie_status ie_rk45_step(ie_run *run, double h, double hmax, int *accepted, double *h_next);
 
@@ -181,6 +184,9 @@ ie_status ie_get_phiref(ie_run *run, double *out);
 /* nsteps iterations of evolve_derivsN + evolve_fieldsN (src/evolution.cpp:1322-1327)
  * kept in registers on the device: one read and one write of the state. */
 ie_status ie_deltaN_steps(ie_run *run, double dN, int64_t nsteps);
+/* nsteps classical RK4 steps of the deltaN system (rk4_step_deltaN,
+ * src/evolution.cpp:1176-1189), also kept in registers per site. */
+ie_status ie_deltaN_rk4_steps(ie_run *run, double h, int64_t nsteps);
 /* The zeta map of saveN (src/output.cpp:1527-1542): mean of deltaN over sites
  * with deltaN <= cutoff, accumulated in float in lattice order and divided by N^3
  * in float as the reference does, then subtracted from every site.  *nmean

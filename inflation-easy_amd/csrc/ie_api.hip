@@ -38,6 +38,7 @@ struct ie_run {
     int hist_bins = 0;
     double *dn_f = nullptr, *dn_fd = nullptr;  // FP64 deltaN state (split runs)
     int in_deltaN = 0;
+    RkWork *rk4 = nullptr, *rk45 = nullptr;    // RK stage buffers, made on first use
     std::vector<double> htab_x, htab_v, htab_dv;   // host copy of the table
     std::vector<void *> allocs;
     std::string err;
@@ -219,6 +220,10 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->P.V0 = c.V0;
     r->P.ns = c.ns;
     r->P.B2 = c.rescale_B * c.rescale_B;
+    r->P.invB2 = 1.0 / r->P.B2;
+    r->P.hV = c.V0 / r->P.B2;
+    r->P.hV4 = c.V0 * ((1.0 - c.ns) / 4.0) / r->P.B2;
+    r->P.hV2 = c.V0 * ((1.0 - c.ns) / 2.0) / r->P.B2;
     r->P.n = 0;
     if (c.numerical_potential) {
         const size_t n = (size_t)c.tab_n;
@@ -250,11 +255,14 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
 ie_status ie_destroy(ie_run *r)
 {
     if (!r) return IE_OK;
-    if (r->stream) hipStreamSynchronize(r->stream);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
     if (r->fft) fft_destroy(r->fft);
-    for (void *p : r->allocs) hipFree(p);
-    if (r->bg_host) hipHostFree(r->bg_host);
-    if (r->stream) hipStreamDestroy(r->stream);
+    rk_free(r->rk4);
+    rk_free(r->rk45);
+    for (void *p : r->allocs) (void)hipFree(p);
+    if (r->hist) (void)hipFree(r->hist);
+    if (r->bg_host) (void)hipHostFree(r->bg_host);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return IE_OK;
 }
@@ -724,7 +732,10 @@ static double ref_potential(const ie_run *r, double v, bool *ok)
         else hi = mid;
     }
     double y[2];
-    hipMemcpy(y, r->table + c.tab_n + hi - 1, 16, hipMemcpyDeviceToHost);
+    if (hipMemcpy(y, r->table + c.tab_n + hi - 1, 16, hipMemcpyDeviceToHost) != hipSuccess) {
+        *ok = false;
+        return 0.0;
+    }
     return (y[1] + (v - x[hi]) * (y[0] - y[1]) / (x[hi - 1] - x[hi])) / B2;
 }
 
@@ -798,6 +809,29 @@ ie_status ie_deltaN_steps(ie_run *r, double dN, int64_t nsteps)
     for (int64_t q = 0; q < nsteps; ++q) {
         r->bg_host->t += dN;
         r->bg_host->a += dN * r->bg_host->ad;
+    }
+    return push_bg(r);
+}
+
+ie_status ie_deltaN_rk4_steps(ie_run *r, double h, int64_t nsteps)
+{
+    if (!r || nsteps < 0) return fail(r, IE_ERR_ARG, "bad argument");
+    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    double *f, *fd;
+    ie_status st = dn_state(r, &f, &fd);
+    if (st != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    bool ok = true;
+    const double Vref = r->cfg.monotonic == IE_GENERIC ? ref_potential(r, b.phiref, &ok) : 0.0;
+    if (!ok) return fail(r, IE_ERR_TABLE_RANGE, "phiref outside the numerical potential range");
+    IE_HIP(r, dn_rk4_steps(r->gs, f, fd, r->F.dn, r->P, r->cfg.monotonic, b.phiref, Vref, h, (long long)nsteps,
+                           &r->bg->error, r->stream));
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    // a advances by h sum_s B_s ad per step, t by h (src/evolution.cpp:1168-1173)
+    const double B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    for (int64_t q = 0; q < nsteps; ++q) {
+        for (int s = 0; s < 4; ++s) r->bg_host->a += h * B[s] * r->bg_host->ad;
+        r->bg_host->t += h;
     }
     return push_bg(r);
 }
@@ -984,19 +1018,84 @@ ie_status ie_batch_destroy(ie_batch *b)
     return IE_OK;
 }
 
+static ie_status rk_ready(ie_run *r, int stages, RkWork **w)
+{
+    if (r->cfg.precision != IE_FP64) return fail(r, IE_ERR_STATE, "RK integrators need FP64 storage");
+    if (r->in_deltaN) return fail(r, IE_ERR_STATE, "run is in its deltaN phase");
+    RkWork **slot = stages == 4 ? &r->rk4 : &r->rk45;
+    if (!*slot) {
+        const hipError_t e = rk_create(r->N, r->logN, r->F.sigw, stages, slot);
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            return fail(r, IE_ERR_OOM, "RK stage buffers do not fit in device memory");
+        IE_HIP(r, e);
+    }
+    *w = *slot;
+    return pull_bg(r);
+}
+
+static void rk_tensors(ie_run *r, float **h, float **hd)
+{
+    const int par = r->bg_host->parity;
+    for (int c = 0; c < 6; ++c) {
+        h[c] = r->F.sigw ? r->F.h[par][c] : nullptr;
+        hd[c] = r->F.sigw ? r->F.hd[c] : nullptr;
+    }
+}
+
 ie_status ie_rk4_step(ie_run *r, double h)
 {
-    (void)h;
-    return fail(r, IE_ERR_STATE, "RK4 inflation stepping is not available on the GPU path yet");
+    if (!r || !(h > 0.0)) return fail(r, IE_ERR_ARG, "bad argument");
+    RkWork *w = nullptr;
+    ie_status st = rk_ready(r, 4, &w);
+    if (st != IE_OK) return st;
+    float *hh[6], *hd[6];
+    rk_tensors(r, hh, hd);
+    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd;
+    IE_HIP(r, rk4_step(w, r->P, r->bg, r->bg_host, f, fd, hh, hd, h, r->stream));
+    const DevBg advanced = *r->bg_host;
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    b.t = advanced.t;
+    b.a = advanced.a;
+    b.ad = advanced.ad;
+    b.steps += 1;
+    b.d = h;
+    return push_bg(r);
 }
 
 ie_status ie_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *h_next)
 {
-    (void)h;
-    (void)hmax;
-    (void)accepted;
-    (void)h_next;
-    return fail(r, IE_ERR_STATE, "RK45 inflation stepping is not available on the GPU path yet");
+    if (!r || !accepted || !h_next || !(h > 0.0)) return fail(r, IE_ERR_ARG, "bad argument");
+    RkWork *w = nullptr;
+    ie_status st = rk_ready(r, 7, &w);
+    if (st != IE_OK) return st;
+    float *hh[6], *hd[6];
+    rk_tensors(r, hh, hd);
+    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd;
+    const ie_config &c = r->cfg;
+    double factor = 1.0, norm = 0.0;
+    int acc = 0;
+    IE_HIP(r, rk45_step(w, r->P, r->bg, r->bg_host, f, fd, hh, hd, h, c.rk45_abs_tol, c.rk45_rel_tol,
+                        c.rk45_safety, &acc, &factor, &norm, r->stream));
+    const DevBg advanced = *r->bg_host;
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    if (!isfinite(norm)) return fail(r, IE_ERR_NONFINITE, "RK45 error norm is not finite");
+    // step-size proposal (src/evolution.cpp:880-904): grow by the factor after an
+    // accepted step, shrink by at most 0.5 (at least 0.1) after a rejection
+    const double hmin = c.rk45_min_dt > 1e-16 ? c.rk45_min_dt : 1e-16;
+    const double hhi = hmax > hmin ? hmax : hmin;
+    double hn = acc ? h * factor : h * std::max(std::min(factor, 0.5), 0.1);
+    hn = hn < hmin ? hmin : (hn > hhi ? hhi : hn);
+    *accepted = acc;
+    *h_next = hn;
+    if (!acc) return IE_OK;
+    DevBg &b = *r->bg_host;
+    b.t = advanced.t;
+    b.a = advanced.a;
+    b.ad = advanced.ad;
+    b.steps += 1;
+    b.d = h;
+    return push_bg(r);
 }
 
 }  // extern "C"

@@ -129,6 +129,103 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f,
     if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(error, 1);
 }
 
+// RHS of the deltaN system at one state (compute_deltaN_rhs,
+// src/evolution.cpp:1065-1113): (dphi, dpi, dN)/dN-time, with the activity test
+// at that state.
+__device__ __forceinline__ void dn_rhs(const DnParams &D, const double *x, const double *yv, const double *ydv, int n,
+                                       double phi, double pi, int *hint, int *ghint, double *kf, double *kfd,
+                                       double *kdn, int *err)
+{
+    double pot, dpot;
+    if (D.P.numerical) {
+        if (!in_table(x, n, phi)) {
+            *err = 1;
+            pot = 1.0;
+            dpot = 0.0;
+        } else {
+            *hint = walk(x, n, phi, *hint);
+            pot = interp(x, yv, phi, *hint) / D.P.B2;
+            dpot = interp(x, ydv, phi, *hint) / D.P.B2;
+        }
+    } else {
+        pot = hill_V(D.P, phi);
+        dpot = hill_dV(D.P, phi);
+    }
+    const double ratio = dpot / pot;
+    bool act;
+    if (D.mono > 0) act = fabs(phi) > fabs(D.phiref);
+    else if (D.mono < 0) act = fabs(phi) < fabs(D.phiref);
+    else act = pot > D.Vref;
+    (void)ghint;
+    *kfd = -(3.0 - 0.5 * (pi * pi)) * (pi + ratio);
+    if (act) {
+        *kf = pi;
+        *kdn = 1.0;
+    } else {
+        *kf = 0.0;
+        *kdn = 0.0;
+    }
+}
+
+// Classical RK4 of the deltaN system, nsteps in registers per site
+// (rk4_step_deltaN, src/evolution.cpp:1176-1189 with the stage states of
+// :1115-1132 and the update of :1146-1174; zero Butcher entries skipped as there).
+__global__ __launch_bounds__(kBlock) void k_deltaN_rk4(double *__restrict__ f, double *__restrict__ fd,
+                                                        double *__restrict__ dn, size_t gs, DnParams D, double h,
+                                                        long long nsteps, int *error)
+{
+    __shared__ double tx[kMaxTable], tv[kMaxTable], tdv[kMaxTable];
+    const int n = D.P.numerical ? D.P.n : 0;
+    const bool lds = n > 0 && n <= kMaxTable;
+    for (int e = threadIdx.x; lds && e < n; e += blockDim.x) {
+        tx[e] = D.P.x[e];
+        tv[e] = D.P.v[e];
+        tdv[e] = D.P.dv[e];
+    }
+    __syncthreads();
+    const double *x = lds ? tx : D.P.x, *yv = lds ? tv : D.P.v, *ydv = lds ? tdv : D.P.dv;
+    const double A[4][4] = {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0}};
+    const double B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    int err = 0;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        double phi = f[id], pi = fd[id], N = dn[id];
+        int hint = n > 0 ? n / 2 : 0, ghint = hint;
+        for (long long s = 0; s < nsteps; ++s) {
+            double kf[4], kfd[4], kdn[4];
+            dn_rhs(D, x, yv, ydv, n, phi, pi, &hint, &ghint, &kf[0], &kfd[0], &kdn[0], &err);
+#pragma unroll
+            for (int st = 1; st < 4; ++st) {
+                double sf = 0.0, sfd = 0.0, sdn = 0.0;
+#pragma unroll
+                for (int q = 0; q < st; ++q) {
+                    const double c = A[st][q];
+                    if (c == 0.0) continue;
+                    sf += c * kf[q];
+                    sfd += c * kfd[q];
+                    sdn += c * kdn[q];
+                }
+                (void)sdn;
+                dn_rhs(D, x, yv, ydv, n, phi + h * sf, pi + h * sfd, &hint, &ghint, &kf[st], &kfd[st], &kdn[st],
+                       &err);
+            }
+            double sf = 0.0, sfd = 0.0, sdn = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                sf += B[q] * kf[q];
+                sfd += B[q] * kfd[q];
+                sdn += B[q] * kdn[q];
+            }
+            phi += h * sf;
+            pi += h * sfd;
+            N += h * sdn;
+        }
+        f[id] = phi;
+        fd[id] = pi;
+        dn[id] = N;
+    }
+    if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(error, 1);
+}
+
 // evolve_fieldsN(d) alone (src/evolution.cpp:1015-1037).
 __global__ __launch_bounds__(kBlock) void k_fieldsN(double *__restrict__ f, const double *__restrict__ fd,
                                                      double *__restrict__ dn, size_t gs, DnParams D, double d)
@@ -390,6 +487,14 @@ hipError_t dn_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, 
 {
     size_t b = (gs + kBlock - 1) / kBlock;
     k_deltaN_steps<<<(int)b, kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), dN, nsteps, error);
+    return hipGetLastError();
+}
+
+hipError_t dn_rk4_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, int mono, double phiref,
+                        double Vref, double h, long long nsteps, int *error, hipStream_t s)
+{
+    size_t b = (gs + kBlock - 1) / kBlock;
+    k_deltaN_rk4<<<(int)b, kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), h, nsteps, error);
     return hipGetLastError();
 }
 

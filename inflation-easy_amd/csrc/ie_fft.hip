@@ -320,10 +320,10 @@ void fft_destroy(FftPlan *p)
 {
     if (!p) return;
     for (int s = 0; s < 6; ++s)
-        if (p->H[s]) hipFree(p->H[s]);
-    if (p->tw) hipFree(p->tw);
-    if (p->partials) hipFree(p->partials);
-    if (p->sums) hipFree(p->sums);
+        if (p->H[s]) (void)hipFree(p->H[s]);
+    if (p->tw) (void)hipFree(p->tw);
+    if (p->partials) (void)hipFree(p->partials);
+    if (p->sums) (void)hipFree(p->sums);
     delete p;
 }
 

@@ -40,6 +40,8 @@ struct Pot {
     int n;
     double V0, ns, B2;
     const double *x, *v, *dv;   // device table, descending x
+    // constants of the lattice kernels: V0/B^2, V0 (1-ns)/4 /B^2, V0 (1-ns)/2 /B^2, 1/B^2
+    double hV, hV4, hV2, invB2;
 };
 
 // Device pointers of one lattice's fields.  Storage of f, fd is double (FP64) or
@@ -166,6 +168,40 @@ __device__ __forceinline__ double pot_dV(const Pot &P, double v, int *err)
     return -P.V0 * (1.0 - P.ns) / 2.0 * v / P.B2;
 }
 
+// Lattice-kernel forms: the division by B^2 folded into host constants (one
+// rounding apart from the reference's per-call expression; the lattice sums are
+// order-dependent at that level anyway).  The deltaN kernel keeps the exact
+// expressions.
+__device__ __forceinline__ void pot_eval_fast(const Pot &P, double v, double *V, double *dV, int *err)
+{
+    if (P.numerical) {
+        const int l = pot_bracket(P, v, err);
+        *V = pot_interp(P, v, l, P.v) * P.invB2;
+        *dV = pot_interp(P, v, l, P.dv) * P.invB2;
+    } else {
+        *V = P.hV - P.hV4 * (v * v);
+        *dV = -P.hV2 * v;
+    }
+}
+
+__device__ __forceinline__ double pot_V_fast(const Pot &P, double v, int *err)
+{
+    if (P.numerical) {
+        const int l = pot_bracket(P, v, err);
+        return pot_interp(P, v, l, P.v) * P.invB2;
+    }
+    return P.hV - P.hV4 * (v * v);
+}
+
+__device__ __forceinline__ double pot_dV_fast(const Pot &P, double v, int *err)
+{
+    if (P.numerical) {
+        const int l = pot_bracket(P, v, err);
+        return pot_interp(P, v, l, P.dv) * P.invB2;
+    }
+    return -P.hV2 * v;
+}
+
 // ---------------------------------------------------------------- launchers
 // (ie_lattice.hip)
 
@@ -214,6 +250,8 @@ hipError_t dn_fieldsN(size_t gs, double *f, const double *fd, double *dn, const 
                       double Vref, double d, hipStream_t s);
 hipError_t dn_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, int mono, double phiref, double Vref,
                     double dN, long long nsteps, int *error, hipStream_t s);
+hipError_t dn_rk4_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, int mono, double phiref,
+                        double Vref, double h, long long nsteps, int *error, hipStream_t s);
 hipError_t dn_phiref(size_t gs, const double *f, const Pot &P, int mono, double *work, double *out, hipStream_t s);
 hipError_t dn_float_mean(const double *dn, size_t gs, double cutoff, float *out, hipStream_t s);
 hipError_t dn_zeta(double *dn, size_t gs, const float *nmean, double cutoff, int zero, hipStream_t s);
@@ -222,5 +260,16 @@ hipError_t field_center(double *g, size_t gs, double *work, double *sum_out, hip
 hipError_t field_minmax(const double *g, size_t gs, double *work, hipStream_t s, int *nb_out);
 hipError_t field_histogram(const double *g, size_t gs, double bmin, double bmax, double df, int nbins,
                            unsigned long long *counts, hipStream_t s);
+
+// RK4 / RK45 inflation steps on FP64 storage (ie_rk.hip).  Stage buffers are
+// allocated once per run (stages x (2 N^3 doubles + 12 N^3 floats with tensors)).
+struct RkWork;
+hipError_t rk_create(int N, int logN, int sigw, int stages, RkWork **out);
+void rk_free(RkWork *w);
+hipError_t rk4_step(RkWork *w, const Pot &P, DevBg *bg, DevBg *bg_host, double *f, double *fd, float *const *h,
+                    float *const *hd, double step, hipStream_t s);
+hipError_t rk45_step(RkWork *w, const Pot &P, DevBg *bg, DevBg *bg_host, double *f, double *fd, float *const *h,
+                     float *const *hd, double step, double abs_tol, double rel_tol, double safety, int *accepted,
+                     double *factor, double *err_norm, hipStream_t s);
 
 }  // namespace ie

@@ -59,6 +59,48 @@ __device__ __forceinline__ Walk site_walk(size_t gs, unsigned G, unsigned b)
 
 __device__ __forceinline__ Walk site_walk(size_t gs) { return site_walk(gs, gridDim.x, blockIdx.x); }
 
+// 32-bit form of the walk for the step kernels (N <= 1024, so N^3 < 2^31):
+// integer index arithmetic stays in single 32-bit VALU instructions.
+struct Walk32 {
+    unsigned begin, stride, end;
+};
+
+__device__ __forceinline__ Walk32 site_walk32(unsigned gs, unsigned G, unsigned b)
+{
+    Walk32 w;
+    if (G % kXcd == 0 && gs % (unsigned)(kXcd * kBlock) == 0) {
+        const unsigned xcd = b % kXcd, lb = b / kXcd, per = G / kXcd;
+        const unsigned slab = gs / kXcd;
+        w.begin = xcd * slab + lb * kBlock + threadIdx.x;
+        w.stride = per * kBlock;
+        w.end = (xcd + 1) * slab;
+    } else {
+        w.begin = b * kBlock + threadIdx.x;
+        w.stride = G * kBlock;
+        w.end = gs;
+    }
+    return w;
+}
+
+struct Nb32 {
+    unsigned ip, im, jp, jm, kp, km;
+};
+
+__device__ __forceinline__ Nb32 nb32(unsigned id, unsigned N, unsigned logN)
+{
+    const unsigned m = N - 1;
+    const unsigned k = id & m, j = (id >> logN) & m, i = id >> (2 * logN);
+    const unsigned base_k = id - k, base_j = id - (j << logN), base_i = id - (i << (2 * logN));
+    Nb32 n;
+    n.kp = base_k + ((k + 1) & m);
+    n.km = base_k + ((k - 1) & m);
+    n.jp = base_j + (((j + 1) & m) << logN);
+    n.jm = base_j + (((j - 1) & m) << logN);
+    n.ip = base_i + (((i + 1) & m) << (2 * logN));
+    n.im = base_i + (((i - 1) & m) << (2 * logN));
+    return n;
+}
+
 // One lattice of a batched launch (ie_batch_steps): its state, its workgroup
 // count G and its first workgroup in the launch.
 struct RunDesc {
@@ -134,26 +176,82 @@ __device__ void finalize_scalars(DevBg *bg, int mode, int split)
 
 // ----------------------------------------------------------------- kernels
 
+// The seven stencil values of U sites of a workgroup's walk, loaded before any
+// arithmetic so U x 7 loads are in flight per thread.  Sites past the end of the
+// walk load a valid site (the last one) and are masked when used.
+template <typename T, int U>
+struct Stencil {
+    T c[U], kp[U], km[U], jp[U], jm[U], ip[U], im[U];
+    unsigned id[U];
+    bool ok[U];
+};
+
+template <typename T, int U>
+__device__ __forceinline__ void load_stencil(const T *__restrict__ f, const Walk32 &w, unsigned base, unsigned N,
+                                             unsigned logN, Stencil<T, U> &s)
+{
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const unsigned raw = base + (unsigned)u * w.stride;
+        s.ok[u] = raw < w.end;
+        const unsigned id = s.ok[u] ? raw : w.end - 1;
+        s.id[u] = id;
+        const Nb32 n = nb32(id, N, logN);
+        s.c[u] = f[id];
+        s.kp[u] = f[n.kp];
+        s.km[u] = f[n.km];
+        s.jp[u] = f[n.jp];
+        s.jm[u] = f[n.jm];
+        s.ip[u] = f[n.ip];
+        s.im[u] = f[n.im];
+    }
+}
+
+// 7-point Laplacian of site u in FP64, summed in the reference's order
+// (src/evolution.cpp:34-51): k+, k-, j+, j-, i+, i-, then -6 f.
+template <typename T, int U>
+__device__ __forceinline__ double stencil_lapl(const Stencil<T, U> &s, int u)
+{
+    return (double)s.kp[u] + (double)s.km[u] + (double)s.jp[u] + (double)s.jm[u] + (double)s.ip[u] +
+           (double)s.im[u] - 6.0 * (double)s.c[u];
+}
+
+template <typename T>
+struct Unroll {
+    static constexpr int value = sizeof(T) == 4 ? 8 : 4;
+};
+
 template <typename T, bool SPLIT>
 __device__ __forceinline__ void friedmann_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials,
                                                       double d_arg, int loop, unsigned G, unsigned blk, double *smem)
 {
+    constexpr int U = Unroll<T>::value;
     if (loop && bg->done) return;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
     const size_t gs = (size_t)F.N * F.N * F.N;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
     const double phibar = SPLIT ? bg->phibar : 0.0;
     double acc[3] = {0.0, 0.0, 0.0};
     int err = 0;
-    const Walk w = site_walk(gs, G, blk);
-    for (size_t id = w.begin; id < w.end; id += w.stride) {
-        const Nb n = neighbours(id, F.N, F.logN);
-        const double c = (double)f[id];
-        const double lap = lapl64(f, id, n);
-        double V, dV;
-        pot_eval(P, SPLIT ? phibar + c : c, &V, &dV, &err);
-        acc[0] -= c * lap;
-        acc[1] += V;
-        acc[2] += dV;
+    const Walk32 w = site_walk32((unsigned)gs, G, blk);
+    for (unsigned base = w.begin; base < w.end; base += (unsigned)U * w.stride) {
+        Stencil<T, U> s;
+        load_stencil<T, U>(f, w, base, N, logN, s);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!s.ok[u]) continue;
+            const double c = (double)s.c[u];
+            const double lap = stencil_lapl<T, U>(s, u);
+            acc[0] -= c * lap;
+            if (SPLIT) {
+                double V, dV;
+                pot_eval_fast(P, phibar + c, &V, &dV, &err);
+                acc[1] += V;
+                acc[2] += dV;
+            } else {
+                acc[1] += pot_V_fast(P, c, &err);
+            }
+        }
     }
     flag_error(bg, err);
     block_sum<3>(acc, smem);
@@ -217,20 +315,34 @@ __device__ __forceinline__ void kick_drift_body(const Fields &F, const Pot &P, D
     const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
     const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
     int err = 0;
-    const Walk w = site_walk(gs, G, blk);
-    for (size_t id = w.begin; id < w.end; id += w.stride) {
-        double v = (double)fd[id];
-        if (MODE & KICK) {
-            const Nb n = neighbours(id, F.N, F.logN);
-            const double c = (double)f[id];
-            const double lap = lapl64(f, id, n);
-            double dV = pot_dV(P, SPLIT ? phibar + c : c, &err);
-            if (SPLIT) dV -= vpmean;
-            v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
-            fd[id] = (T)v;
-            v = (double)(T)v;
+    if (MODE & KICK) {
+        constexpr int U = Unroll<T>::value;
+        const Walk32 w = site_walk32((unsigned)gs, G, blk);
+        for (unsigned base = w.begin; base < w.end; base += (unsigned)U * w.stride) {
+            Stencil<T, U> s;
+            load_stencil<T, U>(f, w, base, (unsigned)F.N, (unsigned)F.logN, s);
+            T vin[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) vin[u] = fd[s.id[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!s.ok[u]) continue;
+                const double c = (double)s.c[u];
+                const double lap = stencil_lapl<T, U>(s, u);
+                double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+                if (SPLIT) dV -= vpmean;
+                double v = (double)vin[u];
+                v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
+                const T vs = (T)v;
+                fd[s.id[u]] = vs;
+                if (MODE & DRIFT) fout[s.id[u]] = (T)(c + d * (double)vs);
+            }
         }
-        if (MODE & DRIFT) fout[id] = (T)((double)f[id] + d * v);
+    } else {
+        // drift alone, in place (pointwise): through one unrestricted pointer
+        T *g = (T *)F.f[par];
+        const Walk w = site_walk(gs, G, blk);
+        for (size_t id = w.begin; id < w.end; id += w.stride) g[id] = (T)((double)g[id] + d * (double)fd[id]);
     }
     flag_error(bg, err);
     if (!finalize) return;
@@ -269,31 +381,39 @@ __device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d
     const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
     const double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
     const double half_over_dx = 0.5 / bg->dx;
-    const Walk w = site_walk(gs, G, blk);
-    for (size_t id = w.begin; id < w.end; id += w.stride) {
-        const Nb n = neighbours(id, F.N, F.logN);
-        double src[6];
-        if (MODE & KICK) {
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
+    if (MODE == DRIFT) {
+        // drift alone, in place (pointwise), through unrestricted pointers
+        const Walk w = site_walk(gs, G, blk);
+        for (int c = 0; c < 6; ++c) {
+            float *h = F.h[par][c];
+            const float *hd = F.hd[c];
+            for (size_t id = w.begin; id < w.end; id += w.stride) h[id] = h[id] + (float)(d * (double)hd[id]);
+        }
+    } else {
+        const Walk32 w = site_walk32((unsigned)gs, G, blk);
+        for (unsigned id = w.begin; id < w.end; id += w.stride) {
+            const Nb32 n = nb32(id, N, logN);
             // central gradient of phi (dfdx, src/evolution.cpp:55-77)
             const double gx = ((double)f[n.ip] - (double)f[n.im]) * half_over_dx;
             const double gy = ((double)f[n.jp] - (double)f[n.jm]) * half_over_dx;
             const double gz = ((double)f[n.kp] - (double)f[n.km]) * half_over_dx;
-            src[0] = gx * gx; src[1] = gy * gy; src[2] = gz * gz;
-            src[3] = gx * gy; src[4] = gx * gz; src[5] = gy * gz;
-        }
+            const double src[6] = {gx * gx, gy * gy, gz * gz, gx * gy, gx * gz, gy * gz};
+            float hc[6], hl[6], hv[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const float *__restrict__ h = F.h[par][c];
-            float *__restrict__ hout = F.h[MODE == KICKDRIFT ? par ^ 1 : par][c];
-            float *__restrict__ hd = F.hd[c];
-            float hv = hd[id];
-            if (MODE & KICK) {
-                const double lap_h = (double)lapl32(h, id, n);
-                const double rhs = d * (laplnorm * lap_h - friction * (double)hv + srcamp * src[c]);
-                hv = hv + (float)rhs;
-                hd[id] = hv;
+            for (int c = 0; c < 6; ++c) {   // all loads of the site first
+                const float *__restrict__ h = F.h[par][c];
+                hc[c] = h[id];
+                hl[c] = h[n.kp] + h[n.km] + h[n.jp] + h[n.jm] + h[n.ip] + h[n.im] - 6.0f * hc[c];
+                hv[c] = F.hd[c][id];
             }
-            if (MODE & DRIFT) hout[id] = h[id] + (float)(d * (double)hv);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const double rhs = d * (laplnorm * (double)hl[c] - friction * (double)hv[c] + srcamp * src[c]);
+                const float hn = hv[c] + (float)rhs;
+                F.hd[c][id] = hn;
+                if (MODE & DRIFT) F.h[par ^ 1][c][id] = hc[c] + (float)(d * (double)hn);
+            }
         }
     }
     if (!last_block(&bg->counter, G)) return;

@@ -22,6 +22,8 @@ IE_FIELD_F, IE_FIELD_FD, IE_FIELD_DELTAN = 0, 1, 2
 
 STATUS = {0: "IE_OK", 1: "IE_ERR_ARG", 2: "IE_ERR_COURANT", 3: "IE_ERR_TABLE_RANGE", 4: "IE_ERR_NONFINITE",
           5: "IE_ERR_RK45", 6: "IE_ERR_OOM", 7: "IE_ERR_HIP", 8: "IE_ERR_STATE"}
+(IE_OK, IE_ERR_ARG, IE_ERR_COURANT, IE_ERR_TABLE_RANGE, IE_ERR_NONFINITE, IE_ERR_RK45, IE_ERR_OOM, IE_ERR_HIP,
+ IE_ERR_STATE) = range(9)
 
 
 class EngineError(RuntimeError):
@@ -142,6 +144,7 @@ def lib() -> C.CDLL:
             "ie_evolve_fieldsN": (st, [P, C.c_double]),
             "ie_get_phiref": (st, [P, dp]),
             "ie_deltaN_steps": (st, [P, C.c_double, C.c_int64]),
+            "ie_deltaN_rk4_steps": (st, [P, C.c_double, C.c_int64]),
             "ie_deltaN_zeta": (st, [P, C.c_double, C.POINTER(C.c_float)]),
             "ie_deltaN_zero_unfinished": (st, [P, C.c_double]),
             "ie_deltaN_center": (st, [P]),
@@ -333,6 +336,15 @@ class Run:
         self._check(lib().ie_leapfrog_steps(self.h, n, C.byref(b)))
         return b.as_dict()
 
+    def rk4_step(self, h: float):
+        self._check(lib().ie_rk4_step(self.h, h))
+
+    def rk45_step(self, h: float, hmax: float) -> tuple[bool, float]:
+        """One Dormand-Prince trial step; returns (accepted, proposed next step)."""
+        acc, hn = C.c_int(), C.c_double()
+        self._check(lib().ie_rk45_step(self.h, h, hmax, C.byref(acc), C.byref(hn)))
+        return bool(acc.value), hn.value
+
     def profile_steps(self, n: int) -> dict:
         p = Profile()
         self._check(lib().ie_profile_steps(self.h, n, C.byref(p)))
@@ -367,6 +379,9 @@ class Run:
 
     def deltaN_steps(self, dN: float, n: int):
         self._check(lib().ie_deltaN_steps(self.h, dN, n))
+
+    def deltaN_rk4_steps(self, h: float, n: int):
+        self._check(lib().ie_deltaN_rk4_steps(self.h, h, n))
 
     def deltaN_zeta(self, cutoff: float) -> float:
         m = C.c_float()

@@ -94,6 +94,34 @@ def test_deltaN_map_bitwise(case, Nend):
     assert np.allclose(v[live], vo[live], rtol=1e-10)
 
 
+@pytest.mark.parametrize("case,Nend", [("tabulated_n16", 0.3), ("hilltop_n16", 0.0001)])
+def test_deltaN_rk4_map_bitwise(case, Nend):
+    """RK4 deltaN integrator (src/evolution.cpp:1176-1189) kept in registers."""
+    st, table = end_of_inflation(case, Nend)
+    c = st.cfg
+    run = E.Run(E.make_config(c.N, numerical=c.numerical_potential, sigw=0, table=table, V0=c.V0, ns=c.ns,
+                              L=c.L, dt=c.dt, af=c.af, dN=c.dN, Nend=c.Nend, monotonic=c.monotonic), table)
+    run.upload_fields(st.field(0), st.field(1))
+    bg = st.background()
+    run.set_background(t=bg["t"], a=bg["a"], ad=bg["ad"], astep=bg["astep"], phiref=0.0)
+    O.lib().ieo_initializeN(st.ptr)
+    phiref = O.lib().ieo_get_phiref(st.ptr)
+    st.set_background(phiref=phiref)
+    Ne, nsteps = 0.0, 0
+    while Ne <= c.Nend:
+        O.lib().ieo_rk4_stepN(st.ptr, c.dN)
+        Ne += c.dN
+        nsteps += 1
+    run.deltaN_init()
+    assert run.get_phiref() == phiref
+    run.set_background(phiref=phiref)
+    run.deltaN_rk4_steps(c.dN, nsteps)
+    assert np.array_equal(st.field(2), run.field(E.IE_FIELD_DELTAN))
+    assert np.array_equal(st.field(0), run.field(E.IE_FIELD_F))
+    assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
+    assert run.background()["a"] == st.background()["a"]
+
+
 def test_histogram_matches_reference_rule():
     st, table = end_of_inflation("hilltop_n16", 0.0002)
     c = st.cfg
