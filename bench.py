@@ -301,8 +301,10 @@ def main() -> int:
         achieved = alg / (ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("kick_drift_split_256"),
-                           "kernel": "k_kick_drift", "kernel_ms": ms, "algorithmic_bytes": alg,
-                           "reduce_ms": prof["reduce_ms"], "step_ms_events": prof["step_ms"]}
+                           "kernel": "k_row_kick (fused kick + drift)", "kernel_ms": ms,
+                           "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
+                           "reduce_frac": (alg / 4) / (prof["reduce_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                           "step_ms_events": prof["step_ms"]}
     if args.ensemble_runs > 0 and not args.only:
         out["ensemble"] = ensemble(args.ensemble_runs)
         print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)

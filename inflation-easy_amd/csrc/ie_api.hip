@@ -478,21 +478,24 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
 {
     if (!r || nsteps < 1 || !out) return fail(r, IE_ERR_ARG, "bad argument");
     const int split = r->cfg.precision == IE_SPLIT;
-    const int nev = r->F.sigw ? 4 : 3;
+    // events bracket each launch of a step: reduce | finish | scalar | tensor | finish
+    const int nev = 6;
     std::vector<hipEvent_t> ev((size_t)nsteps * nev);
     for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
     for (int n = 0; n < nsteps; ++n) {
         hipEvent_t *e = &ev[(size_t)n * nev];
         IE_HIP(r, hipEventRecord(e[0], r->stream));
-        IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+        IE_HIP(r, launch_reduce(r->F, split, r->P, r->bg, r->partials, 1, r->stream));
         IE_HIP(r, hipEventRecord(e[1], r->stream));
-        IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream, 0));
+        IE_HIP(r, launch_friedmann_finish(r->F, r->bg, r->partials, 0.0, 1, r->stream));
         IE_HIP(r, hipEventRecord(e[2], r->stream));
-        if (r->F.sigw) {
-            IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream, 1));
-            IE_HIP(r, hipEventRecord(e[3], r->stream));
-        }
+        IE_HIP(r, launch_scalar_step(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+        IE_HIP(r, hipEventRecord(e[3], r->stream));
+        if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+        IE_HIP(r, hipEventRecord(e[4], r->stream));
+        IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
+        IE_HIP(r, hipEventRecord(e[5], r->stream));
     }
     ie_status st = sync(r);
     if (st != IE_OK) return st;
@@ -502,13 +505,13 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
         float ms = 0.f;
         IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
         red += ms;
-        IE_HIP(r, hipEventElapsedTime(&ms, e[1], e[2]));
+        IE_HIP(r, hipEventElapsedTime(&ms, e[2], e[3]));
         kd += ms;
         if (r->F.sigw) {
-            IE_HIP(r, hipEventElapsedTime(&ms, e[2], e[3]));
+            IE_HIP(r, hipEventElapsedTime(&ms, e[3], e[4]));
             ten += ms;
         }
-        IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[nev - 1]));
+        IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[5]));
         step += ms;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);

@@ -207,16 +207,31 @@ __device__ __forceinline__ double pot_dV_fast(const Pot &P, double v, int *err)
 
 enum KickMode { KICK = 1, DRIFT = 2, KICKDRIFT = 3 };
 
-// Reduction of the Friedmann sums over the current f with the pre-kick scalar
-// update of evolve_derivs (src/evolution.cpp:153-167) in the last workgroup.
-// loop = 0: explicit step d_arg; loop = 1: loop step d = dt astep^{s-1}, and the
-// whole step is skipped once done.
+// Workgroups of the step kernels of an N^3 lattice (the same alone and batched,
+// so a lattice's fixed-order sums do not depend on its company).
+unsigned step_blocks(int N);
+// Partial Friedmann sums of the current f, one triple per workgroup.
+hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, int loop,
+                         hipStream_t s);
+// One workgroup: fixed-order total of the partials and the pre-kick scalar update
+// of evolve_derivs (src/evolution.cpp:153-167).  loop = 0: explicit step d_arg;
+// loop = 1: loop step d = dt astep^{s-1}, skipped once done.
+hipError_t launch_friedmann_finish(const Fields &F, DevBg *bg, const double *partials, double d_arg, int loop,
+                                   hipStream_t s);
+// Kick / drift / fused kick+drift of f, fd; of h, hd (tensor step).
+hipError_t launch_scalar_step(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg, int loop,
+                              hipStream_t s);
+hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s);
+// One thread: the scalar updates that close a kick, a drift or a loop step.
+hipError_t launch_step_finish(DevBg *bg, int split, int mode, double d_arg, int loop, hipStream_t s);
+// launch_reduce + launch_friedmann_finish.
 hipError_t launch_friedmann_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials,
                                    double d_arg, int loop, hipStream_t s);
-// Kick / drift / fused kick+drift of f, fd (and h, hd when sigw).  loop = 1: a
-// step of the device-resident loop (skipped once done, step from DevBg);
-// loop = 0: explicit call with step d_arg (drift) or the step of the last reduce.
-// part: -1 scalar and tensor launches, 0 scalar only, 1 tensor only.
+// Scalar step, tensor step (when sigw) and step finish.  loop = 1: a step of the
+// device-resident loop (skipped once done, step from DevBg); loop = 0: explicit
+// call with step d_arg (drift) or the step of the last reduce.
+// part: -1 all launches, 0 scalar step only (with the finish when !sigw),
+// 1 tensor step and finish.
 hipError_t launch_kick_drift(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg,
                              int loop, hipStream_t s, int part = -1);
 // Sums {sum f, sum fd, sum f^2} (split: about the homogeneous mode), kinetic,

@@ -1,25 +1,33 @@
 // ie_lattice.hip -- leapfrog step of the inflaton lattice on gfx950.
 //
-// One step of run_evolution_loop (src/evolution.cpp:937-945) is three launches on
-// the run's stream, with no host involvement:
-//   1. k_friedmann_reduce: sum_x -f lapl f and sum_x V(f) (gradient_energy and
-//      potential_energy, src/evolution.cpp:130-136, src/potential.cpp:151-171),
-//      and in its last workgroup the implicit Friedmann half-kick of ad
-//      (src/evolution.cpp:153-167);
-//   2. k_kick_drift: fd += d (laplnorm lapl f - friction fd - potnorm V'(f)) and
+// One step of run_evolution_loop (src/evolution.cpp:937-945) is a short chain of
+// launches on the run's stream, with no host involvement:
+//   1. reduce: per-workgroup partial sums of -f lapl f and V(f) (gradient_energy
+//      and potential_energy, src/evolution.cpp:130-136, src/potential.cpp:151-171);
+//   2. friedmann_finish (one workgroup): the fixed-order total and the implicit
+//      Friedmann half-kick of ad (src/evolution.cpp:153-167), the step's
+//      coefficients into DevBg;
+//   3. scalar step: fd += d (laplnorm lapl f - friction fd - potnorm V'(f)) and
 //      f_next = f + d fd (src/evolution.cpp:170-181, 265-270), f ping-ponged
 //      between two buffers so neighbours read the old f;
-//   3. (SIGW) k_tensor: the same for the six h_ij with source 2 d_i phi d_j phi
+//   4. (SIGW) tensor step: the same for the six h_ij with source 2 d_i phi d_j phi
 //      (src/evolution.cpp:185-253, 274-280);
-// the last workgroup of the final launch does the second half-kick of ad, the
-// drift of t and a, flips the buffer parity and evaluates the loop test a <= af.
+//   5. step_finish (one thread): second half-kick of ad, drift of t and a, buffer
+//      parity flip, loop test a <= af (src/evolution.cpp:256-283, 942-944).
 // Every launch starts by reading DevBg::done, so steps queued past af are no-ops.
+// Cross-workgroup results travel only through kernel boundaries: the streaming
+// kernels end with no device-scope fence (on a multi-XCD part a release fence
+// writes back the workgroup's L2, which stalls a streaming kernel).
 //
-// Memory: fields are z-fastest (idx = (i N + j) N + k, src/main.h:34-38); a wave
-// covers 64 consecutive k, so every access is a coalesced 256/512-byte row
-// segment.  The grid is a fixed number of workgroups that stride over the
-// lattice; workgroup b works on the contiguous slab of XCD b % 8, keeping the
-// i +- 1 planes of the stencil inside one XCD's L2.
+// Memory: fields are z-fastest (idx = (i N + j) N + k, src/main.h:34-38).
+// For N >= 64 the step kernels work on rows: a wave owns R consecutive j rows of
+// one (i, k-segment) column, each lane holds E = min(N/64, 4) consecutive k of a
+// 64 E-site segment (one vector load per row), the k +- 1 neighbours come from the
+// lane's own registers and two lane shuffles, the j +- 1 rows from a sliding
+// window in registers, and the i +- 1 rows from the planes that neighbouring
+// workgroups of the same XCD are streaming (L2 hits).  Row and plane offsets are
+// wave-uniform (scalar registers).  Workgroup b works in the contiguous slab of
+// planes of XCD b % 8.  N < 64 uses a per-site walk with the same arithmetic.
 //
 // Storage: FP64 (f, fd double) or split (float fluctuation about the FP64
 // homogeneous mode phibar, pibar in DevBg).  Every right-hand side is evaluated
@@ -59,8 +67,7 @@ __device__ __forceinline__ Walk site_walk(size_t gs, unsigned G, unsigned b)
 
 __device__ __forceinline__ Walk site_walk(size_t gs) { return site_walk(gs, gridDim.x, blockIdx.x); }
 
-// 32-bit form of the walk for the step kernels (N <= 1024, so N^3 < 2^31):
-// integer index arithmetic stays in single 32-bit VALU instructions.
+// 32-bit form of the walk for the small-lattice step kernels.
 struct Walk32 {
     unsigned begin, stride, end;
 };
@@ -101,8 +108,8 @@ __device__ __forceinline__ Nb32 nb32(unsigned id, unsigned N, unsigned logN)
     return n;
 }
 
-// One lattice of a batched launch (ie_batch_steps): its state, its workgroup
-// count G and its first workgroup in the launch.
+// One lattice of a batched launch (ie_batch_steps): its state and its workgroup
+// count G (the same as for a launch of the lattice alone).
 struct RunDesc {
     Fields F;
     Pot P;
@@ -141,10 +148,10 @@ __device__ __forceinline__ double lapl64(const T *__restrict__ g, size_t id, con
            (double)g[n.im] - 6.0 * (double)g[id];
 }
 
-// FP32 Laplacian of a tensor component (lapl<float>, same order).
-__device__ __forceinline__ float lapl32(const float *__restrict__ g, size_t id, const Nb &n)
+__device__ __forceinline__ double lapl_sites(double kp, double km, double jp, double jm, double ip, double im,
+                                             double c)
 {
-    return g[n.kp] + g[n.km] + g[n.jp] + g[n.jm] + g[n.ip] + g[n.im] - 6.0f * g[id];
+    return kp + km + jp + jm + ip + im - 6.0 * c;
 }
 
 __device__ __forceinline__ void flag_error(DevBg *bg, int err)
@@ -152,8 +159,10 @@ __device__ __forceinline__ void flag_error(DevBg *bg, int err)
     if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(&bg->error, 1);
 }
 
-// Scalar updates that close a kick, a drift or a whole step (thread 0 of the
-// last workgroup).  Mirrors src/evolution.cpp:256,262,283 and :942-944.
+// ------------------------------------------------------------ finish kernels
+
+// Scalar updates that close a kick, a drift or a whole step.  Mirrors
+// src/evolution.cpp:256,262,283 and :942-944.
 __device__ void finalize_scalars(DevBg *bg, int mode, int split)
 {
     const double d = bg->d;
@@ -174,7 +183,283 @@ __device__ void finalize_scalars(DevBg *bg, int mode, int split)
     }
 }
 
-// ----------------------------------------------------------------- kernels
+// Fixed-order total of the G partials and the implicit Friedmann half-kick
+// (evolve_derivs, src/evolution.cpp:153-167).  One workgroup.
+__device__ void friedmann_finish_body(DevBg *bg, const double *partials, unsigned G, size_t gs, double d_arg,
+                                      int loop, double *smem)
+{
+    if (loop && bg->done) return;
+    double tot[3];
+    sum_partials<3>(partials, (int)G, tot, smem);
+    if (threadIdx.x != 0) return;
+    bg->red[0] = tot[0];
+    bg->red[1] = tot[1];
+    bg->red[2] = tot[2];
+    const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
+    const double gsd = (double)gs;
+    const double d = loop ? bg->dt * pow(bg->astep, s - 1.0) : d_arg;
+    const double Ge = 0.5 * tot[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
+    const double Pm = tot[1] / gsd;
+    const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
+    bg->d = d;
+    bg->ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
+               (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a +
+                           d * d * sfev1 * pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) / d;
+    bg->ad = ad + 0.5 * d * bg->ad2;
+    bg->friction = (2.0 + s) * bg->ad / a;
+    bg->potnorm = pow(a, 2.0 - 2.0 * s);
+    bg->laplnorm = pow(a, -2.0 * s) / (dx * dx);
+    bg->srcamp = 2.0 * pow(a, -2.0 * s);
+    bg->vpmean = tot[2] / gsd;
+}
+
+__global__ __launch_bounds__(kBlock) void k_friedmann_finish(DevBg *bg, const double *partials, unsigned G,
+                                                              unsigned long long gs, double d_arg, int loop)
+{
+    __shared__ double smem[3 * kWaves];
+    friedmann_finish_body(bg, partials, G, (size_t)gs, d_arg, loop, smem);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_friedmann_finish(const RunDesc *__restrict__ runs)
+{
+    __shared__ double smem[3 * kWaves];
+    const RunDesc &R = runs[blockIdx.x];
+    const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
+    friedmann_finish_body(R.bg, R.partials, R.G, gs, 0.0, 1, smem);
+}
+
+__global__ void k_step_finish(DevBg *bg, int mode, int split, double d_arg, int loop)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (loop && bg->done) return;
+    if (mode == DRIFT && !loop) bg->d = d_arg;
+    finalize_scalars(bg, mode, split);
+}
+
+__global__ void kb_step_finish(const RunDesc *__restrict__ runs, int nruns)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    DevBg *bg = runs[r].bg;
+    if (bg->done) return;
+    finalize_scalars(bg, KICKDRIFT, runs[r].split);
+}
+
+__global__ void kb_loop_begin(const RunDesc *__restrict__ runs, int nruns)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nruns) {
+        DevBg *bg = runs[r].bg;
+        bg->done = !(bg->a <= bg->af) ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------ row kernels
+
+// E consecutive values of one lane, loaded and stored as one vector.
+template <typename T, int E>
+struct alignas(sizeof(T) * E) RowVec {
+    T v[E];
+};
+
+template <typename T, int E>
+__device__ __forceinline__ void ld_row(const T *__restrict__ row, unsigned lane, T (&o)[E])
+{
+    const RowVec<T, E> x = reinterpret_cast<const RowVec<T, E> *>(row)[lane];
+#pragma unroll
+    for (int e = 0; e < E; ++e) o[e] = x.v[e];
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void st_row(T *__restrict__ row, unsigned lane, const T (&v)[E])
+{
+    RowVec<T, E> x;
+#pragma unroll
+    for (int e = 0; e < E; ++e) x.v[e] = v[e];
+    reinterpret_cast<RowVec<T, E> *>(row)[lane] = x;
+}
+
+// Work of one wave: rows j0 .. j0+R-1 of plane i, k-segment [k0, k0 + 64 E).
+struct RowWalk {
+    unsigned i, j0, k0, R, segs;
+};
+
+// G workgroups of 4 waves per lattice; units (i, segment, j-block) in plane-major
+// order, XCD x owning the planes [x N/8, (x+1) N/8).
+__device__ __forceinline__ RowWalk row_walk(unsigned N, unsigned G, unsigned blk, int E)
+{
+    const unsigned segs = N / (64u * (unsigned)E);
+    const unsigned waves = 4u * G;
+    const unsigned R = (N * N * segs) / waves;
+    const unsigned jblocks = N / R;
+    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
+    const unsigned u = xcd * (waves / kXcd) + lb * 4u + (threadIdx.x >> 6);
+    RowWalk w;
+    const unsigned t = u / jblocks;
+    w.j0 = (u % jblocks) * R;
+    w.k0 = (t % segs) * (64u * (unsigned)E);
+    w.i = t / segs;
+    w.R = R;
+    w.segs = segs;
+    return w;
+}
+
+// k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
+// `row`): in-lane, then across lanes, then across segments for N > 64 E.
+template <typename T, int E>
+__device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restrict__ row, const RowWalk &w,
+                                             unsigned N, unsigned lane, T &kp_last, T &km_first)
+{
+    kp_last = __shfl(c[0], (int)((lane + 1) & 63));
+    km_first = __shfl(c[E - 1], (int)((lane + 63) & 63));
+    if (w.segs > 1) {
+        const unsigned m = N - 1;
+        const T lo = row[(w.k0 - 1) & m], hi = row[(w.k0 + 64u * E) & m];
+        if (lane == 0) km_first = lo;
+        if (lane == 63) kp_last = hi;
+    }
+}
+
+template <typename T, bool SPLIT, int E>
+__device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop,
+                                                unsigned G, unsigned blk, double *smem)
+{
+    if (loop && bg->done) return;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
+    const T *__restrict__ f = (const T *)F.f[bg->parity];
+    const double phibar = SPLIT ? bg->phibar : 0.0;
+    const RowWalk w = row_walk(N, G, blk, E);
+    const T *plane = f + (w.i << (2 * logN)) + w.k0;
+    const T *plane_p = f + (((w.i + 1) & m) << (2 * logN)) + w.k0;
+    const T *plane_m = f + (((w.i - 1) & m) << (2 * logN)) + w.k0;
+    T prev[E], cur[E], nxt[E], up[E], dn[E];
+    ld_row<T, E>(plane + (((w.j0 - 1) & m) << logN), lane, prev);
+    ld_row<T, E>(plane + (w.j0 << logN), lane, cur);
+    double acc[3] = {0.0, 0.0, 0.0};
+    int err = 0;
+    for (unsigned r = 0; r < w.R; ++r) {
+        const unsigned j = w.j0 + r;
+        ld_row<T, E>(plane + (((j + 1) & m) << logN), lane, nxt);
+        ld_row<T, E>(plane_p + (j << logN), lane, up);
+        ld_row<T, E>(plane_m + (j << logN), lane, dn);
+        T kp_last, km_first;
+        k_neighbours<T, E>(cur, plane + (j << logN) - w.k0, w, N, lane, kp_last, km_first);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const double c = (double)cur[e];
+            const double kp = (double)(e < E - 1 ? cur[e + 1 < E ? e + 1 : e] : kp_last);
+            const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
+            const double lap = lapl_sites(kp, km, (double)nxt[e], (double)prev[e], (double)up[e], (double)dn[e], c);
+            acc[0] -= c * lap;
+            if (SPLIT) {
+                double V, dV;
+                pot_eval_fast(P, phibar + c, &V, &dV, &err);
+                acc[1] += V;
+                acc[2] += dV;
+            } else {
+                acc[1] += pot_V_fast(P, c, &err);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            prev[e] = cur[e];
+            cur[e] = nxt[e];
+        }
+    }
+    flag_error(bg, err);
+    block_sum<3>(acc, smem);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) partials[(size_t)blk * 3 + q] = acc[q];
+}
+
+template <typename T, bool SPLIT, int E, int MODE>
+__device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, DevBg *bg, int loop, unsigned G,
+                                              unsigned blk)
+{
+    if (loop && bg->done) return;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
+    const int par = bg->parity;
+    const T *__restrict__ f = (const T *)F.f[par];
+    T *__restrict__ fout = (T *)F.f[par ^ 1];
+    T *__restrict__ fd = (T *)F.fd;
+    const double d = bg->d;
+    const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
+    const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
+    const RowWalk w = row_walk(N, G, blk, E);
+    const unsigned poff = (w.i << (2 * logN)) + w.k0;
+    const T *plane = f + poff;
+    const T *plane_p = f + (((w.i + 1) & m) << (2 * logN)) + w.k0;
+    const T *plane_m = f + (((w.i - 1) & m) << (2 * logN)) + w.k0;
+    T prev[E], cur[E], nxt[E], up[E], dn[E], vin[E];
+    ld_row<T, E>(plane + (((w.j0 - 1) & m) << logN), lane, prev);
+    ld_row<T, E>(plane + (w.j0 << logN), lane, cur);
+    int err = 0;
+    for (unsigned r = 0; r < w.R; ++r) {
+        const unsigned j = w.j0 + r;
+        const unsigned roff = poff + (j << logN);
+        ld_row<T, E>(plane + (((j + 1) & m) << logN), lane, nxt);
+        ld_row<T, E>(plane_p + (j << logN), lane, up);
+        ld_row<T, E>(plane_m + (j << logN), lane, dn);
+        ld_row<T, E>(fd + roff, lane, vin);
+        T kp_last, km_first;
+        k_neighbours<T, E>(cur, f + roff - w.k0, w, N, lane, kp_last, km_first);
+        T vout[E], fnew[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const double c = (double)cur[e];
+            const double kp = (double)(e < E - 1 ? cur[e + 1 < E ? e + 1 : e] : kp_last);
+            const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
+            const double lap = lapl_sites(kp, km, (double)nxt[e], (double)prev[e], (double)up[e], (double)dn[e], c);
+            double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+            if (SPLIT) dV -= vpmean;
+            double v = (double)vin[e];
+            v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
+            vout[e] = (T)v;
+            fnew[e] = (T)(c + d * (double)vout[e]);
+        }
+        st_row<T, E>(fd + roff, lane, vout);
+        if (MODE & DRIFT) st_row<T, E>(fout + roff, lane, fnew);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            prev[e] = cur[e];
+            cur[e] = nxt[e];
+        }
+    }
+    flag_error(bg, err);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kBlock) void k_row_reduce(Fields F, Pot P, DevBg *bg, double *partials, int loop)
+{
+    __shared__ double smem[3 * kWaves];
+    row_reduce_body<T, SPLIT, E>(F, P, bg, partials, loop, gridDim.x, blockIdx.x, smem);
+}
+
+template <typename T, bool SPLIT, int E, int MODE>
+__global__ __launch_bounds__(kBlock) void k_row_kick(Fields F, Pot P, DevBg *bg, int loop)
+{
+    row_kick_body<T, SPLIT, E, MODE>(F, P, bg, loop, gridDim.x, blockIdx.x);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kBlock) void kb_row_reduce(const RunDesc *__restrict__ runs,
+                                                         const uint2 *__restrict__ map)
+{
+    __shared__ double smem[3 * kWaves];
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    row_reduce_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, 1, R.G, mp.y, smem);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kBlock) void kb_row_kick(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
+{
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    row_kick_body<T, SPLIT, E, KICKDRIFT>(R.F, R.P, R.bg, 1, R.G, mp.y);
+}
+
+// ------------------------------------------------- small-lattice site kernels
 
 // The seven stencil values of U sites of a workgroup's walk, loaded before any
 // arithmetic so U x 7 loads are in flight per thread.  Sites past the end of the
@@ -207,41 +492,35 @@ __device__ __forceinline__ void load_stencil(const T *__restrict__ f, const Walk
     }
 }
 
-// 7-point Laplacian of site u in FP64, summed in the reference's order
-// (src/evolution.cpp:34-51): k+, k-, j+, j-, i+, i-, then -6 f.
 template <typename T, int U>
 __device__ __forceinline__ double stencil_lapl(const Stencil<T, U> &s, int u)
 {
-    return (double)s.kp[u] + (double)s.km[u] + (double)s.jp[u] + (double)s.jm[u] + (double)s.ip[u] +
-           (double)s.im[u] - 6.0 * (double)s.c[u];
+    return lapl_sites((double)s.kp[u], (double)s.km[u], (double)s.jp[u], (double)s.jm[u], (double)s.ip[u],
+                      (double)s.im[u], (double)s.c[u]);
 }
 
-template <typename T>
-struct Unroll {
-    static constexpr int value = sizeof(T) == 4 ? 8 : 4;
-};
+constexpr int kUnroll = 4;
 
 template <typename T, bool SPLIT>
-__device__ __forceinline__ void friedmann_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials,
-                                                      double d_arg, int loop, unsigned G, unsigned blk, double *smem)
+__device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop,
+                                                 unsigned G, unsigned blk, double *smem)
 {
-    constexpr int U = Unroll<T>::value;
     if (loop && bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
-    const size_t gs = (size_t)F.N * F.N * F.N;
+    const unsigned gs = N * N * N;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
     const double phibar = SPLIT ? bg->phibar : 0.0;
     double acc[3] = {0.0, 0.0, 0.0};
     int err = 0;
-    const Walk32 w = site_walk32((unsigned)gs, G, blk);
-    for (unsigned base = w.begin; base < w.end; base += (unsigned)U * w.stride) {
-        Stencil<T, U> s;
-        load_stencil<T, U>(f, w, base, N, logN, s);
+    const Walk32 w = site_walk32(gs, G, blk);
+    for (unsigned base = w.begin; base < w.end; base += (unsigned)kUnroll * w.stride) {
+        Stencil<T, kUnroll> s;
+        load_stencil<T, kUnroll>(f, w, base, N, logN, s);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
+        for (int u = 0; u < kUnroll; ++u) {
             if (!s.ok[u]) continue;
             const double c = (double)s.c[u];
-            const double lap = stencil_lapl<T, U>(s, u);
+            const double lap = stencil_lapl<T, kUnroll>(s, u);
             acc[0] -= c * lap;
             if (SPLIT) {
                 double V, dV;
@@ -257,78 +536,36 @@ __device__ __forceinline__ void friedmann_reduce_body(const Fields &F, const Pot
     block_sum<3>(acc, smem);
     if (threadIdx.x == 0)
         for (int q = 0; q < 3; ++q) partials[(size_t)blk * 3 + q] = acc[q];
-    if (!last_block(&bg->counter, G)) return;
-    double tot[3];
-    sum_partials<3>(partials, G, tot, smem);
-    if (threadIdx.x != 0) return;
-    bg->red[0] = tot[0];
-    bg->red[1] = tot[1];
-    bg->red[2] = tot[2];
-    const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
-    const double gsd = (double)gs;
-    const double d = loop ? bg->dt * pow(bg->astep, s - 1.0) : d_arg;
-    const double Ge = 0.5 * tot[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
-    const double Pm = tot[1] / gsd;
-    const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
-    bg->d = d;
-    bg->ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
-               (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a +
-                           d * d * sfev1 * pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) / d;
-    bg->ad = ad + 0.5 * d * bg->ad2;
-    bg->friction = (2.0 + s) * bg->ad / a;
-    bg->potnorm = pow(a, 2.0 - 2.0 * s);
-    bg->laplnorm = pow(a, -2.0 * s) / (dx * dx);
-    bg->srcamp = 2.0 * pow(a, -2.0 * s);
-    bg->vpmean = tot[2] / gsd;
-}
-
-template <typename T, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void k_friedmann_reduce(Fields F, Pot P, DevBg *bg,
-                                                              double *partials, double d_arg, int loop)
-{
-    __shared__ double smem[3 * kWaves];
-    friedmann_reduce_body<T, SPLIT>(F, P, bg, partials, d_arg, loop, gridDim.x, blockIdx.x, smem);
-}
-
-template <typename T, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void kb_friedmann_reduce(const RunDesc *__restrict__ runs,
-                                                               const uint2 *__restrict__ map)
-{
-    __shared__ double smem[3 * kWaves];
-    const uint2 m = map[blockIdx.x];
-    const RunDesc &R = runs[m.x];
-    friedmann_reduce_body<T, SPLIT>(R.F, R.P, R.bg, R.partials, 0.0, 1, R.G, m.y, smem);
 }
 
 template <typename T, bool SPLIT, int MODE>
-__device__ __forceinline__ void kick_drift_body(const Fields &F, const Pot &P, DevBg *bg, double d_arg, int loop,
-                                                int finalize, unsigned G, unsigned blk)
+__device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P, DevBg *bg, double d_arg, int loop,
+                                               unsigned G, unsigned blk)
 {
     if (loop && bg->done) return;
-    const size_t gs = (size_t)F.N * F.N * F.N;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
+    const unsigned gs = N * N * N;
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
-    T *__restrict__ fout = (T *)F.f[MODE == KICKDRIFT ? par ^ 1 : par];
+    T *__restrict__ fout = (T *)F.f[par ^ 1];
     T *__restrict__ fd = (T *)F.fd;
-    // explicit drift takes d from the caller; kicks use the step set by the reduce
     const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
     const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
     const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
     int err = 0;
     if (MODE & KICK) {
-        constexpr int U = Unroll<T>::value;
-        const Walk32 w = site_walk32((unsigned)gs, G, blk);
-        for (unsigned base = w.begin; base < w.end; base += (unsigned)U * w.stride) {
-            Stencil<T, U> s;
-            load_stencil<T, U>(f, w, base, (unsigned)F.N, (unsigned)F.logN, s);
-            T vin[U];
+        const Walk32 w = site_walk32(gs, G, blk);
+        for (unsigned base = w.begin; base < w.end; base += (unsigned)kUnroll * w.stride) {
+            Stencil<T, kUnroll> s;
+            load_stencil<T, kUnroll>(f, w, base, N, logN, s);
+            T vin[kUnroll];
 #pragma unroll
-            for (int u = 0; u < U; ++u) vin[u] = fd[s.id[u]];
+            for (int u = 0; u < kUnroll; ++u) vin[u] = fd[s.id[u]];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
+            for (int u = 0; u < kUnroll; ++u) {
                 if (!s.ok[u]) continue;
                 const double c = (double)s.c[u];
-                const double lap = stencil_lapl<T, U>(s, u);
+                const double lap = stencil_lapl<T, kUnroll>(s, u);
                 double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
                 if (SPLIT) dV -= vpmean;
                 double v = (double)vin[u];
@@ -341,38 +578,49 @@ __device__ __forceinline__ void kick_drift_body(const Fields &F, const Pot &P, D
     } else {
         // drift alone, in place (pointwise): through one unrestricted pointer
         T *g = (T *)F.f[par];
-        const Walk w = site_walk(gs, G, blk);
+        const Walk w = site_walk((size_t)gs, G, blk);
         for (size_t id = w.begin; id < w.end; id += w.stride) g[id] = (T)((double)g[id] + d * (double)fd[id]);
     }
     flag_error(bg, err);
-    if (!finalize) return;
-    if (!last_block(&bg->counter, G)) return;
-    if (threadIdx.x == 0) {
-        if (MODE == DRIFT) bg->d = d;
-        finalize_scalars(bg, MODE, SPLIT);
-    }
+}
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_site_reduce(Fields F, Pot P, DevBg *bg, double *partials, int loop)
+{
+    __shared__ double smem[3 * kWaves];
+    site_reduce_body<T, SPLIT>(F, P, bg, partials, loop, gridDim.x, blockIdx.x, smem);
 }
 
 template <typename T, bool SPLIT, int MODE>
-__global__ __launch_bounds__(kBlock) void k_kick_drift(Fields F, Pot P, DevBg *bg, double d_arg,
-                                                        int loop, int finalize)
+__global__ __launch_bounds__(kBlock) void k_site_kick(Fields F, Pot P, DevBg *bg, double d_arg, int loop)
 {
-    kick_drift_body<T, SPLIT, MODE>(F, P, bg, d_arg, loop, finalize, gridDim.x, blockIdx.x);
+    site_kick_body<T, SPLIT, MODE>(F, P, bg, d_arg, loop, gridDim.x, blockIdx.x);
 }
 
-// batched loop step: the scalar kernel closes the step of runs without tensors
 template <typename T, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void kb_kick_drift(const RunDesc *__restrict__ runs,
-                                                         const uint2 *__restrict__ map)
+__global__ __launch_bounds__(kBlock) void kb_site_reduce(const RunDesc *__restrict__ runs,
+                                                          const uint2 *__restrict__ map)
 {
-    const uint2 m = map[blockIdx.x];
-    const RunDesc &R = runs[m.x];
-    kick_drift_body<T, SPLIT, KICKDRIFT>(R.F, R.P, R.bg, 0.0, 1, R.F.sigw ? 0 : 1, R.G, m.y);
+    __shared__ double smem[3 * kWaves];
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    site_reduce_body<T, SPLIT>(R.F, R.P, R.bg, R.partials, 1, R.G, mp.y, smem);
 }
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void kb_site_kick(const RunDesc *__restrict__ runs,
+                                                        const uint2 *__restrict__ map)
+{
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    site_kick_body<T, SPLIT, KICKDRIFT>(R.F, R.P, R.bg, 0.0, 1, R.G, mp.y);
+}
+
+// ------------------------------------------------------------ tensor kernel
 
 template <typename T, int MODE>
-__device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d_arg, int loop, int split,
-                                            unsigned G, unsigned blk)
+__device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d_arg, int loop, unsigned G,
+                                            unsigned blk)
 {
     if (loop && bg->done) return;
     const size_t gs = (size_t)F.N * F.N * F.N;
@@ -416,35 +664,23 @@ __device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d
             }
         }
     }
-    if (!last_block(&bg->counter, G)) return;
-    if (threadIdx.x == 0) {
-        if (MODE == DRIFT) bg->d = d;
-        finalize_scalars(bg, MODE, split);
-    }
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop, int split)
+__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop)
 {
-    tensor_body<T, MODE>(F, bg, d_arg, loop, split, gridDim.x, blockIdx.x);
+    tensor_body<T, MODE>(F, bg, d_arg, loop, gridDim.x, blockIdx.x);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kb_tensor(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
 {
-    const uint2 m = map[blockIdx.x];
-    const RunDesc &R = runs[m.x];
-    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.split, R.G, m.y);
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.G, mp.y);
 }
 
-__global__ void kb_loop_begin(const RunDesc *__restrict__ runs, int nruns)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nruns) {
-        DevBg *bg = runs[r].bg;
-        bg->done = !(bg->a <= bg->af) ? 1 : 0;
-    }
-}
+// ------------------------------------------------------- output utilities
 
 // Volume sums for outputs and for the split upload.
 //   red[0] sum f (split: sum dphi), red[1] sum fd (split: sum dpi),
@@ -531,6 +767,53 @@ __global__ __launch_bounds__(kBlock) void k_reconstitute(Fields F, const DevBg *
     for (size_t id = w.begin; id < w.end; id += w.stride) out[id] = SPLIT ? base + (double)src[id] : (double)src[id];
 }
 
+// Elements per lane of the row kernels, 0 for the site kernels (N < 64).
+int row_elems(int N) { return N < 64 ? 0 : (N >= 256 ? 4 : N / 64); }
+
+template <typename T, bool SPLIT>
+void reduce_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop, unsigned G, hipStream_t s)
+{
+    switch (row_elems(F.N)) {
+    case 1: k_row_reduce<T, SPLIT, 1><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    case 2: k_row_reduce<T, SPLIT, 2><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    case 4: k_row_reduce<T, SPLIT, 4><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    default: k_site_reduce<T, SPLIT><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    }
+}
+
+template <typename T, bool SPLIT, int MODE>
+void row_kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int loop, unsigned G, hipStream_t s)
+{
+    switch (row_elems(F.N)) {
+    case 1: k_row_kick<T, SPLIT, 1, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
+    case 2: k_row_kick<T, SPLIT, 2, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
+    default: k_row_kick<T, SPLIT, 4, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
+    }
+}
+
+template <typename T, bool SPLIT>
+void kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_arg, int loop, unsigned G,
+                   hipStream_t s)
+{
+    if (mode == DRIFT) {
+        k_site_kick<T, SPLIT, DRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
+    } else if (row_elems(F.N) == 0) {
+        if (mode == KICK) k_site_kick<T, SPLIT, KICK><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
+        else k_site_kick<T, SPLIT, KICKDRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
+    } else {
+        if (mode == KICK) row_kick_dispatch<T, SPLIT, KICK>(F, P, bg, loop, G, s);
+        else row_kick_dispatch<T, SPLIT, KICKDRIFT>(F, P, bg, loop, G, s);
+    }
+}
+
+template <typename T>
+void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loop, unsigned G, hipStream_t s)
+{
+    if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+    else if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+    else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+}
+
 }  // namespace
 
 int reduce_blocks(size_t gs)
@@ -541,49 +824,77 @@ int reduce_blocks(size_t gs)
     return (int)(b < 1 ? 1 : b);
 }
 
-hipError_t launch_friedmann_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials,
-                                   double d_arg, int loop, hipStream_t s)
+unsigned step_blocks(int N)
 {
-    const size_t gs = (size_t)F.N * F.N * F.N;
-    const int G = reduce_blocks(gs);
-    if (split) k_friedmann_reduce<float, true><<<G, kBlock, 0, s>>>(F, P, bg, partials, d_arg, loop);
-    else k_friedmann_reduce<double, false><<<G, kBlock, 0, s>>>(F, P, bg, partials, d_arg, loop);
+    const int E = row_elems(N);
+    if (E == 0) {
+        size_t b = (size_t)N * N * N / kBlock;
+        b = (b + kXcd - 1) / kXcd * kXcd;
+        return (unsigned)(b < kXcd ? kXcd : (b > 2048 ? 2048 : b));
+    }
+    // row kernels: N^2 * segments / 32 workgroups (8 rows per wave), at most 2048
+    const size_t segs = (size_t)N / (64 * (size_t)E);
+    const size_t b = (size_t)N * N * segs / 32;
+    return (unsigned)(b > 2048 ? 2048 : b);
+}
+
+hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, int loop,
+                         hipStream_t s)
+{
+    const unsigned G = step_blocks(F.N);
+    if (split) reduce_dispatch<float, true>(F, P, bg, partials, loop, G, s);
+    else reduce_dispatch<double, false>(F, P, bg, partials, loop, G, s);
     return hipGetLastError();
 }
 
-template <typename T, bool SPLIT>
-static void kd_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_arg, int loop, int fin,
-                        int G, hipStream_t s)
+hipError_t launch_friedmann_finish(const Fields &F, DevBg *bg, const double *partials, double d_arg, int loop,
+                                   hipStream_t s)
 {
-    if (mode == KICK) k_kick_drift<T, SPLIT, KICK><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop, fin);
-    else if (mode == DRIFT) k_kick_drift<T, SPLIT, DRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop, fin);
-    else k_kick_drift<T, SPLIT, KICKDRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop, fin);
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    k_friedmann_finish<<<1, kBlock, 0, s>>>(bg, partials, step_blocks(F.N), (unsigned long long)gs, d_arg, loop);
+    return hipGetLastError();
 }
 
-template <typename T>
-static void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loop, int split, int G,
-                            hipStream_t s)
+hipError_t launch_scalar_step(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg, int loop,
+                              hipStream_t s)
 {
-    if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, split);
-    else if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, split);
-    else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, split);
+    const unsigned G = step_blocks(F.N);
+    if (split) kick_dispatch<float, true>(F, P, bg, mode, d_arg, loop, G, s);
+    else kick_dispatch<double, false>(F, P, bg, mode, d_arg, loop, G, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s)
+{
+    const unsigned G = step_blocks(F.N);
+    if (split) tensor_dispatch<float>(F, bg, mode, d_arg, loop, G, s);
+    else tensor_dispatch<double>(F, bg, mode, d_arg, loop, G, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_finish(DevBg *bg, int split, int mode, double d_arg, int loop, hipStream_t s)
+{
+    k_step_finish<<<1, 64, 0, s>>>(bg, mode, split, d_arg, loop);
+    return hipGetLastError();
+}
+
+hipError_t launch_friedmann_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials,
+                                   double d_arg, int loop, hipStream_t s)
+{
+    hipError_t e = launch_reduce(F, split, P, bg, partials, loop, s);
+    if (e != hipSuccess) return e;
+    return launch_friedmann_finish(F, bg, partials, d_arg, loop, s);
 }
 
 hipError_t launch_kick_drift(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg,
                              int loop, hipStream_t s, int part)
 {
-    const size_t gs = (size_t)F.N * F.N * F.N;
-    const int G = reduce_blocks(gs);
-    const int fin_scalar = F.sigw ? 0 : 1;
-    if (part != 1) {
-        if (split) kd_dispatch<float, true>(F, P, bg, mode, d_arg, loop, fin_scalar, G, s);
-        else kd_dispatch<double, false>(F, P, bg, mode, d_arg, loop, fin_scalar, G, s);
-    }
-    if (F.sigw && part != 0) {
-        if (split) tensor_dispatch<float>(F, bg, mode, d_arg, loop, split, G, s);
-        else tensor_dispatch<double>(F, bg, mode, d_arg, loop, split, G, s);
-    }
-    return hipGetLastError();
+    hipError_t e = hipSuccess;
+    if (part != 1) e = launch_scalar_step(F, split, P, bg, mode, d_arg, loop, s);
+    if (e == hipSuccess && F.sigw && part != 0) e = launch_tensor_step(F, split, bg, mode, d_arg, loop, s);
+    if (e == hipSuccess && (part == -1 || (part == 0 && !F.sigw) || part == 1))
+        e = launch_step_finish(bg, split, mode, d_arg, loop, s);
+    return e;
 }
 
 hipError_t launch_field_sums(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
@@ -616,52 +927,61 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
 
 // ------------------------------------------------------------------ batches
 
-struct BatchDev {
-    RunDesc *runs = nullptr;
+// Runs of one storage kind, grouped by kernel shape (row elements per lane); one
+// launch per group and kernel, one finish launch over all runs.
+struct BatchGroup {
+    int E = 0;
     uint2 *map = nullptr, *tmap = nullptr;
-    int nruns = 0, nblocks = 0, ntblocks = 0, split = 0;
+    int nblocks = 0, ntblocks = 0;
 };
 
-// Workgroups per lattice in a batched launch: a multiple of 8 (whole XCD slabs),
-// at most 256 so the last workgroup's fixed-order sum stays short.
-static unsigned batch_blocks(size_t gs)
-{
-    size_t b = gs / kBlock;
-    if (b > 256) b = 256;
-    b = (b + kXcd - 1) / kXcd * kXcd;
-    return (unsigned)(b < kXcd ? kXcd : b);
-}
+struct BatchDev {
+    RunDesc *runs = nullptr;
+    std::vector<BatchGroup> groups;
+    int nruns = 0, split = 0;
+};
 
 hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
 {
     *out = nullptr;
     std::vector<RunDesc> d(n);
-    std::vector<uint2> map, tmap;
+    BatchDev *B = new BatchDev();
+    B->nruns = n;
+    B->split = split;
+    const int shapes[4] = {0, 1, 2, 4};
+    std::vector<std::vector<uint2>> maps(4), tmaps(4);
     for (int r = 0; r < n; ++r) {
-        const size_t gs = (size_t)runs[r].F.N * runs[r].F.N * runs[r].F.N;
         d[r].F = runs[r].F;
         d[r].P = runs[r].P;
         d[r].bg = runs[r].bg;
         d[r].partials = runs[r].partials;
-        d[r].G = batch_blocks(gs);
+        d[r].G = step_blocks(runs[r].F.N);
         d[r].split = split;
+        const int E = row_elems(runs[r].F.N);
+        const int g = E == 0 ? 0 : (E == 1 ? 1 : (E == 2 ? 2 : 3));
         for (unsigned b = 0; b < d[r].G; ++b) {
-            map.push_back(make_uint2((unsigned)r, b));
-            if (runs[r].F.sigw) tmap.push_back(make_uint2((unsigned)r, b));
+            maps[g].push_back(make_uint2((unsigned)r, b));
+            if (runs[r].F.sigw) tmaps[g].push_back(make_uint2((unsigned)r, b));
         }
     }
-    BatchDev *B = new BatchDev();
-    B->nruns = n;
-    B->nblocks = (int)map.size();
-    B->ntblocks = (int)tmap.size();
-    B->split = split;
     hipError_t e = hipMalloc(&B->runs, sizeof(RunDesc) * n);
-    if (e == hipSuccess) e = hipMalloc(&B->map, sizeof(uint2) * map.size());
-    if (e == hipSuccess && !tmap.empty()) e = hipMalloc(&B->tmap, sizeof(uint2) * tmap.size());
     if (e == hipSuccess) e = hipMemcpy(B->runs, d.data(), sizeof(RunDesc) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(B->map, map.data(), sizeof(uint2) * map.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !tmap.empty())
-        e = hipMemcpy(B->tmap, tmap.data(), sizeof(uint2) * tmap.size(), hipMemcpyHostToDevice);
+    for (int g = 0; g < 4 && e == hipSuccess; ++g) {
+        if (maps[g].empty()) continue;
+        BatchGroup G;
+        G.E = shapes[g];
+        G.nblocks = (int)maps[g].size();
+        G.ntblocks = (int)tmaps[g].size();
+        e = hipMalloc(&G.map, sizeof(uint2) * maps[g].size());
+        if (e == hipSuccess)
+            e = hipMemcpy(G.map, maps[g].data(), sizeof(uint2) * maps[g].size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && G.ntblocks) {
+            e = hipMalloc(&G.tmap, sizeof(uint2) * tmaps[g].size());
+            if (e == hipSuccess)
+                e = hipMemcpy(G.tmap, tmaps[g].data(), sizeof(uint2) * tmaps[g].size(), hipMemcpyHostToDevice);
+        }
+        B->groups.push_back(G);
+    }
     if (e != hipSuccess) {
         batch_free(B);
         return e;
@@ -674,8 +994,10 @@ void batch_free(BatchDev *B)
 {
     if (!B) return;
     if (B->runs) (void)hipFree(B->runs);
-    if (B->map) (void)hipFree(B->map);
-    if (B->tmap) (void)hipFree(B->tmap);
+    for (BatchGroup &G : B->groups) {
+        if (G.map) (void)hipFree(G.map);
+        if (G.tmap) (void)hipFree(G.tmap);
+    }
     delete B;
 }
 
@@ -685,17 +1007,41 @@ hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename T, bool SPLIT>
+static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream_t s)
+{
+    switch (G.E) {
+    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    default: kb_site_reduce<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    }
+}
+
+template <typename T, bool SPLIT>
+static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t s)
+{
+    switch (G.E) {
+    case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    }
+    if (G.ntblocks) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+}
+
 hipError_t batch_step(const BatchDev *B, hipStream_t s)
 {
-    if (B->split) {
-        kb_friedmann_reduce<float, true><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
-        kb_kick_drift<float, true><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
-        if (B->ntblocks) kb_tensor<float><<<B->ntblocks, kBlock, 0, s>>>(B->runs, B->tmap);
-    } else {
-        kb_friedmann_reduce<double, false><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
-        kb_kick_drift<double, false><<<B->nblocks, kBlock, 0, s>>>(B->runs, B->map);
-        if (B->ntblocks) kb_tensor<double><<<B->ntblocks, kBlock, 0, s>>>(B->runs, B->tmap);
+    for (const BatchGroup &G : B->groups) {
+        if (B->split) batch_group_reduce<float, true>(B, G, s);
+        else batch_group_reduce<double, false>(B, G, s);
     }
+    kb_friedmann_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
+    for (const BatchGroup &G : B->groups) {
+        if (B->split) batch_group_kick<float, true>(B, G, s);
+        else batch_group_kick<double, false>(B, G, s);
+    }
+    kb_step_finish<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
     return hipGetLastError();
 }
 

@@ -25,8 +25,10 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def oracle_state(case: str, **over):
+def oracle_state(case: str, N: int | None = None, **over):
     cfg, params, table = golden_config(case)
+    if N is not None and N != cfg.N:
+        cfg = O.make_config(N, cfg.numerical_potential, params, table)
     for k, v in over.items():
         setattr(cfg, k, v)
     st = O.State(cfg, table)
@@ -68,9 +70,12 @@ def test_energies_match_oracle(case):
     assert s[2] == pytest.approx((f * f).sum(), rel=1e-13)
 
 
-@pytest.mark.parametrize("case,nsteps", [("hilltop_n32", 200), ("tabulated_n16", 200)])
-def test_leapfrog_fp64_matches_oracle(case, nsteps):
-    st, _, table = oracle_state(case, calculate_sigw=0)
+# N >= 64 runs the row kernels (E = 1, 2 sites per lane), N < 64 the site kernels
+@pytest.mark.parametrize("case,N,nsteps", [("hilltop_n32", 32, 200), ("tabulated_n16", 16, 200),
+                                           ("hilltop_n32", 64, 100), ("tabulated_n16", 128, 20)])
+def test_leapfrog_fp64_matches_oracle(case, N, nsteps):
+    print(f"leapfrog parity {case} N={N} steps={nsteps}", flush=True)
+    st, _, table = oracle_state(case, N=N, calculate_sigw=0)
     run = engine_for(st, table, sigw=0)
     dt = st.cfg.dt
     O.lib().ieo_evolve_fields(st.ptr, 0.5 * dt)
@@ -167,8 +172,9 @@ def test_upload_modes_matches_oracle_initial_conditions():
     assert fluct_err(st2.field(1), fd) < 1e-10
 
 
-def test_split_storage_matches_fp64_spectra():
-    st, _, table = oracle_state("hilltop_n32", calculate_sigw=0)
+@pytest.mark.parametrize("N", [32, 64])
+def test_split_storage_matches_fp64_spectra(N):
+    st, _, table = oracle_state("hilltop_n32", N=N, calculate_sigw=0)
     r64 = engine_for(st, table, precision=E.IE_FP64, sigw=0)
     r32 = engine_for(st, table, precision=E.IE_SPLIT, sigw=0)
     dt = st.cfg.dt
@@ -177,8 +183,44 @@ def test_split_storage_matches_fp64_spectra():
         r.leapfrog_steps(200)
     b64, b32 = r64.background(), r32.background()
     assert b32["a"] == pytest.approx(b64["a"], rel=1e-12)
-    norm1 = (st.cfg.L / st.cfg.rescale_B) ** 3 / 32 ** 6
+    norm1 = (st.cfg.L / st.cfg.rescale_B) ** 3 / N ** 6
     _, _, v64 = r64.spectrum(E.IE_FIELD_F, norm1)
     _, _, v32 = r32.spectrum(E.IE_FIELD_F, norm1)
     assert np.allclose(v32[1:], v64[1:], rtol=1e-4)
     assert v32[0] == pytest.approx(v64[0], rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", [E.IE_FP64, E.IE_SPLIT])
+def test_segmented_rows_match_oracle_n512(precision):
+    """N = 512: each row is two 256-site segments per wave; the k +- 1 neighbours
+    across segment edges come from the row itself.  One explicit kick and drift of
+    seeded random fields against the oracle (FP64: 1e-10 of the fluctuation rms;
+    split storage: FP32 rounding of the stored fluctuation, 1e-6)."""
+    N = 512
+    print(f"segmented rows N={N} precision={precision}", flush=True)
+    cfg, params, _ = golden_config("hilltop_n32")
+    cfg = O.make_config(N, 0, params, None)
+    cfg.calculate_sigw = 0
+    st = O.State(cfg, None)
+    rng = np.random.default_rng(512)
+    f0 = cfg.initial_field + 1e-5 * rng.standard_normal(N ** 3)
+    fd0 = cfg.initial_derivative + 1e-5 * rng.standard_normal(N ** 3)
+    st.field(0)[:] = f0
+    st.field(1)[:] = fd0
+    st.set_background(t=0.0, a=1.0, ad=0.01, astep=1.0)
+    run = E.Run(E.make_config(N, precision=precision, sigw=0, V0=cfg.V0, ns=cfg.ns, L=cfg.L, dt=cfg.dt,
+                              af=cfg.af, rescale_s=cfg.rescale_s, rescale_B=cfg.rescale_B), None)
+    run.upload_fields(f0, fd0)
+    run.set_background(t=0.0, a=1.0, ad=0.01, astep=1.0)
+    d = cfg.dt
+    O.lib().ieo_evolve_derivs(st.ptr, d)
+    O.lib().ieo_evolve_fields(st.ptr, d)
+    run.evolve_derivs(d)
+    run.evolve_fields(d)
+    bo, b = st.background(), run.background()
+    for key in ("a", "ad", "t"):
+        assert b[key] == pytest.approx(bo[key], rel=1e-12), key
+    tol = 1e-10 if precision == E.IE_FP64 else 1e-6
+    assert fluct_err(st.field(0), run.field(E.IE_FIELD_F)) < tol
+    assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < tol
+    run.close()
