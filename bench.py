@@ -55,12 +55,15 @@ def time_steps(run, steps: int, warmup: int) -> tuple[float, dict]:
 
 def bench_case(N: int, precision: int, sigw: int, steps: int, warmup: int, profile_steps: int):
     run = make_run(N, precision, sigw)
+    # kernel shapes of the kick / tensor launches picked by the engine's tuner on
+    # real loop steps, before the untimed warmup
+    tuning = run.tune(8)
     secs, bg = time_steps(run, steps, warmup)
-    if bg["steps"] != steps + warmup + 0 or bg["done"]:
+    if bg["steps"] != steps + warmup + tuning["steps"] or bg["done"]:
         raise RuntimeError(f"unexpected loop state {bg}")
     prof = run.profile_steps(profile_steps) if profile_steps else None
     run.close()
-    return secs, prof
+    return secs, prof, tuning
 
 
 def read_traffic(name: str):
@@ -118,7 +121,7 @@ def cpu_baseline(N: int = 256, target_steps_af: float = 1.25) -> dict | None:
 
 
 def serve_stream(nreq: int, rate: float, seed: int = 20261203) -> dict:
-    """Serving stream (BASELINE config 5): Poisson arrivals, N in {32, 64, 128,
+    """Serving stream: Poisson arrivals, N in {32, 64, 128,
     256} with weights 0.4/0.3/0.2/0.1, half hilltop / half tabulated USR, 20% with
     SIGW, 5% with an FP64 twin; each request is a whole run (ICs, inflation to
     af = 2N, deltaN map, P_zeta)."""
@@ -172,7 +175,7 @@ def serve_stream(nreq: int, rate: float, seed: int = 20261203) -> dict:
 
 
 def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203) -> dict:
-    """Batched ensemble (BASELINE config 4): nruns hilltop lattices of N^3, seeds
+    """Batched ensemble: nruns hilltop lattices of N^3, seeds
     1000 + r, V0 log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split
     storage.  The same number of steps is timed once as one batch (one launch per
     kernel per step, HIP graph replay) and once as sequential per-run launches."""
@@ -247,8 +250,8 @@ def main() -> int:
         copy_calibration()
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
-              "(ensembles shard by run, see DESIGN.md).", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; runs on several GPUs are not supported "
+              "(ensembles and serving would shard by run, with no communication).", file=sys.stderr)
         return 2
     import ie_engine as E
     if E.lib().ie_device_count() < 1:
@@ -261,10 +264,11 @@ def main() -> int:
     res = {}
     for name in todo:
         prec, sigw = cases[name]
-        secs, prof = bench_case(N, prec, sigw, K, W, args.profile_steps)
-        res[name] = {"secs": secs, "sups": sites * K / secs, "ms_per_step": 1e3 * secs / K, "profile": prof}
+        secs, prof, tuning = bench_case(N, prec, sigw, K, W, args.profile_steps)
+        res[name] = {"secs": secs, "sups": sites * K / secs, "ms_per_step": 1e3 * secs / K, "profile": prof,
+                     "tuning": tuning}
         print(f"# {name}: {res[name]['sups']:.4e} site-updates/s, {res[name]['ms_per_step']:.4f} ms/step, "
-              f"profile {prof}", file=sys.stderr, flush=True)
+              f"profile {prof}, tuning {tuning}", file=sys.stderr, flush=True)
     head = res.get("split") or next(iter(res.values()))
     out = {
         "metric": METRIC,
@@ -291,6 +295,9 @@ def main() -> int:
     if "sigw" in res:
         out["sigw_split_value"] = res["sigw"]["sups"]
         out["sigw_ms_per_step"] = res["sigw"]["ms_per_step"]
+        if res["sigw"]["profile"]:
+            out["sigw_tensor_ms"] = res["sigw"]["profile"]["tensor_ms"]
+    out["autotune"] = {k: v["tuning"] for k, v in res.items()}
     prof = head["profile"]
     if prof:
         # dominant kernel: fused kick + drift; algorithmic bytes per site: read dphi and

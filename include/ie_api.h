@@ -154,6 +154,25 @@ typedef struct ie_profile {
     int64_t steps;
 } ie_profile;
 ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
+
+/* Shape autotuner of the step kernels (no counterpart in the reference: the
+ * reference has no kernel shapes).  The kick and tensor kernels of N >= 128
+ * lattices can hold 1, 2 or 4 sites per lane of a row; their results do not depend
+ * on it (no cross-site sums), so the tuner times real loop steps: for each
+ * candidate, steps_per_candidate steps of the device-resident loop with events
+ * around the two launches, then keeps the fastest of each.  The choice is stored
+ * process-wide per (N, storage) and taken by later runs and batches of that kind;
+ * ie_leapfrog_steps tunes once per kind by itself on a call of >= 256 steps.  The
+ * reduce kernel keeps one shape, since it fixes the summation order. */
+typedef struct {
+    int ncand;                 /* candidates timed */
+    int elems[3];              /* sites per lane of each candidate */
+    double kick_ms[3];         /* mean kick launch time per candidate */
+    double tensor_ms[3];       /* mean tensor launch time (0 without SIGW) */
+    int kick_elems, tensor_elems;   /* chosen */
+    int64_t steps;             /* loop steps taken while tuning */
+} ie_tuning;
+ie_status ie_tune(ie_run *run, int steps_per_candidate, ie_tuning *out);
 /* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904), FP64
  * storage only (IE_ERR_STATE for split storage).  RK45 returns *accepted and the
  * proposed next step clamped to [rk45_min_dt, hmax]; a rejected step leaves the
@@ -187,6 +206,12 @@ ie_status ie_deltaN_steps(ie_run *run, double dN, int64_t nsteps);
 /* nsteps classical RK4 steps of the deltaN system (rk4_step_deltaN,
  * src/evolution.cpp:1176-1189), also kept in registers per site. */
 ie_status ie_deltaN_rk4_steps(ie_run *run, double h, int64_t nsteps);
+/* One Dormand-Prince trial step of the deltaN system (rk45_step_deltaN,
+ * src/evolution.cpp:1191-1267): the weighted RMS error over (phi, pi, N) of every
+ * site and a; *accepted and the proposed next step as for ie_rk45_step.  The
+ * stages stay in registers; three N^3 FP64 buffers for the trial state are
+ * allocated on the first call. */
+ie_status ie_deltaN_rk45_step(ie_run *run, double h, double hmax, int *accepted, double *h_next);
 /* The zeta map of saveN (src/output.cpp:1527-1542): mean of deltaN over sites
  * with deltaN <= cutoff, accumulated in float in lattice order and divided by N^3
  * in float as the reference does, then subtracted from every site.  *nmean

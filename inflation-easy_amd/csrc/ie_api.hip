@@ -10,7 +10,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ie_fft.h"
@@ -37,6 +40,7 @@ struct ie_run {
     unsigned long long *hist = nullptr;
     int hist_bins = 0;
     double *dn_f = nullptr, *dn_fd = nullptr;  // FP64 deltaN state (split runs)
+    double *dn5[3] = {nullptr, nullptr, nullptr};   // deltaN RK45 trial state (f, fd, N)
     int in_deltaN = 0;
     RkWork *rk4 = nullptr, *rk45 = nullptr;    // RK stage buffers, made on first use
     std::vector<double> htab_x, htab_v, htab_dv;   // host copy of the table
@@ -103,6 +107,45 @@ ie_status check_kernel_error(ie_run *r)
         return fail(r, IE_ERR_TABLE_RANGE, "field value outside the numerical potential range");
     }
     return IE_OK;
+}
+
+// Accept test and step-size proposal of an RK45 trial step (src/evolution.cpp:
+// 880-904, 1250-1267): accept when the weighted RMS error is <= 1; grow by
+// safety * err^{-1/5} (within [0.1, 5]) after an accepted step, shrink by at most
+// 0.5 (at least 0.1) after a rejection, clamped to [max(rk45_min_dt, 1e-16), hmax].
+int rk45_propose(const ie_config &c, double h, double hmax, double err_norm, double *h_next)
+{
+    const double safe = err_norm > 1e-16 ? err_norm : 1e-16;
+    double factor = c.rk45_safety * pow(safe, -0.2);
+    factor = factor < 0.1 ? 0.1 : (factor > 5.0 ? 5.0 : factor);
+    const int acc = err_norm <= 1.0;
+    const double hmin = c.rk45_min_dt > 1e-16 ? c.rk45_min_dt : 1e-16;
+    const double hhi = hmax > hmin ? hmax : hmin;
+    double hn = acc ? h * factor : h * std::max(std::min(factor, 0.5), 0.1);
+    hn = hn < hmin ? hmin : (hn > hhi ? hhi : hn);
+    *h_next = hn;
+    return acc;
+}
+
+// Kernel shapes chosen by the tuner, per (N, split storage): sites per lane of the
+// kick and of the tensor launches (0: not tuned yet).
+std::mutex g_shape_mu;
+std::map<std::pair<int, int>, std::pair<int, int>> g_shapes;
+
+void tuned_shapes(int N, int split, int *kick_e, int *tensor_e)
+{
+    std::lock_guard<std::mutex> lock(g_shape_mu);
+    auto it = g_shapes.find({N, split});
+    *kick_e = it == g_shapes.end() ? 0 : it->second.first;
+    *tensor_e = it == g_shapes.end() ? 0 : it->second.second;
+}
+
+void store_shapes(int N, int split, int kick_e, int tensor_e)
+{
+    std::lock_guard<std::mutex> lock(g_shape_mu);
+    auto &v = g_shapes[{N, split}];
+    if (kick_e > 0) v.first = kick_e;
+    if (tensor_e > 0) v.second = tensor_e;
 }
 
 __global__ void k_loop_begin(DevBg *bg)
@@ -188,6 +231,7 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->F.N = c.N;
     r->F.logN = r->logN;
     r->F.sigw = c.calculate_sigw ? 1 : 0;
+    tuned_shapes(c.N, c.precision == IE_SPLIT, &r->F.kick_e, &r->F.tensor_e);
     for (int p = 0; p < 2; ++p)
         if ((st = dev_alloc(r, &r->F.f[p], r->gs * es)) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, &r->F.fd, r->gs * es)) != IE_OK) return bail(st);
@@ -462,6 +506,15 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
 {
     if (!r || max_steps < 0) return fail(r, IE_ERR_ARG, "bad argument");
     const int split = r->cfg.precision == IE_SPLIT;
+    // first long loop of a kind: tune the kick / tensor shapes on its first steps
+    int ke = 0, te = 0;
+    tuned_shapes(r->N, split, &ke, &te);
+    if (max_steps >= 256 && row_elems(r->N) >= 2 && (ke == 0 || (r->F.sigw && te == 0))) {
+        ie_tuning tun;
+        ie_status st = ie_tune(r, 8, &tun);
+        if (st != IE_OK) return st;
+        max_steps -= (int)std::min<int64_t>(tun.steps, max_steps);
+    }
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
     IE_HIP(r, hipGetLastError());
     for (int n = 0; n < max_steps; ++n) {
@@ -521,6 +574,70 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
     out->tensor_ms = ten / nsteps;
     out->step_ms = step / nsteps;
     out->steps = r->bg_host->steps;
+    return IE_OK;
+}
+
+ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
+{
+    if (!r || K < 1 || !out) return fail(r, IE_ERR_ARG, "bad argument");
+    memset(out, 0, sizeof(*out));
+    const int split = r->cfg.precision == IE_SPLIT;
+    const int def = row_elems(r->N);
+    out->kick_elems = out->tensor_elems = def;
+    for (int e = 1; e <= 4 && def > 0; e *= 2)
+        if (e <= def) out->elems[out->ncand++] = e;
+    if (out->ncand <= 1) {
+        r->F.kick_e = r->F.tensor_e = def;
+        store_shapes(r->N, split, def, r->F.sigw ? def : 0);
+        return IE_OK;
+    }
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int64_t steps0 = r->bg_host->steps;
+    const int nc = out->ncand;
+    std::vector<hipEvent_t> ev((size_t)nc * K * 4);
+    for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
+    k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+    for (int c = 0; c < nc; ++c) {
+        r->F.kick_e = r->F.tensor_e = out->elems[c];
+        for (int q = 0; q < K; ++q) {
+            hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
+            IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+            IE_HIP(r, hipEventRecord(e[0], r->stream));
+            IE_HIP(r, launch_scalar_step(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+            IE_HIP(r, hipEventRecord(e[1], r->stream));
+            if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+            IE_HIP(r, hipEventRecord(e[2], r->stream));
+            IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
+            IE_HIP(r, hipEventRecord(e[3], r->stream));
+        }
+    }
+    if ((st = sync(r)) != IE_OK) return st;
+    int bk = 0, bt = 0;
+    for (int c = 0; c < nc; ++c) {
+        double kk = 0.0, tt = 0.0;
+        for (int q = 0; q < K; ++q) {
+            hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
+            float ms = 0.f;
+            IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
+            kk += ms;
+            IE_HIP(r, hipEventElapsedTime(&ms, e[1], e[2]));
+            tt += ms;
+        }
+        out->kick_ms[c] = kk / K;
+        out->tensor_ms[c] = r->F.sigw ? tt / K : 0.0;
+        if (out->kick_ms[c] < out->kick_ms[bk]) bk = c;
+        if (out->tensor_ms[c] < out->tensor_ms[bt]) bt = c;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    out->steps = r->bg_host->steps - steps0;
+    out->kick_elems = out->elems[bk];
+    out->tensor_elems = r->F.sigw ? out->elems[bt] : def;
+    r->F.kick_e = out->kick_elems;
+    r->F.tensor_e = out->tensor_elems;
+    // timings of a loop that reached af part way are not kept
+    if (out->steps == (int64_t)nc * K) store_shapes(r->N, split, out->kick_elems, r->F.sigw ? out->tensor_elems : 0);
     return IE_OK;
 }
 
@@ -839,6 +956,63 @@ ie_status ie_deltaN_rk4_steps(ie_run *r, double h, int64_t nsteps)
     return push_bg(r);
 }
 
+ie_status ie_deltaN_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *h_next)
+{
+    if (!r || !accepted || !h_next || !(h > 0.0)) return fail(r, IE_ERR_ARG, "bad argument");
+    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    double *f, *fd;
+    ie_status st = dn_state(r, &f, &fd);
+    if (st != IE_OK) return st;
+    if (!r->dn5[0])
+        for (int q = 0; q < 3; ++q)
+            if ((st = dev_alloc(r, (void **)&r->dn5[q], r->gs * 8)) != IE_OK) return st;
+    const ie_config &c = r->cfg;
+    DevBg &b = *r->bg_host;
+    bool ok = true;
+    const double Vref = c.monotonic == IE_GENERIC ? ref_potential(r, b.phiref, &ok) : 0.0;
+    if (!ok) return fail(r, IE_ERR_TABLE_RANGE, "phiref outside the numerical potential range");
+    double *total = r->work + 4000;
+    IE_HIP(r, dn_rk45_trial(r->gs, f, fd, r->F.dn, r->dn5[0], r->dn5[1], r->dn5[2], r->P, c.monotonic, b.phiref,
+                            Vref, h, c.rk45_abs_tol, c.rk45_rel_tol, r->work, total, &r->bg->error, r->stream));
+    double acc_sum = 0.0;
+    IE_HIP(r, hipMemcpyAsync(&acc_sum, total, 8, hipMemcpyDeviceToHost, r->stream));
+    if ((st = check_kernel_error(r)) != IE_OK) return st;
+    // the scale factor's error term: a' = ad in every stage (src/evolution.cpp:1113)
+    const double Bw[7] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0};
+    const double Ew[7] = {35.0 / 384.0 - 5179.0 / 57600.0,       0,
+                          500.0 / 1113.0 - 7571.0 / 16695.0,     125.0 / 192.0 - 393.0 / 640.0,
+                          -2187.0 / 6784.0 + 92097.0 / 339200.0, 11.0 / 84.0 - 187.0 / 2100.0,
+                          -1.0 / 40.0};
+    double a5 = b.a, erra = 0.0;
+    for (int q = 0; q < 7; ++q) {
+        if (Bw[q] != 0.0) a5 += h * Bw[q] * b.ad;
+        if (Ew[q] != 0.0) erra += h * Ew[q] * b.ad;
+    }
+    const double sa = c.rk45_abs_tol + c.rk45_rel_tol * std::max(fabs(b.a), fabs(a5));
+    acc_sum += (erra / sa) * (erra / sa);
+    const double norm = sqrt(acc_sum / (double)(3 * r->gs + 1));
+    if (!isfinite(norm)) return fail(r, IE_ERR_NONFINITE, "RK45 error norm is not finite");
+    const int acc = rk45_propose(c, h, hmax, norm, h_next);
+    *accepted = acc;
+    if (!acc) return IE_OK;
+    // adopt the trial state: the previous buffers become the next trial's
+    if (c.precision == IE_SPLIT) {
+        std::swap(r->dn_f, r->dn5[0]);
+        std::swap(r->dn_fd, r->dn5[1]);
+    } else {
+        double *pf = (double *)r->F.f[b.parity], *pfd = (double *)r->F.fd;
+        r->F.f[b.parity] = r->dn5[0];
+        r->F.fd = r->dn5[1];
+        r->dn5[0] = pf;
+        r->dn5[1] = pfd;
+    }
+    std::swap(r->F.dn, r->dn5[2]);
+    r->scratch = r->F.dn;
+    b.a = a5;
+    b.t += h;
+    return push_bg(r);
+}
+
 ie_status ie_deltaN_zeta(ie_run *r, double cutoff, float *nmean)
 {
     if (!r || !nmean) return fail(r, IE_ERR_ARG, "null argument");
@@ -1083,14 +1257,9 @@ ie_status ie_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *
     const DevBg advanced = *r->bg_host;
     if ((st = check_kernel_error(r)) != IE_OK) return st;
     if (!isfinite(norm)) return fail(r, IE_ERR_NONFINITE, "RK45 error norm is not finite");
-    // step-size proposal (src/evolution.cpp:880-904): grow by the factor after an
-    // accepted step, shrink by at most 0.5 (at least 0.1) after a rejection
-    const double hmin = c.rk45_min_dt > 1e-16 ? c.rk45_min_dt : 1e-16;
-    const double hhi = hmax > hmin ? hmax : hmin;
-    double hn = acc ? h * factor : h * std::max(std::min(factor, 0.5), 0.1);
-    hn = hn < hmin ? hmin : (hn > hhi ? hhi : hn);
+    (void)factor;
+    acc = rk45_propose(c, h, hmax, norm, h_next);
     *accepted = acc;
-    *h_next = hn;
     if (!acc) return IE_OK;
     DevBg &b = *r->bg_host;
     b.t = advanced.t;

@@ -226,6 +226,86 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk4(double *__restrict__ f, d
     if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(error, 1);
 }
 
+// One Dormand-Prince 5(4) trial step of the deltaN system (rk45_step_deltaN,
+// src/evolution.cpp:1191-1267).  The system is per site with one global step, so
+// the seven stages stay in registers: the fifth-order state goes to (f5, fd5, dn5)
+// and each workgroup writes its partial of the weighted squared error
+// sum (e / (abs_tol + rel_tol max(|y|, |y5|)))^2 over (phi, pi, N).
+__global__ __launch_bounds__(kBlock) void k_deltaN_rk45_trial(
+    const double *__restrict__ f, const double *__restrict__ fd, const double *__restrict__ dn,
+    double *__restrict__ f5, double *__restrict__ fd5, double *__restrict__ dn5, size_t gs, DnParams D, double h,
+    double abs_tol, double rel_tol, double *__restrict__ partials, int *error)
+{
+    __shared__ double tx[kMaxTable], tv[kMaxTable], tdv[kMaxTable];
+    __shared__ double smem[kWaves];
+    const int n = D.P.numerical ? D.P.n : 0;
+    const bool lds = n > 0 && n <= kMaxTable;
+    for (int e = threadIdx.x; lds && e < n; e += blockDim.x) {
+        tx[e] = D.P.x[e];
+        tv[e] = D.P.v[e];
+        tdv[e] = D.P.dv[e];
+    }
+    __syncthreads();
+    const double *x = lds ? tx : D.P.x, *yv = lds ? tv : D.P.v, *ydv = lds ? tdv : D.P.dv;
+    const double A[7][7] = {{0, 0, 0, 0, 0, 0, 0},
+                            {1.0 / 5.0, 0, 0, 0, 0, 0, 0},
+                            {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0, 0},
+                            {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0, 0},
+                            {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0, 0},
+                            {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0, 0},
+                            {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0}};
+    const double B[7] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0};
+    const double E[7] = {35.0 / 384.0 - 5179.0 / 57600.0,     0,
+                         500.0 / 1113.0 - 7571.0 / 16695.0,   125.0 / 192.0 - 393.0 / 640.0,
+                         -2187.0 / 6784.0 + 92097.0 / 339200.0, 11.0 / 84.0 - 187.0 / 2100.0,
+                         -1.0 / 40.0};
+    double acc = 0.0;
+    int err = 0;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        const double phi = f[id], pi = fd[id], N = dn[id];
+        int hint = n > 0 ? n / 2 : 0, ghint = hint;
+        double kf[7], kfd[7], kdn[7];
+        dn_rhs(D, x, yv, ydv, n, phi, pi, &hint, &ghint, &kf[0], &kfd[0], &kdn[0], &err);
+#pragma unroll
+        for (int st = 1; st < 7; ++st) {
+            double sf = 0.0, sfd = 0.0;
+#pragma unroll
+            for (int q = 0; q < st; ++q) {
+                const double c = A[st][q];
+                if (c == 0.0) continue;
+                sf += c * kf[q];
+                sfd += c * kfd[q];
+            }
+            dn_rhs(D, x, yv, ydv, n, phi + h * sf, pi + h * sfd, &hint, &ghint, &kf[st], &kfd[st], &kdn[st], &err);
+        }
+        double y5f = phi, y5fd = pi, y5dn = N, ef = 0.0, efd = 0.0, edn = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            if (B[q] != 0.0) {
+                y5f += h * B[q] * kf[q];
+                y5fd += h * B[q] * kfd[q];
+                y5dn += h * B[q] * kdn[q];
+            }
+            if (E[q] != 0.0) {
+                ef += h * E[q] * kf[q];
+                efd += h * E[q] * kfd[q];
+                edn += h * E[q] * kdn[q];
+            }
+        }
+        f5[id] = y5f;
+        fd5[id] = y5fd;
+        dn5[id] = y5dn;
+        const double s1 = abs_tol + rel_tol * fmax(fabs(phi), fabs(y5f));
+        const double s2 = abs_tol + rel_tol * fmax(fabs(pi), fabs(y5fd));
+        const double s3 = abs_tol + rel_tol * fmax(fabs(N), fabs(y5dn));
+        acc += (ef / s1) * (ef / s1) + (efd / s2) * (efd / s2) + (edn / s3) * (edn / s3);
+    }
+    if (__any(err) && (threadIdx.x & 63) == 0) atomicOr(error, 1);
+    double v[1] = {acc};
+    block_sum<1>(v, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
 // evolve_fieldsN(d) alone (src/evolution.cpp:1015-1037).
 __global__ __launch_bounds__(kBlock) void k_fieldsN(double *__restrict__ f, const double *__restrict__ fd,
                                                      double *__restrict__ dn, size_t gs, DnParams D, double d)
@@ -495,6 +575,17 @@ hipError_t dn_rk4_steps(size_t gs, double *f, double *fd, double *dn, const Pot 
 {
     size_t b = (gs + kBlock - 1) / kBlock;
     k_deltaN_rk4<<<(int)b, kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), h, nsteps, error);
+    return hipGetLastError();
+}
+
+hipError_t dn_rk45_trial(size_t gs, const double *f, const double *fd, const double *dn, double *f5, double *fd5,
+                         double *dn5, const Pot &P, int mono, double phiref, double Vref, double h, double abs_tol,
+                         double rel_tol, double *work, double *out, int *error, hipStream_t s)
+{
+    const int nb = grid_for(gs) > 2048 ? 2048 : grid_for(gs);
+    k_deltaN_rk45_trial<<<nb, kBlock, 0, s>>>(f, fd, dn, f5, fd5, dn5, gs, params(P, mono, phiref, Vref), h, abs_tol,
+                                               rel_tol, work, error);
+    k_sum_final<<<1, 64, 0, s>>>(work, nb, out);
     return hipGetLastError();
 }
 

@@ -17,8 +17,9 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kMaxRed = 8;       // reduction slots per block
 
 // Background of one run, resident on the device.  Kernels of a step read it at
-// their start; the last workgroup of a kernel (arrival counter) updates it, so a
-// whole step sequence runs without the host.
+// their start; one-workgroup finish kernels (or the last workgroup of an output
+// reduction, by arrival counter) update it, so a whole step sequence runs without
+// the host.
 struct DevBg {
     double t, a, ad, ad2, astep, Ne, phiref, hubble_init;
     double d;                                   // step of the current kick/drift
@@ -54,6 +55,7 @@ struct Fields {
     double *dn;
     int N, logN;
     int sigw;
+    int kick_e, tensor_e;   // sites per lane of the kick / tensor row kernels (0: default)
 };
 
 // ---------------------------------------------------------------------- helpers
@@ -210,6 +212,10 @@ enum KickMode { KICK = 1, DRIFT = 2, KICKDRIFT = 3 };
 // Workgroups of the step kernels of an N^3 lattice (the same alone and batched,
 // so a lattice's fixed-order sums do not depend on its company).
 unsigned step_blocks(int N);
+// Sites per lane of the row kernels (0 below N = 64) and of a kick / tensor launch
+// asked for `want` sites per lane.
+int row_elems(int N);
+int shape_elems(int N, int want);
 // Partial Friedmann sums of the current f, one triple per workgroup.
 hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, int loop,
                          hipStream_t s);
@@ -267,6 +273,11 @@ hipError_t dn_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, 
                     double dN, long long nsteps, int *error, hipStream_t s);
 hipError_t dn_rk4_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, int mono, double phiref,
                         double Vref, double h, long long nsteps, int *error, hipStream_t s);
+// One RK45 trial step of every site into (f5, fd5, dn5); *out receives the
+// fixed-order sum of the weighted squared errors (work: >= 2048 partials).
+hipError_t dn_rk45_trial(size_t gs, const double *f, const double *fd, const double *dn, double *f5, double *fd5,
+                         double *dn5, const Pot &P, int mono, double phiref, double Vref, double h, double abs_tol,
+                         double rel_tol, double *work, double *out, int *error, hipStream_t s);
 hipError_t dn_phiref(size_t gs, const double *f, const Pot &P, int mono, double *work, double *out, hipStream_t s);
 hipError_t dn_float_mean(const double *dn, size_t gs, double cutoff, float *out, hipStream_t s);
 hipError_t dn_zeta(double *dn, size_t gs, const float *nmean, double cutoff, int zero, hipStream_t s);

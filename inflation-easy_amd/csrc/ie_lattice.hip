@@ -428,6 +428,101 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, Dev
     flag_error(bg, err);
 }
 
+// Tensor kick (+ drift) of the six h_ij on rows (src/evolution.cpp:185-253,
+// 274-280): source 2 d_i phi d_j phi from the central gradient of the old f (a
+// j-window of f rows in registers), the FP32 Laplacian of each component from its
+// five rows and two lane shuffles, the right-hand side in FP64.
+template <typename T, int E, int MODE>
+__device__ __forceinline__ void row_tensor_body(const Fields &F, DevBg *bg, int loop, unsigned G, unsigned blk)
+{
+    if (loop && bg->done) return;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
+    const int par = bg->parity;
+    const T *__restrict__ f = (const T *)F.f[par];
+    const double d = bg->d;
+    const double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    const double hx = 0.5 / bg->dx;
+    const RowWalk w = row_walk(N, G, blk, E);
+    const unsigned poff = (w.i << (2 * logN)) + w.k0;
+    const unsigned pp = (((w.i + 1) & m) << (2 * logN)) + w.k0, pm = (((w.i - 1) & m) << (2 * logN)) + w.k0;
+    T prev[E], cur[E], nxt[E];
+    ld_row<T, E>(f + poff + (((w.j0 - 1) & m) << logN), lane, prev);
+    ld_row<T, E>(f + poff + (w.j0 << logN), lane, cur);
+    for (unsigned r = 0; r < w.R; ++r) {
+        const unsigned j = w.j0 + r;
+        const unsigned rj = j << logN, rjp = ((j + 1) & m) << logN, rjm = ((j - 1) & m) << logN;
+        T up[E], dn[E];
+        ld_row<T, E>(f + poff + rjp, lane, nxt);
+        ld_row<T, E>(f + pp + rj, lane, up);
+        ld_row<T, E>(f + pm + rj, lane, dn);
+        T kp_last, km_first;
+        k_neighbours<T, E>(cur, f + poff + rj - w.k0, w, N, lane, kp_last, km_first);
+        double gx[E], gy[E], gz[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const double kp = (double)(e < E - 1 ? cur[e + 1 < E ? e + 1 : e] : kp_last);
+            const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
+            gx[e] = ((double)up[e] - (double)dn[e]) * hx;
+            gy[e] = ((double)nxt[e] - (double)prev[e]) * hx;
+            gz[e] = (kp - km) * hx;
+        }
+#pragma unroll 1
+        for (int c = 0; c < 6; ++c) {
+            const float *__restrict__ h = F.h[par][c];
+            float hc[E], hjp[E], hjm[E], hip[E], him[E], hv[E];
+            ld_row<float, E>(h + poff + rj, lane, hc);
+            ld_row<float, E>(h + poff + rjp, lane, hjp);
+            ld_row<float, E>(h + poff + rjm, lane, hjm);
+            ld_row<float, E>(h + pp + rj, lane, hip);
+            ld_row<float, E>(h + pm + rj, lane, him);
+            ld_row<float, E>(F.hd[c] + poff + rj, lane, hv);
+            float hkp, hkm;
+            k_neighbours<float, E>(hc, h + poff + rj - w.k0, w, N, lane, hkp, hkm);
+            float hdn[E], hn[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float kp = e < E - 1 ? hc[e + 1 < E ? e + 1 : e] : hkp;
+                const float km = e > 0 ? hc[e > 0 ? e - 1 : 0] : hkm;
+                const float hl = kp + km + hjp[e] + hjm[e] + hip[e] + him[e] - 6.0f * hc[e];
+                double src;
+                switch (c) {
+                case 0: src = gx[e] * gx[e]; break;
+                case 1: src = gy[e] * gy[e]; break;
+                case 2: src = gz[e] * gz[e]; break;
+                case 3: src = gx[e] * gy[e]; break;
+                case 4: src = gx[e] * gz[e]; break;
+                default: src = gy[e] * gz[e]; break;
+                }
+                const double rhs = d * (laplnorm * (double)hl - friction * (double)hv[e] + srcamp * src);
+                hdn[e] = hv[e] + (float)rhs;
+                hn[e] = hc[e] + (float)(d * (double)hdn[e]);
+            }
+            st_row<float, E>(F.hd[c] + poff + rj, lane, hdn);
+            if (MODE & DRIFT) st_row<float, E>(F.h[par ^ 1][c] + poff + rj, lane, hn);
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            prev[e] = cur[e];
+            cur[e] = nxt[e];
+        }
+    }
+}
+
+template <typename T, int E, int MODE>
+__global__ __launch_bounds__(kBlock) void k_row_tensor(Fields F, DevBg *bg, int loop)
+{
+    row_tensor_body<T, E, MODE>(F, bg, loop, gridDim.x, blockIdx.x);
+}
+
+template <typename T, int E>
+__global__ __launch_bounds__(kBlock) void kb_row_tensor(const RunDesc *__restrict__ runs,
+                                                         const uint2 *__restrict__ map)
+{
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.G, mp.y);
+}
+
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kBlock) void k_row_reduce(Fields F, Pot P, DevBg *bg, double *partials, int loop)
 {
@@ -767,9 +862,6 @@ __global__ __launch_bounds__(kBlock) void k_reconstitute(Fields F, const DevBg *
     for (size_t id = w.begin; id < w.end; id += w.stride) out[id] = SPLIT ? base + (double)src[id] : (double)src[id];
 }
 
-// Elements per lane of the row kernels, 0 for the site kernels (N < 64).
-int row_elems(int N) { return N < 64 ? 0 : (N >= 256 ? 4 : N / 64); }
-
 template <typename T, bool SPLIT>
 void reduce_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop, unsigned G, hipStream_t s)
 {
@@ -784,7 +876,7 @@ void reduce_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials,
 template <typename T, bool SPLIT, int MODE>
 void row_kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int loop, unsigned G, hipStream_t s)
 {
-    switch (row_elems(F.N)) {
+    switch (shape_elems(F.N, F.kick_e)) {
     case 1: k_row_kick<T, SPLIT, 1, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
     case 2: k_row_kick<T, SPLIT, 2, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
     default: k_row_kick<T, SPLIT, 4, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
@@ -806,15 +898,44 @@ void kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_
     }
 }
 
+template <typename T, int MODE>
+void row_tensor_dispatch(const Fields &F, DevBg *bg, int loop, unsigned G, hipStream_t s)
+{
+    switch (shape_elems(F.N, F.tensor_e)) {
+    case 1: k_row_tensor<T, 1, MODE><<<G, kBlock, 0, s>>>(F, bg, loop); break;
+    case 2: k_row_tensor<T, 2, MODE><<<G, kBlock, 0, s>>>(F, bg, loop); break;
+    default: k_row_tensor<T, 4, MODE><<<G, kBlock, 0, s>>>(F, bg, loop); break;
+    }
+}
+
 template <typename T>
 void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loop, unsigned G, hipStream_t s)
 {
-    if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
-    else if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
-    else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+    if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+    else if (row_elems(F.N) == 0) {
+        if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+        else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+    } else {
+        if (mode == KICK) row_tensor_dispatch<T, KICK>(F, bg, loop, G, s);
+        else row_tensor_dispatch<T, KICKDRIFT>(F, bg, loop, G, s);
+    }
 }
 
 }  // namespace
+
+// Elements per lane of the row kernels, 0 for the site kernels (N < 64).
+int row_elems(int N) { return N < 64 ? 0 : (N >= 256 ? 4 : N / 64); }
+
+// Elements per lane of a kick or tensor launch: `want` (1, 2 or 4; 0 = the
+// reduce's) within what one row holds.  These kernels have no cross-site sums,
+// so their shape never changes a result.
+int shape_elems(int N, int want)
+{
+    const int def = row_elems(N);
+    if (def == 0 || want <= 0) return def;
+    const int e = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
+    return e <= def ? e : def;
+}
 
 int reduce_blocks(size_t gs)
 {
@@ -930,7 +1051,7 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
 // Runs of one storage kind, grouped by kernel shape (row elements per lane); one
 // launch per group and kernel, one finish launch over all runs.
 struct BatchGroup {
-    int E = 0;
+    int E = 0, kE = 0, tE = 0;   // elements per lane: reduce, kick, tensor
     uint2 *map = nullptr, *tmap = nullptr;
     int nblocks = 0, ntblocks = 0;
 };
@@ -948,8 +1069,9 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     BatchDev *B = new BatchDev();
     B->nruns = n;
     B->split = split;
-    const int shapes[4] = {0, 1, 2, 4};
-    std::vector<std::vector<uint2>> maps(4), tmaps(4);
+    // groups of runs launched together: same kernel shapes (reduce, kick, tensor)
+    std::vector<int> keys;
+    std::vector<std::vector<uint2>> maps, tmaps;
     for (int r = 0; r < n; ++r) {
         d[r].F = runs[r].F;
         d[r].P = runs[r].P;
@@ -957,8 +1079,15 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         d[r].partials = runs[r].partials;
         d[r].G = step_blocks(runs[r].F.N);
         d[r].split = split;
-        const int E = row_elems(runs[r].F.N);
-        const int g = E == 0 ? 0 : (E == 1 ? 1 : (E == 2 ? 2 : 3));
+        const int N = runs[r].F.N;
+        const int key = row_elems(N) * 100 + shape_elems(N, runs[r].F.kick_e) * 10 + shape_elems(N, runs[r].F.tensor_e);
+        size_t g = 0;
+        while (g < keys.size() && keys[g] != key) ++g;
+        if (g == keys.size()) {
+            keys.push_back(key);
+            maps.emplace_back();
+            tmaps.emplace_back();
+        }
         for (unsigned b = 0; b < d[r].G; ++b) {
             maps[g].push_back(make_uint2((unsigned)r, b));
             if (runs[r].F.sigw) tmaps[g].push_back(make_uint2((unsigned)r, b));
@@ -966,10 +1095,11 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     }
     hipError_t e = hipMalloc(&B->runs, sizeof(RunDesc) * n);
     if (e == hipSuccess) e = hipMemcpy(B->runs, d.data(), sizeof(RunDesc) * n, hipMemcpyHostToDevice);
-    for (int g = 0; g < 4 && e == hipSuccess; ++g) {
-        if (maps[g].empty()) continue;
+    for (size_t g = 0; g < keys.size() && e == hipSuccess; ++g) {
         BatchGroup G;
-        G.E = shapes[g];
+        G.E = keys[g] / 100;
+        G.kE = keys[g] / 10 % 10;
+        G.tE = keys[g] % 10;
         G.nblocks = (int)maps[g].size();
         G.ntblocks = (int)tmaps[g].size();
         e = hipMalloc(&G.map, sizeof(uint2) * maps[g].size());
@@ -1021,13 +1151,19 @@ static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream
 template <typename T, bool SPLIT>
 static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
-    switch (G.E) {
+    switch (G.kE) {
     case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     }
-    if (G.ntblocks) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+    if (!G.ntblocks) return;
+    switch (G.tE) {
+    case 1: kb_row_tensor<T, 1><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
+    case 2: kb_row_tensor<T, 2><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
+    case 4: kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
+    default: kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
+    }
 }
 
 hipError_t batch_step(const BatchDev *B, hipStream_t s)

@@ -329,9 +329,7 @@ ie_status ie_server_pump(ie_server *s, int *active, int *pending)
         }
     }
     s->live.swap(keep);
-    int n_active = 0;
-    for (Job *j : s->live) n_active += 1;
-    if (active) *active = n_active;
+    if (active) *active = (int)s->live.size();
     if (pending) *pending = (int)s->pending.size();
     return IE_OK;
 }

@@ -447,13 +447,95 @@ struct Driver {
         }
     }
 
+    // RK45 step clamp and cap (src/evolution.cpp:330-345)
+    double clamp_rk45(double h, double hmax) const
+    {
+        const double hmin = p.rk45_min_dt > 1e-16 ? p.rk45_min_dt : 1e-16;
+        const double hhi = hmax > hmin ? hmax : hmin;
+        return h < hmin ? hmin : (h > hhi ? hhi : h);
+    }
+    double rk45_hmax(double base) const
+    {
+        const double lo = p.rk45_min_dt > 2.0 * base ? p.rk45_min_dt : 2.0 * base;
+        return p.rk45_max_dt < lo ? p.rk45_max_dt : lo;
+    }
+    // accept/reject driver (src/evolution.cpp:346-362): trial steps until one is
+    // accepted; returns the proposed next step
+    double rk45_drive(double h, double hmax, bool deltaN_phase)
+    {
+        int accepted = 0, attempts = 0;
+        while (!accepted) {
+            double hs = h;
+            if (deltaN_phase) CHECK(run, ie_deltaN_rk45_step(run, h, hmax, &accepted, &hs));
+            else CHECK(run, ie_rk45_step(run, h, hmax, &accepted, &hs));
+            h = hs;
+            ++attempts;
+            if (!accepted && (attempts > 25 || h <= p.rk45_min_dt * (1.0 + 1e-12))) {
+                fprintf(stderr, "RK45 failed to converge (attempts=%d, h=%e)\n", attempts, h);
+                exit(1);
+            }
+        }
+        return h;
+    }
+
+    void progress(long long n, double a)
+    {
+        if (log) {
+            fprintf(log, "scale factor a = %f\n", a);
+            fprintf(log, "numsteps %lld\n\n", n);
+            fflush(log);
+        }
+        if (p.screen_updates && n % (p.output_freq > 0 ? p.output_freq : 1) == 0) {
+            printf("scale factor a = %f\n", a);
+            printf("numsteps %lld\n\n", n);
+        }
+    }
+
+    // run_evolution_loop, RK4 and RK45 branches (src/evolution.cpp:947-989): one
+    // host-driven step at a time, dt astep^{s-1} from the previous step's a
+    void inflation_rk()
+    {
+        long long n = 0;
+        auto step_base = [&]() { return p.dt * pow(bg().astep, p.rescale_s - 1.0); };
+        double h = clamp_rk45(step_base(), p.rk45_max_dt);
+        for (;;) {
+            ie_background b = bg();
+            if (!(b.a <= p.af)) break;
+            if (p.integrator == IE_RK4) {
+                CHECK(run, ie_rk4_step(run, step_base()));
+            } else {
+                const double hmax = rk45_hmax(step_base());
+                h = clamp_rk45(h, hmax);
+                h = rk45_drive(h, hmax, false);
+            }
+            ++n;
+            b = bg();
+            progress(n, b.a);
+            if (n % p.output_freq == 0 && b.a < p.af) save((n % p.output_infrequent_freq == 0) ? 1 : 0, 0.0);
+            b.astep = b.a;
+            CHECK(run, ie_set_background(run, &b));
+        }
+        printf("Saving final inflaton data\n");
+        save(1, 0.0);
+        save_modes();
+    }
+
+    void save_modes()
+    {
+        // save_last: modes.dat (src/output.cpp:1505-1519)
+        const int nb = (int)(sqrt(3.0) * (N / 2)) + 1;
+        std::vector<double> md(nb);
+        CHECK(run, ie_modes(N, p.L, p.rescale_B, md.data()));
+        FILE *fp = out.get("modes");
+        for (double x : md) fprintf(fp, "%e\n", x);
+        fprintf(fp, "\n");
+        fflush(fp);
+    }
+
     // run_evolution_loop, leapfrog branch (src/evolution.cpp:909-946,990-997)
     void inflation()
     {
-        if (p.integrator != IE_LEAPFROG) {
-            fprintf(stderr, "inflation_integrator %d: only leapfrog runs on the GPU path\n", p.integrator);
-            exit(1);
-        }
+        if (p.integrator != IE_LEAPFROG) return inflation_rk();
         CHECK(run, ie_evolve_fields(run, 0.5 * p.dt));
         long long n = 0;
         const int chunk = p.output_freq > 0 ? p.output_freq : 1;
@@ -463,15 +545,7 @@ struct Driver {
             const long long taken = b.steps - n;
             n = b.steps;
             if (taken <= 0) break;
-            if (log) {
-                fprintf(log, "scale factor a = %f\n", b.a);
-                fprintf(log, "numsteps %lld\n\n", n);
-                fflush(log);
-            }
-            if (p.screen_updates && n % chunk == 0) {
-                printf("scale factor a = %f\n", b.a);
-                printf("numsteps %lld\n\n", n);
-            }
+            progress(n, b.a);
             if (n % p.output_freq == 0 && b.a < p.af) {
                 // re-synchronise by half the step just taken, dt astep_old^{s-1}
                 // (src/output.cpp:1471-1472: astep is not yet updated there)
@@ -484,23 +558,14 @@ struct Driver {
         const double half = 0.5 * p.dt * pow(b.astep, p.rescale_s - 1.0);
         save(1, half);
         CHECK(run, ie_evolve_fields(run, -half));
-        // save_last: modes.dat (src/output.cpp:1505-1519)
-        const int nb = (int)(sqrt(3.0) * (N / 2)) + 1;
-        std::vector<double> md(nb);
-        CHECK(run, ie_modes(N, p.L, p.rescale_B, md.data()));
-        FILE *fp = out.get("modes");
-        for (double x : md) fprintf(fp, "%e\n", x);
-        fprintf(fp, "\n");
-        fflush(fp);
+        save_modes();
     }
 
-    // run_deltaN_loop, leapfrog branch (src/evolution.cpp:1291-1375) with saveN
+    // run_deltaN_loop (src/evolution.cpp:1291-1375) with saveN; the leapfrog is
+    // staggered by half a step, RK4 and RK45 are not
     void deltaN()
     {
-        if (p.deltaN_integrator != IE_LEAPFROG) {
-            fprintf(stderr, "deltaN_integrator %d: only leapfrog runs on the GPU path\n", p.deltaN_integrator);
-            exit(1);
-        }
+        const bool staggered = p.deltaN_integrator == IE_LEAPFROG;
         printf("Starting deltaN calculation\n");
         if (log) fprintf(log, "Starting deltaN calculation\n");
         CHECK(run, ie_deltaN_init(run));
@@ -508,23 +573,34 @@ struct Driver {
         b.phiref = 0.0;
         b.Ne = 0.0;
         CHECK(run, ie_set_background(run, &b));
-        CHECK(run, ie_evolve_fieldsN(run, 0.5 * p.dN));
+        if (staggered) CHECK(run, ie_evolve_fieldsN(run, 0.5 * p.dN));
         double phiref = p.phiref_manual_value;
         if (!p.use_phiref_manual) CHECK(run, ie_get_phiref(run, &phiref));
         b = bg();
         b.phiref = phiref;
         CHECK(run, ie_set_background(run, &b));
         double Ne = 0.0;
-        long long nsteps = 0;
-        while (Ne <= p.Nend) {   // the step count of the reference loop, exactly
-            Ne += p.dN;
-            ++nsteps;
+        if (p.deltaN_integrator == IE_RK45) {
+            double h = clamp_rk45(p.dN, p.rk45_max_dt);
+            while (Ne <= p.Nend) {
+                const double hmax = rk45_hmax(p.dN);
+                h = clamp_rk45(h, hmax);
+                h = rk45_drive(h, hmax, true);
+                Ne += h;   // the proposed next step, as the reference counts
+            }
+        } else {
+            long long nsteps = 0;
+            while (Ne <= p.Nend) {   // the step count of the reference loop, exactly
+                Ne += p.dN;
+                ++nsteps;
+            }
+            if (staggered) CHECK(run, ie_deltaN_steps(run, p.dN, nsteps));
+            else CHECK(run, ie_deltaN_rk4_steps(run, p.dN, nsteps));
         }
-        CHECK(run, ie_deltaN_steps(run, p.dN, nsteps));
         if (log) fprintf(log, "N = %f\n\n", Ne);
         // saveN (src/output.cpp:1522-1560)
-        CHECK(run, ie_evolve_fieldsN(run, -.5 * p.dN));
-        const double cutoff = p.Nend - p.dN;
+        if (staggered) CHECK(run, ie_evolve_fieldsN(run, -.5 * p.dN));
+        const double cutoff = p.Nend - (staggered ? p.dN : 0.0);
         float nmean = 0.f;
         CHECK(run, ie_deltaN_zeta(run, cutoff, &nmean));
         b = bg();
@@ -549,8 +625,10 @@ struct Driver {
             CHECK(run, ie_spectrum(run, IE_FIELD_DELTAN, norm1, pp.data(), cc.data(), vv.data()));
             spectra_block(out.get("spectraN"), pp, cc, vv);
         }
-        CHECK(run, ie_evolve_fieldsN(run, .5 * p.dN));
-        CHECK(run, ie_evolve_fieldsN(run, -0.5 * p.dN));
+        if (staggered) {
+            CHECK(run, ie_evolve_fieldsN(run, .5 * p.dN));
+            CHECK(run, ie_evolve_fieldsN(run, -0.5 * p.dN));
+        }
     }
 
     int main(const char *params_path)

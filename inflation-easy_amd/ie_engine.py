@@ -86,6 +86,20 @@ class Profile(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """Mirror of ``ie_tuning``."""
+
+    _fields_ = [("ncand", C.c_int), ("elems", C.c_int * 3), ("kick_ms", C.c_double * 3),
+                ("tensor_ms", C.c_double * 3), ("kick_elems", C.c_int), ("tensor_elems", C.c_int),
+                ("steps", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        n = self.ncand
+        return {"candidates": list(self.elems[:n]), "kick_ms": list(self.kick_ms[:n]),
+                "tensor_ms": list(self.tensor_ms[:n]), "kick_elems": self.kick_elems,
+                "tensor_elems": self.tensor_elems, "steps": self.steps}
+
+
 class Background(C.Structure):
     """Mirror of ``ie_background``."""
 
@@ -135,6 +149,7 @@ def lib() -> C.CDLL:
             "ie_evolve_fields": (st, [P, C.c_double]),
             "ie_leapfrog_steps": (st, [P, C.c_int, C.POINTER(Background)]),
             "ie_profile_steps": (st, [P, C.c_int, C.POINTER(Profile)]),
+            "ie_tune": (st, [P, C.c_int, C.POINTER(Tuning)]),
             "ie_rk4_step": (st, [P, C.c_double]),
             "ie_rk45_step": (st, [P, C.c_double, C.c_double, ip, dp]),
             "ie_spectrum_numbins": (C.c_int, [C.c_int]),
@@ -145,6 +160,7 @@ def lib() -> C.CDLL:
             "ie_get_phiref": (st, [P, dp]),
             "ie_deltaN_steps": (st, [P, C.c_double, C.c_int64]),
             "ie_deltaN_rk4_steps": (st, [P, C.c_double, C.c_int64]),
+            "ie_deltaN_rk45_step": (st, [P, C.c_double, C.c_double, ip, dp]),
             "ie_deltaN_zeta": (st, [P, C.c_double, C.POINTER(C.c_float)]),
             "ie_deltaN_zero_unfinished": (st, [P, C.c_double]),
             "ie_deltaN_center": (st, [P]),
@@ -345,6 +361,11 @@ class Run:
         self._check(lib().ie_rk45_step(self.h, h, hmax, C.byref(acc), C.byref(hn)))
         return bool(acc.value), hn.value
 
+    def tune(self, steps_per_candidate: int = 8) -> dict:
+        t = Tuning()
+        self._check(lib().ie_tune(self.h, steps_per_candidate, C.byref(t)))
+        return t.as_dict()
+
     def profile_steps(self, n: int) -> dict:
         p = Profile()
         self._check(lib().ie_profile_steps(self.h, n, C.byref(p)))
@@ -382,6 +403,12 @@ class Run:
 
     def deltaN_rk4_steps(self, h: float, n: int):
         self._check(lib().ie_deltaN_rk4_steps(self.h, h, n))
+
+    def deltaN_rk45_step(self, h: float, hmax: float) -> tuple[bool, float]:
+        """One Dormand-Prince trial step of the deltaN system; (accepted, next step)."""
+        acc, hn = C.c_int(), C.c_double()
+        self._check(lib().ie_deltaN_rk45_step(self.h, h, hmax, C.byref(acc), C.byref(hn)))
+        return bool(acc.value), hn.value
 
     def deltaN_zeta(self, cutoff: float) -> float:
         m = C.c_float()

@@ -122,6 +122,43 @@ def test_deltaN_rk4_map_bitwise(case, Nend):
     assert run.background()["a"] == st.background()["a"]
 
 
+@pytest.mark.parametrize("case,Nend", [("tabulated_n16", 0.3), ("hilltop_n16", 0.0001)])
+def test_deltaN_rk45_matches_oracle(case, Nend):
+    """Dormand-Prince deltaN trial steps (src/evolution.cpp:1191-1267).  Both sides
+    take the oracle's step sequence, so the per-site maps agree bit for bit; the
+    accept decisions must agree and the proposed steps to 1e-10 (the error norm is
+    summed in another order)."""
+    import ctypes
+    st, table = end_of_inflation(case, Nend)
+    c = st.cfg
+    run = E.Run(E.make_config(c.N, numerical=c.numerical_potential, sigw=0, table=table, V0=c.V0, ns=c.ns,
+                              L=c.L, dt=c.dt, af=c.af, dN=c.dN, Nend=c.Nend, monotonic=c.monotonic), table)
+    run.upload_fields(st.field(0), st.field(1))
+    bg = st.background()
+    run.set_background(t=bg["t"], a=bg["a"], ad=bg["ad"], astep=bg["astep"], phiref=0.0)
+    O.lib().ieo_initializeN(st.ptr)
+    phiref = O.lib().ieo_get_phiref(st.ptr)
+    st.set_background(phiref=phiref)
+    run.deltaN_init()
+    assert run.get_phiref() == phiref
+    run.set_background(phiref=phiref)
+    h, hmax = c.dN, 8.0 * c.dN
+    accepted = 0
+    for _ in range(30):
+        hn = ctypes.c_double()
+        acc_o = O.lib().ieo_rk45_stepN(st.ptr, h, hmax, ctypes.byref(hn))
+        acc_g, hn_g = run.deltaN_rk45_step(h, hmax)
+        assert acc_g == bool(acc_o)
+        assert hn_g == pytest.approx(hn.value, rel=1e-10)
+        accepted += acc_g
+        h = hn.value
+    assert accepted > 0
+    assert np.array_equal(st.field(2), run.field(E.IE_FIELD_DELTAN))
+    assert np.array_equal(st.field(0), run.field(E.IE_FIELD_F))
+    assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
+    assert run.background()["a"] == st.background()["a"]
+
+
 def test_histogram_matches_reference_rule():
     st, table = end_of_inflation("hilltop_n16", 0.0002)
     c = st.cfg

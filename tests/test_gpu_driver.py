@@ -55,7 +55,9 @@ TEXT_FILES = ["means", "velocity", "variance", "sf", "energy", "conservation", "
               "histogramtimes", "modes"]
 
 
-@pytest.mark.parametrize("case", ["hilltop_n16", "tabulated_n16"])
+# the *_rk4 / *_rk45 cases run both the inflation and the deltaN loops with that
+# integrator (inflation_integrator, deltaN_integrator in params.txt)
+@pytest.mark.parametrize("case", ["hilltop_n16", "tabulated_n16", "tabulated_n16_rk4", "tabulated_n16_rk45"])
 def test_driver_matches_reference_files(case, tmp_path):
     g = run_driver(case, tmp_path)
     res = tmp_path / "results"

@@ -132,14 +132,17 @@ def test_spectrum_matches_oracle(case):
     assert np.allclose(v[1:], vo[1:], rtol=1e-8, atol=0)
 
 
-def test_sigw_tensors_and_gw_spectra_match_oracle():
-    st, _, table = oracle_state("hilltop_n16")
+# N = 16: site kernels; N = 64: row kernels (one site per lane)
+@pytest.mark.parametrize("N,nsteps", [(16, 100), (64, 30)])
+def test_sigw_tensors_and_gw_spectra_match_oracle(N, nsteps):
+    print(f"sigw parity N={N} steps={nsteps}", flush=True)
+    st, _, table = oracle_state("hilltop_n16", N=N)
     run = engine_for(st, table, sigw=1)
     dt = st.cfg.dt
     O.lib().ieo_evolve_fields(st.ptr, 0.5 * dt)
     run.evolve_fields(0.5 * dt)
-    O.lib().ieo_leapfrog_steps(st.ptr, 100)
-    run.leapfrog_steps(100)
+    O.lib().ieo_leapfrog_steps(st.ptr, nsteps)
+    run.leapfrog_steps(nsteps)
     for comp in (0, 3, 5, 6, 9, 11):
         ho, hg = st.tensor(comp).astype(np.float64), run.tensor(comp).astype(np.float64)
         scale = np.sqrt(np.mean(ho ** 2))
@@ -224,3 +227,26 @@ def test_segmented_rows_match_oracle_n512(precision):
     assert fluct_err(st.field(0), run.field(E.IE_FIELD_F)) < tol
     assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < tol
     run.close()
+
+
+def test_tuned_shapes_leave_results_unchanged():
+    """The tuner times real loop steps with 1 and 2 sites per lane (N = 128); the
+    kick and tensor kernels have no cross-site sums, so a run that changes shape
+    between steps matches one that never does, bit for bit."""
+    st, _, table = oracle_state("hilltop_n32", N=128, calculate_sigw=1)
+    ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
+    rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
+    dt = st.cfg.dt
+    for r in (ra, rb):
+        r.evolve_fields(0.5 * dt)
+    t = ra.tune(4)
+    assert t["candidates"] == [1, 2] and t["steps"] == 8
+    assert t["kick_elems"] in (1, 2) and t["tensor_elems"] in (1, 2)
+    assert all(x > 0 for x in t["kick_ms"] + t["tensor_ms"])
+    ra.leapfrog_steps(12)
+    rb.leapfrog_steps(20)
+    assert ra.background()["steps"] == rb.background()["steps"]
+    assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
+    assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
+    for comp in range(12):
+        assert np.array_equal(ra.tensor(comp), rb.tensor(comp)), comp
