@@ -509,7 +509,8 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
     // first long loop of a kind: tune the kick / tensor shapes on its first steps
     int ke = 0, te = 0;
     tuned_shapes(r->N, split, &ke, &te);
-    if (max_steps >= 256 && row_elems(r->N) >= 2 && (ke == 0 || (r->F.sigw && te == 0))) {
+    const bool untuned = march_ok(r->N) ? (r->F.sigw && te == 0) : (ke == 0 || (r->F.sigw && te == 0));
+    if (max_steps >= 256 && row_elems(r->N) >= 2 && untuned) {
         ie_tuning tun;
         ie_status st = ie_tune(r, 8, &tun);
         if (st != IE_OK) return st;
@@ -517,9 +518,16 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
     }
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
     IE_HIP(r, hipGetLastError());
-    for (int n = 0; n < max_steps; ++n) {
-        IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
-        IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+    if (march_ok(r->N)) {
+        // fused steps: one march launch per step computes the next step's sums
+        if (max_steps > 0) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
+        for (int n = 0; n < max_steps; ++n)
+            IE_HIP(r, launch_fused_step(r->F, split, r->P, r->bg, r->partials, r->stream));
+    } else {
+        for (int n = 0; n < max_steps; ++n) {
+            IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+            IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+        }
     }
     ie_status st = check_kernel_error(r);
     if (st != IE_OK) return st;
@@ -536,8 +544,23 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
     std::vector<hipEvent_t> ev((size_t)nsteps * nev);
     for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+    const bool fused = march_ok(r->N);
+    if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
     for (int n = 0; n < nsteps; ++n) {
         hipEvent_t *e = &ev[(size_t)n * nev];
+        if (fused) {
+            // the march kernel is the kick, the drift and the next step's sums
+            IE_HIP(r, hipEventRecord(e[0], r->stream));
+            IE_HIP(r, hipEventRecord(e[1], r->stream));
+            IE_HIP(r, hipEventRecord(e[2], r->stream));
+            IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, r->stream));
+            IE_HIP(r, hipEventRecord(e[3], r->stream));
+            if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, 1));
+            IE_HIP(r, hipEventRecord(e[4], r->stream));
+            IE_HIP(r, launch_step_close(r->F, split, r->bg, r->partials, r->stream));
+            IE_HIP(r, hipEventRecord(e[5], r->stream));
+            continue;
+        }
         IE_HIP(r, hipEventRecord(e[0], r->stream));
         IE_HIP(r, launch_reduce(r->F, split, r->P, r->bg, r->partials, 1, r->stream));
         IE_HIP(r, hipEventRecord(e[1], r->stream));
@@ -598,17 +621,23 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     std::vector<hipEvent_t> ev((size_t)nc * K * 4);
     for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+    // fused steps (64 <= N <= 256): the march kernel has one shape; the tensor's is tuned
+    const bool fused = march_ok(r->N);
+    if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
     for (int c = 0; c < nc; ++c) {
         r->F.kick_e = r->F.tensor_e = out->elems[c];
         for (int q = 0; q < K; ++q) {
             hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
-            IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+            if (!fused) IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[0], r->stream));
-            IE_HIP(r, launch_scalar_step(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+            if (fused) IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, r->stream));
+            else IE_HIP(r, launch_scalar_step(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[1], r->stream));
-            if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+            if (r->F.sigw)
+                IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, fused ? 1 : 0));
             IE_HIP(r, hipEventRecord(e[2], r->stream));
-            IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
+            if (fused) IE_HIP(r, launch_step_close(r->F, split, r->bg, r->partials, r->stream));
+            else IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[3], r->stream));
         }
     }
@@ -632,6 +661,7 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     for (auto &e : ev) (void)hipEventDestroy(e);
     if ((st = check_kernel_error(r)) != IE_OK) return st;
     out->steps = r->bg_host->steps - steps0;
+    if (fused) bk = nc - 1;   // the march kernel's one shape (row_elems), whatever was asked
     out->kick_elems = out->elems[bk];
     out->tensor_elems = r->F.sigw ? out->elems[bt] : def;
     r->F.kick_e = out->kick_elems;

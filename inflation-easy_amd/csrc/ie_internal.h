@@ -227,7 +227,19 @@ hipError_t launch_friedmann_finish(const Fields &F, DevBg *bg, const double *par
 // Kick / drift / fused kick+drift of f, fd; of h, hd (tensor step).
 hipError_t launch_scalar_step(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg, int loop,
                               hipStream_t s);
-hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s);
+hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s,
+                              int fresh = 0);
+// Fused loop steps for 64 <= N <= 256: the march kernel kicks, drifts and sums the
+// new f for the next step's Friedmann update; the close kernel applies the
+// step's scalar updates.  A loop starts with launch_march_begin (a reduce of the
+// current f into bg->red); then launch_fused_step per step (march, tensor step
+// with the same scalars when sigw, close).
+bool march_ok(int N);
+unsigned march_blocks(int N);
+hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s);
+hipError_t launch_march(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s);
+hipError_t launch_step_close(const Fields &F, int split, DevBg *bg, const double *partials, hipStream_t s);
+hipError_t launch_fused_step(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s);
 // One thread: the scalar updates that close a kick, a drift or a loop step.
 hipError_t launch_step_finish(DevBg *bg, int split, int mode, double d_arg, int loop, hipStream_t s);
 // launch_reduce + launch_friedmann_finish.

@@ -115,8 +115,10 @@ struct RunDesc {
     Pot P;
     DevBg *bg;
     double *partials;
-    unsigned G;
+    unsigned G;        // workgroups of the row / site kernels
+    unsigned Gm;       // workgroups of the fused march kernel (0: not fused)
     int split;
+    int march;         // loop steps fused (march kernel + close)
 };
 
 struct Nb {
@@ -213,6 +215,87 @@ __device__ void friedmann_finish_body(DevBg *bg, const double *partials, unsigne
     bg->vpmean = tot[2] / gsd;
 }
 
+// Scalars of one loop step as a pure function of the background between steps
+// and the sums of the current f in bg->red (run_evolution_loop's step
+// dt astep^{s-1}, the implicit Friedmann half-kick of evolve_derivs, the kick
+// coefficients, and for split storage the homogeneous mode's kick and drift;
+// src/evolution.cpp:153-181, 256-283, 938).  Every kernel of a fused step
+// evaluates it and gets the same bits: no contraction, same operations.
+struct StepScalars {
+    double d, ad2, ad_half, ad_next, friction, potnorm, laplnorm, srcamp, vpmean, pibar_next, phibar_next;
+};
+
+#pragma clang fp contract(off)
+__device__ __forceinline__ StepScalars step_scalars(const DevBg *bg, double gsd)
+{
+    StepScalars S;
+    const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
+    const double d = bg->dt * pow(bg->astep, s - 1.0);
+    const double Ge = 0.5 * bg->red[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
+    const double Pm = bg->red[1] / gsd;
+    const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
+    S.d = d;
+    S.ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
+             (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a + d * d * sfev1 * pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) /
+            d;
+    S.ad_half = ad + 0.5 * d * S.ad2;
+    S.ad_next = S.ad_half + 0.5 * d * S.ad2;
+    S.friction = (2.0 + s) * S.ad_half / a;
+    S.potnorm = pow(a, 2.0 - 2.0 * s);
+    S.laplnorm = pow(a, -2.0 * s) / (dx * dx);
+    S.srcamp = 2.0 * pow(a, -2.0 * s);
+    S.vpmean = bg->red[2] / gsd;
+    S.pibar_next = bg->pibar + d * (-S.friction * bg->pibar - S.potnorm * S.vpmean);
+    S.phibar_next = bg->phibar + d * S.pibar_next;
+    return S;
+}
+#pragma clang fp contract(fast)
+
+// Closes a fused loop step: applies the step's scalar updates (the same values
+// its kernels used) and stores the fixed-order sums of the new f for the next
+// step.  One workgroup.
+__device__ void step_close_body(DevBg *bg, const double *partials, unsigned G, size_t gs, int split, double *smem)
+{
+    if (bg->done) return;
+    double tot[3];
+    sum_partials<3>(partials, (int)G, tot, smem);
+    if (threadIdx.x != 0) return;
+    const StepScalars S = step_scalars(bg, (double)gs);
+    bg->d = S.d;
+    bg->ad2 = S.ad2;
+    bg->friction = S.friction;
+    bg->potnorm = S.potnorm;
+    bg->laplnorm = S.laplnorm;
+    bg->srcamp = S.srcamp;
+    bg->vpmean = S.vpmean;
+    bg->ad = S.ad_next;
+    bg->t += S.d;
+    if (split) {
+        bg->pibar = S.pibar_next;
+        bg->phibar = S.phibar_next;
+    }
+    bg->a += S.d * bg->ad;
+    bg->parity ^= 1;
+    bg->steps += 1;
+    bg->astep = bg->a;
+    if (!(bg->a <= bg->af)) bg->done = 1;
+    bg->red[0] = tot[0];
+    bg->red[1] = tot[1];
+    bg->red[2] = tot[2];
+}
+
+// Sums of the current f into bg->red at the start of a fused loop (from a reduce).
+__device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, double *smem)
+{
+    if (bg->done) return;
+    double tot[3];
+    sum_partials<3>(partials, (int)G, tot, smem);
+    if (threadIdx.x != 0) return;
+    bg->red[0] = tot[0];
+    bg->red[1] = tot[1];
+    bg->red[2] = tot[2];
+}
+
 __global__ __launch_bounds__(kBlock) void k_friedmann_finish(DevBg *bg, const double *partials, unsigned G,
                                                               unsigned long long gs, double d_arg, int loop)
 {
@@ -224,6 +307,7 @@ __global__ __launch_bounds__(kBlock) void kb_friedmann_finish(const RunDesc *__r
 {
     __shared__ double smem[3 * kWaves];
     const RunDesc &R = runs[blockIdx.x];
+    if (R.march) return;
     const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
     friedmann_finish_body(R.bg, R.partials, R.G, gs, 0.0, 1, smem);
 }
@@ -239,7 +323,7 @@ __global__ void k_step_finish(DevBg *bg, int mode, int split, double d_arg, int 
 __global__ void kb_step_finish(const RunDesc *__restrict__ runs, int nruns)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nruns) return;
+    if (r >= nruns || runs[r].march) return;
     DevBg *bg = runs[r].bg;
     if (bg->done) return;
     finalize_scalars(bg, KICKDRIFT, runs[r].split);
@@ -433,14 +517,22 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, Dev
 // j-window of f rows in registers), the FP32 Laplacian of each component from its
 // five rows and two lane shuffles, the right-hand side in FP64.
 template <typename T, int E, int MODE>
-__device__ __forceinline__ void row_tensor_body(const Fields &F, DevBg *bg, int loop, unsigned G, unsigned blk)
+__device__ __forceinline__ void row_tensor_body(const Fields &F, DevBg *bg, int loop, int fresh, unsigned G,
+                                                unsigned blk)
 {
     if (loop && bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
-    const double d = bg->d;
-    const double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    // fused loop steps: the step's scalars from the background (as the march kernel)
+    double d = bg->d, laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    if (fresh) {
+        const StepScalars S = step_scalars(bg, (double)N * N * N);
+        d = S.d;
+        laplnorm = S.laplnorm;
+        friction = S.friction;
+        srcamp = S.srcamp;
+    }
     const double hx = 0.5 / bg->dx;
     const RowWalk w = row_walk(N, G, blk, E);
     const unsigned poff = (w.i << (2 * logN)) + w.k0;
@@ -509,9 +601,9 @@ __device__ __forceinline__ void row_tensor_body(const Fields &F, DevBg *bg, int 
 }
 
 template <typename T, int E, int MODE>
-__global__ __launch_bounds__(kBlock) void k_row_tensor(Fields F, DevBg *bg, int loop)
+__global__ __launch_bounds__(kBlock) void k_row_tensor(Fields F, DevBg *bg, int loop, int fresh)
 {
-    row_tensor_body<T, E, MODE>(F, bg, loop, gridDim.x, blockIdx.x);
+    row_tensor_body<T, E, MODE>(F, bg, loop, fresh, gridDim.x, blockIdx.x);
 }
 
 template <typename T, int E>
@@ -520,7 +612,7 @@ __global__ __launch_bounds__(kBlock) void kb_row_tensor(const RunDesc *__restric
 {
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.G, mp.y);
+    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.march, R.G, mp.y);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -552,6 +644,183 @@ __global__ __launch_bounds__(kBlock) void kb_row_kick(const RunDesc *__restrict_
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
     row_kick_body<T, SPLIT, E, KICKDRIFT>(R.F, R.P, R.bg, 1, R.G, mp.y);
+}
+
+// ------------------------------------------------------ fused march kernel
+//
+// One loop step of the scalar sector for 64 <= N <= 256 in one pass that also
+// produces the next step's Friedmann sums.  A workgroup of 9 waves owns a tile of
+// 8 j rows x the whole k row and marches along i over TI planes.  At each plane
+// every wave kicks and drifts its row (the row kernel's arithmetic), stores it
+// (waves 0-7 only), and puts the new row in LDS.  The sums of the new f follow
+// from  -sum_x f lapl f = sum_x sum_d (f(x + e_d) - f(x))^2  on the periodic
+// lattice: the k pair from a lane shuffle, the j pair from the next wave's LDS
+// row, the i pair from the wave's previous plane.  Wave 8 recomputes the first
+// row of the next tile (the partner of row 7's j pair) and one extra plane step
+// gives the last i pair; both are stored by their owners, not here.  V and V' of
+// the new f (about the new homogeneous mode for split storage) are summed per
+// owned site.  Partials per workgroup, in a fixed order.
+constexpr int kTileJ = 8;
+constexpr int kMarchWaves = kTileJ + 1;
+constexpr int kMarchThreads = 64 * kMarchWaves;
+
+__device__ __forceinline__ unsigned march_ti(unsigned N) { return N >= 256 ? 16u : 8u; }
+
+template <typename T, bool SPLIT, int E>
+__device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned blk,
+                                           T (*lds)[kMarchWaves][64 * E], double *smem)
+{
+    if (bg->done) return;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
+    const unsigned w = threadIdx.x >> 6;
+    const StepScalars S = step_scalars(bg, (double)N * N * N);
+    const int par = bg->parity;
+    const T *__restrict__ f = (const T *)F.f[par];
+    T *__restrict__ fout = (T *)F.f[par ^ 1];
+    T *__restrict__ fd = (T *)F.fd;
+    const double d = S.d, laplnorm = S.laplnorm, friction = S.friction, potnorm = S.potnorm;
+    const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? S.vpmean : 0.0;
+    const double phibar_next = SPLIT ? S.phibar_next : 0.0;
+    // tile of this workgroup: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
+    const unsigned TI = march_ti(N), jblocks = N / kTileJ;
+    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
+    const unsigned i0 = xcd * (N / kXcd) + (lb / jblocks) * TI;
+    const unsigned j = ((lb % jblocks) * kTileJ + w) & m;
+    const bool owner = w < (unsigned)kTileJ;
+    const unsigned rj = j << logN, rjp = ((j + 1) & m) << logN, rjm = ((j - 1) & m) << logN;
+    T fdn[E], fcur[E], fup[E], fjp[E], fjm[E], vin[E], fprev[E];
+    ld_row<T, E>(f + (((i0 - 1) & m) << (2 * logN)) + rj, lane, fdn);
+    ld_row<T, E>(f + ((i0 & m) << (2 * logN)) + rj, lane, fcur);
+#pragma unroll
+    for (int e = 0; e < E; ++e) fprev[e] = (T)0;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    int err = 0;
+    RowWalk wk;   // k neighbours within the whole row (one segment)
+    wk.segs = 1;
+    wk.k0 = 0;
+    for (unsigned t = 0; t <= TI; ++t) {
+        const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
+        ld_row<T, E>(f + pu + rj, lane, fup);
+        ld_row<T, E>(f + pi + rjp, lane, fjp);
+        ld_row<T, E>(f + pi + rjm, lane, fjm);
+        ld_row<T, E>(fd + pi + rj, lane, vin);
+        T kp_last, km_first;
+        k_neighbours<T, E>(fcur, f + pi + rj, wk, N, lane, kp_last, km_first);
+        T vout[E], fnew[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const double c = (double)fcur[e];
+            const double kp = (double)(e < E - 1 ? fcur[e + 1 < E ? e + 1 : e] : kp_last);
+            const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : km_first);
+            const double lap = lapl_sites(kp, km, (double)fjp[e], (double)fjm[e], (double)fup[e], (double)fdn[e], c);
+            double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+            if (SPLIT) dV -= vpmean;
+            double v = (double)vin[e];
+            v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
+            vout[e] = (T)v;
+            fnew[e] = (T)(c + d * (double)vout[e]);
+        }
+        if (owner && t < TI) {
+            st_row<T, E>(fd + pi + rj, lane, vout);
+            st_row<T, E>(fout + pi + rj, lane, fnew);
+        }
+        T(*buf)[64 * E] = lds[t & 1];
+#pragma unroll
+        for (int e = 0; e < E; ++e) buf[w][lane * E + e] = fnew[e];
+        __syncthreads();
+        if (owner) {
+            if (t < TI) {
+                const T knext = __shfl(fnew[0], (int)((lane + 1) & 63));
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const double c = (double)fnew[e];
+                    const double dk = (double)(e < E - 1 ? fnew[e + 1 < E ? e + 1 : e] : knext) - c;
+                    const double dj = (double)buf[w + 1][lane * E + e] - c;
+                    acc0 += dk * dk + dj * dj;
+                    double V, dV;
+                    pot_eval_fast(P, SPLIT ? phibar_next + c : c, &V, &dV, &err);
+                    acc1 += V;
+                    acc2 += dV;
+                }
+            }
+            if (t >= 1) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const double di = (double)fnew[e] - (double)fprev[e];
+                    acc0 += di * di;
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            fprev[e] = fnew[e];
+            fdn[e] = fcur[e];
+            fcur[e] = fup[e];
+        }
+    }
+    flag_error(bg, err);
+    // fixed-order workgroup sum over the 9 waves
+    double acc[3] = {acc0, acc1, acc2};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0)
+        for (int q = 0; q < 3; ++q) smem[q * kMarchWaves + w] = acc[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) {
+            double tot = 0.0;
+            for (int ww = 0; ww < kMarchWaves; ++ww) tot += smem[q * kMarchWaves + ww];
+            partials[(size_t)blk * 3 + q] = tot;
+        }
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg *bg, double *partials)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem);
+}
+
+__global__ __launch_bounds__(kBlock) void k_step_close(DevBg *bg, const double *partials, unsigned G,
+                                                        unsigned long long gs, int split)
+{
+    __shared__ double smem[3 * kWaves];
+    step_close_body(bg, partials, G, (size_t)gs, split, smem);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sums_finish(DevBg *bg, const double *partials, unsigned G)
+{
+    __shared__ double smem[3 * kWaves];
+    sums_finish_body(bg, partials, G, smem);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) void kb_march(const RunDesc *__restrict__ runs,
+                                                           const uint2 *__restrict__ map)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_step_close(const RunDesc *__restrict__ runs)
+{
+    __shared__ double smem[3 * kWaves];
+    const RunDesc &R = runs[blockIdx.x];
+    if (!R.march) return;
+    const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
+    step_close_body(R.bg, R.partials, R.Gm, gs, R.split, smem);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restrict__ runs)
+{
+    __shared__ double smem[3 * kWaves];
+    const RunDesc &R = runs[blockIdx.x];
+    if (!R.march) return;
+    sums_finish_body(R.bg, R.partials, R.G, smem);
 }
 
 // ------------------------------------------------- small-lattice site kernels
@@ -714,15 +983,22 @@ __global__ __launch_bounds__(kBlock) void kb_site_kick(const RunDesc *__restrict
 // ------------------------------------------------------------ tensor kernel
 
 template <typename T, int MODE>
-__device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d_arg, int loop, unsigned G,
+__device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d_arg, int loop, int fresh, unsigned G,
                                             unsigned blk)
 {
     if (loop && bg->done) return;
     const size_t gs = (size_t)F.N * F.N * F.N;
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
-    const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
-    const double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
+    double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    if (fresh) {
+        const StepScalars S = step_scalars(bg, (double)gs);
+        d = S.d;
+        laplnorm = S.laplnorm;
+        friction = S.friction;
+        srcamp = S.srcamp;
+    }
     const double half_over_dx = 0.5 / bg->dx;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
     if (MODE == DRIFT) {
@@ -762,9 +1038,9 @@ __device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop)
+__global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d_arg, int loop, int fresh)
 {
-    tensor_body<T, MODE>(F, bg, d_arg, loop, gridDim.x, blockIdx.x);
+    tensor_body<T, MODE>(F, bg, d_arg, loop, fresh, gridDim.x, blockIdx.x);
 }
 
 template <typename T>
@@ -772,7 +1048,7 @@ __global__ __launch_bounds__(kBlock) void kb_tensor(const RunDesc *__restrict__ 
 {
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.G, mp.y);
+    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.march, R.G, mp.y);
 }
 
 // ------------------------------------------------------- output utilities
@@ -899,25 +1175,36 @@ void kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_
 }
 
 template <typename T, int MODE>
-void row_tensor_dispatch(const Fields &F, DevBg *bg, int loop, unsigned G, hipStream_t s)
+void row_tensor_dispatch(const Fields &F, DevBg *bg, int loop, int fresh, unsigned G, hipStream_t s)
 {
     switch (shape_elems(F.N, F.tensor_e)) {
-    case 1: k_row_tensor<T, 1, MODE><<<G, kBlock, 0, s>>>(F, bg, loop); break;
-    case 2: k_row_tensor<T, 2, MODE><<<G, kBlock, 0, s>>>(F, bg, loop); break;
-    default: k_row_tensor<T, 4, MODE><<<G, kBlock, 0, s>>>(F, bg, loop); break;
+    case 1: k_row_tensor<T, 1, MODE><<<G, kBlock, 0, s>>>(F, bg, loop, fresh); break;
+    case 2: k_row_tensor<T, 2, MODE><<<G, kBlock, 0, s>>>(F, bg, loop, fresh); break;
+    default: k_row_tensor<T, 4, MODE><<<G, kBlock, 0, s>>>(F, bg, loop, fresh); break;
     }
 }
 
 template <typename T>
-void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loop, unsigned G, hipStream_t s)
+void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loop, int fresh, unsigned G,
+                     hipStream_t s)
 {
-    if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+    if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, 0);
     else if (row_elems(F.N) == 0) {
-        if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
-        else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop);
+        if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, fresh);
+        else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, fresh);
     } else {
-        if (mode == KICK) row_tensor_dispatch<T, KICK>(F, bg, loop, G, s);
-        else row_tensor_dispatch<T, KICKDRIFT>(F, bg, loop, G, s);
+        if (mode == KICK) row_tensor_dispatch<T, KICK>(F, bg, loop, fresh, G, s);
+        else row_tensor_dispatch<T, KICKDRIFT>(F, bg, loop, fresh, G, s);
+    }
+}
+
+template <typename T, bool SPLIT>
+void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
+{
+    switch (row_elems(F.N)) {
+    case 1: k_march<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials); break;
+    case 2: k_march<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials); break;
+    default: k_march<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials); break;
     }
 }
 
@@ -985,12 +1272,48 @@ hipError_t launch_scalar_step(const Fields &F, int split, const Pot &P, DevBg *b
     return hipGetLastError();
 }
 
-hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s)
+hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s,
+                              int fresh)
 {
     const unsigned G = step_blocks(F.N);
-    if (split) tensor_dispatch<float>(F, bg, mode, d_arg, loop, G, s);
-    else tensor_dispatch<double>(F, bg, mode, d_arg, loop, G, s);
+    if (split) tensor_dispatch<float>(F, bg, mode, d_arg, loop, fresh, G, s);
+    else tensor_dispatch<double>(F, bg, mode, d_arg, loop, fresh, G, s);
     return hipGetLastError();
+}
+
+bool march_ok(int N) { return N >= 64 && N <= 256; }
+
+unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (N >= 256 ? 16 : 8))); }
+
+hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
+{
+    hipError_t e = launch_reduce(F, split, P, bg, partials, 1, s);
+    if (e != hipSuccess) return e;
+    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, step_blocks(F.N));
+    return hipGetLastError();
+}
+
+hipError_t launch_march(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
+{
+    const unsigned G = march_blocks(F.N);
+    if (split) march_dispatch<float, true>(F, P, bg, partials, G, s);
+    else march_dispatch<double, false>(F, P, bg, partials, G, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_close(const Fields &F, int split, DevBg *bg, const double *partials, hipStream_t s)
+{
+    const size_t gs = (size_t)F.N * F.N * F.N;
+    k_step_close<<<1, kBlock, 0, s>>>(bg, partials, march_blocks(F.N), (unsigned long long)gs, split);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_step(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
+{
+    hipError_t e = launch_march(F, split, P, bg, partials, s);
+    if (e == hipSuccess && F.sigw) e = launch_tensor_step(F, split, bg, KICKDRIFT, 0.0, 1, s, 1);
+    if (e == hipSuccess) e = launch_step_close(F, split, bg, partials, s);
+    return e;
 }
 
 hipError_t launch_step_finish(DevBg *bg, int split, int mode, double d_arg, int loop, hipStream_t s)
@@ -1052,14 +1375,17 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
 // launch per group and kernel, one finish launch over all runs.
 struct BatchGroup {
     int E = 0, kE = 0, tE = 0;   // elements per lane: reduce, kick, tensor
-    uint2 *map = nullptr, *tmap = nullptr;
-    int nblocks = 0, ntblocks = 0;
+    int march = 0;               // fused steps (march kernel) for this group
+    uint2 *map = nullptr;        // kick / march workgroups
+    uint2 *rmap = nullptr;       // reduce workgroups (march groups: loop start only)
+    uint2 *tmap = nullptr;       // tensor workgroups
+    int nblocks = 0, nrblocks = 0, ntblocks = 0;
 };
 
 struct BatchDev {
     RunDesc *runs = nullptr;
     std::vector<BatchGroup> groups;
-    int nruns = 0, split = 0;
+    int nruns = 0, split = 0, has_march = 0, has_plain = 0;
 };
 
 hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
@@ -1069,47 +1395,56 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     BatchDev *B = new BatchDev();
     B->nruns = n;
     B->split = split;
-    // groups of runs launched together: same kernel shapes (reduce, kick, tensor)
+    // groups of runs launched together: same step structure and kernel shapes
     std::vector<int> keys;
-    std::vector<std::vector<uint2>> maps, tmaps;
+    std::vector<std::vector<uint2>> maps, rmaps, tmaps;
     for (int r = 0; r < n; ++r) {
+        const int N = runs[r].F.N;
         d[r].F = runs[r].F;
         d[r].P = runs[r].P;
         d[r].bg = runs[r].bg;
         d[r].partials = runs[r].partials;
-        d[r].G = step_blocks(runs[r].F.N);
+        d[r].G = step_blocks(N);
+        d[r].march = march_ok(N) ? 1 : 0;
+        d[r].Gm = d[r].march ? march_blocks(N) : 0;
         d[r].split = split;
-        const int N = runs[r].F.N;
-        const int key = row_elems(N) * 100 + shape_elems(N, runs[r].F.kick_e) * 10 + shape_elems(N, runs[r].F.tensor_e);
+        (d[r].march ? B->has_march : B->has_plain) = 1;
+        const int kE = d[r].march ? row_elems(N) : shape_elems(N, runs[r].F.kick_e);
+        const int key = d[r].march * 1000 + row_elems(N) * 100 + kE * 10 + shape_elems(N, runs[r].F.tensor_e);
         size_t g = 0;
         while (g < keys.size() && keys[g] != key) ++g;
         if (g == keys.size()) {
             keys.push_back(key);
             maps.emplace_back();
+            rmaps.emplace_back();
             tmaps.emplace_back();
         }
         for (unsigned b = 0; b < d[r].G; ++b) {
-            maps[g].push_back(make_uint2((unsigned)r, b));
+            rmaps[g].push_back(make_uint2((unsigned)r, b));
             if (runs[r].F.sigw) tmaps[g].push_back(make_uint2((unsigned)r, b));
         }
+        const unsigned Gk = d[r].march ? d[r].Gm : d[r].G;
+        for (unsigned b = 0; b < Gk; ++b) maps[g].push_back(make_uint2((unsigned)r, b));
     }
     hipError_t e = hipMalloc(&B->runs, sizeof(RunDesc) * n);
     if (e == hipSuccess) e = hipMemcpy(B->runs, d.data(), sizeof(RunDesc) * n, hipMemcpyHostToDevice);
+    auto upload = [&](const std::vector<uint2> &v, uint2 **p) {
+        if (e != hipSuccess || v.empty()) return;
+        e = hipMalloc(p, sizeof(uint2) * v.size());
+        if (e == hipSuccess) e = hipMemcpy(*p, v.data(), sizeof(uint2) * v.size(), hipMemcpyHostToDevice);
+    };
     for (size_t g = 0; g < keys.size() && e == hipSuccess; ++g) {
         BatchGroup G;
-        G.E = keys[g] / 100;
+        G.march = keys[g] / 1000;
+        G.E = keys[g] / 100 % 10;
         G.kE = keys[g] / 10 % 10;
         G.tE = keys[g] % 10;
         G.nblocks = (int)maps[g].size();
+        G.nrblocks = (int)rmaps[g].size();
         G.ntblocks = (int)tmaps[g].size();
-        e = hipMalloc(&G.map, sizeof(uint2) * maps[g].size());
-        if (e == hipSuccess)
-            e = hipMemcpy(G.map, maps[g].data(), sizeof(uint2) * maps[g].size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess && G.ntblocks) {
-            e = hipMalloc(&G.tmap, sizeof(uint2) * tmaps[g].size());
-            if (e == hipSuccess)
-                e = hipMemcpy(G.tmap, tmaps[g].data(), sizeof(uint2) * tmaps[g].size(), hipMemcpyHostToDevice);
-        }
+        upload(maps[g], &G.map);
+        upload(rmaps[g], &G.rmap);
+        upload(tmaps[g], &G.tmap);
         B->groups.push_back(G);
     }
     if (e != hipSuccess) {
@@ -1126,14 +1461,27 @@ void batch_free(BatchDev *B)
     if (B->runs) (void)hipFree(B->runs);
     for (BatchGroup &G : B->groups) {
         if (G.map) (void)hipFree(G.map);
+        if (G.rmap) (void)hipFree(G.rmap);
         if (G.tmap) (void)hipFree(G.tmap);
     }
     delete B;
 }
 
+template <typename T, bool SPLIT>
+static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream_t s);
+
 hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
 {
     kb_loop_begin<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
+    if (B->has_march) {
+        // fused groups start from the sums of the current f
+        for (const BatchGroup &G : B->groups) {
+            if (!G.march) continue;
+            if (B->split) batch_group_reduce<float, true>(B, G, s);
+            else batch_group_reduce<double, false>(B, G, s);
+        }
+        kb_sums_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
+    }
     return hipGetLastError();
 }
 
@@ -1141,17 +1489,23 @@ template <typename T, bool SPLIT>
 static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
     switch (G.E) {
-    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
-    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
-    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
-    default: kb_site_reduce<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    default: kb_site_reduce<T, SPLIT><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
     }
 }
 
 template <typename T, bool SPLIT>
 static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
-    switch (G.kE) {
+    if (G.march) {
+        switch (G.E) {
+        case 1: kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map); break;
+        case 2: kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map); break;
+        default: kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map); break;
+        }
+    } else switch (G.kE) {
     case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
@@ -1168,16 +1522,20 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t
 
 hipError_t batch_step(const BatchDev *B, hipStream_t s)
 {
-    for (const BatchGroup &G : B->groups) {
-        if (B->split) batch_group_reduce<float, true>(B, G, s);
-        else batch_group_reduce<double, false>(B, G, s);
+    if (B->has_plain) {
+        for (const BatchGroup &G : B->groups) {
+            if (G.march) continue;
+            if (B->split) batch_group_reduce<float, true>(B, G, s);
+            else batch_group_reduce<double, false>(B, G, s);
+        }
+        kb_friedmann_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
     }
-    kb_friedmann_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
     for (const BatchGroup &G : B->groups) {
         if (B->split) batch_group_kick<float, true>(B, G, s);
         else batch_group_kick<double, false>(B, G, s);
     }
-    kb_step_finish<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
+    if (B->has_plain) kb_step_finish<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
+    if (B->has_march) kb_step_close<<<B->nruns, kBlock, 0, s>>>(B->runs);
     return hipGetLastError();
 }
 

@@ -55,8 +55,8 @@ def test_alone_equals_in_mix_bitwise():
 
 
 def test_graph_replay_equals_direct_launches():
-    r1 = [new_run(32, 21), new_run(16, 22)]
-    r2 = [new_run(32, 21), new_run(16, 22)]
+    r1 = [new_run(32, 21), new_run(16, 22), new_run(64, 23, sigw=1)]
+    r2 = [new_run(32, 21), new_run(16, 22), new_run(64, 23, sigw=1)]
     b1, b2 = E.Batch(r1), E.Batch(r2)
     for _ in range(4):   # several replays with host work in between
         b1.steps(25, graph_steps=25)
@@ -67,8 +67,8 @@ def test_graph_replay_equals_direct_launches():
 
 
 def test_batch_matches_single_run_path():
-    rb = [new_run(32, 31, precision=E.IE_FP64), new_run(32, 32, precision=E.IE_FP64)]
-    rs = [new_run(32, 31, precision=E.IE_FP64), new_run(32, 32, precision=E.IE_FP64)]
+    rb = [new_run(32, 31, precision=E.IE_FP64), new_run(64, 32, precision=E.IE_FP64)]
+    rs = [new_run(32, 31, precision=E.IE_FP64), new_run(64, 32, precision=E.IE_FP64)]
     E.Batch(rb).steps(100, graph_steps=50)
     for r in rs:
         r.leapfrog_steps(100)
@@ -85,8 +85,13 @@ def test_batch_stops_each_run_at_its_af():
     short = E.Run(cfg_short)
     short.initialize(41, 0.0935, 0.000796)
     short.evolve_fields(0.5 * HILL["dt"])
+    cfg_short64 = E.make_config(64, precision=E.IE_SPLIT, af=1.005, **HILL)
+    short64 = E.Run(cfg_short64)
+    short64.initialize(43, 0.0935, 0.000796)
+    short64.evolve_fields(0.5 * HILL["dt"])
     long_ = new_run(16, 42)
-    E.Batch([short, long_]).steps(40, graph_steps=10)
-    bs, bl = short.background(), long_.background()
+    E.Batch([short, long_, short64]).steps(40, graph_steps=10)
+    bs, bl, b64 = short.background(), long_.background(), short64.background()
     assert bs["done"] == 1 and bs["steps"] < 40 and bs["a"] > 1.005
+    assert b64["done"] == 1 and b64["steps"] == bs["steps"] and b64["a"] > 1.005
     assert bl["done"] == 0 and bl["steps"] == 40

@@ -105,8 +105,10 @@ def test_explicit_kick_and_drift_match_oracle():
     assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < 1e-10
 
 
-def test_loop_stops_at_af():
-    st, _, table = oracle_state("hilltop_n16", calculate_sigw=0, af=1.01)
+# N = 64: fused steps (march + close kernels) stop in the close kernel
+@pytest.mark.parametrize("N", [16, 64])
+def test_loop_stops_at_af(N):
+    st, _, table = oracle_state("hilltop_n16", N=N, calculate_sigw=0, af=1.01)
     run = engine_for(st, table, sigw=0)
     run.evolve_fields(0.5 * st.cfg.dt)
     O.lib().ieo_evolve_fields(st.ptr, 0.5 * st.cfg.dt)
