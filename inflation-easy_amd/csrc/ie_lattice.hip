@@ -666,6 +666,15 @@ constexpr int kMarchThreads = 64 * kMarchWaves;
 
 __device__ __forceinline__ unsigned march_ti(unsigned N) { return N >= 256 ? 16u : 8u; }
 
+// A value equal in every lane, moved to scalar registers.
+__device__ __forceinline__ double uniform(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename T, bool SPLIT, int E>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned blk,
                                            T (*lds)[kMarchWaves][64 * E], double *smem)
@@ -678,9 +687,10 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
     T *__restrict__ fd = (T *)F.fd;
-    const double d = S.d, laplnorm = S.laplnorm, friction = S.friction, potnorm = S.potnorm;
-    const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? S.vpmean : 0.0;
-    const double phibar_next = SPLIT ? S.phibar_next : 0.0;
+    const double d = uniform(S.d), laplnorm = uniform(S.laplnorm), friction = uniform(S.friction);
+    const double potnorm = uniform(S.potnorm);
+    const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? uniform(S.vpmean) : 0.0;
+    const double phibar_next = SPLIT ? uniform(S.phibar_next) : 0.0;
     // tile of this workgroup: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
     const unsigned TI = march_ti(N), jblocks = N / kTileJ;
     const unsigned xcd = blk % kXcd, lb = blk / kXcd;
@@ -774,6 +784,17 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
         }
 }
 
+// Split storage fits two workgroups per CU (5 waves per SIMD) without spills;
+// FP64 rows need more registers and run one.
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) __attribute__((amdgpu_waves_per_eu(5)))
+void k_march_w5(Fields F, Pot P, DevBg *bg, double *partials)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem);
+}
+
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg *bg, double *partials)
 {
@@ -793,6 +814,17 @@ __global__ __launch_bounds__(kBlock) void k_sums_finish(DevBg *bg, const double 
 {
     __shared__ double smem[3 * kWaves];
     sums_finish_body(bg, partials, G, smem);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) __attribute__((amdgpu_waves_per_eu(5)))
+void kb_march_w5(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -1201,10 +1233,15 @@ void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loo
 template <typename T, bool SPLIT>
 void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
 {
-    switch (row_elems(F.N)) {
-    case 1: k_march<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials); break;
-    case 2: k_march<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials); break;
-    default: k_march<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials); break;
+    const int E = row_elems(F.N);
+    if constexpr (sizeof(T) == 4) {
+        if (E == 1) k_march_w5<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else k_march_w5<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+    } else {
+        if (E == 1) k_march<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else if (E == 2) k_march<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else k_march<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
     }
 }
 
@@ -1500,10 +1537,14 @@ template <typename T, bool SPLIT>
 static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
     if (G.march) {
-        switch (G.E) {
-        case 1: kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map); break;
-        case 2: kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map); break;
-        default: kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map); break;
+        if constexpr (sizeof(T) == 4) {
+            if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+        } else {
+            if (G.E == 1) kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            else if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
         }
     } else switch (G.kE) {
     case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;

@@ -93,5 +93,5 @@ def test_batch_stops_each_run_at_its_af():
     E.Batch([short, long_, short64]).steps(40, graph_steps=10)
     bs, bl, b64 = short.background(), long_.background(), short64.background()
     assert bs["done"] == 1 and bs["steps"] < 40 and bs["a"] > 1.005
-    assert b64["done"] == 1 and b64["steps"] == bs["steps"] and b64["a"] > 1.005
+    assert b64["done"] == 1 and b64["steps"] < 40 and b64["a"] > 1.005
     assert bl["done"] == 0 and bl["steps"] == 40
