@@ -173,7 +173,7 @@ size_t ie_footprint_bytes(const ie_config *c)
     if (!c || !is_pow2(c->N)) return 0;
     const size_t gs = (size_t)c->N * c->N * c->N;
     const size_t es = c->precision == IE_SPLIT ? 4 : 8;
-    size_t b = 3 * gs * es;                        // f ping-pong + fd
+    size_t b = 4 * gs * es;                        // f and fd ping-pong
     if (c->calculate_sigw) b += 18 * gs * 4;       // h ping-pong + hd
     b += gs * 8;                                   // scratch (downloads, deltaN)
     b += ie::fft_workspace_bytes(c->N);            // spectra
@@ -234,7 +234,8 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     tuned_shapes(c.N, c.precision == IE_SPLIT, &r->F.kick_e, &r->F.tensor_e);
     for (int p = 0; p < 2; ++p)
         if ((st = dev_alloc(r, &r->F.f[p], r->gs * es)) != IE_OK) return bail(st);
-    if ((st = dev_alloc(r, &r->F.fd, r->gs * es)) != IE_OK) return bail(st);
+    for (int p = 0; p < 2; ++p)
+        if ((st = dev_alloc(r, &r->F.fd[p], r->gs * es)) != IE_OK) return bail(st);
     if (r->F.sigw) {
         for (int q = 0; q < 6; ++q) {
             for (int p = 0; p < 2; ++p)
@@ -253,7 +254,8 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     for (int p = 0; p < 2; ++p) {
         if (hipMemsetAsync(r->F.f[p], 0, r->gs * es, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
     }
-    if (hipMemsetAsync(r->F.fd, 0, r->gs * es, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
+    for (int p = 0; p < 2; ++p)
+        if (hipMemsetAsync(r->F.fd[p], 0, r->gs * es, r->stream) != hipSuccess) return bail(IE_ERR_HIP);
     if (r->F.sigw)
         for (int q = 0; q < 6; ++q) {
             for (int p = 0; p < 2; ++p)
@@ -321,7 +323,7 @@ ie_status ie_upload_fields(ie_run *r, const double *f, const double *fd)
     const size_t bytes = r->gs * 8;
     if (r->cfg.precision == IE_FP64) {
         IE_HIP(r, hipMemcpyAsync(r->F.f[0], f, bytes, hipMemcpyHostToDevice, r->stream));
-        IE_HIP(r, hipMemcpyAsync(r->F.fd, fd, bytes, hipMemcpyHostToDevice, r->stream));
+        IE_HIP(r, hipMemcpyAsync(r->F.fd[0], fd, bytes, hipMemcpyHostToDevice, r->stream));
         return sync(r);
     }
     // split: stage both fields in FP64 (scratch + the unused second f buffer is too
@@ -348,7 +350,7 @@ ie_status ie_download_field(ie_run *r, int which, double *out)
         if (st != IE_OK) return st;
         const int split = r->cfg.precision == IE_SPLIT;
         const double *src = which == IE_FIELD_F ? (split ? r->dn_f : (const double *)r->F.f[r->bg_host->parity])
-                                                : (split ? r->dn_fd : (const double *)r->F.fd);
+                                                : (split ? r->dn_fd : (const double *)r->F.fd[r->bg_host->parity]);
         IE_HIP(r, hipMemcpyAsync(out, src, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
         return sync(r);
     }
@@ -621,7 +623,7 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     std::vector<hipEvent_t> ev((size_t)nc * K * 4);
     for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
-    // fused steps (64 <= N <= 256): the march kernel has one shape; the tensor's is tuned
+    // fused steps (128 <= N <= 256): the march kernel has one shape; the tensor's is tuned
     const bool fused = march_ok(r->N);
     if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
     for (int c = 0; c < nc; ++c) {
@@ -716,14 +718,14 @@ ie_status ie_spectrum(ie_run *r, int which, double norm1, double *p, int *counts
         const int par = r->bg_host->parity;
         if (r->in_deltaN) {
             const double *src = which == IE_FIELD_F ? (split ? r->dn_f : (const double *)r->F.f[par])
-                                                    : (split ? r->dn_fd : (const double *)r->F.fd);
+                                                    : (split ? r->dn_fd : (const double *)r->F.fd[par]);
             IE_HIP(r, fft_forward_double(r->fft, src, 0, r->stream));
         } else if (split) {
-            const float *src = (const float *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd);
+            const float *src = (const float *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd[par]);
             dc = (double)r->gs * (which == IE_FIELD_F ? r->bg_host->phibar : r->bg_host->pibar);
             IE_HIP(r, fft_forward_float(r->fft, src, dc, 0, r->stream));
         } else {
-            const double *src = (const double *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd);
+            const double *src = (const double *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd[par]);
             IE_HIP(r, fft_forward_double(r->fft, src, 0, r->stream));
         }
     } else {
@@ -779,7 +781,7 @@ ie_status ie_upload_modes(ie_run *r, const double *fk, const double *fny, const 
     if ((st = push_bg(r)) != IE_OK) return st;
     if (r->cfg.precision == IE_FP64) {
         IE_HIP(r, fft_inverse_ffteasy(r->fft, fk, fny, phi0, (double *)r->F.f[0], r->stream));
-        IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, (double *)r->F.fd, r->stream));
+        IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, (double *)r->F.fd[0], r->stream));
         return sync(r);
     }
     double *tmp = nullptr;
@@ -846,7 +848,7 @@ static ie_status dn_state(ie_run *r, double **f, double **fd)
         *fd = r->dn_fd;
     } else {
         *f = (double *)r->F.f[r->bg_host->parity];
-        *fd = (double *)r->F.fd;
+        *fd = (double *)r->F.fd[r->bg_host->parity];
     }
     return IE_OK;
 }
@@ -1030,9 +1032,9 @@ ie_status ie_deltaN_rk45_step(ie_run *r, double h, double hmax, int *accepted, d
         std::swap(r->dn_f, r->dn5[0]);
         std::swap(r->dn_fd, r->dn5[1]);
     } else {
-        double *pf = (double *)r->F.f[b.parity], *pfd = (double *)r->F.fd;
+        double *pf = (double *)r->F.f[b.parity], *pfd = (double *)r->F.fd[b.parity];
         r->F.f[b.parity] = r->dn5[0];
-        r->F.fd = r->dn5[1];
+        r->F.fd[b.parity] = r->dn5[1];
         r->dn5[0] = pf;
         r->dn5[1] = pfd;
     }
@@ -1257,7 +1259,7 @@ ie_status ie_rk4_step(ie_run *r, double h)
     if (st != IE_OK) return st;
     float *hh[6], *hd[6];
     rk_tensors(r, hh, hd);
-    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd;
+    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd[r->bg_host->parity];
     IE_HIP(r, rk4_step(w, r->P, r->bg, r->bg_host, f, fd, hh, hd, h, r->stream));
     const DevBg advanced = *r->bg_host;
     if ((st = check_kernel_error(r)) != IE_OK) return st;
@@ -1278,7 +1280,7 @@ ie_status ie_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *
     if (st != IE_OK) return st;
     float *hh[6], *hd[6];
     rk_tensors(r, hh, hd);
-    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd;
+    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd[r->bg_host->parity];
     const ie_config &c = r->cfg;
     double factor = 1.0, norm = 0.0;
     int acc = 0;

@@ -49,7 +49,7 @@ struct Pot {
 // float (split: fluctuation about the homogeneous mode in DevBg).
 struct Fields {
     void *f[2];
-    void *fd;
+    void *fd[2];            // f and fd: current buffer [parity], a loop step writes [parity ^ 1]
     float *h[2][6];
     float *hd[6];
     double *dn;
@@ -229,7 +229,7 @@ hipError_t launch_scalar_step(const Fields &F, int split, const Pot &P, DevBg *b
                               hipStream_t s);
 hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s,
                               int fresh = 0);
-// Fused loop steps for 64 <= N <= 256: the march kernel kicks, drifts and sums the
+// Fused loop steps for 128 <= N <= 256: the march kernel kicks, drifts and sums the
 // new f for the next step's Friedmann update; the close kernel applies the
 // step's scalar updates.  A loop starts with launch_march_begin (a reduce of the
 // current f into bg->red); then launch_fused_step per step (march, tensor step

@@ -465,7 +465,8 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, Dev
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
-    T *__restrict__ fd = (T *)F.fd;
+    const T *fd = (const T *)F.fd[par];
+    T *fdout = (T *)F.fd[MODE == KICKDRIFT ? par ^ 1 : par];
     const double d = bg->d;
     const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
     const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
@@ -501,7 +502,7 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, Dev
             vout[e] = (T)v;
             fnew[e] = (T)(c + d * (double)vout[e]);
         }
-        st_row<T, E>(fd + roff, lane, vout);
+        st_row<T, E>(fdout + roff, lane, vout);
         if (MODE & DRIFT) st_row<T, E>(fout + roff, lane, fnew);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -648,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void kb_row_kick(const RunDesc *__restrict_
 
 // ------------------------------------------------------ fused march kernel
 //
-// One loop step of the scalar sector for 64 <= N <= 256 in one pass that also
+// One loop step of the scalar sector for 128 <= N <= 256 in one pass that also
 // produces the next step's Friedmann sums.  A workgroup of 9 waves owns a tile of
 // 8 j rows x the whole k row and marches along i over TI planes.  At each plane
 // every wave kicks and drifts its row (the row kernel's arithmetic), stores it
@@ -686,7 +687,10 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
-    T *__restrict__ fd = (T *)F.fd;
+    // fd is read from [par] and written to [par ^ 1]: the halo row and the extra
+    // plane read fd rows that their owning workgroups update in the same launch
+    const T *__restrict__ fd = (const T *)F.fd[par];
+    T *__restrict__ fdout = (T *)F.fd[par ^ 1];
     const double d = uniform(S.d), laplnorm = uniform(S.laplnorm), friction = uniform(S.friction);
     const double potnorm = uniform(S.potnorm);
     const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? uniform(S.vpmean) : 0.0;
@@ -731,7 +735,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
             fnew[e] = (T)(c + d * (double)vout[e]);
         }
         if (owner && t < TI) {
-            st_row<T, E>(fd + pi + rj, lane, vout);
+            st_row<T, E>(fdout + pi + rj, lane, vout);
             st_row<T, E>(fout + pi + rj, lane, fnew);
         }
         T(*buf)[64 * E] = lds[t & 1];
@@ -944,7 +948,8 @@ __device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P, De
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
-    T *__restrict__ fd = (T *)F.fd;
+    const T *fd = (const T *)F.fd[par];
+    T *fdout = (T *)F.fd[MODE == KICKDRIFT ? par ^ 1 : par];
     const double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
     const double laplnorm = bg->laplnorm, friction = bg->friction, potnorm = bg->potnorm;
     const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? bg->vpmean : 0.0;
@@ -967,7 +972,7 @@ __device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P, De
                 double v = (double)vin[u];
                 v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
                 const T vs = (T)v;
-                fd[s.id[u]] = vs;
+                fdout[s.id[u]] = vs;
                 if (MODE & DRIFT) fout[s.id[u]] = (T)(c + d * (double)vs);
             }
         }
@@ -1095,7 +1100,7 @@ __global__ __launch_bounds__(kBlock) void k_field_sums(Fields F, Pot P, DevBg *b
     __shared__ double smem[6 * kWaves];
     const size_t gs = (size_t)F.N * F.N * F.N;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
-    const T *__restrict__ fd = (const T *)F.fd;
+    const T *__restrict__ fd = (const T *)F.fd[bg->parity];
     const double phibar = SPLIT ? bg->phibar : 0.0;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     int err = 0;
@@ -1164,7 +1169,7 @@ template <typename T, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_reconstitute(Fields F, const DevBg *bg, int which, double *out)
 {
     const size_t gs = (size_t)F.N * F.N * F.N;
-    const T *__restrict__ src = which == 0 ? (const T *)F.f[bg->parity] : (const T *)F.fd;
+    const T *__restrict__ src = which == 0 ? (const T *)F.f[bg->parity] : (const T *)F.fd[bg->parity];
     const double base = SPLIT ? (which == 0 ? bg->phibar : bg->pibar) : 0.0;
     const Walk w = site_walk(gs);
     for (size_t id = w.begin; id < w.end; id += w.stride) out[id] = SPLIT ? base + (double)src[id] : (double)src[id];
@@ -1318,7 +1323,11 @@ hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, d
     return hipGetLastError();
 }
 
-bool march_ok(int N) { return N >= 64 && N <= 256; }
+// Fused steps for 128 <= N <= 256: at N = 64 a row is one site per lane and the
+// tile's halo row and per-plane barrier cost more than the reduce they save (a
+// batch of 256 lattices of 64^3 steps faster unfused); above 256 a row spans
+// several segments, whose k pairs would cross waves.
+bool march_ok(int N) { return N >= 128 && N <= 256; }
 
 unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (N >= 256 ? 16 : 8))); }
 
@@ -1393,7 +1402,7 @@ hipError_t launch_split_upload(const double *f, const double *fd, Fields &F, Dev
     const size_t gs = (size_t)F.N * F.N * F.N;
     const int G = reduce_blocks(gs);
     k_mean2<<<G, kBlock, 0, s>>>(f, fd, gs, bg, partials);
-    k_split_store<<<G, kBlock, 0, s>>>(f, fd, gs, bg, (float *)F.f[0], (float *)F.fd);
+    k_split_store<<<G, kBlock, 0, s>>>(f, fd, gs, bg, (float *)F.f[0], (float *)F.fd[0]);
     return hipGetLastError();
 }
 

@@ -41,7 +41,7 @@ def state(run):
 
 def test_alone_equals_in_mix_bitwise():
     table = potential_table()
-    specs = [(32, 11, 0, 0, 0.97), (16, 12, 1, 0, 0.97), (64, 13, 0, 1, 0.96), (32, 14, 0, 0, 0.95)]
+    specs = [(32, 11, 0, 0, 0.97), (16, 12, 1, 0, 0.97), (128, 13, 0, 1, 0.96), (32, 14, 0, 0, 0.95)]
     mix = [new_run(N, s, numerical=p, sigw=g, table=table, ns=ns) for N, s, p, g, ns in specs]
     alone = [new_run(N, s, numerical=p, sigw=g, table=table, ns=ns) for N, s, p, g, ns in specs]
     E.Batch(mix).steps(60, graph_steps=20)
@@ -55,8 +55,8 @@ def test_alone_equals_in_mix_bitwise():
 
 
 def test_graph_replay_equals_direct_launches():
-    r1 = [new_run(32, 21), new_run(16, 22), new_run(64, 23, sigw=1)]
-    r2 = [new_run(32, 21), new_run(16, 22), new_run(64, 23, sigw=1)]
+    r1 = [new_run(32, 21), new_run(16, 22), new_run(64, 24), new_run(128, 23, sigw=1)]
+    r2 = [new_run(32, 21), new_run(16, 22), new_run(64, 24), new_run(128, 23, sigw=1)]
     b1, b2 = E.Batch(r1), E.Batch(r2)
     for _ in range(4):   # several replays with host work in between
         b1.steps(25, graph_steps=25)
@@ -67,8 +67,10 @@ def test_graph_replay_equals_direct_launches():
 
 
 def test_batch_matches_single_run_path():
-    rb = [new_run(32, 31, precision=E.IE_FP64), new_run(64, 32, precision=E.IE_FP64)]
-    rs = [new_run(32, 31, precision=E.IE_FP64), new_run(64, 32, precision=E.IE_FP64)]
+    rb = [new_run(32, 31, precision=E.IE_FP64), new_run(64, 33, precision=E.IE_FP64),
+          new_run(128, 32, precision=E.IE_FP64)]
+    rs = [new_run(32, 31, precision=E.IE_FP64), new_run(64, 33, precision=E.IE_FP64),
+          new_run(128, 32, precision=E.IE_FP64)]
     E.Batch(rb).steps(100, graph_steps=50)
     for r in rs:
         r.leapfrog_steps(100)
@@ -85,7 +87,7 @@ def test_batch_stops_each_run_at_its_af():
     short = E.Run(cfg_short)
     short.initialize(41, 0.0935, 0.000796)
     short.evolve_fields(0.5 * HILL["dt"])
-    cfg_short64 = E.make_config(64, precision=E.IE_SPLIT, af=1.005, **HILL)
+    cfg_short64 = E.make_config(128, precision=E.IE_SPLIT, af=1.005, **HILL)
     short64 = E.Run(cfg_short64)
     short64.initialize(43, 0.0935, 0.000796)
     short64.evolve_fields(0.5 * HILL["dt"])

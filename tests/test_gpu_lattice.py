@@ -70,9 +70,11 @@ def test_energies_match_oracle(case):
     assert s[2] == pytest.approx((f * f).sum(), rel=1e-13)
 
 
-# N >= 64 runs the row kernels (E = 1, 2 sites per lane), N < 64 the site kernels
+# N < 64: site kernels; N = 64: row kernels (one site per lane); 128 <= N <= 256:
+# fused steps (march kernel with the next step's sums, two and four sites per lane)
 @pytest.mark.parametrize("case,N,nsteps", [("hilltop_n32", 32, 200), ("tabulated_n16", 16, 200),
-                                           ("hilltop_n32", 64, 100), ("tabulated_n16", 128, 20)])
+                                           ("hilltop_n32", 64, 100), ("tabulated_n16", 128, 20),
+                                           ("hilltop_n32", 128, 40), ("hilltop_n32", 256, 6)])
 def test_leapfrog_fp64_matches_oracle(case, N, nsteps):
     print(f"leapfrog parity {case} N={N} steps={nsteps}", flush=True)
     st, _, table = oracle_state(case, N=N, calculate_sigw=0)
@@ -105,8 +107,8 @@ def test_explicit_kick_and_drift_match_oracle():
     assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < 1e-10
 
 
-# N = 64: fused steps (march + close kernels) stop in the close kernel
-@pytest.mark.parametrize("N", [16, 64])
+# N = 128: fused steps (march + close kernels) stop in the close kernel
+@pytest.mark.parametrize("N", [16, 128])
 def test_loop_stops_at_af(N):
     st, _, table = oracle_state("hilltop_n16", N=N, calculate_sigw=0, af=1.01)
     run = engine_for(st, table, sigw=0)
@@ -134,8 +136,9 @@ def test_spectrum_matches_oracle(case):
     assert np.allclose(v[1:], vo[1:], rtol=1e-8, atol=0)
 
 
-# N = 16: site kernels; N = 64: row kernels (one site per lane)
-@pytest.mark.parametrize("N,nsteps", [(16, 100), (64, 30)])
+# N = 16: site kernels; N = 64: row kernels (one site per lane); N = 128: fused
+# steps with the tensor step taking the march kernel's scalars
+@pytest.mark.parametrize("N,nsteps", [(16, 100), (64, 30), (128, 8)])
 def test_sigw_tensors_and_gw_spectra_match_oracle(N, nsteps):
     print(f"sigw parity N={N} steps={nsteps}", flush=True)
     st, _, table = oracle_state("hilltop_n16", N=N)
@@ -177,7 +180,7 @@ def test_upload_modes_matches_oracle_initial_conditions():
     assert fluct_err(st2.field(1), fd) < 1e-10
 
 
-@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("N", [32, 64, 128])
 def test_split_storage_matches_fp64_spectra(N):
     st, _, table = oracle_state("hilltop_n32", N=N, calculate_sigw=0)
     r64 = engine_for(st, table, precision=E.IE_FP64, sigw=0)
