@@ -300,17 +300,21 @@ def main() -> int:
     out["autotune"] = {k: v["tuning"] for k, v in res.items()}
     prof = head["profile"]
     if prof:
-        # dominant kernel: fused kick + drift; algorithmic bytes per site: read dphi and
-        # dpi, write dpi and the next dphi (4 B each in split storage, 8 B in FP64)
+        # dominant kernel: the fused step kernel (k_march: kick, drift and the next
+        # step's Friedmann sums, 128 <= N <= 256) or the row kick; algorithmic bytes
+        # per site: read dphi and dpi, write dpi and the next dphi (4 B each in split
+        # storage, 8 B in FP64); the sums read nothing more
         per_site = 16 if head is res.get("split") else 32
         alg = per_site * sites
         ms = prof["kick_drift_ms"]
         achieved = alg / (ms * 1e-3) / 1e9
+        fused = 128 <= N <= 256
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("kick_drift_split_256"),
-                           "kernel": "k_row_kick (fused kick + drift)", "kernel_ms": ms,
+                           "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": read_traffic(f"{'march' if fused else 'kick_drift'}_split_{N}"),
+                           "kernel": "k_march_w5<float,true,4> (fused kick + drift + next-step sums)" if fused
+                           else "k_row_kick (kick + drift)", "kernel_ms": ms,
                            "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
-                           "reduce_frac": (alg / 4) / (prof["reduce_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
                            "step_ms_events": prof["step_ms"]}
     if args.ensemble_runs > 0 and not args.only:
         out["ensemble"] = ensemble(args.ensemble_runs)

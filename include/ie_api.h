@@ -148,8 +148,9 @@ ie_status ie_leapfrog_steps(ie_run *run, int max_steps, ie_background *out);
  * counterpart: the reference has wall-clock timing only (src/output.cpp:1451-1461). */
 typedef struct ie_profile {
     double step_ms;            /* whole step, first to last event */
-    double reduce_ms;          /* Friedmann reduction (gradient/potential sums) */
-    double kick_drift_ms;      /* scalar kick + drift */
+    double reduce_ms;          /* Friedmann reduction (gradient/potential sums); 0 for
+                                  fused steps (128 <= N <= 256), whose kick kernel sums */
+    double kick_drift_ms;      /* scalar kick + drift (fused steps: with the sums) */
     double tensor_ms;          /* tensor kick + drift (0 without SIGW) */
     int64_t steps;
 } ie_profile;

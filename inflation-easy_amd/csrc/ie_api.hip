@@ -551,10 +551,9 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
     for (int n = 0; n < nsteps; ++n) {
         hipEvent_t *e = &ev[(size_t)n * nev];
         if (fused) {
-            // the march kernel is the kick, the drift and the next step's sums
+            // the march kernel is the kick, the drift and the next step's sums;
+            // e[1], e[2] stay unrecorded (no reduce or Friedmann launch)
             IE_HIP(r, hipEventRecord(e[0], r->stream));
-            IE_HIP(r, hipEventRecord(e[1], r->stream));
-            IE_HIP(r, hipEventRecord(e[2], r->stream));
             IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, r->stream));
             IE_HIP(r, hipEventRecord(e[3], r->stream));
             if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, 1));
@@ -581,9 +580,11 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
     for (int n = 0; n < nsteps; ++n) {
         hipEvent_t *e = &ev[(size_t)n * nev];
         float ms = 0.f;
-        IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
-        red += ms;
-        IE_HIP(r, hipEventElapsedTime(&ms, e[2], e[3]));
+        if (!fused) {
+            IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
+            red += ms;
+        }
+        IE_HIP(r, hipEventElapsedTime(&ms, fused ? e[0] : e[2], e[3]));
         kd += ms;
         if (r->F.sigw) {
             IE_HIP(r, hipEventElapsedTime(&ms, e[3], e[4]));
