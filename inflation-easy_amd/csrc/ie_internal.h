@@ -216,6 +216,9 @@ unsigned step_blocks(int N);
 // asked for `want` sites per lane.
 int row_elems(int N);
 int shape_elems(int N, int want);
+// Sites per lane and workgroups of the tensor march kernel (loop steps, N >= 64).
+int tensor_march_elems(int N, int want);
+unsigned tensor_march_blocks(int N, int E);
 // Partial Friedmann sums of the current f, one triple per workgroup.
 hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, int loop,
                          hipStream_t s);

@@ -607,14 +607,6 @@ __global__ __launch_bounds__(kBlock) void k_row_tensor(Fields F, DevBg *bg, int 
     row_tensor_body<T, E, MODE>(F, bg, loop, fresh, gridDim.x, blockIdx.x);
 }
 
-template <typename T, int E>
-__global__ __launch_bounds__(kBlock) void kb_row_tensor(const RunDesc *__restrict__ runs,
-                                                         const uint2 *__restrict__ map)
-{
-    const uint2 mp = map[blockIdx.x];
-    const RunDesc &R = runs[mp.x];
-    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.march, R.G, mp.y);
-}
 
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kBlock) void k_row_reduce(Fields F, Pot P, DevBg *bg, double *partials, int loop)
@@ -857,6 +849,139 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
     const RunDesc &R = runs[blockIdx.x];
     if (!R.march) return;
     sums_finish_body(R.bg, R.partials, R.G, smem);
+}
+
+// ----------------------------------------------------- tensor march kernel
+//
+// Tensor kick + drift of a loop step (src/evolution.cpp:185-253, 274-280) for
+// N >= 64: a workgroup of 8 waves owns 8 j rows x one k segment and marches along
+// i over TI planes.  Each wave keeps the i-1, i, i+1 rows of f and of the six h
+// components in registers, so every h row comes from HBM once; the j +- 1 rows are
+// the neighbouring waves' rows of the same plane (cache hits).  hdot is updated
+// in place (pointwise), h goes to the other buffer.  Same arithmetic as the row
+// tensor kernel; no cross-site sums, so the shape (E sites per lane) is tunable.
+constexpr int kTensorWaves = 8;
+
+__device__ __forceinline__ unsigned tmarch_ti(unsigned N) { return N >= 128 ? 16u : 8u; }
+
+template <typename T, int E>
+__device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, int fresh, unsigned blk)
+{
+    if (bg->done) return;
+    const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
+    const unsigned w = threadIdx.x >> 6;
+    const int par = bg->parity;
+    const T *__restrict__ f = (const T *)F.f[par];
+    double d = bg->d, laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
+    if (fresh) {
+        const StepScalars S = step_scalars(bg, (double)N * N * N);
+        d = S.d;
+        laplnorm = S.laplnorm;
+        friction = S.friction;
+        srcamp = S.srcamp;
+    }
+    d = uniform(d);
+    laplnorm = uniform(laplnorm);
+    friction = uniform(friction);
+    srcamp = uniform(srcamp);
+    const double hx = 0.5 / bg->dx;
+    // tile: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
+    const unsigned TI = tmarch_ti(N), jblocks = N / kTensorWaves, segs = N / (64u * E);
+    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
+    const unsigned jb = lb % jblocks, t2 = lb / jblocks;
+    RowWalk wk;
+    wk.segs = segs;
+    wk.k0 = (t2 % segs) * (64u * E);
+    const unsigned i0 = xcd * (N / kXcd) + (t2 / segs) * TI;
+    const unsigned j = jb * kTensorWaves + w;
+    const unsigned rj = (j << logN) + wk.k0, rjp = (((j + 1) & m) << logN) + wk.k0,
+                   rjm = (((j - 1) & m) << logN) + wk.k0;
+    const unsigned row0 = j << logN;   // start of the whole row (segment edges)
+    T fdn[E], fcur[E];
+    float hdn[6][E], hcur[6][E];
+    {
+        const unsigned pm = ((i0 - 1) & m) << (2 * logN), p0 = (i0 & m) << (2 * logN);
+        ld_row<T, E>(f + pm + rj, lane, fdn);
+        ld_row<T, E>(f + p0 + rj, lane, fcur);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            ld_row<float, E>(F.h[par][c] + pm + rj, lane, hdn[c]);
+            ld_row<float, E>(F.h[par][c] + p0 + rj, lane, hcur[c]);
+        }
+    }
+    for (unsigned t = 0; t < TI; ++t) {
+        const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
+        T fup[E], fjp[E], fjm[E];
+        ld_row<T, E>(f + pu + rj, lane, fup);
+        ld_row<T, E>(f + pi + rjp, lane, fjp);
+        ld_row<T, E>(f + pi + rjm, lane, fjm);
+        T fkp, fkm;
+        k_neighbours<T, E>(fcur, f + pi + row0, wk, N, lane, fkp, fkm);
+        double gx[E], gy[E], gz[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            // central gradient of phi (dfdx, src/evolution.cpp:55-77)
+            const double kp = (double)(e < E - 1 ? fcur[e + 1 < E ? e + 1 : e] : fkp);
+            const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : fkm);
+            gx[e] = ((double)fup[e] - (double)fdn[e]) * hx;
+            gy[e] = ((double)fjp[e] - (double)fjm[e]) * hx;
+            gz[e] = (kp - km) * hx;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float *__restrict__ h = F.h[par][c];
+            float hup[E], hjp[E], hjm[E], hv[E];
+            ld_row<float, E>(h + pu + rj, lane, hup);
+            ld_row<float, E>(h + pi + rjp, lane, hjp);
+            ld_row<float, E>(h + pi + rjm, lane, hjm);
+            ld_row<float, E>(F.hd[c] + pi + rj, lane, hv);
+            float hkp, hkm;
+            k_neighbours<float, E>(hcur[c], h + pi + row0, wk, N, lane, hkp, hkm);
+            float hdn_new[E], hn[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float kp = e < E - 1 ? hcur[c][e + 1 < E ? e + 1 : e] : hkp;
+                const float km = e > 0 ? hcur[c][e > 0 ? e - 1 : 0] : hkm;
+                const float hl = kp + km + hjp[e] + hjm[e] + hup[e] + hdn[c][e] - 6.0f * hcur[c][e];
+                const double src = c == 0 ? gx[e] * gx[e]
+                                 : c == 1 ? gy[e] * gy[e]
+                                 : c == 2 ? gz[e] * gz[e]
+                                 : c == 3 ? gx[e] * gy[e]
+                                 : c == 4 ? gx[e] * gz[e]
+                                          : gy[e] * gz[e];
+                const double rhs = d * (laplnorm * (double)hl - friction * (double)hv[e] + srcamp * src);
+                hdn_new[e] = hv[e] + (float)rhs;
+                hn[e] = hcur[c][e] + (float)(d * (double)hdn_new[e]);
+            }
+            st_row<float, E>(F.hd[c] + pi + rj, lane, hdn_new);
+            st_row<float, E>(F.h[par ^ 1][c] + pi + rj, lane, hn);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                hdn[c][e] = hcur[c][e];
+                hcur[c][e] = hup[e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            fdn[e] = fcur[e];
+            fcur[e] = fup[e];
+        }
+    }
+}
+
+template <typename T, int E>
+__global__ __launch_bounds__(64 * kTensorWaves) void k_tensor_march(Fields F, DevBg *bg, int fresh)
+{
+    tensor_march_body<T, E>(F, bg, fresh, blockIdx.x);
+}
+
+template <typename T, int E>
+__global__ __launch_bounds__(64 * kTensorWaves) void kb_tensor_march(const RunDesc *__restrict__ runs,
+                                                                      const uint2 *__restrict__ map)
+{
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    tensor_march_body<T, E>(R.F, R.bg, R.march, mp.y);
 }
 
 // ------------------------------------------------- small-lattice site kernels
@@ -1226,7 +1351,12 @@ void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loo
                      hipStream_t s)
 {
     if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, 0);
-    else if (row_elems(F.N) == 0) {
+    else if (mode == KICKDRIFT && loop && row_elems(F.N) > 0) {
+        const int E = tensor_march_elems(F.N, F.tensor_e);
+        const unsigned Gt = tensor_march_blocks(F.N, E);
+        if (E == 1) k_tensor_march<T, 1><<<Gt, 64 * kTensorWaves, 0, s>>>(F, bg, fresh);
+        else k_tensor_march<T, 2><<<Gt, 64 * kTensorWaves, 0, s>>>(F, bg, fresh);
+    } else if (row_elems(F.N) == 0) {
         if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, fresh);
         else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, fresh);
     } else {
@@ -1264,6 +1394,21 @@ int shape_elems(int N, int want)
     if (def == 0 || want <= 0) return def;
     const int e = want >= 4 ? 4 : (want >= 2 ? 2 : 1);
     return e <= def ? e : def;
+}
+
+// Sites per lane of the tensor march kernel: the tuned shape, at most 2 (six
+// components' three-plane windows must stay in registers).
+int tensor_march_elems(int N, int want)
+{
+    const int e = shape_elems(N, want);
+    return e > 2 ? 2 : e;
+}
+
+unsigned tensor_march_blocks(int N, int E)
+{
+    const unsigned segs = (unsigned)N / (64u * (unsigned)E);
+    const unsigned TI = N >= 128 ? 16u : 8u;
+    return (unsigned)(N / kTensorWaves) * segs * ((unsigned)N / TI);
 }
 
 int reduce_blocks(size_t gs)
@@ -1456,7 +1601,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         d[r].split = split;
         (d[r].march ? B->has_march : B->has_plain) = 1;
         const int kE = d[r].march ? row_elems(N) : shape_elems(N, runs[r].F.kick_e);
-        const int key = d[r].march * 1000 + row_elems(N) * 100 + kE * 10 + shape_elems(N, runs[r].F.tensor_e);
+        const int tE = row_elems(N) > 0 ? tensor_march_elems(N, runs[r].F.tensor_e) : 0;
+        const int key = d[r].march * 1000 + row_elems(N) * 100 + kE * 10 + tE;
         size_t g = 0;
         while (g < keys.size() && keys[g] != key) ++g;
         if (g == keys.size()) {
@@ -1465,9 +1611,12 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
             rmaps.emplace_back();
             tmaps.emplace_back();
         }
-        for (unsigned b = 0; b < d[r].G; ++b) {
-            rmaps[g].push_back(make_uint2((unsigned)r, b));
-            if (runs[r].F.sigw) tmaps[g].push_back(make_uint2((unsigned)r, b));
+        for (unsigned b = 0; b < d[r].G; ++b) rmaps[g].push_back(make_uint2((unsigned)r, b));
+        if (runs[r].F.sigw) {
+            // tensor step: march kernel (N >= 64) or site kernel
+            const unsigned Gt = row_elems(N) > 0 ? tensor_march_blocks(N, tensor_march_elems(N, runs[r].F.tensor_e))
+                                                 : d[r].G;
+            for (unsigned b = 0; b < Gt; ++b) tmaps[g].push_back(make_uint2((unsigned)r, b));
         }
         const unsigned Gk = d[r].march ? d[r].Gm : d[r].G;
         for (unsigned b = 0; b < Gk; ++b) maps[g].push_back(make_uint2((unsigned)r, b));
@@ -1562,12 +1711,9 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t
     default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     }
     if (!G.ntblocks) return;
-    switch (G.tE) {
-    case 1: kb_row_tensor<T, 1><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
-    case 2: kb_row_tensor<T, 2><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
-    case 4: kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
-    default: kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap); break;
-    }
+    if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+    else if (G.tE == 1) kb_tensor_march<T, 1><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
+    else kb_tensor_march<T, 2><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
 }
 
 hipError_t batch_step(const BatchDev *B, hipStream_t s)
