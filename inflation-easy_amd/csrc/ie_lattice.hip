@@ -390,6 +390,19 @@ __device__ __forceinline__ RowWalk row_walk(unsigned N, unsigned G, unsigned blk
 
 // k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
 // `row`): in-lane, then across lanes, then across segments for N > 64 E.
+// The value of the first active lane, in scalar registers (the same in every lane).
+__device__ __forceinline__ float first_lane(float x)
+{
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(x)));
+}
+__device__ __forceinline__ double first_lane(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename T, int E>
 __device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restrict__ row, const RowWalk &w,
                                              unsigned N, unsigned lane, T &kp_last, T &km_first)
@@ -397,10 +410,12 @@ __device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restric
     kp_last = __shfl(c[0], (int)((lane + 1) & 63));
     km_first = __shfl(c[E - 1], (int)((lane + 63) & 63));
     if (w.segs > 1) {
+        // the two edge sites, loaded by every lane and kept wave-uniform, then
+        // selected per lane without a branch
         const unsigned m = N - 1;
-        const T lo = row[(w.k0 - 1) & m], hi = row[(w.k0 + 64u * E) & m];
-        if (lane == 0) km_first = lo;
-        if (lane == 63) kp_last = hi;
+        const T lo = first_lane(row[(w.k0 - 1) & m]), hi = first_lane(row[(w.k0 + 64u * E) & m]);
+        km_first = lane == 0 ? lo : km_first;
+        kp_last = lane == 63 ? hi : kp_last;
     }
 }
 
@@ -701,7 +716,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     for (int e = 0; e < E; ++e) fprev[e] = (T)0;
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     int err = 0;
-    RowWalk wk;   // k neighbours within the whole row (one segment)
+    RowWalk wk{};   // k neighbours within the whole row (one segment)
     wk.segs = 1;
     wk.k0 = 0;
     for (unsigned t = 0; t <= TI; ++t) {
@@ -856,16 +871,26 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
 // Tensor kick + drift of a loop step (src/evolution.cpp:185-253, 274-280) for
 // N >= 64: a workgroup of 8 waves owns 8 j rows x one k segment and marches along
 // i over TI planes.  Each wave keeps the i-1, i, i+1 rows of f and of the six h
-// components in registers, so every h row comes from HBM once; the j +- 1 rows are
-// the neighbouring waves' rows of the same plane (cache hits).  hdot is updated
-// in place (pointwise), h goes to the other buffer.  Same arithmetic as the row
+// components in registers and puts its plane-i rows in LDS (double-buffered, with
+// the tile's two halo rows), where the neighbouring waves read their j +- 1 rows:
+// every h row comes from HBM about once (the j +- 1 rows streamed from L2 instead
+// were evicted between the two uses, doubling the reads).  hdot is updated in
+// place (pointwise), h goes to the other buffer.  Same arithmetic as the row
 // tensor kernel; no cross-site sums, so the shape (E sites per lane) is tunable.
 constexpr int kTensorWaves = 8;
 
 __device__ __forceinline__ unsigned tmarch_ti(unsigned N) { return N >= 128 ? 16u : 8u; }
 
+// LDS tile of one plane: rows j0-1 .. j0+8 of the six h components and of f.
 template <typename T, int E>
-__device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, int fresh, unsigned blk)
+struct TensorTile {
+    float h[2][6][kTensorWaves + 2][64 * E];
+    T f[2][kTensorWaves + 2][64 * E];
+};
+
+template <typename T, int E>
+__device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, int fresh, unsigned blk,
+                                                  TensorTile<T, E> &L)
 {
     if (bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
@@ -889,14 +914,17 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
     const unsigned TI = tmarch_ti(N), jblocks = N / kTensorWaves, segs = N / (64u * E);
     const unsigned xcd = blk % kXcd, lb = blk / kXcd;
     const unsigned jb = lb % jblocks, t2 = lb / jblocks;
-    RowWalk wk;
+    RowWalk wk{};
     wk.segs = segs;
     wk.k0 = (t2 % segs) * (64u * E);
     const unsigned i0 = xcd * (N / kXcd) + (t2 / segs) * TI;
-    const unsigned j = jb * kTensorWaves + w;
-    const unsigned rj = (j << logN) + wk.k0, rjp = (((j + 1) & m) << logN) + wk.k0,
-                   rjm = (((j - 1) & m) << logN) + wk.k0;
+    const unsigned j0 = jb * kTensorWaves, j = j0 + w;
+    const unsigned rj = (j << logN) + wk.k0;
     const unsigned row0 = j << logN;   // start of the whole row (segment edges)
+    // waves 0 and 7 also bring the tile's halo rows j0-1 and j0+8 into LDS
+    const bool halo = w == 0 || w == kTensorWaves - 1;
+    const unsigned rh = ((w == 0 ? j0 - 1 : j0 + kTensorWaves) & m) << logN;
+    const unsigned sh = w == 0 ? 0u : (unsigned)kTensorWaves + 1;
     T fdn[E], fcur[E];
     float hdn[6][E], hcur[6][E];
     {
@@ -910,11 +938,31 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
         }
     }
     for (unsigned t = 0; t < TI; ++t) {
+        const unsigned b = t & 1;
         const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
-        T fup[E], fjp[E], fjm[E];
+        // plane i rows into LDS: the wave's own, and the halo rows
+#pragma unroll
+        for (int e = 0; e < E; ++e) L.f[b][w + 1][lane * E + e] = fcur[e];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) L.h[b][c][w + 1][lane * E + e] = hcur[c][e];
+        if (halo) {
+            T fh[E];
+            ld_row<T, E>(f + pi + rh + wk.k0, lane, fh);
+#pragma unroll
+            for (int e = 0; e < E; ++e) L.f[b][sh][lane * E + e] = fh[e];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                float hh[E];
+                ld_row<float, E>(F.h[par][c] + pi + rh + wk.k0, lane, hh);
+#pragma unroll
+                for (int e = 0; e < E; ++e) L.h[b][c][sh][lane * E + e] = hh[e];
+            }
+        }
+        T fup[E];
         ld_row<T, E>(f + pu + rj, lane, fup);
-        ld_row<T, E>(f + pi + rjp, lane, fjp);
-        ld_row<T, E>(f + pi + rjm, lane, fjm);
+        __syncthreads();
         T fkp, fkm;
         k_neighbours<T, E>(fcur, f + pi + row0, wk, N, lane, fkp, fkm);
         double gx[E], gy[E], gz[E];
@@ -924,16 +972,14 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
             const double kp = (double)(e < E - 1 ? fcur[e + 1 < E ? e + 1 : e] : fkp);
             const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : fkm);
             gx[e] = ((double)fup[e] - (double)fdn[e]) * hx;
-            gy[e] = ((double)fjp[e] - (double)fjm[e]) * hx;
+            gy[e] = ((double)L.f[b][w + 2][lane * E + e] - (double)L.f[b][w][lane * E + e]) * hx;
             gz[e] = (kp - km) * hx;
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             const float *__restrict__ h = F.h[par][c];
-            float hup[E], hjp[E], hjm[E], hv[E];
+            float hup[E], hv[E];
             ld_row<float, E>(h + pu + rj, lane, hup);
-            ld_row<float, E>(h + pi + rjp, lane, hjp);
-            ld_row<float, E>(h + pi + rjm, lane, hjm);
             ld_row<float, E>(F.hd[c] + pi + rj, lane, hv);
             float hkp, hkm;
             k_neighbours<float, E>(hcur[c], h + pi + row0, wk, N, lane, hkp, hkm);
@@ -942,7 +988,8 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
             for (int e = 0; e < E; ++e) {
                 const float kp = e < E - 1 ? hcur[c][e + 1 < E ? e + 1 : e] : hkp;
                 const float km = e > 0 ? hcur[c][e > 0 ? e - 1 : 0] : hkm;
-                const float hl = kp + km + hjp[e] + hjm[e] + hup[e] + hdn[c][e] - 6.0f * hcur[c][e];
+                const float jp = L.h[b][c][w + 2][lane * E + e], jm = L.h[b][c][w][lane * E + e];
+                const float hl = kp + km + jp + jm + hup[e] + hdn[c][e] - 6.0f * hcur[c][e];
                 const double src = c == 0 ? gx[e] * gx[e]
                                  : c == 1 ? gy[e] * gy[e]
                                  : c == 2 ? gz[e] * gz[e]
@@ -970,18 +1017,21 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
 }
 
 template <typename T, int E>
-__global__ __launch_bounds__(64 * kTensorWaves) void k_tensor_march(Fields F, DevBg *bg, int fresh)
+__global__ __launch_bounds__(64 * kTensorWaves)
+void k_tensor_march(Fields F, DevBg *bg, int fresh)
 {
-    tensor_march_body<T, E>(F, bg, fresh, blockIdx.x);
+    __shared__ TensorTile<T, E> L;
+    tensor_march_body<T, E>(F, bg, fresh, blockIdx.x, L);
 }
 
 template <typename T, int E>
-__global__ __launch_bounds__(64 * kTensorWaves) void kb_tensor_march(const RunDesc *__restrict__ runs,
-                                                                      const uint2 *__restrict__ map)
+__global__ __launch_bounds__(64 * kTensorWaves)
+void kb_tensor_march(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
 {
+    __shared__ TensorTile<T, E> L;
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    tensor_march_body<T, E>(R.F, R.bg, R.march, mp.y);
+    tensor_march_body<T, E>(R.F, R.bg, R.march, mp.y, L);
 }
 
 // ------------------------------------------------- small-lattice site kernels
