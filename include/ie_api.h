@@ -157,20 +157,26 @@ typedef struct ie_profile {
 ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
 
 /* Shape autotuner of the step kernels (no counterpart in the reference: the
- * reference has no kernel shapes).  The kick and tensor kernels of N >= 128
- * lattices can hold 1, 2 or 4 sites per lane of a row; their results do not depend
- * on it (no cross-site sums), so the tuner times real loop steps: for each
- * candidate, steps_per_candidate steps of the device-resident loop with events
- * around the two launches, then keeps the fastest of each.  The choice is stored
- * process-wide per (N, storage) and taken by later runs and batches of that kind;
- * ie_leapfrog_steps tunes once per kind by itself on a call of >= 256 steps.  The
- * reduce kernel keeps one shape, since it fixes the summation order. */
+ * reference has no kernel shapes).  The unfused kick kernel of N >= 128 lattices
+ * can hold 1, 2 or 4 sites per lane of a row; the tensor step can run as the row
+ * kernel (1, 2 or 4 sites per lane) or as the plane-marching kernel (1 or 2).
+ * None of these changes a result (no cross-site sums), so the tuner times real
+ * loop steps: round r uses kick candidate r mod ncand and tensor candidate
+ * r mod ntensor for steps_per_candidate steps of the device-resident loop, with
+ * events around the two launches, then keeps the fastest of each.  The choice is
+ * stored process-wide per (N, storage) and taken by later runs and batches of
+ * that kind; ie_leapfrog_steps tunes once per kind by itself on a call of >= 256
+ * steps.  The reduce and fused step kernels keep one shape, since theirs fixes the
+ * summation order. */
 typedef struct {
-    int ncand;                 /* candidates timed */
-    int elems[3];              /* sites per lane of each candidate */
+    int ncand;                 /* kick candidates timed */
+    int elems[3];              /* sites per lane of each kick candidate */
     double kick_ms[3];         /* mean kick launch time per candidate */
-    double tensor_ms[3];       /* mean tensor launch time (0 without SIGW) */
-    int kick_elems, tensor_elems;   /* chosen */
+    int ntensor;               /* tensor candidates timed (0 without SIGW) */
+    int tensor_code[5];        /* 1, 2, 4: row kernel; 17, 18: march kernel with 1, 2 */
+    double tensor_ms[5];       /* mean tensor launch time per candidate */
+    int kick_elems;            /* chosen kick shape */
+    int tensor_code_chosen;    /* chosen tensor shape code (0 without SIGW) */
     int64_t steps;             /* loop steps taken while tuning */
 } ie_tuning;
 ie_status ie_tune(ie_run *run, int steps_per_candidate, ie_tuning *out);

@@ -609,26 +609,36 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     memset(out, 0, sizeof(*out));
     const int split = r->cfg.precision == IE_SPLIT;
     const int def = row_elems(r->N);
-    out->kick_elems = out->tensor_elems = def;
+    const bool fused = march_ok(r->N);
+    out->kick_elems = def;
     for (int e = 1; e <= 4 && def > 0; e *= 2)
         if (e <= def) out->elems[out->ncand++] = e;
-    if (out->ncand <= 1) {
-        r->F.kick_e = r->F.tensor_e = def;
-        store_shapes(r->N, split, def, r->F.sigw ? def : 0);
+    if (r->F.sigw && def > 0) {
+        for (int e = 1; e <= 4; e *= 2)
+            if (e <= def) out->tensor_code[out->ntensor++] = e;
+        for (int e = 1; e <= 2; e *= 2)
+            if (e <= def) out->tensor_code[out->ntensor++] = kTensorMarch | e;
+    }
+    // fused steps have one kick shape: only the tensor's is tuned there
+    const int nk = fused ? 1 : out->ncand, nt = out->ntensor;
+    const int rounds = std::max(nk, nt);
+    if (rounds <= 1) {
+        r->F.kick_e = def;
+        r->F.tensor_e = nt == 1 ? out->tensor_code[0] : 0;
+        out->tensor_code_chosen = r->F.tensor_e;
+        store_shapes(r->N, split, def, r->F.sigw ? std::max(r->F.tensor_e, 1) : 0);
         return IE_OK;
     }
     ie_status st = pull_bg(r);
     if (st != IE_OK) return st;
     const int64_t steps0 = r->bg_host->steps;
-    const int nc = out->ncand;
-    std::vector<hipEvent_t> ev((size_t)nc * K * 4);
+    std::vector<hipEvent_t> ev((size_t)rounds * K * 4);
     for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
-    // fused steps (128 <= N <= 256): the march kernel has one shape; the tensor's is tuned
-    const bool fused = march_ok(r->N);
     if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
-    for (int c = 0; c < nc; ++c) {
-        r->F.kick_e = r->F.tensor_e = out->elems[c];
+    for (int c = 0; c < rounds; ++c) {
+        r->F.kick_e = fused ? def : out->elems[c % nk];
+        r->F.tensor_e = nt ? out->tensor_code[c % nt] : 0;
         for (int q = 0; q < K; ++q) {
             hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
             if (!fused) IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
@@ -645,32 +655,40 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
         }
     }
     if ((st = sync(r)) != IE_OK) return st;
-    int bk = 0, bt = 0;
-    for (int c = 0; c < nc; ++c) {
-        double kk = 0.0, tt = 0.0;
+    std::vector<double> ksum(3, 0.0), tsum(5, 0.0);
+    std::vector<int> kcnt(3, 0), tcnt(5, 0);
+    for (int c = 0; c < rounds; ++c)
         for (int q = 0; q < K; ++q) {
             hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
             float ms = 0.f;
             IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
-            kk += ms;
-            IE_HIP(r, hipEventElapsedTime(&ms, e[1], e[2]));
-            tt += ms;
+            ksum[c % nk] += ms;
+            kcnt[c % nk] += 1;
+            if (nt) {
+                IE_HIP(r, hipEventElapsedTime(&ms, e[1], e[2]));
+                tsum[c % nt] += ms;
+                tcnt[c % nt] += 1;
+            }
         }
-        out->kick_ms[c] = kk / K;
-        out->tensor_ms[c] = r->F.sigw ? tt / K : 0.0;
-        if (out->kick_ms[c] < out->kick_ms[bk]) bk = c;
-        if (out->tensor_ms[c] < out->tensor_ms[bt]) bt = c;
-    }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if ((st = check_kernel_error(r)) != IE_OK) return st;
+    int bk = 0, bt = 0;
+    for (int c = 0; c < out->ncand; ++c) {
+        const int i = fused ? 0 : c;
+        out->kick_ms[c] = kcnt[i] ? ksum[i] / kcnt[i] : 0.0;
+        if (out->kick_ms[c] < out->kick_ms[bk]) bk = c;
+    }
+    for (int c = 0; c < nt; ++c) {
+        out->tensor_ms[c] = tcnt[c] ? tsum[c] / tcnt[c] : 0.0;
+        if (out->tensor_ms[c] < out->tensor_ms[bt]) bt = c;
+    }
     out->steps = r->bg_host->steps - steps0;
-    if (fused) bk = nc - 1;   // the march kernel's one shape (row_elems), whatever was asked
-    out->kick_elems = out->elems[bk];
-    out->tensor_elems = r->F.sigw ? out->elems[bt] : def;
+    out->kick_elems = fused ? def : out->elems[bk];
+    out->tensor_code_chosen = nt ? out->tensor_code[bt] : 0;
     r->F.kick_e = out->kick_elems;
-    r->F.tensor_e = out->tensor_elems;
+    r->F.tensor_e = out->tensor_code_chosen;
     // timings of a loop that reached af part way are not kept
-    if (out->steps == (int64_t)nc * K) store_shapes(r->N, split, out->kick_elems, r->F.sigw ? out->tensor_elems : 0);
+    if (out->steps == (int64_t)rounds * K) store_shapes(r->N, split, out->kick_elems, out->tensor_code_chosen);
     return IE_OK;
 }
 

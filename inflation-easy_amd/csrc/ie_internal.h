@@ -55,7 +55,7 @@ struct Fields {
     double *dn;
     int N, logN;
     int sigw;
-    int kick_e, tensor_e;   // sites per lane of the kick / tensor row kernels (0: default)
+    int kick_e, tensor_e;   // kick sites per lane / tensor shape code (0: default)
 };
 
 // ---------------------------------------------------------------------- helpers
@@ -216,8 +216,11 @@ unsigned step_blocks(int N);
 // asked for `want` sites per lane.
 int row_elems(int N);
 int shape_elems(int N, int want);
-// Sites per lane and workgroups of the tensor march kernel (loop steps, N >= 64).
-int tensor_march_elems(int N, int want);
+// Tensor step of loop steps (N >= 64): Fields::tensor_e is a shape code, 0 or
+// 1/2/4 for the row kernel, kTensorMarch | E (E = 1, 2) for the march kernel.
+constexpr int kTensorMarch = 16;
+bool tensor_is_march(int code);
+int tensor_march_elems(int N, int code);
 unsigned tensor_march_blocks(int N, int E);
 // Partial Friedmann sums of the current f, one triple per workgroup.
 hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, int loop,

@@ -622,6 +622,15 @@ __global__ __launch_bounds__(kBlock) void k_row_tensor(Fields F, DevBg *bg, int 
     row_tensor_body<T, E, MODE>(F, bg, loop, fresh, gridDim.x, blockIdx.x);
 }
 
+template <typename T, int E>
+__global__ __launch_bounds__(kBlock) void kb_row_tensor(const RunDesc *__restrict__ runs,
+                                                         const uint2 *__restrict__ map)
+{
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.march, R.G, mp.y);
+}
+
 
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kBlock) void k_row_reduce(Fields F, Pot P, DevBg *bg, double *partials, int loop)
@@ -683,14 +692,16 @@ __device__ __forceinline__ double uniform(double x)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-template <typename T, bool SPLIT, int E>
+// SUMS: the sums alone, of the current f taken as the new f (loop start), with
+// the same partition and operation order as a step's, so that a loop split into
+// several calls gives the same bits as one call.
+template <typename T, bool SPLIT, int E, bool SUMS = false>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned blk,
                                            T (*lds)[kMarchWaves][64 * E], double *smem)
 {
     if (bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const unsigned w = threadIdx.x >> 6;
-    const StepScalars S = step_scalars(bg, (double)N * N * N);
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
@@ -698,10 +709,18 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     // plane read fd rows that their owning workgroups update in the same launch
     const T *__restrict__ fd = (const T *)F.fd[par];
     T *__restrict__ fdout = (T *)F.fd[par ^ 1];
-    const double d = uniform(S.d), laplnorm = uniform(S.laplnorm), friction = uniform(S.friction);
-    const double potnorm = uniform(S.potnorm);
-    const double phibar = SPLIT ? bg->phibar : 0.0, vpmean = SPLIT ? uniform(S.vpmean) : 0.0;
-    const double phibar_next = SPLIT ? uniform(S.phibar_next) : 0.0;
+    double d = 0.0, laplnorm = 0.0, friction = 0.0, potnorm = 0.0, vpmean = 0.0;
+    const double phibar = SPLIT ? bg->phibar : 0.0;
+    double phibar_next = phibar;
+    if constexpr (!SUMS) {
+        const StepScalars S = step_scalars(bg, (double)N * N * N);
+        d = uniform(S.d);
+        laplnorm = uniform(S.laplnorm);
+        friction = uniform(S.friction);
+        potnorm = uniform(S.potnorm);
+        vpmean = SPLIT ? uniform(S.vpmean) : 0.0;
+        phibar_next = SPLIT ? uniform(S.phibar_next) : 0.0;
+    }
     // tile of this workgroup: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
     const unsigned TI = march_ti(N), jblocks = N / kTileJ;
     const unsigned xcd = blk % kXcd, lb = blk / kXcd;
@@ -722,28 +741,35 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     for (unsigned t = 0; t <= TI; ++t) {
         const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
         ld_row<T, E>(f + pu + rj, lane, fup);
-        ld_row<T, E>(f + pi + rjp, lane, fjp);
-        ld_row<T, E>(f + pi + rjm, lane, fjm);
-        ld_row<T, E>(fd + pi + rj, lane, vin);
-        T kp_last, km_first;
-        k_neighbours<T, E>(fcur, f + pi + rj, wk, N, lane, kp_last, km_first);
-        T vout[E], fnew[E];
+        T fnew[E];
+        if constexpr (SUMS) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const double c = (double)fcur[e];
-            const double kp = (double)(e < E - 1 ? fcur[e + 1 < E ? e + 1 : e] : kp_last);
-            const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : km_first);
-            const double lap = lapl_sites(kp, km, (double)fjp[e], (double)fjm[e], (double)fup[e], (double)fdn[e], c);
-            double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
-            if (SPLIT) dV -= vpmean;
-            double v = (double)vin[e];
-            v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
-            vout[e] = (T)v;
-            fnew[e] = (T)(c + d * (double)vout[e]);
-        }
-        if (owner && t < TI) {
-            st_row<T, E>(fdout + pi + rj, lane, vout);
-            st_row<T, E>(fout + pi + rj, lane, fnew);
+            for (int e = 0; e < E; ++e) fnew[e] = fcur[e];
+        } else {
+            ld_row<T, E>(f + pi + rjp, lane, fjp);
+            ld_row<T, E>(f + pi + rjm, lane, fjm);
+            ld_row<T, E>(fd + pi + rj, lane, vin);
+            T kp_last, km_first;
+            k_neighbours<T, E>(fcur, f + pi + rj, wk, N, lane, kp_last, km_first);
+            T vout[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double c = (double)fcur[e];
+                const double kp = (double)(e < E - 1 ? fcur[e + 1 < E ? e + 1 : e] : kp_last);
+                const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : km_first);
+                const double lap =
+                    lapl_sites(kp, km, (double)fjp[e], (double)fjm[e], (double)fup[e], (double)fdn[e], c);
+                double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+                if (SPLIT) dV -= vpmean;
+                double v = (double)vin[e];
+                v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
+                vout[e] = (T)v;
+                fnew[e] = (T)(c + d * (double)vout[e]);
+            }
+            if (owner && t < TI) {
+                st_row<T, E>(fdout + pi + rj, lane, vout);
+                st_row<T, E>(fout + pi + rj, lane, fnew);
+            }
         }
         T(*buf)[64 * E] = lds[t & 1];
 #pragma unroll
@@ -814,6 +840,25 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg 
     march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem);
 }
 
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) void k_march_sums(Fields F, Pot P, DevBg *bg, double *partials)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    march_body<T, SPLIT, E, true>(F, P, bg, partials, blockIdx.x, lds, smem);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) void kb_march_sums(const RunDesc *__restrict__ runs,
+                                                                const uint2 *__restrict__ map)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    march_body<T, SPLIT, E, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
+}
+
 __global__ __launch_bounds__(kBlock) void k_step_close(DevBg *bg, const double *partials, unsigned G,
                                                         unsigned long long gs, int split)
 {
@@ -863,7 +908,7 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
     __shared__ double smem[3 * kWaves];
     const RunDesc &R = runs[blockIdx.x];
     if (!R.march) return;
-    sums_finish_body(R.bg, R.partials, R.G, smem);
+    sums_finish_body(R.bg, R.partials, R.Gm, smem);
 }
 
 // ----------------------------------------------------- tensor march kernel
@@ -1389,7 +1434,7 @@ void kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_
 template <typename T, int MODE>
 void row_tensor_dispatch(const Fields &F, DevBg *bg, int loop, int fresh, unsigned G, hipStream_t s)
 {
-    switch (shape_elems(F.N, F.tensor_e)) {
+    switch (shape_elems(F.N, F.tensor_e & 7)) {
     case 1: k_row_tensor<T, 1, MODE><<<G, kBlock, 0, s>>>(F, bg, loop, fresh); break;
     case 2: k_row_tensor<T, 2, MODE><<<G, kBlock, 0, s>>>(F, bg, loop, fresh); break;
     default: k_row_tensor<T, 4, MODE><<<G, kBlock, 0, s>>>(F, bg, loop, fresh); break;
@@ -1401,7 +1446,7 @@ void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loo
                      hipStream_t s)
 {
     if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, 0);
-    else if (mode == KICKDRIFT && loop && row_elems(F.N) > 0) {
+    else if (mode == KICKDRIFT && loop && row_elems(F.N) > 0 && tensor_is_march(F.tensor_e)) {
         const int E = tensor_march_elems(F.N, F.tensor_e);
         const unsigned Gt = tensor_march_blocks(F.N, E);
         if (E == 1) k_tensor_march<T, 1><<<Gt, 64 * kTensorWaves, 0, s>>>(F, bg, fresh);
@@ -1446,11 +1491,14 @@ int shape_elems(int N, int want)
     return e <= def ? e : def;
 }
 
-// Sites per lane of the tensor march kernel: the tuned shape, at most 2 (six
-// components' three-plane windows must stay in registers).
-int tensor_march_elems(int N, int want)
+// Tensor step shape codes: 0 default (row kernel), 1/2/4 row kernel with that many
+// sites per lane, kTensorMarch | E the march kernel with E (1 or 2) sites per lane
+// (six components' three-plane windows must stay in registers).
+bool tensor_is_march(int code) { return (code & kTensorMarch) != 0; }
+
+int tensor_march_elems(int N, int code)
 {
-    const int e = shape_elems(N, want);
+    const int e = shape_elems(N, code & 7);
     return e > 2 ? 2 : e;
 }
 
@@ -1528,9 +1576,16 @@ unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (N >= 256 ?
 
 hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
 {
-    hipError_t e = launch_reduce(F, split, P, bg, partials, 1, s);
-    if (e != hipSuccess) return e;
-    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, step_blocks(F.N));
+    const unsigned G = march_blocks(F.N);
+    const int E = row_elems(F.N);
+    if (split) {
+        if (E == 2) k_march_sums<float, true, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else k_march_sums<float, true, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+    } else {
+        if (E == 2) k_march_sums<double, false, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else k_march_sums<double, false, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+    }
+    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G);
     return hipGetLastError();
 }
 
@@ -1651,8 +1706,10 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         d[r].split = split;
         (d[r].march ? B->has_march : B->has_plain) = 1;
         const int kE = d[r].march ? row_elems(N) : shape_elems(N, runs[r].F.kick_e);
-        const int tE = row_elems(N) > 0 ? tensor_march_elems(N, runs[r].F.tensor_e) : 0;
-        const int key = d[r].march * 1000 + row_elems(N) * 100 + kE * 10 + tE;
+        const int tc = runs[r].F.tensor_e;
+        const int tE = row_elems(N) == 0 ? 0
+                       : tensor_is_march(tc) ? kTensorMarch | tensor_march_elems(N, tc) : shape_elems(N, tc);
+        const int key = d[r].march * 100000 + row_elems(N) * 10000 + kE * 100 + tE;
         size_t g = 0;
         while (g < keys.size() && keys[g] != key) ++g;
         if (g == keys.size()) {
@@ -1664,8 +1721,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         for (unsigned b = 0; b < d[r].G; ++b) rmaps[g].push_back(make_uint2((unsigned)r, b));
         if (runs[r].F.sigw) {
             // tensor step: march kernel (N >= 64) or site kernel
-            const unsigned Gt = row_elems(N) > 0 ? tensor_march_blocks(N, tensor_march_elems(N, runs[r].F.tensor_e))
-                                                 : d[r].G;
+            const unsigned Gt = (row_elems(N) > 0 && tensor_is_march(tc)) ? tensor_march_blocks(N, tensor_march_elems(N, tc))
+                                                                          : d[r].G;
             for (unsigned b = 0; b < Gt; ++b) tmaps[g].push_back(make_uint2((unsigned)r, b));
         }
         const unsigned Gk = d[r].march ? d[r].Gm : d[r].G;
@@ -1680,10 +1737,10 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     };
     for (size_t g = 0; g < keys.size() && e == hipSuccess; ++g) {
         BatchGroup G;
-        G.march = keys[g] / 1000;
-        G.E = keys[g] / 100 % 10;
-        G.kE = keys[g] / 10 % 10;
-        G.tE = keys[g] % 10;
+        G.march = keys[g] / 100000;
+        G.E = keys[g] / 10000 % 10;
+        G.kE = keys[g] / 100 % 100;
+        G.tE = keys[g] % 100;
         G.nblocks = (int)maps[g].size();
         G.nrblocks = (int)rmaps[g].size();
         G.ntblocks = (int)tmaps[g].size();
@@ -1719,11 +1776,16 @@ hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
 {
     kb_loop_begin<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
     if (B->has_march) {
-        // fused groups start from the sums of the current f
+        // fused groups start from the sums of the current f, in a step's order
         for (const BatchGroup &G : B->groups) {
             if (!G.march) continue;
-            if (B->split) batch_group_reduce<float, true>(B, G, s);
-            else batch_group_reduce<double, false>(B, G, s);
+            if (B->split) {
+                if (G.E == 2) kb_march_sums<float, true, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+                else kb_march_sums<float, true, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            } else {
+                if (G.E == 2) kb_march_sums<double, false, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+                else kb_march_sums<double, false, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            }
         }
         kb_sums_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
     }
@@ -1762,8 +1824,13 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t
     }
     if (!G.ntblocks) return;
     if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
-    else if (G.tE == 1) kb_tensor_march<T, 1><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
-    else kb_tensor_march<T, 2><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
+    else if (G.tE == (kTensorMarch | 1))
+        kb_tensor_march<T, 1><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
+    else if (G.tE == (kTensorMarch | 2))
+        kb_tensor_march<T, 2><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
+    else if (G.tE == 1) kb_row_tensor<T, 1><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+    else if (G.tE == 2) kb_row_tensor<T, 2><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+    else kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
 }
 
 hipError_t batch_step(const BatchDev *B, hipStream_t s)

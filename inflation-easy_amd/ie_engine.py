@@ -90,14 +90,14 @@ class Tuning(C.Structure):
     """Mirror of ``ie_tuning``."""
 
     _fields_ = [("ncand", C.c_int), ("elems", C.c_int * 3), ("kick_ms", C.c_double * 3),
-                ("tensor_ms", C.c_double * 3), ("kick_elems", C.c_int), ("tensor_elems", C.c_int),
-                ("steps", C.c_int64)]
+                ("ntensor", C.c_int), ("tensor_code", C.c_int * 5), ("tensor_ms", C.c_double * 5),
+                ("kick_elems", C.c_int), ("tensor_code_chosen", C.c_int), ("steps", C.c_int64)]
 
     def as_dict(self) -> dict:
-        n = self.ncand
+        n, t = self.ncand, self.ntensor
         return {"candidates": list(self.elems[:n]), "kick_ms": list(self.kick_ms[:n]),
-                "tensor_ms": list(self.tensor_ms[:n]), "kick_elems": self.kick_elems,
-                "tensor_elems": self.tensor_elems, "steps": self.steps}
+                "tensor_codes": list(self.tensor_code[:t]), "tensor_ms": list(self.tensor_ms[:t]),
+                "kick_elems": self.kick_elems, "tensor_code": self.tensor_code_chosen, "steps": self.steps}
 
 
 class Background(C.Structure):

@@ -235,9 +235,10 @@ def test_segmented_rows_match_oracle_n512(precision):
 
 
 def test_tuned_shapes_leave_results_unchanged():
-    """The tuner times real loop steps with 1 and 2 sites per lane (N = 128); the
-    kick and tensor kernels have no cross-site sums, so a run that changes shape
-    between steps matches one that never does, bit for bit."""
+    """The tuner times real loop steps with the tensor step as the row kernel (1 and
+    2 sites per lane) and as the plane-marching kernel (1 and 2), N = 128 with fused
+    scalar steps; the tensor kernels have no cross-site sums, so a run that changes
+    shape between steps matches one that never does, bit for bit."""
     st, _, table = oracle_state("hilltop_n32", N=128, calculate_sigw=1)
     ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
     rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
@@ -245,13 +246,33 @@ def test_tuned_shapes_leave_results_unchanged():
     for r in (ra, rb):
         r.evolve_fields(0.5 * dt)
     t = ra.tune(4)
-    assert t["candidates"] == [1, 2] and t["steps"] == 8
-    assert t["kick_elems"] in (1, 2) and t["tensor_elems"] in (1, 2)
-    assert all(x > 0 for x in t["kick_ms"] + t["tensor_ms"])
+    assert t["tensor_codes"] == [1, 2, 17, 18] and t["steps"] == 16
+    assert t["tensor_code"] in (1, 2, 17, 18)
+    assert all(x > 0 for x in t["tensor_ms"])
     ra.leapfrog_steps(12)
-    rb.leapfrog_steps(20)
+    rb.leapfrog_steps(28)
     assert ra.background()["steps"] == rb.background()["steps"]
     assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
     assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
     for comp in range(12):
         assert np.array_equal(ra.tensor(comp), rb.tensor(comp)), comp
+
+
+@pytest.mark.parametrize("N", [64, 128])
+def test_loop_in_chunks_equals_one_call(N):
+    """A loop split into several ie_leapfrog_steps calls gives the same bits as one
+    call: fused steps (N = 128) start each call with the sums of the current f in
+    a step's own partition and order."""
+    st, _, table = oracle_state("hilltop_n32", N=N, calculate_sigw=0)
+    ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=0)
+    rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=0)
+    for r in (ra, rb):
+        r.evolve_fields(0.5 * st.cfg.dt)
+    for _ in range(3):
+        ra.leapfrog_steps(10)
+    rb.leapfrog_steps(30)
+    ba, bb = ra.background(), rb.background()
+    assert ba["steps"] == bb["steps"] == 30
+    assert ba["a"] == bb["a"] and ba["ad"] == bb["ad"]
+    assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
+    assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
