@@ -159,7 +159,9 @@ ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
 /* Shape autotuner of the step kernels (no counterpart in the reference: the
  * reference has no kernel shapes).  The unfused kick kernel of N >= 128 lattices
  * can hold 1, 2 or 4 sites per lane of a row; the tensor step can run as the row
- * kernel (1, 2 or 4 sites per lane) or as the plane-marching kernel (1 or 2).
+ * kernel (1, 2 or 4 sites per lane) or as the plane-marching kernel (1 or 2); the
+ * fused step kernel can load the next plane before or after the current one's
+ * arithmetic (variants 0, 1; the same sums in the same order).
  * None of these changes a result (no cross-site sums), so the tuner times real
  * loop steps: round r uses kick candidate r mod ncand and tensor candidate
  * r mod ntensor for steps_per_candidate steps of the device-resident loop, with
@@ -175,8 +177,11 @@ typedef struct {
     int ntensor;               /* tensor candidates timed (0 without SIGW) */
     int tensor_code[5];        /* 1, 2, 4: row kernel; 17, 18: march kernel with 1, 2 */
     double tensor_ms[5];       /* mean tensor launch time per candidate */
+    int nmarch;                /* fused step kernel variants timed (128 <= N <= 256) */
+    double march_ms[2];        /* mean fused step kernel time: plain, early loads */
     int kick_elems;            /* chosen kick shape */
     int tensor_code_chosen;    /* chosen tensor shape code (0 without SIGW) */
+    int march_variant;         /* chosen fused step kernel variant */
     int64_t steps;             /* loop steps taken while tuning */
 } ie_tuning;
 ie_status ie_tune(ie_run *run, int steps_per_candidate, ie_tuning *out);

@@ -129,23 +129,34 @@ int rk45_propose(const ie_config &c, double h, double hmax, double err_norm, dou
 
 // Kernel shapes chosen by the tuner, per (N, split storage): sites per lane of the
 // kick and of the tensor launches (0: not tuned yet).
+struct Shapes {
+    int kick_e = 0, tensor_e = 0;   // 0: not tuned
+    int march_pf = -1;              // -1: not tuned
+};
 std::mutex g_shape_mu;
-std::map<std::pair<int, int>, std::pair<int, int>> g_shapes;
+std::map<std::pair<int, int>, Shapes> g_shapes;
 
-void tuned_shapes(int N, int split, int *kick_e, int *tensor_e)
+Shapes tuned_shapes(int N, int split)
 {
     std::lock_guard<std::mutex> lock(g_shape_mu);
     auto it = g_shapes.find({N, split});
-    *kick_e = it == g_shapes.end() ? 0 : it->second.first;
-    *tensor_e = it == g_shapes.end() ? 0 : it->second.second;
+    return it == g_shapes.end() ? Shapes() : it->second;
 }
 
-void store_shapes(int N, int split, int kick_e, int tensor_e)
+void apply_shapes(Fields &F, const Shapes &sh)
+{
+    F.kick_e = sh.kick_e;
+    F.tensor_e = sh.tensor_e;
+    F.march_pf = sh.march_pf > 0 ? 1 : 0;
+}
+
+void store_shapes(int N, int split, int kick_e, int tensor_e, int march_pf)
 {
     std::lock_guard<std::mutex> lock(g_shape_mu);
-    auto &v = g_shapes[{N, split}];
-    if (kick_e > 0) v.first = kick_e;
-    if (tensor_e > 0) v.second = tensor_e;
+    Shapes &v = g_shapes[{N, split}];
+    if (kick_e > 0) v.kick_e = kick_e;
+    if (tensor_e > 0) v.tensor_e = tensor_e;
+    if (march_pf >= 0) v.march_pf = march_pf;
 }
 
 __global__ void k_loop_begin(DevBg *bg)
@@ -231,7 +242,7 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->F.N = c.N;
     r->F.logN = r->logN;
     r->F.sigw = c.calculate_sigw ? 1 : 0;
-    tuned_shapes(c.N, c.precision == IE_SPLIT, &r->F.kick_e, &r->F.tensor_e);
+    apply_shapes(r->F, tuned_shapes(c.N, c.precision == IE_SPLIT));
     for (int p = 0; p < 2; ++p)
         if ((st = dev_alloc(r, &r->F.f[p], r->gs * es)) != IE_OK) return bail(st);
     for (int p = 0; p < 2; ++p)
@@ -509,9 +520,9 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
     if (!r || max_steps < 0) return fail(r, IE_ERR_ARG, "bad argument");
     const int split = r->cfg.precision == IE_SPLIT;
     // first long loop of a kind: tune the kick / tensor shapes on its first steps
-    int ke = 0, te = 0;
-    tuned_shapes(r->N, split, &ke, &te);
-    const bool untuned = march_ok(r->N) ? (r->F.sigw && te == 0) : (ke == 0 || (r->F.sigw && te == 0));
+    const Shapes sh = tuned_shapes(r->N, split);
+    const bool untuned = march_ok(r->N) ? (sh.march_pf < 0 || (r->F.sigw && sh.tensor_e == 0))
+                                        : (sh.kick_e == 0 || (r->F.sigw && sh.tensor_e == 0));
     if (max_steps >= 256 && row_elems(r->N) >= 2 && untuned) {
         ie_tuning tun;
         ie_status st = ie_tune(r, 8, &tun);
@@ -619,14 +630,17 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
         for (int e = 1; e <= 2; e *= 2)
             if (e <= def) out->tensor_code[out->ntensor++] = kTensorMarch | e;
     }
-    // fused steps have one kick shape: only the tensor's is tuned there
-    const int nk = fused ? 1 : out->ncand, nt = out->ntensor;
+    // fused steps have one kick shape and two kernel variants; unfused steps have
+    // kick shapes
+    if (fused) out->nmarch = 2;
+    const int nk = fused ? out->nmarch : out->ncand, nt = out->ntensor;
     const int rounds = std::max(nk, nt);
     if (rounds <= 1) {
         r->F.kick_e = def;
         r->F.tensor_e = nt == 1 ? out->tensor_code[0] : 0;
+        r->F.march_pf = 0;
         out->tensor_code_chosen = r->F.tensor_e;
-        store_shapes(r->N, split, def, r->F.sigw ? std::max(r->F.tensor_e, 1) : 0);
+        store_shapes(r->N, split, def, r->F.sigw ? std::max(r->F.tensor_e, 1) : 0, 0);
         return IE_OK;
     }
     ie_status st = pull_bg(r);
@@ -638,6 +652,7 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
     for (int c = 0; c < rounds; ++c) {
         r->F.kick_e = fused ? def : out->elems[c % nk];
+        r->F.march_pf = fused ? c % nk : 0;
         r->F.tensor_e = nt ? out->tensor_code[c % nt] : 0;
         for (int q = 0; q < K; ++q) {
             hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
@@ -672,11 +687,17 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
         }
     for (auto &e : ev) (void)hipEventDestroy(e);
     if ((st = check_kernel_error(r)) != IE_OK) return st;
-    int bk = 0, bt = 0;
-    for (int c = 0; c < out->ncand; ++c) {
-        const int i = fused ? 0 : c;
-        out->kick_ms[c] = kcnt[i] ? ksum[i] / kcnt[i] : 0.0;
-        if (out->kick_ms[c] < out->kick_ms[bk]) bk = c;
+    int bk = 0, bt = 0, bm = 0;
+    if (fused) {
+        for (int c = 0; c < out->nmarch; ++c) {
+            out->march_ms[c] = kcnt[c] ? ksum[c] / kcnt[c] : 0.0;
+            if (out->march_ms[c] < out->march_ms[bm]) bm = c;
+        }
+    } else {
+        for (int c = 0; c < out->ncand; ++c) {
+            out->kick_ms[c] = kcnt[c] ? ksum[c] / kcnt[c] : 0.0;
+            if (out->kick_ms[c] < out->kick_ms[bk]) bk = c;
+        }
     }
     for (int c = 0; c < nt; ++c) {
         out->tensor_ms[c] = tcnt[c] ? tsum[c] / tcnt[c] : 0.0;
@@ -685,10 +706,13 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     out->steps = r->bg_host->steps - steps0;
     out->kick_elems = fused ? def : out->elems[bk];
     out->tensor_code_chosen = nt ? out->tensor_code[bt] : 0;
+    out->march_variant = fused ? bm : 0;
     r->F.kick_e = out->kick_elems;
     r->F.tensor_e = out->tensor_code_chosen;
+    r->F.march_pf = out->march_variant;
     // timings of a loop that reached af part way are not kept
-    if (out->steps == (int64_t)rounds * K) store_shapes(r->N, split, out->kick_elems, out->tensor_code_chosen);
+    if (out->steps == (int64_t)rounds * K)
+        store_shapes(r->N, split, out->kick_elems, out->tensor_code_chosen, fused ? bm : -1);
     return IE_OK;
 }
 

@@ -56,6 +56,7 @@ struct Fields {
     int N, logN;
     int sigw;
     int kick_e, tensor_e;   // kick sites per lane / tensor shape code (0: default)
+    int march_pf;           // fused step kernel variant: 1 loads the next plane early
 };
 
 // ---------------------------------------------------------------------- helpers

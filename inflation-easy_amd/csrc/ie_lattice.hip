@@ -695,7 +695,9 @@ __device__ __forceinline__ double uniform(double x)
 // SUMS: the sums alone, of the current f taken as the new f (loop start), with
 // the same partition and operation order as a step's, so that a loop split into
 // several calls gives the same bits as one call.
-template <typename T, bool SPLIT, int E, bool SUMS = false>
+// PF: the next plane's rows are loaded before the current plane's arithmetic
+// (more registers, loads in flight across the barrier); the same results.
+template <typename T, bool SPLIT, int E, bool SUMS = false, bool PF = false>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned blk,
                                            T (*lds)[kMarchWaves][64 * E], double *smem)
 {
@@ -738,17 +740,44 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     RowWalk wk{};   // k neighbours within the whole row (one segment)
     wk.segs = 1;
     wk.k0 = 0;
+    T nup[E], njp[E], njm[E], nvin[E];
+    if constexpr (PF && !SUMS) {
+        const unsigned p0 = (i0 & m) << (2 * logN), p1 = ((i0 + 1) & m) << (2 * logN);
+        ld_row<T, E>(f + p1 + rj, lane, nup);
+        ld_row<T, E>(f + p0 + rjp, lane, njp);
+        ld_row<T, E>(f + p0 + rjm, lane, njm);
+        ld_row<T, E>(fd + p0 + rj, lane, nvin);
+    }
     for (unsigned t = 0; t <= TI; ++t) {
         const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
-        ld_row<T, E>(f + pu + rj, lane, fup);
+        if constexpr (PF && !SUMS) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                fup[e] = nup[e];
+                fjp[e] = njp[e];
+                fjm[e] = njm[e];
+                vin[e] = nvin[e];
+            }
+            if (t < TI) {
+                const unsigned pn = ((i0 + t + 2) & m) << (2 * logN);
+                ld_row<T, E>(f + pn + rj, lane, nup);
+                ld_row<T, E>(f + pu + rjp, lane, njp);
+                ld_row<T, E>(f + pu + rjm, lane, njm);
+                ld_row<T, E>(fd + pu + rj, lane, nvin);
+            }
+        } else {
+            ld_row<T, E>(f + pu + rj, lane, fup);
+        }
         T fnew[E];
         if constexpr (SUMS) {
 #pragma unroll
             for (int e = 0; e < E; ++e) fnew[e] = fcur[e];
         } else {
-            ld_row<T, E>(f + pi + rjp, lane, fjp);
-            ld_row<T, E>(f + pi + rjm, lane, fjm);
-            ld_row<T, E>(fd + pi + rj, lane, vin);
+            if constexpr (!PF) {
+                ld_row<T, E>(f + pi + rjp, lane, fjp);
+                ld_row<T, E>(f + pi + rjm, lane, fjm);
+                ld_row<T, E>(fd + pi + rj, lane, vin);
+            }
             T kp_last, km_first;
             k_neighbours<T, E>(fcur, f + pi + rj, wk, N, lane, kp_last, km_first);
             T vout[E];
@@ -838,6 +867,25 @@ __global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg 
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
     march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) void k_march_pf(Fields F, Pot P, DevBg *bg, double *partials)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    march_body<T, SPLIT, E, false, true>(F, P, bg, partials, blockIdx.x, lds, smem);
+}
+
+template <typename T, bool SPLIT, int E>
+__global__ __launch_bounds__(kMarchThreads) void kb_march_pf(const RunDesc *__restrict__ runs,
+                                                              const uint2 *__restrict__ map)
+{
+    __shared__ T lds[2][kMarchWaves][64 * E];
+    __shared__ double smem[3 * kMarchWaves];
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    march_body<T, SPLIT, E, false, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -1464,6 +1512,11 @@ template <typename T, bool SPLIT>
 void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
 {
     const int E = row_elems(F.N);
+    if (F.march_pf) {
+        if (E == 2) k_march_pf<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        else k_march_pf<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        return;
+    }
     if constexpr (sizeof(T) == 4) {
         if (E == 1) k_march_w5<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
         else if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
@@ -1672,6 +1725,7 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
 struct BatchGroup {
     int E = 0, kE = 0, tE = 0;   // elements per lane: reduce, kick, tensor
     int march = 0;               // fused steps (march kernel) for this group
+    int pf = 0;                  // the march kernel variant with early loads
     uint2 *map = nullptr;        // kick / march workgroups
     uint2 *rmap = nullptr;       // reduce workgroups (march groups: loop start only)
     uint2 *tmap = nullptr;       // tensor workgroups
@@ -1709,7 +1763,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         const int tc = runs[r].F.tensor_e;
         const int tE = row_elems(N) == 0 ? 0
                        : tensor_is_march(tc) ? kTensorMarch | tensor_march_elems(N, tc) : shape_elems(N, tc);
-        const int key = d[r].march * 100000 + row_elems(N) * 10000 + kE * 100 + tE;
+        const int pf = d[r].march && runs[r].F.march_pf ? 1 : 0;
+        const int key = pf * 1000000 + d[r].march * 100000 + row_elems(N) * 10000 + kE * 100 + tE;
         size_t g = 0;
         while (g < keys.size() && keys[g] != key) ++g;
         if (g == keys.size()) {
@@ -1737,7 +1792,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     };
     for (size_t g = 0; g < keys.size() && e == hipSuccess; ++g) {
         BatchGroup G;
-        G.march = keys[g] / 100000;
+        G.pf = keys[g] / 1000000;
+        G.march = keys[g] / 100000 % 10;
         G.E = keys[g] / 10000 % 10;
         G.kE = keys[g] / 100 % 100;
         G.tE = keys[g] % 100;
@@ -1806,7 +1862,10 @@ static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream
 template <typename T, bool SPLIT>
 static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
-    if (G.march) {
+    if (G.march && G.pf) {
+        if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+        else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+    } else if (G.march) {
         if constexpr (sizeof(T) == 4) {
             if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
             else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);

@@ -91,12 +91,15 @@ class Tuning(C.Structure):
 
     _fields_ = [("ncand", C.c_int), ("elems", C.c_int * 3), ("kick_ms", C.c_double * 3),
                 ("ntensor", C.c_int), ("tensor_code", C.c_int * 5), ("tensor_ms", C.c_double * 5),
-                ("kick_elems", C.c_int), ("tensor_code_chosen", C.c_int), ("steps", C.c_int64)]
+                ("nmarch", C.c_int), ("march_ms", C.c_double * 2),
+                ("kick_elems", C.c_int), ("tensor_code_chosen", C.c_int), ("march_variant", C.c_int),
+                ("steps", C.c_int64)]
 
     def as_dict(self) -> dict:
         n, t = self.ncand, self.ntensor
         return {"candidates": list(self.elems[:n]), "kick_ms": list(self.kick_ms[:n]),
                 "tensor_codes": list(self.tensor_code[:t]), "tensor_ms": list(self.tensor_ms[:t]),
+                "march_ms": list(self.march_ms[:self.nmarch]), "march_variant": self.march_variant,
                 "kick_elems": self.kick_elems, "tensor_code": self.tensor_code_chosen, "steps": self.steps}
 
 

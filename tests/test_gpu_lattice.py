@@ -236,9 +236,10 @@ def test_segmented_rows_match_oracle_n512(precision):
 
 def test_tuned_shapes_leave_results_unchanged():
     """The tuner times real loop steps with the tensor step as the row kernel (1 and
-    2 sites per lane) and as the plane-marching kernel (1 and 2), N = 128 with fused
-    scalar steps; the tensor kernels have no cross-site sums, so a run that changes
-    shape between steps matches one that never does, bit for bit."""
+    2 sites per lane) and as the plane-marching kernel (1 and 2), and both variants
+    of the fused step kernel, N = 128; none changes an operation or a sum order, so
+    a run that changes shape between steps matches one that never does, bit for
+    bit."""
     st, _, table = oracle_state("hilltop_n32", N=128, calculate_sigw=1)
     ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
     rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
@@ -247,8 +248,8 @@ def test_tuned_shapes_leave_results_unchanged():
         r.evolve_fields(0.5 * dt)
     t = ra.tune(4)
     assert t["tensor_codes"] == [1, 2, 17, 18] and t["steps"] == 16
-    assert t["tensor_code"] in (1, 2, 17, 18)
-    assert all(x > 0 for x in t["tensor_ms"])
+    assert t["tensor_code"] in (1, 2, 17, 18) and t["march_variant"] in (0, 1)
+    assert all(x > 0 for x in t["tensor_ms"] + t["march_ms"]) and len(t["march_ms"]) == 2
     ra.leapfrog_steps(12)
     rb.leapfrog_steps(28)
     assert ra.background()["steps"] == rb.background()["steps"]
