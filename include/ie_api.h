@@ -205,6 +205,20 @@ ie_status ie_spectrum(ie_run *run, int which, double norm1, double *p, int *coun
  * numbins = int(sqrt(3)*(N/2))+1; bins >= floor(0.8 numbins) are empty. */
 ie_status ie_gw_spectrum(ie_run *run, int hdot, double *p, int *counts, double *vals);
 
+/* --- curvature and tensor power spectra ------------------------------------ */
+/* The analysis notebook's formulas (notebooks/plot.ipynb cells 9, 23) on binned
+ * spectra, each row paired with the modes.dat row of the same index (k[b], from
+ * ie_modes): P_zeta = P_phi k^3/(2 pi^2) (phidot/H)^-2 (phidot and H as written
+ * to velocity.dat and sf.dat); P_zeta = P_dN k^3/(2 pi^2); Delta_h^2 =
+ * k^3/(2 pi^2) P_h; Omega_GW = k^3/(2 pi^2) (P_hdot + (k/a)^2 P_h) / (12 H^2).
+ * Host functions; out has nbins entries. */
+ie_status ie_pzeta_linear(int nbins, const double *k, const double *p_phi, double phidot, double hubble,
+                          double *out);
+ie_status ie_pzeta_deltaN(int nbins, const double *k, const double *p_dn, double *out);
+ie_status ie_delta_h2(int nbins, const double *k, const double *p_h, double *out);
+ie_status ie_omega_gw(int nbins, const double *k, const double *p_h, const double *p_hdot, double a, double hubble,
+                      double *out);
+
 /* --- deltaN ----------------------------------------------------------------- */
 /* initializeN (src/initialize.cpp:219-232): fd -> dphi/dN, deltaN = 0. */
 ie_status ie_deltaN_init(ie_run *run);
@@ -275,7 +289,7 @@ typedef struct ie_result {
     int64_t steps;             /* inflation steps taken */
     double a_end;
     int32_t nbins;             /* P_zeta shells written to pzeta (<= 512) */
-    double pzeta[512];         /* spectraN values (norm1 <|zeta_k|^2>) */
+    double pzeta[512];         /* P_zeta(k) = k^3/(2 pi^2) spectraN on the modes.dat k (ie_pzeta_deltaN) */
     double fp64_max_rel_err;   /* max over shells of |P_split - P_fp64| / P_fp64, -1 if not checked */
 } ie_result;
 ie_status ie_server_create(size_t hbm_budget_bytes, int quantum_steps, int finish_threads, ie_server **out);

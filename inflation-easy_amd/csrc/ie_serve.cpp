@@ -98,8 +98,13 @@ ie_status finish_run(ie_run *run, const ie_config &c, int deltaN, double *out, i
     std::vector<int> cnt(bins);
     const double norm1 = pow(c.L / c.rescale_B, 3) / pow((double)c.N, 6);
     if ((st = ie_spectrum(run, IE_FIELD_DELTAN, norm1, p.data(), cnt.data(), v.data())) != IE_OK) return st;
-    *nb = std::min(bins, 512);
-    for (int q = 0; q < *nb; ++q) out[q] = v[q];
+    // P_zeta(k) = k^3/(2 pi^2) P_dN on the modes.dat k (notebooks/plot.ipynb cell 9)
+    const int nm = (int)(sqrt(3.0) * (c.N / 2)) + 1;
+    std::vector<double> k(nm), pz(nm);
+    if ((st = ie_modes(c.N, c.L, c.rescale_B, k.data())) != IE_OK) return st;
+    *nb = std::min(std::min(bins, nm), 512);
+    if ((st = ie_pzeta_deltaN(*nb, k.data(), v.data(), pz.data())) != IE_OK) return st;
+    for (int q = 0; q < *nb; ++q) out[q] = pz[q];
     return IE_OK;
 }
 

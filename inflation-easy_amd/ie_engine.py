@@ -178,6 +178,10 @@ def lib() -> C.CDLL:
             "ie_server_pop_result": (st, [P, C.POINTER(Result)]),
             "ie_server_clock": (C.c_double, [P]),
             "ie_server_destroy": (st, [P]),
+            "ie_pzeta_linear": (st, [C.c_int, dp, dp, C.c_double, C.c_double, dp]),
+            "ie_pzeta_deltaN": (st, [C.c_int, dp, dp, dp]),
+            "ie_delta_h2": (st, [C.c_int, dp, dp, dp]),
+            "ie_omega_gw": (st, [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp]),
             "ie_version": (C.c_char_p, []),
             "ie_device_count": (C.c_int, []),
         }
@@ -523,3 +527,46 @@ def modes(N: int, L: float, B: float) -> np.ndarray:
     if st:
         raise EngineError(st, lib().ie_last_error(None).decode())
     return out
+
+
+def _f64(x) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _spectrum_call(fn, k, *arrays_and_scalars) -> np.ndarray:
+    k = _f64(k)
+    out = np.empty_like(k)
+    args = [len(k), _dp(k)]
+    keep = [k]
+    for a in arrays_and_scalars:
+        if isinstance(a, (int, float, np.floating)):
+            args.append(float(a))
+        else:
+            arr = _f64(a)
+            keep.append(arr)
+            args.append(_dp(arr))
+    args.append(_dp(out))
+    status = fn(*args)
+    if status != 0:
+        raise EngineError(status, "bad argument")
+    return out
+
+
+def pzeta_linear(k, p_phi, phidot: float, hubble: float) -> np.ndarray:
+    """ie_pzeta_linear: P_phi k^3/(2 pi^2) (phidot/H)^-2."""
+    return _spectrum_call(lib().ie_pzeta_linear, k, p_phi, phidot, hubble)
+
+
+def pzeta_deltaN(k, p_dn) -> np.ndarray:
+    """ie_pzeta_deltaN: P_dN k^3/(2 pi^2)."""
+    return _spectrum_call(lib().ie_pzeta_deltaN, k, p_dn)
+
+
+def delta_h2(k, p_h) -> np.ndarray:
+    """ie_delta_h2: k^3/(2 pi^2) P_h."""
+    return _spectrum_call(lib().ie_delta_h2, k, p_h)
+
+
+def omega_gw(k, p_h, p_hdot, a: float, hubble: float) -> np.ndarray:
+    """ie_omega_gw: k^3/(2 pi^2) (P_hdot + (k/a)^2 P_h) / (12 H^2)."""
+    return _spectrum_call(lib().ie_omega_gw, k, p_h, p_hdot, a, hubble)

@@ -336,3 +336,35 @@ def read_blocks(path: str | os.PathLike) -> list[np.ndarray]:
     if cur:
         blocks.append(np.array(cur, dtype=np.float64))
     return blocks
+
+
+# ------------------------------------------------------------------ analysis
+# The reference's analysis notebook (notebooks/plot.ipynb) turns the written
+# spectra into curvature and tensor power spectra; each spectrum row pairs with
+# the modes.dat row of the same index.
+
+def pzeta_linear(k, p_phi, phidot, hubble):
+    """P_zeta = P_phi k^3/(2 pi^2) (phidot/H)^-2 (notebooks/plot.ipynb cell 9, with
+    phidot = velocity.dat[-1, 2] and H = sf.dat[-1, 2])."""
+    k = np.asarray(k, dtype=np.float64)
+    return np.asarray(p_phi, dtype=np.float64) * k ** 3 / (2 * np.pi ** 2) * (phidot / hubble) ** (-2)
+
+
+def pzeta_deltaN(k, p_dn):
+    """P_zeta = P_dN k^3/(2 pi^2) (notebooks/plot.ipynb cell 9)."""
+    k = np.asarray(k, dtype=np.float64)
+    return np.asarray(p_dn, dtype=np.float64) * k ** 3 / (2 * np.pi ** 2)
+
+
+def delta_h2(k, p_h):
+    """Delta_h^2 = k^3/(2 pi^2) P_h (notebooks/plot.ipynb cell 23)."""
+    k = np.asarray(k, dtype=np.float64)
+    return (k ** 3 / (2.0 * np.pi ** 2)) * np.asarray(p_h, dtype=np.float64)
+
+
+def omega_gw(k, p_h, p_hdot, a, hubble):
+    """Omega_GW = 1/(12 H^2) k^3/(2 pi^2) (P_hdot + (k/a)^2 P_h) (notebooks/plot.ipynb
+    cell 23, with a = sf.dat[t, 1], H = sf.dat[t, 2])."""
+    k = np.asarray(k, dtype=np.float64)
+    term2 = (k / a) ** 2 * np.asarray(p_h, dtype=np.float64)
+    return (1.0 / (12.0 * hubble ** 2)) * (k ** 3 / (2.0 * np.pi ** 2)) * (np.asarray(p_hdot, dtype=np.float64) + term2)
