@@ -12,8 +12,8 @@
 # written, to oracle/_ref/.  The temporary directory is removed afterwards.  The flags follow Makefile:2,28-31 (-std=c++17 -O3 -DNDEBUG, plus
 # -fopenmp when available).
 #
-# usage: oracle/build_ref.sh N NUMERICAL(0|1) SIGW(0|1) DELTAN(0|1) [OMP(0|1)]
-# output: oracle/_ref/ie_ref_n${N}_p${NUMERICAL}_g${SIGW}_d${DELTAN}_o${OMP}
+# usage: oracle/build_ref.sh N NUMERICAL(0|1) SIGW(0|1) DELTAN(0|1) [OMP(0|1)] [POST(0|1)]
+# output: oracle/_ref/ie_ref_n${N}_p${NUMERICAL}_g${SIGW}_d${DELTAN}_o${OMP}[_post]
 set -euo pipefail
 
 REF=${IE_REFERENCE_DIR:-/root/reference}
@@ -25,6 +25,7 @@ NUM=${2:?numerical_potential}
 SIGW=${3:?calculate_SIGW}
 DN=${4:?perform_deltaN}
 OMP=${5:-1}
+POST=${6:-0}
 
 if [ ! -f "$REF/src/parameters.h" ]; then
     echo "reference sources not found under $REF" >&2
@@ -33,19 +34,21 @@ fi
 
 mkdir -p "$OUT"
 TAG="n${N}_p${NUM}_g${SIGW}_d${DN}_o${OMP}"
+[ "$POST" = "1" ] && TAG="${TAG}_post"
 BIN="$OUT/ie_ref_${TAG}"
 
 TMP=$(mktemp -d /tmp/ie_ref_cfg.XXXXXX)
 trap 'rm -rf "$TMP"' EXIT
 
-# Compile-time edits only: N, the three feature toggles, post_inflation off
-# (post-inflation is outside this engine), and the OpenMP toggle.
+# Compile-time edits only: N, the three feature toggles, the post-inflation
+# toggle (it requires calculate_SIGW = 1, src/parameters.h:33-35) and the OpenMP
+# toggle.
 sed -E \
     -e "s/^const int N = [0-9]+;/const int N = ${N};/" \
     -e "s/^#define numerical_potential [01]/#define numerical_potential ${NUM}/" \
     -e "s/^#define perform_deltaN [01]/#define perform_deltaN ${DN}/" \
     -e "s/^#define calculate_SIGW [01]/#define calculate_SIGW ${SIGW}/" \
-    -e "s/^#define post_inflation [01]/#define post_inflation 0/" \
+    -e "s/^#define post_inflation [01]/#define post_inflation ${POST}/" \
     -e "s/^#define parallel_calculation [01]/#define parallel_calculation ${OMP}/" \
     "$REF/src/parameters.h" > "$TMP/parameters.h"
 

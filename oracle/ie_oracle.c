@@ -56,6 +56,7 @@ struct ieo_state {
     double dx, pi;
     double t, t0, a, ad, ad2, astep, Ne, phiref, hubble_init;
     double phiref_potential;
+    int post_rhs;              /* RK stages use the post-inflation right-hand side */
     double *f, *fd, *dn;
     double *fny, *fdny;
     float *h[6], *hd[6];
@@ -124,6 +125,12 @@ void ieo_config_defaults(ieo_config *c, int N, int numerical)
     c->nbins = 256;
     c->int_err = 5;
     c->int_errN = 5;
+    c->post_inflation = 0;
+    c->horizon_factor = 1.0;
+    c->omega = 1.0 / 3.0;
+    c->dt_post = 0.001;
+    c->af_post = 2.0 * N;
+    c->post_integrator = IEO_LEAPFROG;
 }
 
 /* src/runtime_parameters.cpp:342-353 and :184-192 */
@@ -524,6 +531,135 @@ void ieo_evolve_fields(ieo_state *st, double d)
     st->a += d * st->ad;
 }
 
+/* ---------------------------------------------------------- post-inflation */
+
+static inline int wrap_up(int N, int x) { return x == N - 1 ? 0 : x + 1; }
+static inline int wrap_dn(int N, int x) { return x == 0 ? N - 1 : x - 1; }
+
+/* d2_same_state (src/evolution.cpp:79-91): second difference along dim. */
+static inline double d2_same(int N, double dx, const double *g, int dim, int i, int j, int k)
+{
+    const double inv_dx2 = 1.0 / (dx * dx);
+    const double c = g[IX(N, i, j, k)];
+    if (dim == 0) return (g[IX(N, wrap_up(N, i), j, k)] - 2.0 * c + g[IX(N, wrap_dn(N, i), j, k)]) * inv_dx2;
+    if (dim == 1) return (g[IX(N, i, wrap_up(N, j), k)] - 2.0 * c + g[IX(N, i, wrap_dn(N, j), k)]) * inv_dx2;
+    return (g[IX(N, i, j, wrap_up(N, k))] - 2.0 * c + g[IX(N, i, j, wrap_dn(N, k))]) * inv_dx2;
+}
+
+/* d2_cross_state (src/evolution.cpp:93-106).  Each lattice direction named by d1
+ * or d2 is stepped "+" in the first two samples' first slot and "-" in the other
+ * two, with every named direction moving together: (x, y) and (x, z) are the usual
+ * four-corner cross differences, while for (y, z) the samples are f(i, j+1, k+1),
+ * f(i, j-1, k-1), f(i, j+1, k+1), f(i, j-1, k-1), whose combination
+ * ((A - B) - A) + B is the rounding residue of A - B; it is kept as written. */
+static inline double d2_cross(int N, double dx, const double *g, int d1, int d2, int i, int j, int k)
+{
+    const int xi = d1 == 0 || d2 == 0, yj = d1 == 1 || d2 == 1, zk = d1 == 2 || d2 == 2;
+    const int ip = xi ? wrap_up(N, i) : i, im = xi ? wrap_dn(N, i) : i;
+    const int jp = yj ? wrap_up(N, j) : j, jm = yj ? wrap_dn(N, j) : j;
+    const int kp = zk ? wrap_up(N, k) : k, km = zk ? wrap_dn(N, k) : k;
+    const double fpp = g[IX(N, ip, jp, kp)];
+    const double fpm = g[IX(N, ip, jm, km)];
+    const double fmp = g[IX(N, im, jp, kp)];
+    const double fmm = g[IX(N, im, jm, km)];
+    return (fpp - fpm - fmp + fmm) / (4.0 * dx * dx);
+}
+
+/* Bare tensor source of one site (src/evolution.cpp:1474-1503, 549-570): gradients
+ * of Phi and Phi', the Hessian of Phi, in the order xx yy zz xy xz yz. */
+static void post_source(int N, double dx, const double *f, const double *fd, int i, int j, int k,
+                        double invH, double coeffU, double S[6])
+{
+    double g[3], gd[3];
+    grad_site(N, dx, f, i, j, k, g);
+    grad_site(N, dx, fd, i, j, k, gd);
+    const double H[6] = {d2_same(N, dx, f, 0, i, j, k), d2_same(N, dx, f, 1, i, j, k),
+                         d2_same(N, dx, f, 2, i, j, k), d2_cross(N, dx, f, 0, 1, i, j, k),
+                         d2_cross(N, dx, f, 0, 2, i, j, k), d2_cross(N, dx, f, 1, 2, i, j, k)};
+    const double Phi = f[IX(N, i, j, k)];
+    const double du[3] = {gd[0] * invH + g[0], gd[1] * invH + g[1], gd[2] * invH + g[2]};
+    static const int L[6] = {0, 1, 2, 0, 0, 1}, M[6] = {0, 1, 2, 1, 2, 2};
+    for (int c = 0; c < 6; ++c)
+        S[c] = 4.0 * Phi * H[c] + 2.0 * g[L[c]] * g[M[c]] - coeffU * (du[L[c]] * du[M[c]]);
+}
+
+/* initialize_post_inflation (src/initialize.cpp:246-291) without its save: Phi
+ * from the deltaN map (or, without deltaN, from f and the mean velocity), fd kept
+ * as the previous phase left it, tensors zeroed, t = 0, a = 1,
+ * ad = horizon_factor N 2 pi / L. */
+void ieo_post_initialize(ieo_state *st)
+{
+    const ieo_config *c = &st->cfg;
+    const size_t gs = st->gs;
+    st->t0 = 0.0;
+    st->t = 0.0;
+    for (int q = 0; q < 6; ++q) {
+        memset(st->h[q], 0, gs * sizeof(float));
+        memset(st->hd[q], 0, gs * sizeof(float));
+    }
+    const double Phi_from_zeta = 3.0 * (1.0 + c->omega) / (5.0 + 3.0 * c->omega);
+    if (c->perform_deltaN) {
+        double Nmean = 0.;
+        for (size_t id = 0; id < gs; ++id) Nmean += st->dn[id];
+        Nmean = Nmean / (double)gs;
+        for (size_t id = 0; id < gs; ++id) st->f[id] = Phi_from_zeta * (st->dn[id] - Nmean);
+    } else {
+        double fmean = 0.0, fdmean = 0.0;
+        for (size_t id = 0; id < gs; ++id) {
+            fmean += st->f[id];
+            fdmean += st->fd[id];
+        }
+        fmean = fmean / (double)gs;
+        fdmean = fdmean / (double)gs;
+        const double s = c->rescale_s;
+        for (size_t id = 0; id < gs; ++id)
+            st->f[id] = Phi_from_zeta * (st->f[id] - fmean)
+                      / (fdmean * pow(st->a, s - 1.0) / (st->ad * pow(st->a, s - 2.0)));
+    }
+    st->a = 1.0;
+    st->ad = c->horizon_factor * st->N * 2.0 * st->pi / c->L;
+}
+
+/* evolve_derivs_post_inflation (src/evolution.cpp:1437-1564): half kicks of ad
+ * with the fluid ad2, the Phi kick, then the tensor kick whose source uses the
+ * kicked Phi' of every site. */
+void ieo_post_evolve_derivs(ieo_state *st, double d)
+{
+    const int N = st->N;
+    const double s = st->cfg.rescale_s, w = st->cfg.omega;
+    const double laplnorm = pow(st->a, -2.0 * s) / sq(st->dx);
+    st->ad2 = -(s - 0.5 * (1.0 - 3.0 * w)) * sq(st->ad) / st->a;
+    st->ad += 0.5 * d * st->ad2;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                const size_t id = IX(N, i, j, k);
+                st->fd[id] += d * (w * laplnorm * lapl_d(N, st->f, i, j, k)
+                                   - (3.0 * (1.0 + w) + s) * st->ad * st->fd[id] / st->a);
+            }
+    const double Htilde = st->ad / st->a;
+    const double invH = 1.0 / Htilde;
+    const double coeffU = 4.0 / (3.0 * (1.0 + w));
+    const double srcAmp = 2.0 * pow(st->a, -2.0 * s);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                const size_t id = IX(N, i, j, k);
+                double S[6];
+                post_source(N, st->dx, st->f, st->fd, i, j, k, invH, coeffU, S);
+                for (int c = 0; c < 6; ++c) {
+                    const double lap_h = (double)lapl_f(N, st->h[c], i, j, k);
+                    const double rhs = d * (laplnorm * lap_h
+                                            - (2.0 + s) * st->ad * (double)st->hd[c][id] / st->a
+                                            + srcAmp * S[c]);
+                    st->hd[c][id] += (float)rhs;
+                }
+            }
+    st->ad += 0.5 * d * st->ad2;
+}
+
 static double rescaled_step(const ieo_state *st, double astep_value)
 {
     return st->cfg.dt * pow(astep_value, st->cfg.rescale_s - 1.0);
@@ -678,6 +814,40 @@ static void inflation_rhs(ieo_state *st, const double *fs, const double *fds,
     *daddt = pow(a_s, 3.0 - 2.0 * s) * source - (s + 1.0) * sq(ad_s) / a_s;
 }
 
+/* Post-inflation right-hand side (src/evolution.cpp:520-592). */
+static void post_rhs(ieo_state *st, const double *fs, const double *fds,
+                     float *const *hs, float *const *hds,
+                     double *dfdt, double *dfddt, float **dhdt, float **dhddt,
+                     double a_s, double ad_s, double *dadt, double *daddt)
+{
+    const int N = st->N;
+    const double s = st->cfg.rescale_s, w = st->cfg.omega;
+    const double laplnorm = pow(a_s, -2.0 * s) / sq(st->dx);
+    const double scalar_friction = (3.0 * (1.0 + w) + s) * ad_s / a_s;
+    const double tensor_friction = (2.0 + s) * ad_s / a_s;
+    const double Htilde = ad_s / a_s;
+    const double invH = 1.0 / Htilde;
+    const double coeffU = 4.0 / (3.0 * (1.0 + w));
+    const double srcAmp = 2.0 * pow(a_s, -2.0 * s);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                const size_t id = IX(N, i, j, k);
+                const double lap_f = lapl_d(N, fs, i, j, k);
+                dfdt[id] = fds[id];
+                dfddt[id] = w * laplnorm * lap_f - scalar_friction * fds[id];
+                double S[6];
+                post_source(N, st->dx, fs, fds, i, j, k, invH, coeffU, S);
+                for (int c = 0; c < 6; ++c) dhdt[c][id] = hds[c][id];
+                for (int c = 0; c < 6; ++c)
+                    dhddt[c][id] = (float)(laplnorm * (double)lapl_f(N, hs[c], i, j, k)
+                                           - tensor_friction * (double)hds[c][id] + srcAmp * S[c]);
+            }
+    *dadt = ad_s;
+    *daddt = -(s - 0.5 * (1.0 - 3.0 * w)) * sq(ad_s) / a_s;
+}
+
 /* Stage state y + h * sum_s A[stage][s] k_s (src/evolution.cpp:639-713). */
 static void rk_stage(ieo_state *st, rk_scratch *sc, double h, int stage, const double *A, int ns,
                      const double *ka, const double *kad, double *a_s, double *ad_s)
@@ -720,13 +890,18 @@ static void rk_stage(ieo_state *st, rk_scratch *sc, double h, int stage, const d
 static void rk_eval_stages(ieo_state *st, rk_scratch *sc, double h, const double *A, int ns,
                            double *ka, double *kad)
 {
-    inflation_rhs(st, st->f, st->fd, st->h, st->hd, sc->kf[0], sc->kfd[0], sc->kh[0], sc->khd[0],
-                  st->a, st->ad, &ka[0], &kad[0]);
+    /* compute_inflation_rhs dispatches on the post-inflation mode flag
+     * (src/evolution.cpp:325,364-371,439) */
+    void (*rhs)(ieo_state *, const double *, const double *, float *const *, float *const *, double *,
+                double *, float **, float **, double, double, double *, double *) =
+        st->post_rhs ? post_rhs : inflation_rhs;
+    rhs(st, st->f, st->fd, st->h, st->hd, sc->kf[0], sc->kfd[0], sc->kh[0], sc->khd[0],
+        st->a, st->ad, &ka[0], &kad[0]);
     for (int stage = 1; stage < ns; ++stage) {
         double a_s, ad_s;
         rk_stage(st, sc, h, stage, A, ns, ka, kad, &a_s, &ad_s);
-        inflation_rhs(st, sc->ftmp, sc->fdtmp, sc->htmp, sc->hdtmp, sc->kf[stage], sc->kfd[stage],
-                      sc->kh[stage], sc->khd[stage], a_s, ad_s, &ka[stage], &kad[stage]);
+        rhs(st, sc->ftmp, sc->fdtmp, sc->htmp, sc->hdtmp, sc->kf[stage], sc->kfd[stage],
+            sc->kh[stage], sc->khd[stage], a_s, ad_s, &ka[stage], &kad[stage]);
     }
 }
 
@@ -872,6 +1047,23 @@ int ieo_rk45_step(ieo_state *st, double h, double hmax, double *h_next)
     return accepted;
 }
 
+/* rk4_step_inflation / rk45_step_inflation under ScopedPostInflationRhsMode
+ * (src/evolution.cpp:1606-1640). */
+void ieo_post_rk4_step(ieo_state *st, double h)
+{
+    st->post_rhs = 1;
+    ieo_rk4_step(st, h);
+    st->post_rhs = 0;
+}
+
+int ieo_post_rk45_step(ieo_state *st, double h, double hmax, double *h_next)
+{
+    st->post_rhs = 1;
+    const int acc = ieo_rk45_step(st, h, hmax, h_next);
+    st->post_rhs = 0;
+    return acc;
+}
+
 /* ---------------------------------------------------------------- spectra */
 
 int ieo_spectrum_numbins(int N) { return (int)(1.73205 * (N / 2)) + 1; }
@@ -991,6 +1183,9 @@ void ieo_gw_spectrum(ieo_state *st, int hdot, double *p_out, int *counts, double
     float **h = hdot ? st->hd : st->h;
     float **hn = hdot ? st->hdny : st->hny;
     const double to_phys = hdot ? st->cfg.rescale_B * pow(st->a, st->cfg.rescale_s - 1.0) : 1.0;
+    /* the post-inflation variant rounds the factor to float and multiplies in
+     * float (spectraGWdot_post_inflation, src/output.cpp:662-669) */
+    const float tp = (float)(st->cfg.rescale_B * pow(st->a, st->cfg.rescale_s - 1.f));
     float *f2 = (float *)calloc(nb, sizeof(float));
     for (int b = 0; b < nb; ++b) counts[b] = 0;
     for (int c = 0; c < 6; ++c) ieo_fftrn_f(h[c], hn[c], N, 1);
@@ -1024,7 +1219,10 @@ void ieo_gw_spectrum(ieo_state *st, int hdot, double *p_out, int *counts, double
                             re = hn[comp][(size_t)i * 2 * N + 2 * j];
                             im = hn[comp][(size_t)i * 2 * N + 2 * j + 1];
                         }
-                        if (hdot) {
+                        if (hdot == 2) {
+                            re *= tp;
+                            im *= tp;
+                        } else if (hdot) {
                             re = (float)((double)re * to_phys);
                             im = (float)((double)im * to_phys);
                         }
@@ -1435,8 +1633,9 @@ static void out_close(ieo_out *o)
     o->hist_buf = NULL;
 }
 
-/* meansvars (src/output.cpp:204-269). Returns 1 when the mean is not finite. */
-static int out_meansvars(ieo_state *st, ieo_out *o)
+/* meansvars (src/output.cpp:204-269; the post-inflation copy, :1575-1636, tests
+ * against FLT_MAX). Returns 1 when the mean is not finite. */
+static int out_meansvars(ieo_state *st, ieo_out *o, double big)
 {
     FILE *m = out_open(o, &o->means, "means"), *v = out_open(o, &o->vars, "variance"),
          *w = out_open(o, &o->vel, "velocity");
@@ -1455,7 +1654,7 @@ static int out_meansvars(ieo_state *st, ieo_out *o)
     fprintf(m, " %e", av);
     fprintf(w, " %e", vel);
     fprintf(v, " %e", var - sq(av));
-    if (av + DBL_MAX == av || (av != 0. && av / av != 1.)) {
+    if (av + big == av || (av != 0. && av / av != 1.)) {
         printf("Unstable solution developed. Scalar field not numerical at t=%f\n", st->t);
         fflush(m); fflush(v);
         return 1;
@@ -1536,7 +1735,7 @@ static int program_save(ieo_state *st, ieo_out *o, int infrequent)
     const int N = st->N;
     const int staggered = c->integrator == IEO_LEAPFROG;
     if (staggered && st->t > 0.) ieo_evolve_fields(st, -.5 * c->dt * pow(st->astep, c->rescale_s - 1));
-    if (out_meansvars(st, o)) return 1;
+    if (out_meansvars(st, o, DBL_MAX)) return 1;
     out_scale(st, o);
     if (infrequent) {
         if (c->output_box3D) {
@@ -1588,12 +1787,13 @@ static int program_save(ieo_state *st, ieo_out *o, int infrequent)
 
 /* RK45 accept/reject driver (src/evolution.cpp:346-362). Returns the next step or
  * a negative value on failure. */
-static double rk45_drive(ieo_state *st, double h, double hmax, int deltaN_phase)
+static double rk45_drive(ieo_state *st, double h, double hmax, int phase)
 {
     int accepted = 0, attempts = 0;
     while (!accepted) {
         double hs = h;
-        accepted = deltaN_phase ? ieo_rk45_stepN(st, h, hmax, &hs) : ieo_rk45_step(st, h, hmax, &hs);
+        accepted = phase == 1 ? ieo_rk45_stepN(st, h, hmax, &hs)
+                 : phase == 2 ? ieo_post_rk45_step(st, h, hmax, &hs) : ieo_rk45_step(st, h, hmax, &hs);
         h = hs;
         ++attempts;
         if (!accepted && (attempts > 25 || h <= st->cfg.rk45_min_dt * (1.0 + 1e-12))) {
@@ -1745,6 +1945,87 @@ static int program_deltaN(ieo_state *st, ieo_out *o)
     return 0;
 }
 
+/* save_post_inflation (src/output.cpp:1839-1866): the same outputs as save()
+ * under post_inflation/, with the float factor of spectraGWdot_post_inflation. */
+static int program_save_post(ieo_state *st, ieo_out *o, int infrequent)
+{
+    const ieo_config *c = &st->cfg;
+    const int N = st->N;
+    const int staggered = c->post_integrator == IEO_LEAPFROG;
+    if (staggered && st->t > 0.) ieo_evolve_fields(st, -.5 * c->dt_post);
+    if (out_meansvars(st, o, FLT_MAX)) return 1;
+    out_scale(st, o);
+    if (infrequent) {
+        if (c->output_spectra) {
+            const int nb = ieo_spectrum_numbins(N);
+            double *p = calloc(nb, sizeof(double)), *v = calloc(nb, sizeof(double));
+            int *cnt = calloc(nb, sizeof(int));
+            ieo_spectrum_f(st, p, cnt, v);
+            out_bins(out_open(o, &o->spectra, "spectra"), nb, p, cnt, v);
+            fprintf(out_open(o, &o->spectratimes, "spectratimes"), "%f %e\n", st->t, st->a);
+            const int ngw = (int)(sqrt(3.0) * (N / 2)) + 1;
+            double *pg = calloc(ngw, sizeof(double)), *vg = calloc(ngw, sizeof(double));
+            int *cg = calloc(ngw, sizeof(int));
+            ieo_gw_spectrum(st, 0, pg, cg, vg);
+            out_bins(out_open(o, &o->gw, "spectraGW"), ngw, pg, cg, vg);
+            ieo_gw_spectrum(st, 2, pg, cg, vg);
+            out_bins(out_open(o, &o->gwdot, "spectraGWdot"), ngw, pg, cg, vg);
+            free(pg); free(vg); free(cg);
+            free(p); free(v); free(cnt);
+        }
+        if (c->output_histogram) {
+            FILE *ht = out_open(o, &o->histtimes, "histogramtimes");
+            fprintf(ht, "%f", st->t);
+            fprintf(ht, " %e", st->a);
+            out_histogram(st, o, st->f, out_open(o, &o->hist, "histogram"), ht);
+        }
+    }
+    if (staggered && st->t > 0.) ieo_evolve_fields(st, .5 * c->dt_post);
+    return 0;
+}
+
+/* run_post_inflation_loop (src/evolution.cpp:1568-1651) */
+static int program_post(ieo_state *st, ieo_out *o)
+{
+    const ieo_config *c = &st->cfg;
+    ieo_post_initialize(st);
+    if (program_save_post(st, o, 1)) return 1;
+    int n = 0;
+#define REPORT_POST()                                                                     \
+    do {                                                                                  \
+        if (n % c->output_freq == 0)                                                      \
+            if (program_save_post(st, o, (n % c->output_infrequent_freq == 0) ? 1 : 0))  \
+                return 1;                                                                 \
+    } while (0)
+    if (c->post_integrator == IEO_LEAPFROG) {
+        ieo_evolve_fields(st, 0.5 * c->dt_post);
+        while (st->a <= c->af_post) {
+            ieo_post_evolve_derivs(st, c->dt_post);
+            ieo_evolve_fields(st, c->dt_post);
+            n++;
+            REPORT_POST();
+        }
+    } else if (c->post_integrator == IEO_RK4) {
+        while (st->a <= c->af_post) {
+            ieo_post_rk4_step(st, c->dt_post);
+            n++;
+            REPORT_POST();
+        }
+    } else {
+        double h = clamp_rk45(st, c->dt_post, c->rk45_max_dt);
+        while (st->a <= c->af_post) {
+            const double hmax = rk45_hmax(c, c->dt_post);
+            h = clamp_rk45(st, h, hmax);
+            h = rk45_drive(st, h, hmax, 2);
+            if (h < 0) return 2;
+            n++;
+            REPORT_POST();
+        }
+    }
+#undef REPORT_POST
+    return program_save_post(st, o, 1);
+}
+
 int ieo_program(const ieo_config *cfg, const char *results_dir)
 {
     ieo_state *st = ieo_create(cfg);
@@ -1758,6 +2039,12 @@ int ieo_program(const ieo_config *cfg, const char *results_dir)
     if (!rc) { st->t = st->t0; rc = program_inflation(st, &o); }
     if (!rc && cfg->perform_deltaN) rc = program_deltaN(st, &o);
     out_close(&o);
+    if (!rc && cfg->post_inflation) {
+        snprintf(o.dir, sizeof(o.dir), "%s/post_inflation", results_dir);
+        mkdir(o.dir, 0777);
+        rc = program_post(st, &o);
+        out_close(&o);
+    }
     ieo_destroy(st);
     return rc;
 }

@@ -12,6 +12,8 @@
  *   - potential              src/potential.cpp:11-204
  *   - leapfrog / RK stepping src/evolution.cpp:21-284,286-997
  *   - deltaN loop            src/evolution.cpp:999-1376
+ *   - post-inflation loop    src/evolution.cpp:79-106,520-592,1378-1651,
+ *                            src/initialize.cpp:246-291
  *   - FFT (FFTEASY layout)   src/ffteasy.hpp:21-204
  *   - outputs                src/output.cpp:63-151,156-164,204-293,297-406,
  *                            434-648,674-738,983-1026,1070-1135,1140-1277,1469-1560
@@ -57,6 +59,10 @@ typedef struct ieo_config {
     /* tabulated potential (descending field grid) */
     int tab_n;
     const double *tab_x, *tab_v, *tab_dv;
+    /* post-inflation phase (src/parameters.h:31,148-155; off unless set) */
+    int post_inflation;
+    double horizon_factor, omega, dt_post, af_post;
+    int post_integrator;
 } ieo_config;
 
 typedef struct ieo_state ieo_state;
@@ -119,7 +125,8 @@ void ieo_bin_spectrum(int N, double L, const double *ffield, const double *fny,
                       double norm1, double *p, int *counts, double *vals);
 /* spectraf without the cutoff branch: forward FFT, bin, inverse FFT of f. */
 void ieo_spectrum_f(ieo_state *st, double *p, int *counts, double *vals);
-/* GW TT spectrum (src/output.cpp:466-625). hdot=0: h_ij, 1: hdot_ij (DoubleThenFloat). */
+/* GW TT spectrum (src/output.cpp:466-625). hdot=0: h_ij, 1: hdot_ij (DoubleThenFloat),
+ * 2: hdot_ij with the post-inflation float factor (FloatMultiply, :662-669). */
 void ieo_gw_spectrum(ieo_state *st, int hdot, double *p, int *counts, double *vals);
 /* modes.dat (src/output.cpp:674-738) */
 void ieo_modes(int N, double L, double rescale_B, double *out);
@@ -132,7 +139,15 @@ double ieo_get_phiref(ieo_state *st);
 void ieo_rk4_stepN(ieo_state *st, double h);
 int ieo_rk45_stepN(ieo_state *st, double h, double hmax, double *h_next);
 
-/* Whole program (src/main.cpp:103-146 without post-inflation), writing the
+/* post-inflation (src/initialize.cpp:246-291 without its save; src/evolution.cpp:
+ * 1437-1564; the RK steps with the post-inflation right-hand side :520-592). */
+void ieo_post_initialize(ieo_state *st);
+void ieo_post_evolve_derivs(ieo_state *st, double d);
+void ieo_post_rk4_step(ieo_state *st, double h);
+int ieo_post_rk45_step(ieo_state *st, double h, double hmax, double *h_next);
+
+/* Whole program (src/main.cpp:103-146; the post-inflation phase when
+ * cfg->post_inflation, into results_dir/post_inflation), writing the
  * results/ files of the reference under results_dir.  Returns 0 or an error. */
 int ieo_program(const ieo_config *cfg, const char *results_dir);
 

@@ -71,6 +71,12 @@ class Config(C.Structure):
         ("tab_x", C.POINTER(C.c_double)),
         ("tab_v", C.POINTER(C.c_double)),
         ("tab_dv", C.POINTER(C.c_double)),
+        ("post_inflation", C.c_int),
+        ("horizon_factor", C.c_double),
+        ("omega", C.c_double),
+        ("dt_post", C.c_double),
+        ("af_post", C.c_double),
+        ("post_integrator", C.c_int),
     ]
 
 
@@ -129,6 +135,10 @@ def lib() -> C.CDLL:
             "ieo_get_phiref": (C.c_double, [P]),
             "ieo_rk4_stepN": (None, [P, C.c_double]),
             "ieo_rk45_stepN": (C.c_int, [P, C.c_double, C.c_double, dp]),
+            "ieo_post_initialize": (None, [P]),
+            "ieo_post_evolve_derivs": (None, [P, C.c_double]),
+            "ieo_post_rk4_step": (None, [P, C.c_double]),
+            "ieo_post_rk45_step": (C.c_int, [P, C.c_double, C.c_double, dp]),
             "ieo_program": (C.c_int, [C.POINTER(Config), C.c_char_p]),
             "ieo_num_threads": (C.c_int, []),
         }
@@ -180,20 +190,29 @@ class Table:
         self.dv = np.ascontiguousarray(dv, dtype=np.float64)
 
 
+_POST_KEYS = ("horizon_factor", "omega", "dt_post", "af_post", "post_integrator")
+
+
 def make_config(N: int, numerical: int, params: dict | None = None, table: Table | None = None,
-                sigw: int = 1, deltaN: int = 1) -> Config:
+                sigw: int = 1, deltaN: int = 1, post: int = 0) -> Config:
+    """Oracle config; the post-inflation keys of params.txt are read only with
+    post = 1, as the reference ignores them when built without that phase
+    (src/runtime_parameters.cpp:327-332)."""
     cfg = Config()
     lib().ieo_config_defaults(C.byref(cfg), N, numerical)
     cfg.calculate_sigw = sigw
     cfg.perform_deltaN = deltaN
+    cfg.post_inflation = post
     params = dict(params or {})
     rescale_B_set = "rescale_B" in params
     af_set = "af" in params
-    remap = {"inflation_integrator": "integrator"}
+    remap = {"inflation_integrator": "integrator", "dt_post_inflation": "dt_post",
+             "af_post_inflation": "af_post", "post_inflation_integrator": "post_integrator"}
     for k, v in params.items():
         k = remap.get(k, k)
-        if k in ("output_bispectrum", "output_LOG", "eta_log", "screen_updates", "horizon_factor",
-                 "omega", "dt_post_inflation", "af_post_inflation", "post_inflation_integrator"):
+        if k in ("output_bispectrum", "output_LOG", "eta_log", "screen_updates"):
+            continue
+        if k in _POST_KEYS and not post:
             continue
         if not hasattr(cfg, k):
             raise KeyError(k)

@@ -37,6 +37,11 @@ if str(ENGINE_DIR) not in sys.path:
     sys.path.insert(0, str(ENGINE_DIR))
 
 
+def golden_toggles(g: dict) -> dict:
+    """Build toggles of a golden case (fixtures without the keys: deltaN on, post off)."""
+    return {"deltaN": int(g.get("perform_deltaN", 1)), "post": int(g.get("post_inflation", 0))}
+
+
 def golden_config(case: str):
     """Oracle config and parsed params of a golden case."""
     from oracle import oracle as O
@@ -45,4 +50,4 @@ def golden_config(case: str):
     numerical = int(g["numerical_potential"])
     params = O.parse_params_text(str(g["params_text"]))
     table = potential_table() if numerical else None
-    return O.make_config(N, numerical, params, table), params, table
+    return O.make_config(N, numerical, params, table, **golden_toggles(g)), params, table

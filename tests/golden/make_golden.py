@@ -9,6 +9,10 @@ params.numerical.txt) with the overrides listed below, and stores the numeric
 results/*.dat files as arrays in a compressed .npz next to this script.  The
 fixtures are data only: the parameters used, and the reference's outputs.
 
+Cases that build the reference with its post-inflation phase on (src/parameters.h:31)
+store only the results/post_inflation/*.dat files, under keys prefixed "post_"; the
+files of the earlier phases are the same as those of the case without it.
+
 It also stores the tabulated-potential input tables (inputs/*.dat of the
 reference) as tests/golden/potential_table.npz, since tests on machines without
 the reference need them.
@@ -33,7 +37,7 @@ REF = Path(os.environ.get("IE_REFERENCE_DIR", "/root/reference"))
 sys.path.insert(0, str(REPO))
 from oracle.oracle import read_blocks  # noqa: E402
 
-# name: (N, numerical_potential, preset, overrides)
+# name: (N, numerical_potential, preset, overrides[, build toggles])
 CASES = {
     "hilltop_n16": (16, 0, "params.analytic.txt", {"output_box3D": 1, "output_box2D": 1}),
     "tabulated_n16": (16, 1, "params.numerical.txt", {"output_box3D": 1, "output_box2D": 1}),
@@ -45,6 +49,14 @@ CASES = {
                             "af": 4, "Nend": 0.5}),
     "hilltop_n32": (32, 0, "params.analytic.txt", {"output_box2D": 1}),
     "tabulated_n32": (32, 1, "params.numerical.txt", {"output_box2D": 1}),
+    "tabulated_n16_post": (16, 1, "params.numerical.txt", {}, {"post": 1}),
+    "tabulated_n16_post_rk4": (16, 1, "params.numerical.txt",
+                               {"post_inflation_integrator": "rk4", "af_post_inflation": 4}, {"post": 1}),
+    "tabulated_n16_post_rk45": (16, 1, "params.numerical.txt",
+                                {"post_inflation_integrator": "rk45", "af_post_inflation": 4}, {"post": 1}),
+    "tabulated_n16_post_nodn": (16, 1, "params.numerical.txt", {"af_post_inflation": 8},
+                                {"post": 1, "deltaN": 0}),
+    "hilltop_n16_post": (16, 0, "params.analytic.txt", {"af_post_inflation": 8}, {"post": 1}),
 }
 
 # files whose first block only is kept (large text dumps)
@@ -63,15 +75,35 @@ def params_text(preset: str, overrides: dict) -> str:
     return "\n".join(lines) + "\n"
 
 
-def build(N: int, numerical: int) -> Path:
-    out = subprocess.run([str(REPO / "oracle" / "build_ref.sh"), str(N), str(numerical), "1", "1", "0"],
-                         check=True, capture_output=True, text=True).stdout.strip().splitlines()[-1]
+def build(N: int, numerical: int, deltaN: int = 1, post: int = 0) -> Path:
+    out = subprocess.run([str(REPO / "oracle" / "build_ref.sh"), str(N), str(numerical), "1", str(deltaN), "0",
+                          str(post)], check=True, capture_output=True, text=True).stdout.strip().splitlines()[-1]
     return Path(out)
 
 
+SERIES = ("means", "variance", "velocity", "sf", "energy", "conservation", "spectratimes",
+          "histogramtimes", "histogramtimesN")
+
+
+def store(arrays: dict, folder: Path, prefix: str) -> None:
+    for f in sorted(folder.glob("*.dat")):
+        stem = f.stem
+        if stem == "info":
+            continue
+        blocks = read_blocks(f)
+        if stem in FIRST_BLOCK_ONLY:
+            blocks = blocks[:1]
+        if stem in SERIES:
+            arrays[prefix + stem] = np.concatenate(blocks, axis=0)
+        else:
+            arrays[prefix + stem] = np.stack(blocks, axis=0)
+
+
 def run_case(name: str) -> None:
-    N, numerical, preset, overrides = CASES[name]
-    binary = build(N, numerical)
+    N, numerical, preset, overrides = CASES[name][:4]
+    toggles = CASES[name][4] if len(CASES[name]) > 4 else {}
+    post = toggles.get("post", 0)
+    binary = build(N, numerical, toggles.get("deltaN", 1), post)
     text = params_text(preset, overrides)
     with tempfile.TemporaryDirectory(prefix=f"ie_golden_{name}_") as tmp:
         tmp = Path(tmp)
@@ -79,21 +111,14 @@ def run_case(name: str) -> None:
         os.symlink(REF / "inputs", tmp / "inputs")
         env = dict(os.environ, OMP_NUM_THREADS="1")
         subprocess.run([str(binary)], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL)
-        arrays = {"params_text": np.array(text), "N": np.array(N), "numerical_potential": np.array(numerical)}
-        for f in sorted((tmp / "results").glob("*.dat")):
-            stem = f.stem
-            if stem == "info":
-                continue
-            blocks = read_blocks(f)
-            if stem in FIRST_BLOCK_ONLY:
-                blocks = blocks[:1]
-            if stem in ("means", "variance", "velocity", "sf", "energy", "conservation", "spectratimes",
-                        "histogramtimes", "histogramtimesN"):
-                arrays[stem] = np.concatenate(blocks, axis=0)
-            else:
-                arrays[stem] = np.stack(blocks, axis=0)
+        arrays = {"params_text": np.array(text), "N": np.array(N), "numerical_potential": np.array(numerical),
+                  "perform_deltaN": np.array(toggles.get("deltaN", 1)), "post_inflation": np.array(post)}
+        if post:
+            store(arrays, tmp / "results" / "post_inflation", "post_")
+        else:
+            store(arrays, tmp / "results", "")
     np.savez_compressed(HERE / f"{name}.npz", **arrays)
-    print(f"{name}: {len(arrays) - 3} files")
+    print(f"{name}: {len(arrays) - 5} files")
 
 
 def write_table() -> None:

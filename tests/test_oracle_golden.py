@@ -9,28 +9,34 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import load_golden, potential_table
+from conftest import golden_toggles, load_golden, potential_table
 from oracle import oracle as O
 from oracle.oracle import read_blocks
 
 CASES = ["hilltop_n16", "tabulated_n16", "tabulated_n16_rk4", "tabulated_n16_rk45",
          "hilltop_n32", "tabulated_n32"]
+# post-inflation phase (results/post_inflation/*.dat): leapfrog, RK4, RK45, Phi
+# from f without deltaN, and the analytic potential
+POST_CASES = ["tabulated_n16_post", "tabulated_n16_post_rk4", "tabulated_n16_post_rk45",
+              "tabulated_n16_post_nodn", "hilltop_n16_post"]
+META = ("params_text", "N", "numerical_potential", "perform_deltaN", "post_inflation")
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", CASES + POST_CASES)
 def test_oracle_reproduces_reference(case, tmp_path):
     g = load_golden(case)
     N = int(g["N"])
     numerical = int(g["numerical_potential"])
     params = O.parse_params_text(str(g["params_text"]))
     table = potential_table() if numerical else None
-    cfg = O.make_config(N, numerical, params, table)
+    cfg = O.make_config(N, numerical, params, table, **golden_toggles(g))
     assert O.run_program(cfg, tmp_path) == 0
     checked = 0
     for key, ref in g.items():
-        if key in ("params_text", "N", "numerical_potential"):
+        if key in META:
             continue
-        blocks = read_blocks(tmp_path / f"{key}.dat")
+        path = tmp_path / "post_inflation" / f"{key[5:]}.dat" if key.startswith("post_") else tmp_path / f"{key}.dat"
+        blocks = read_blocks(path)
         if key == "box":
             blocks = blocks[:1]
         if ref.ndim == 2 and key not in ("box",):
@@ -40,4 +46,4 @@ def test_oracle_reproduces_reference(case, tmp_path):
         assert got.shape == ref.shape, key
         assert np.array_equal(got, ref), f"{case}/{key}: max |diff| {np.max(np.abs(got - ref))}"
         checked += 1
-    assert checked >= 18
+    assert checked >= (10 if golden_toggles(g)["post"] else 18)
