@@ -91,6 +91,9 @@ ie_status ie_upload_modes(ie_run *run, const double *fk, const double *fny,
                           const double *fdk, const double *fdny, double phi0, double dphi0);
 /* Download a field as FP64 (N^3); IE_FIELD_F reconstitutes phibar + dphi in split mode. */
 ie_status ie_download_field(ie_run *run, int which, double *out);
+/* Upload a field as FP64 (N^3): IE_FIELD_DELTAN the deltaN map; IE_FIELD_F and
+ * IE_FIELD_FD the FP64 state of the deltaN and post-inflation phases. */
+ie_status ie_upload_field(ie_run *run, int which, const double *src);
 /* Tensors: comp 0..5 h_ij, 6..11 hdot_ij (N^3 floats each). */
 ie_status ie_upload_tensor(ie_run *run, int comp, const float *h);
 ie_status ie_download_tensor(ie_run *run, int comp, float *out);
@@ -136,7 +139,8 @@ ie_status ie_histogram(ie_run *run, int which, int nbins, double *bmin, double *
 /* evolve_derivs(d): Friedmann half-kicks around the field (and tensor) kick
  * (src/evolution.cpp:150-257). */
 ie_status ie_evolve_derivs(ie_run *run, double d);
-/* evolve_fields(d): drift of f (and h), t and a (src/evolution.cpp:260-284). */
+/* evolve_fields(d): drift of f (and h), t and a (src/evolution.cpp:260-284); in
+ * the post-inflation phase the drift of Phi and h. */
 ie_status ie_evolve_fields(ie_run *run, double d);
 /* Up to max_steps iterations of the leapfrog loop body of run_evolution_loop
  * (src/evolution.cpp:937-945) with the loop test a <= af evaluated on the device:
@@ -186,7 +190,8 @@ typedef struct {
 } ie_tuning;
 ie_status ie_tune(ie_run *run, int steps_per_candidate, ie_tuning *out);
 /* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904), FP64
- * storage only (IE_ERR_STATE for split storage).  RK45 returns *accepted and the
+ * storage only (IE_ERR_STATE for split storage) in the inflation phase; in the
+ * post-inflation phase both kinds, with the post-inflation right-hand side.  RK45 returns *accepted and the
  * proposed next step clamped to [rk45_min_dt, hmax]; a rejected step leaves the
  * state as it was.  Stage buffers (4 or 7 stages of the state) are allocated on
  * the first call. */
@@ -248,6 +253,37 @@ ie_status ie_deltaN_zeta(ie_run *run, double cutoff, float *nmean);
 ie_status ie_deltaN_zero_unfinished(ie_run *run, double cutoff);
 /* subtract the FP64 mean of deltaN (build_deltaN_linear, src/output.cpp:156-164) */
 ie_status ie_deltaN_center(ie_run *run);
+
+/* --- post-inflation --------------------------------------------------------- */
+/* The fluid-era phase after inflation and the deltaN map (src/parameters.h:31,
+ * 148-155; src/evolution.cpp:1378-1651): the Bardeen potential Phi and its
+ * derivative in FP64 (for both storage kinds: split runs allocate two more N^3
+ * double buffers), the tensors in FP32, the background (a, a') advanced by the
+ * host from its closed equation.  Needs calculate_sigw = 1, as the reference
+ * (src/parameters.h:33-35). */
+typedef struct ie_post_params {
+    double horizon_factor, omega, dt, af;  /* src/runtime_parameters.cpp:48-52 (af <= 0: 2N) */
+    int integrator;                        /* post_inflation_integrator: IE_LEAPFROG, IE_RK4, IE_RK45 */
+    int from_deltaN;                       /* 1: Phi from the deltaN map (perform_deltaN = 1) */
+} ie_post_params;
+/* initialize_post_inflation (src/initialize.cpp:246-291) without its save:
+ * tensors zeroed; Phi = 3(1+w)/(5+3w) (deltaN - <deltaN>) with the mean summed
+ * in lattice order, or without deltaN Phi = 3(1+w)/(5+3w) (f - <f>) / (<fd>
+ * a^{s-1} / (ad a^{s-2})); fd kept as the previous phase left it; t = 0, a = 1,
+ * ad = horizon_factor N 2 pi / L.  from_deltaN needs the deltaN phase
+ * (ie_deltaN_init).  Afterwards ie_evolve_fields drifts the post-inflation state,
+ * ie_rk4_step / ie_rk45_step use its right-hand side (src/evolution.cpp:520-592,
+ * 1606-1640), the output entry points (ie_field_sums, ie_spectrum, ie_histogram,
+ * ie_gw_spectrum with spectraGWdot_post_inflation's float factor,
+ * src/output.cpp:662-669) read it, and the inflation and deltaN steps refuse. */
+ie_status ie_post_init(ie_run *run, const ie_post_params *params, ie_background *out);
+/* evolve_derivs_post_inflation (src/evolution.cpp:1437-1564): Phi kick, then the
+ * tensor kick whose source uses the kicked Phi' (no drift). */
+ie_status ie_post_evolve_derivs(ie_run *run, double d);
+/* Up to max_steps iterations of the leapfrog loop body of run_post_inflation_loop
+ * (src/evolution.cpp:1598-1604) while a <= af: two launches per step, no host
+ * synchronisation.  out->done is set once a > af. */
+ie_status ie_post_leapfrog_steps(ie_run *run, int max_steps, ie_background *out);
 
 /* --- batches ---------------------------------------------------------------- */
 /* Many runs advanced together: each step of the leapfrog loop

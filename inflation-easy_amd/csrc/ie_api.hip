@@ -41,7 +41,10 @@ struct ie_run {
     int hist_bins = 0;
     double *dn_f = nullptr, *dn_fd = nullptr;  // FP64 deltaN state (split runs)
     double *dn5[3] = {nullptr, nullptr, nullptr};   // deltaN RK45 trial state (f, fd, N)
-    int in_deltaN = 0;
+    int in_deltaN = 0;          // FP64 state (dn_f / F.f): deltaN and post-inflation phases
+    int in_post = 0;            // post-inflation phase
+    ie_post_params post{};
+    double *post_f2 = nullptr, *post_fd2 = nullptr;   // second Phi, Phi' buffers (split runs)
     RkWork *rk4 = nullptr, *rk45 = nullptr;    // RK stage buffers, made on first use
     std::vector<double> htab_x, htab_v, htab_dv;   // host copy of the table
     std::vector<void *> allocs;
@@ -373,6 +376,27 @@ ie_status ie_download_field(ie_run *r, int which, double *out)
     return sync(r);
 }
 
+static ie_status dn_state(ie_run *r, double **f, double **fd);
+
+ie_status ie_upload_field(ie_run *r, int which, const double *src)
+{
+    if (!r || !src) return fail(r, IE_ERR_ARG, "null argument");
+    double *dst = nullptr;
+    if (which == IE_FIELD_DELTAN) {
+        dst = r->F.dn;
+    } else if (which == IE_FIELD_F || which == IE_FIELD_FD) {
+        if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "inflation phase: use ie_upload_fields");
+        double *f, *fd;
+        ie_status st = dn_state(r, &f, &fd);
+        if (st != IE_OK) return st;
+        dst = which == IE_FIELD_F ? f : fd;
+    } else {
+        return fail(r, IE_ERR_ARG, "bad field");
+    }
+    IE_HIP(r, hipMemcpyAsync(dst, src, r->gs * 8, hipMemcpyHostToDevice, r->stream));
+    return sync(r);
+}
+
 ie_status ie_upload_tensor(ie_run *r, int comp, const float *h)
 {
     if (!r || !h || comp < 0 || comp > 11) return fail(r, IE_ERR_ARG, "bad argument");
@@ -481,6 +505,15 @@ ie_status ie_potential_energy(ie_run *r, double *out)
 ie_status ie_field_sums(ie_run *r, double out[3])
 {
     if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    if (r->in_deltaN) {
+        // FP64 state of the deltaN and post-inflation phases
+        double *f, *fd;
+        ie_status st = dn_state(r, &f, &fd);
+        if (st != IE_OK) return st;
+        IE_HIP(r, post_sums3(f, fd, r->gs, r->work, r->work + 4000, r->stream));
+        IE_HIP(r, hipMemcpyAsync(out, r->work + 4000, 3 * 8, hipMemcpyDeviceToHost, r->stream));
+        return sync(r);
+    }
     ie_status st = sums(r);
     if (st != IE_OK) return st;
     const DevBg &b = *r->bg_host;
@@ -497,19 +530,31 @@ ie_status ie_field_sums(ie_run *r, double out[3])
     return IE_OK;
 }
 
+// the inflation steps need the inflation state
+static ie_status inflation_phase(ie_run *r)
+{
+    if (r->in_post) return fail(r, IE_ERR_STATE, "run is in its post-inflation phase");
+    if (r->in_deltaN) return fail(r, IE_ERR_STATE, "run is in its deltaN phase");
+    return IE_OK;
+}
+
 ie_status ie_evolve_derivs(ie_run *r, double d)
 {
     if (!r || !(d != 0.0)) return fail(r, IE_ERR_ARG, "bad step");
+    if (inflation_phase(r) != IE_OK) return IE_ERR_STATE;
     const int split = r->cfg.precision == IE_SPLIT;
     IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, d, 0, r->stream));
     IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICK, d, 0, r->stream));
     return check_kernel_error(r);
 }
 
+static ie_status post_drift_call(ie_run *r, double d);
+
 ie_status ie_evolve_fields(ie_run *r, double d)
 {
     if (!r) return fail(r, IE_ERR_ARG, "null argument");
     if (d == 0.0) return IE_OK;
+    if (r->in_post) return post_drift_call(r, d);
     const int split = r->cfg.precision == IE_SPLIT;
     IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, DRIFT, d, 0, r->stream));
     return sync(r);
@@ -518,6 +563,7 @@ ie_status ie_evolve_fields(ie_run *r, double d)
 ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
 {
     if (!r || max_steps < 0) return fail(r, IE_ERR_ARG, "bad argument");
+    if (inflation_phase(r) != IE_OK) return IE_ERR_STATE;
     const int split = r->cfg.precision == IE_SPLIT;
     // first long loop of a kind: tune the kick / tensor shapes on its first steps
     const Shapes sh = tuned_shapes(r->N, split);
@@ -551,6 +597,7 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
 ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
 {
     if (!r || nsteps < 1 || !out) return fail(r, IE_ERR_ARG, "bad argument");
+    if (inflation_phase(r) != IE_OK) return IE_ERR_STATE;
     const int split = r->cfg.precision == IE_SPLIT;
     // events bracket each launch of a step: reduce | finish | scalar | tensor | finish
     const int nev = 6;
@@ -617,6 +664,7 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
 ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
 {
     if (!r || K < 1 || !out) return fail(r, IE_ERR_ARG, "bad argument");
+    if (inflation_phase(r) != IE_OK) return IE_ERR_STATE;
     memset(out, 0, sizeof(*out));
     const int split = r->cfg.precision == IE_SPLIT;
     const int def = row_elems(r->N);
@@ -797,7 +845,10 @@ ie_status ie_gw_spectrum(ie_run *r, int hdot, double *p, int *counts, double *va
     const int par = r->bg_host->parity;
     for (int c = 0; c < 6; ++c)
         IE_HIP(r, fft_forward_float(r->fft, hdot ? r->F.hd[c] : r->F.h[par][c], 0.0, c, r->stream));
-    const double to_phys = hdot ? r->cfg.rescale_B * pow(r->bg_host->a, r->cfg.rescale_s - 1.0) : 1.0;
+    // spectraGWdot (src/output.cpp:644-648): B a^{s-1} in double; its
+    // post-inflation copy rounds the factor to float (:662-669)
+    double to_phys = hdot ? r->cfg.rescale_B * pow(r->bg_host->a, r->cfg.rescale_s - 1.0) : 1.0;
+    if (hdot && r->in_post) to_phys = (double)(float)(r->cfg.rescale_B * pow(r->bg_host->a, r->cfg.rescale_s - 1.f));
     const int nb = fft_gw_numbins(r->N);
     std::vector<double> sums(nb);
     IE_HIP(r, fft_bin_tt(r->fft, r->dx, to_phys, sums.data(), r->stream));
@@ -960,7 +1011,8 @@ ie_status ie_deltaN_init(ie_run *r)
 ie_status ie_evolve_fieldsN(ie_run *r, double d)
 {
     if (!r) return fail(r, IE_ERR_ARG, "null argument");
-    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    if (!r->in_deltaN || r->in_post)
+        return fail(r, IE_ERR_STATE, r->in_post ? "run is in its post-inflation phase" : "deltaN phase not initialised");
     double *f, *fd;
     ie_status st = dn_state(r, &f, &fd);
     if (st != IE_OK) return st;
@@ -977,7 +1029,8 @@ ie_status ie_evolve_fieldsN(ie_run *r, double d)
 ie_status ie_get_phiref(ie_run *r, double *out)
 {
     if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
-    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    if (!r->in_deltaN || r->in_post)
+        return fail(r, IE_ERR_STATE, r->in_post ? "run is in its post-inflation phase" : "deltaN phase not initialised");
     double *f, *fd;
     ie_status st = dn_state(r, &f, &fd);
     if (st != IE_OK) return st;
@@ -989,7 +1042,8 @@ ie_status ie_get_phiref(ie_run *r, double *out)
 ie_status ie_deltaN_steps(ie_run *r, double dN, int64_t nsteps)
 {
     if (!r || nsteps < 0) return fail(r, IE_ERR_ARG, "bad argument");
-    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    if (!r->in_deltaN || r->in_post)
+        return fail(r, IE_ERR_STATE, r->in_post ? "run is in its post-inflation phase" : "deltaN phase not initialised");
     double *f, *fd;
     ie_status st = dn_state(r, &f, &fd);
     if (st != IE_OK) return st;
@@ -1011,7 +1065,8 @@ ie_status ie_deltaN_steps(ie_run *r, double dN, int64_t nsteps)
 ie_status ie_deltaN_rk4_steps(ie_run *r, double h, int64_t nsteps)
 {
     if (!r || nsteps < 0) return fail(r, IE_ERR_ARG, "bad argument");
-    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    if (!r->in_deltaN || r->in_post)
+        return fail(r, IE_ERR_STATE, r->in_post ? "run is in its post-inflation phase" : "deltaN phase not initialised");
     double *f, *fd;
     ie_status st = dn_state(r, &f, &fd);
     if (st != IE_OK) return st;
@@ -1034,7 +1089,8 @@ ie_status ie_deltaN_rk4_steps(ie_run *r, double h, int64_t nsteps)
 ie_status ie_deltaN_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *h_next)
 {
     if (!r || !accepted || !h_next || !(h > 0.0)) return fail(r, IE_ERR_ARG, "bad argument");
-    if (!r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    if (!r->in_deltaN || r->in_post)
+        return fail(r, IE_ERR_STATE, r->in_post ? "run is in its post-inflation phase" : "deltaN phase not initialised");
     double *f, *fd;
     ie_status st = dn_state(r, &f, &fd);
     if (st != IE_OK) return st;
@@ -1160,6 +1216,187 @@ ie_status ie_histogram(ie_run *r, int which, int nbins, double *bmin, double *bm
     return IE_OK;
 }
 
+/* ------------------------------------------------------ post-inflation */
+
+// Phi, Phi' (FP64) and h of the current step and the buffers its results go to
+static ie_status post_state(ie_run *r, PostState *S)
+{
+    double *f, *fd;
+    ie_status st = dn_state(r, &f, &fd);
+    if (st != IE_OK) return st;
+    const int par = r->bg_host->parity;
+    S->f = f;
+    S->fd = fd;
+    if (r->cfg.precision == IE_SPLIT) {
+        S->f_out = r->post_f2;
+        S->fd_out = r->post_fd2;
+    } else {
+        S->f_out = (double *)r->F.f[par ^ 1];
+        S->fd_out = (double *)r->F.fd[par ^ 1];
+    }
+    for (int c = 0; c < 6; ++c) {
+        S->h[c] = r->F.h[par][c];
+        S->h_out[c] = r->F.h[par ^ 1][c];
+        S->hd[c] = r->F.hd[c];
+    }
+    return IE_OK;
+}
+
+// after a step: the written buffers become current
+static void post_flip(ie_run *r)
+{
+    r->bg_host->parity ^= 1;
+    if (r->cfg.precision == IE_SPLIT) {
+        std::swap(r->dn_f, r->post_f2);
+        std::swap(r->dn_fd, r->post_fd2);
+    }
+}
+
+// Scalars of one kick of evolve_derivs_post_inflation (src/evolution.cpp:1440-1464)
+// at the background b, whose ad2 and first half kick of ad it applies.
+static PostStep post_kick_scalars(ie_run *r, DevBg &b, double d, double dd)
+{
+    const double s = r->cfg.rescale_s, w = r->post.omega, dx = r->dx;
+    PostStep P{};
+    P.d = d;
+    P.dd = dd;
+    P.laplnorm = pow(b.a, -2.0 * s) / (dx * dx);
+    b.ad2 = -(s - 0.5 * (1.0 - 3.0 * w)) * (b.ad * b.ad) / b.a;
+    b.ad += 0.5 * d * b.ad2;
+    P.A = w * P.laplnorm;
+    P.c1ad = (3.0 * (1.0 + w) + s) * b.ad;
+    P.a = b.a;
+    const double Htilde = b.ad / b.a;
+    P.invH = 1.0 / Htilde;
+    P.coeffU = 4.0 / (3.0 * (1.0 + w));
+    P.srcamp = 2.0 * pow(b.a, -2.0 * s);
+    P.c2ad = (2.0 + s) * b.ad;
+    P.hx = 0.5 / dx;
+    P.inv_dx2 = 1.0 / (dx * dx);
+    P.cross_den = 4.0 * dx * dx;
+    return P;
+}
+
+static ie_status post_drift_call(ie_run *r, double d)
+{
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    PostState S;
+    if ((st = post_state(r, &S)) != IE_OK) return st;
+    IE_HIP(r, post_drift(S, r->gs, d, r->stream));
+    DevBg &b = *r->bg_host;
+    b.t += d;
+    b.a += d * b.ad;
+    return push_bg(r);
+}
+
+extern "C" {
+
+ie_status ie_post_init(ie_run *r, const ie_post_params *p, ie_background *out)
+{
+    if (!r || !p) return fail(r, IE_ERR_ARG, "null argument");
+    if (!r->F.sigw) return fail(r, IE_ERR_STATE, "the post-inflation phase needs calculate_sigw = 1");
+    if (!(p->dt > 0.0) || !(1.0 + p->omega != 0.0) || !(5.0 + 3.0 * p->omega != 0.0) ||
+        (p->integrator != IE_LEAPFROG && p->integrator != IE_RK4 && p->integrator != IE_RK45))
+        return fail(r, IE_ERR_ARG, "bad post-inflation parameters");
+    if (r->in_post) return fail(r, IE_ERR_STATE, "run is already in its post-inflation phase");
+    if (p->from_deltaN && !r->in_deltaN) return fail(r, IE_ERR_STATE, "deltaN phase not initialised");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int split = r->cfg.precision == IE_SPLIT;
+    if (split) {
+        if (!r->dn_f) {
+            if ((st = dev_alloc(r, (void **)&r->dn_f, r->gs * 8)) != IE_OK) return st;
+            if ((st = dev_alloc(r, (void **)&r->dn_fd, r->gs * 8)) != IE_OK) return st;
+        }
+        if (!r->in_deltaN) {
+            IE_HIP(r, launch_reconstitute(r->F, 1, 0, r->bg, r->dn_f, r->stream));
+            IE_HIP(r, launch_reconstitute(r->F, 1, 1, r->bg, r->dn_fd, r->stream));
+        }
+        if (!r->post_f2) {
+            if ((st = dev_alloc(r, (void **)&r->post_f2, r->gs * 8)) != IE_OK) return st;
+            if ((st = dev_alloc(r, (void **)&r->post_fd2, r->gs * 8)) != IE_OK) return st;
+        }
+    }
+    r->in_deltaN = 1;
+    double *f, *fd;
+    if ((st = dn_state(r, &f, &fd)) != IE_OK) return st;
+    for (int q = 0; q < 2; ++q)
+        for (int c = 0; c < 6; ++c) IE_HIP(r, hipMemsetAsync(r->F.h[q][c], 0, r->gs * 4, r->stream));
+    for (int c = 0; c < 6; ++c) IE_HIP(r, hipMemsetAsync(r->F.hd[c], 0, r->gs * 4, r->stream));
+    const double w = p->omega, s = r->cfg.rescale_s;
+    const double Phi_from_zeta = 3.0 * (1.0 + w) / (5.0 + 3.0 * w);
+    double sums2[2];
+    IE_HIP(r, serial_sums(p->from_deltaN ? r->F.dn : f, p->from_deltaN ? nullptr : fd, r->gs, r->work + 4000,
+                          r->stream));
+    IE_HIP(r, hipMemcpyAsync(sums2, r->work + 4000, 16, hipMemcpyDeviceToHost, r->stream));
+    if ((st = sync(r)) != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    if (p->from_deltaN) {
+        const double Nmean = sums2[0] / (double)r->gs;
+        IE_HIP(r, post_phi(f, r->F.dn, r->gs, Phi_from_zeta, Nmean, 1.0, 0, r->stream));
+    } else {
+        const double fmean = sums2[0] / (double)r->gs, fdmean = sums2[1] / (double)r->gs;
+        const double den = fdmean * pow(b.a, s - 1.0) / (b.ad * pow(b.a, s - 2.0));
+        IE_HIP(r, post_phi(f, f, r->gs, Phi_from_zeta, fmean, den, 1, r->stream));
+    }
+    r->post = *p;
+    if (!(r->post.af > 0.0)) r->post.af = 2.0 * r->N;
+    r->in_post = 1;
+    b.t = 0.0;
+    b.a = 1.0;
+    b.ad = p->horizon_factor * r->N * 2.0 * (2.0 * asin(1.0)) / r->cfg.L;
+    b.steps = 0;
+    b.done = 0;
+    if ((st = push_bg(r)) != IE_OK) return st;
+    if (out) return ie_get_background(r, out);
+    return IE_OK;
+}
+
+ie_status ie_post_evolve_derivs(ie_run *r, double d)
+{
+    if (!r || !(d != 0.0)) return fail(r, IE_ERR_ARG, "bad step");
+    if (!r->in_post) return fail(r, IE_ERR_STATE, "post-inflation phase not initialised");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    PostState S;
+    if ((st = post_state(r, &S)) != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    const PostStep P = post_kick_scalars(r, b, d, 0.0);
+    IE_HIP(r, post_step(S, r->N, r->logN, P, r->stream));
+    b.ad += 0.5 * d * b.ad2;
+    post_flip(r);
+    return push_bg(r);
+}
+
+ie_status ie_post_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
+{
+    if (!r || max_steps < 0) return fail(r, IE_ERR_ARG, "bad argument");
+    if (!r->in_post) return fail(r, IE_ERR_STATE, "post-inflation phase not initialised");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    DevBg &b = *r->bg_host;
+    const double d = r->post.dt;
+    for (int n = 0; n < max_steps && b.a <= r->post.af; ++n) {
+        PostState S;
+        if ((st = post_state(r, &S)) != IE_OK) return st;
+        const PostStep P = post_kick_scalars(r, b, d, d);
+        IE_HIP(r, post_step(S, r->N, r->logN, P, r->stream));
+        b.ad += 0.5 * d * b.ad2;
+        b.t += d;
+        b.a += d * b.ad;
+        b.steps += 1;
+        b.d = d;
+        post_flip(r);
+    }
+    b.done = !(b.a <= r->post.af) ? 1 : 0;
+    if ((st = push_bg(r)) != IE_OK) return st;
+    if (out) return ie_get_background(r, out);
+    return IE_OK;
+}
+
+}  // extern "C"
+
 /* ------------------------------------------------------------- batches */
 
 }  // extern "C"
@@ -1272,8 +1509,10 @@ ie_status ie_batch_destroy(ie_batch *b)
 
 static ie_status rk_ready(ie_run *r, int stages, RkWork **w)
 {
-    if (r->cfg.precision != IE_FP64) return fail(r, IE_ERR_STATE, "RK integrators need FP64 storage");
-    if (r->in_deltaN) return fail(r, IE_ERR_STATE, "run is in its deltaN phase");
+    if (!r->in_post) {
+        if (r->cfg.precision != IE_FP64) return fail(r, IE_ERR_STATE, "RK integrators need FP64 storage");
+        if (r->in_deltaN) return fail(r, IE_ERR_STATE, "run is in its deltaN phase");
+    }
     RkWork **slot = stages == 4 ? &r->rk4 : &r->rk45;
     if (!*slot) {
         const hipError_t e = rk_create(r->N, r->logN, r->F.sigw, stages, slot);
@@ -1282,7 +1521,17 @@ static ie_status rk_ready(ie_run *r, int stages, RkWork **w)
         IE_HIP(r, e);
     }
     *w = *slot;
+    rk_set_post(*w, r->in_post, r->post.omega);
     return pull_bg(r);
+}
+
+// y of an RK step: the inflation state, or the FP64 post-inflation state
+static ie_status rk_fields(ie_run *r, double **f, double **fd)
+{
+    if (r->in_post) return dn_state(r, f, fd);
+    *f = (double *)r->F.f[r->bg_host->parity];
+    *fd = (double *)r->F.fd[r->bg_host->parity];
+    return IE_OK;
 }
 
 static void rk_tensors(ie_run *r, float **h, float **hd)
@@ -1302,7 +1551,8 @@ ie_status ie_rk4_step(ie_run *r, double h)
     if (st != IE_OK) return st;
     float *hh[6], *hd[6];
     rk_tensors(r, hh, hd);
-    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd[r->bg_host->parity];
+    double *f, *fd;
+    if ((st = rk_fields(r, &f, &fd)) != IE_OK) return st;
     IE_HIP(r, rk4_step(w, r->P, r->bg, r->bg_host, f, fd, hh, hd, h, r->stream));
     const DevBg advanced = *r->bg_host;
     if ((st = check_kernel_error(r)) != IE_OK) return st;
@@ -1323,7 +1573,8 @@ ie_status ie_rk45_step(ie_run *r, double h, double hmax, int *accepted, double *
     if (st != IE_OK) return st;
     float *hh[6], *hd[6];
     rk_tensors(r, hh, hd);
-    double *f = (double *)r->F.f[r->bg_host->parity], *fd = (double *)r->F.fd[r->bg_host->parity];
+    double *f, *fd;
+    if ((st = rk_fields(r, &f, &fd)) != IE_OK) return st;
     const ie_config &c = r->cfg;
     double factor = 1.0, norm = 0.0;
     int acc = 0;

@@ -306,11 +306,164 @@ hipError_t field_minmax(const double *g, size_t gs, double *work, hipStream_t s,
 hipError_t field_histogram(const double *g, size_t gs, double bmin, double bmax, double df, int nbins,
                            unsigned long long *counts, hipStream_t s);
 
+// ------------------------------------------------------------ row helpers
+// Shared by the row kernels of the inflation and post-inflation steps.
+
+constexpr int kXcd = 8;          // XCDs of the MI355X; workgroup b runs on XCD b mod 8
+
+// E consecutive values of one lane, loaded and stored as one vector.
+template <typename T, int E>
+struct alignas(sizeof(T) * E) RowVec {
+    T v[E];
+};
+
+template <typename T, int E>
+__device__ __forceinline__ void ld_row(const T *__restrict__ row, unsigned lane, T (&o)[E])
+{
+    const RowVec<T, E> x = reinterpret_cast<const RowVec<T, E> *>(row)[lane];
+#pragma unroll
+    for (int e = 0; e < E; ++e) o[e] = x.v[e];
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void st_row(T *__restrict__ row, unsigned lane, const T (&v)[E])
+{
+    RowVec<T, E> x;
+#pragma unroll
+    for (int e = 0; e < E; ++e) x.v[e] = v[e];
+    reinterpret_cast<RowVec<T, E> *>(row)[lane] = x;
+}
+
+// Work of one wave: rows j0 .. j0+R-1 of plane i, k-segment [k0, k0 + 64 E).
+struct RowWalk {
+    unsigned i, j0, k0, R, segs;
+};
+
+// G workgroups of 4 waves per lattice; units (i, segment, j-block) in plane-major
+// order, XCD x owning the planes [x N/8, (x+1) N/8).
+__device__ __forceinline__ RowWalk row_walk(unsigned N, unsigned G, unsigned blk, int E)
+{
+    const unsigned segs = N / (64u * (unsigned)E);
+    const unsigned waves = 4u * G;
+    const unsigned R = (N * N * segs) / waves;
+    const unsigned jblocks = N / R;
+    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
+    const unsigned u = xcd * (waves / kXcd) + lb * 4u + (threadIdx.x >> 6);
+    RowWalk w;
+    const unsigned t = u / jblocks;
+    w.j0 = (u % jblocks) * R;
+    w.k0 = (t % segs) * (64u * (unsigned)E);
+    w.i = t / segs;
+    w.R = R;
+    w.segs = segs;
+    return w;
+}
+
+// k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
+// `row`): in-lane, then across lanes, then across segments for N > 64 E.
+// The value of the first active lane, in scalar registers (the same in every lane).
+__device__ __forceinline__ float first_lane(float x)
+{
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(x)));
+}
+__device__ __forceinline__ double first_lane(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restrict__ row, const RowWalk &w,
+                                             unsigned N, unsigned lane, T &kp_last, T &km_first)
+{
+    kp_last = __shfl(c[0], (int)((lane + 1) & 63));
+    km_first = __shfl(c[E - 1], (int)((lane + 63) & 63));
+    if (w.segs > 1) {
+        // the two edge sites, loaded by every lane and kept wave-uniform, then
+        // selected per lane without a branch
+        const unsigned m = N - 1;
+        const T lo = first_lane(row[(w.k0 - 1) & m]), hi = first_lane(row[(w.k0 + 64u * E) & m]);
+        km_first = lane == 0 ? lo : km_first;
+        kp_last = lane == 63 ? hi : kp_last;
+    }
+}
+
+// Post-inflation steps (ie_postinf.hip).  Scalars of one leapfrog step of
+// evolve_derivs_post_inflation + evolve_fields, computed by the host in the
+// reference's order (src/evolution.cpp:1437-1564).
+struct PostStep {
+    double d, dd, A, c1ad, a;                      // kick step, drift step (0: none), w laplnorm, (3(1+w)+s) a', a
+    double laplnorm, c2ad, srcamp, invH, coeffU;   // tensor kick: laplnorm, (2+s) a', 2 a^-2s, a/a', 4/(3(1+w))
+    double hx, inv_dx2, cross_den;                 // 0.5/dx, 1/(dx dx), 4 dx dx
+};
+// FP64 Phi, Phi' and FP32 h_ij, h_ij' of a step: read f, fd, h; write f_out,
+// fd_out, h_out; hd in place.  post_drift uses f, fd, h, hd only (in place).
+struct PostState {
+    double *f, *f_out, *fd, *fd_out;
+    float *h[6], *h_out[6], *hd[6];
+};
+#pragma clang fp contract(off)
+// Stencil of one site for the post-inflation tensor source, and the source
+// S_ij (xx yy zz xy xz yz) in the reference's operation order
+// (src/evolution.cpp:1474-1503, 549-570); contraction off, as the reference build.
+struct SiteStencil {
+    double c;                   // Phi(x)
+    double fxp, fxm, fyp, fym, fzp, fzm;      // Phi at the six faces
+    double fxpyp, fxpym, fxmyp, fxmym;        // Phi(i +- 1, j +- 1, k)
+    double fxpzp, fxpzm, fxmzp, fxmzm;        // Phi(i +- 1, j, k +- 1)
+    double fypzp, fymzm;                      // Phi(i, j+1, k+1), Phi(i, j-1, k-1)
+    double dc, dxp, dxm, dyp, dym, dzp, dzm;  // kicked Phi' at x and its faces
+};
+
+__device__ __forceinline__ void post_source(const SiteStencil &q, const PostStep &P, double S[6])
+{
+    const double gx = (q.fxp - q.fxm) * P.hx;
+    const double gy = (q.fyp - q.fym) * P.hx;
+    const double gz = (q.fzp - q.fzm) * P.hx;
+    const double gdx = (q.dxp - q.dxm) * P.hx;
+    const double gdy = (q.dyp - q.dym) * P.hx;
+    const double gdz = (q.dzp - q.dzm) * P.hx;
+    const double Hxx = (q.fxp - 2.0 * q.c + q.fxm) * P.inv_dx2;
+    const double Hyy = (q.fyp - 2.0 * q.c + q.fym) * P.inv_dx2;
+    const double Hzz = (q.fzp - 2.0 * q.c + q.fzm) * P.inv_dx2;
+    const double Hxy = (q.fxpyp - q.fxpym - q.fxmyp + q.fxmym) / P.cross_den;
+    const double Hxz = (q.fxpzp - q.fxpzm - q.fxmzp + q.fxmzm) / P.cross_den;
+    const double Hyz = (q.fypzp - q.fymzm - q.fypzp + q.fymzm) / P.cross_den;
+    const double dux = gdx * P.invH + gx;
+    const double duy = gdy * P.invH + gy;
+    const double duz = gdz * P.invH + gz;
+    const double Phi = q.c;
+    S[0] = 4.0 * Phi * Hxx + 2.0 * gx * gx - P.coeffU * (dux * dux);
+    S[1] = 4.0 * Phi * Hyy + 2.0 * gy * gy - P.coeffU * (duy * duy);
+    S[2] = 4.0 * Phi * Hzz + 2.0 * gz * gz - P.coeffU * (duz * duz);
+    S[3] = 4.0 * Phi * Hxy + 2.0 * gx * gy - P.coeffU * (dux * duy);
+    S[4] = 4.0 * Phi * Hxz + 2.0 * gx * gz - P.coeffU * (dux * duz);
+    S[5] = 4.0 * Phi * Hyz + 2.0 * gy * gz - P.coeffU * (duy * duz);
+}
+
+#pragma clang fp contract(fast)
+
+int post_row_elems(int N);
+unsigned post_row_blocks(int N, int E);
+hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s);
+hipError_t post_drift(const PostState &S, size_t gs, double d, hipStream_t s);
+// serial sums of g1 (and g2 when non-null) in lattice order into out[0], out[1]
+hipError_t serial_sums(const double *g1, const double *g2, size_t gs, double *out, hipStream_t s);
+hipError_t post_phi(double *f, const double *src, size_t gs, double P, double mean, double den, int divide,
+                    hipStream_t s);
+// {sum f, sum fd, sum f^2} in fixed order (work: >= 3072 doubles)
+hipError_t post_sums3(const double *f, const double *fd, size_t gs, double *work, double *out, hipStream_t s);
+
 // RK4 / RK45 inflation steps on FP64 storage (ie_rk.hip).  Stage buffers are
 // allocated once per run (stages x (2 N^3 doubles + 12 N^3 floats with tensors)).
 struct RkWork;
 hipError_t rk_create(int N, int logN, int sigw, int stages, RkWork **out);
 void rk_free(RkWork *w);
+// post = 1: later steps use the post-inflation right-hand side with this w
+// (src/evolution.cpp:520-592), as the reference's ScopedPostInflationRhsMode
+void rk_set_post(RkWork *w, int post, double omega);
 hipError_t rk4_step(RkWork *w, const Pot &P, DevBg *bg, DevBg *bg_host, double *f, double *fd, float *const *h,
                     float *const *hd, double step, hipStream_t s);
 hipError_t rk45_step(RkWork *w, const Pot &P, DevBg *bg, DevBg *bg_host, double *f, double *fd, float *const *h,

@@ -40,8 +40,6 @@ namespace ie {
 
 namespace {
 
-constexpr int kXcd = 8;
-
 struct Walk {
     size_t begin, stride, end;
 };
@@ -339,85 +337,6 @@ __global__ void kb_loop_begin(const RunDesc *__restrict__ runs, int nruns)
 }
 
 // ------------------------------------------------------------ row kernels
-
-// E consecutive values of one lane, loaded and stored as one vector.
-template <typename T, int E>
-struct alignas(sizeof(T) * E) RowVec {
-    T v[E];
-};
-
-template <typename T, int E>
-__device__ __forceinline__ void ld_row(const T *__restrict__ row, unsigned lane, T (&o)[E])
-{
-    const RowVec<T, E> x = reinterpret_cast<const RowVec<T, E> *>(row)[lane];
-#pragma unroll
-    for (int e = 0; e < E; ++e) o[e] = x.v[e];
-}
-
-template <typename T, int E>
-__device__ __forceinline__ void st_row(T *__restrict__ row, unsigned lane, const T (&v)[E])
-{
-    RowVec<T, E> x;
-#pragma unroll
-    for (int e = 0; e < E; ++e) x.v[e] = v[e];
-    reinterpret_cast<RowVec<T, E> *>(row)[lane] = x;
-}
-
-// Work of one wave: rows j0 .. j0+R-1 of plane i, k-segment [k0, k0 + 64 E).
-struct RowWalk {
-    unsigned i, j0, k0, R, segs;
-};
-
-// G workgroups of 4 waves per lattice; units (i, segment, j-block) in plane-major
-// order, XCD x owning the planes [x N/8, (x+1) N/8).
-__device__ __forceinline__ RowWalk row_walk(unsigned N, unsigned G, unsigned blk, int E)
-{
-    const unsigned segs = N / (64u * (unsigned)E);
-    const unsigned waves = 4u * G;
-    const unsigned R = (N * N * segs) / waves;
-    const unsigned jblocks = N / R;
-    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
-    const unsigned u = xcd * (waves / kXcd) + lb * 4u + (threadIdx.x >> 6);
-    RowWalk w;
-    const unsigned t = u / jblocks;
-    w.j0 = (u % jblocks) * R;
-    w.k0 = (t % segs) * (64u * (unsigned)E);
-    w.i = t / segs;
-    w.R = R;
-    w.segs = segs;
-    return w;
-}
-
-// k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
-// `row`): in-lane, then across lanes, then across segments for N > 64 E.
-// The value of the first active lane, in scalar registers (the same in every lane).
-__device__ __forceinline__ float first_lane(float x)
-{
-    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(x)));
-}
-__device__ __forceinline__ double first_lane(double x)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-template <typename T, int E>
-__device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restrict__ row, const RowWalk &w,
-                                             unsigned N, unsigned lane, T &kp_last, T &km_first)
-{
-    kp_last = __shfl(c[0], (int)((lane + 1) & 63));
-    km_first = __shfl(c[E - 1], (int)((lane + 63) & 63));
-    if (w.segs > 1) {
-        // the two edge sites, loaded by every lane and kept wave-uniform, then
-        // selected per lane without a branch
-        const unsigned m = N - 1;
-        const T lo = first_lane(row[(w.k0 - 1) & m]), hi = first_lane(row[(w.k0 + 64u * E) & m]);
-        km_first = lane == 0 ? lo : km_first;
-        kp_last = lane == 63 ? hi : kp_last;
-    }
-}
 
 template <typename T, bool SPLIT, int E>
 __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop,

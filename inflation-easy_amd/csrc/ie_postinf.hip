@@ -1,0 +1,452 @@
+// ie_postinf.hip -- post-inflation (fluid era) steps of the gfx950 engine.
+//
+// After inflation and the deltaN map the reference evolves the Bardeen potential
+// Phi in a fluid of equation of state w, and the tensors it sources
+// (src/evolution.cpp:1378-1651, src/initialize.cpp:246-291):
+//   Phi'' = w laplnorm lapl Phi - (3 (1 + w) + s) (a'/a) Phi'
+//   h_ij'' = laplnorm lapl h_ij - (2 + s) (a'/a) h_ij' + 2 a^{-2s} S_ij
+//   S_ij = 4 Phi d_i d_j Phi + 2 d_i Phi d_j Phi - 4/(3(1+w)) d_i U d_j U,  U = Phi'/(a'/a) + Phi
+// with a'' = -(s - (1 - 3w)/2) a'^2 / a, a closed equation: no lattice sums enter
+// the step, so the host advances (a, a') and hands each launch its scalars.
+//
+// One leapfrog step (evolve_derivs_post_inflation then evolve_fields) is two
+// launches over the FP64 Phi, Phi' and the six FP32 h_ij, h_ij':
+//   k_post_kick    Phi' += d (...) from the 7-point Laplacian of Phi   24 B/site
+//   k_post_tensor  the tensor kick with S_ij from the old Phi (gradient and
+//                  Hessian: 15 points) and the kicked Phi' (gradient), then the
+//                  drifts of h_ij and Phi                              120 B/site
+// Phi, Phi' and h_ij are double-buffered (neighbours are read while the new value
+// is written); h_ij' is updated in place.  Rows of N >= 64 lattices are walked
+// by waves as in the inflation kernels (vector loads, k +- 1 by lane shuffles,
+// XCD-aware plane slabs); N < 64 lattices use one site per thread.
+//
+// The site arithmetic keeps the reference's operation order, with FP contraction
+// off, so every value equals the reference build bit for bit; that includes the
+// (y, z) entry of the Hessian, which the reference forms from f(i, j+1, k+1) and
+// f(i, j-1, k-1) only (src/evolution.cpp:93-106): ((A - B) - A) + B.
+#include <math.h>
+
+#include "ie_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace ie {
+
+namespace {
+
+struct PostTensorArgs {
+    const double *f;    // Phi before the step (read)
+    const double *fd;   // Phi' after the kick (read)
+    double *f_out;      // Phi after the drift
+    const float *h[6];
+    float *h_out[6];
+    float *hd[6];       // in place
+};
+
+// hd' = hd + d (laplnorm lapl h - (2+s) a' hd / a + srcAmp S), h' = h + dd hd'
+__device__ __forceinline__ void post_tensor_update(float hl, float hc, float hv, double S, const PostStep &P,
+                                                   float *hd_new, float *h_new)
+{
+    const double rhs = P.d * (P.laplnorm * (double)hl - P.c2ad * (double)hv / P.a + P.srcamp * S);
+    const float hdn = hv + (float)rhs;
+    *hd_new = hdn;
+    *h_new = hc + (float)(P.dd * (double)hdn);
+}
+
+__device__ __forceinline__ double post_kick_value(double fdv, double lap, const PostStep &P)
+{
+    return fdv + P.d * (P.A * lap - P.c1ad * fdv / P.a);
+}
+
+__device__ __forceinline__ void site_ijk(unsigned id, unsigned logN, unsigned m, unsigned &i, unsigned &j, unsigned &k)
+{
+    k = id & m;
+    j = (id >> logN) & m;
+    i = id >> (2 * logN);
+}
+
+__device__ __forceinline__ unsigned sidx(unsigned i, unsigned j, unsigned k, unsigned logN, unsigned m)
+{
+    return ((i & m) << (2 * logN)) | ((j & m) << logN) | (k & m);
+}
+
+// ---------------------------------------------------------------- site kernels
+
+__global__ __launch_bounds__(kBlock) void k_post_kick_site(const double *__restrict__ f,
+                                                            const double *__restrict__ fd,
+                                                            double *__restrict__ fd_out, unsigned N, unsigned logN,
+                                                            PostStep P)
+{
+    const unsigned m = N - 1, gs = N * N * N;
+    for (unsigned id = blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += gridDim.x * blockDim.x) {
+        unsigned i, j, k;
+        site_ijk(id, logN, m, i, j, k);
+        const double c = f[id];
+        const double lap = f[sidx(i, j, k + 1, logN, m)] + f[sidx(i, j, k - 1, logN, m)] +
+                           f[sidx(i, j + 1, k, logN, m)] + f[sidx(i, j - 1, k, logN, m)] +
+                           f[sidx(i + 1, j, k, logN, m)] + f[sidx(i - 1, j, k, logN, m)] - 6.0 * c;
+        fd_out[id] = post_kick_value(fd[id], lap, P);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_post_tensor_site(PostTensorArgs T, unsigned N, unsigned logN, PostStep P)
+{
+    const unsigned m = N - 1, gs = N * N * N;
+    const double *f = T.f, *fd = T.fd;
+    for (unsigned id = blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += gridDim.x * blockDim.x) {
+        unsigned i, j, k;
+        site_ijk(id, logN, m, i, j, k);
+        SiteStencil q;
+        q.c = f[id];
+        q.fxp = f[sidx(i + 1, j, k, logN, m)];
+        q.fxm = f[sidx(i - 1, j, k, logN, m)];
+        q.fyp = f[sidx(i, j + 1, k, logN, m)];
+        q.fym = f[sidx(i, j - 1, k, logN, m)];
+        q.fzp = f[sidx(i, j, k + 1, logN, m)];
+        q.fzm = f[sidx(i, j, k - 1, logN, m)];
+        q.fxpyp = f[sidx(i + 1, j + 1, k, logN, m)];
+        q.fxpym = f[sidx(i + 1, j - 1, k, logN, m)];
+        q.fxmyp = f[sidx(i - 1, j + 1, k, logN, m)];
+        q.fxmym = f[sidx(i - 1, j - 1, k, logN, m)];
+        q.fxpzp = f[sidx(i + 1, j, k + 1, logN, m)];
+        q.fxpzm = f[sidx(i + 1, j, k - 1, logN, m)];
+        q.fxmzp = f[sidx(i - 1, j, k + 1, logN, m)];
+        q.fxmzm = f[sidx(i - 1, j, k - 1, logN, m)];
+        q.fypzp = f[sidx(i, j + 1, k + 1, logN, m)];
+        q.fymzm = f[sidx(i, j - 1, k - 1, logN, m)];
+        q.dc = fd[id];
+        q.dxp = fd[sidx(i + 1, j, k, logN, m)];
+        q.dxm = fd[sidx(i - 1, j, k, logN, m)];
+        q.dyp = fd[sidx(i, j + 1, k, logN, m)];
+        q.dym = fd[sidx(i, j - 1, k, logN, m)];
+        q.dzp = fd[sidx(i, j, k + 1, logN, m)];
+        q.dzm = fd[sidx(i, j, k - 1, logN, m)];
+        double S[6];
+        post_source(q, P, S);
+#pragma unroll 1
+        for (int c = 0; c < 6; ++c) {
+            const float *h = T.h[c];
+            const float hc = h[id];
+            const float hl = h[sidx(i, j, k + 1, logN, m)] + h[sidx(i, j, k - 1, logN, m)] +
+                             h[sidx(i, j + 1, k, logN, m)] + h[sidx(i, j - 1, k, logN, m)] +
+                             h[sidx(i + 1, j, k, logN, m)] + h[sidx(i - 1, j, k, logN, m)] - 6.0f * hc;
+            float hdn, hn;
+            post_tensor_update(hl, hc, T.hd[c][id], S[c], P, &hdn, &hn);
+            T.hd[c][id] = hdn;
+            T.h_out[c][id] = hn;
+        }
+        T.f_out[id] = q.c + P.dd * q.dc;
+    }
+}
+
+// ----------------------------------------------------------------- row kernels
+
+// kp / km of element e of a row held as E values per lane
+template <typename T, int E>
+__device__ __forceinline__ T kplus(const T (&c)[E], int e, T kp_last)
+{
+    return e < E - 1 ? c[e + 1 < E ? e + 1 : e] : kp_last;
+}
+template <typename T, int E>
+__device__ __forceinline__ T kminus(const T (&c)[E], int e, T km_first)
+{
+    return e > 0 ? c[e > 0 ? e - 1 : 0] : km_first;
+}
+
+template <int E>
+__global__ __launch_bounds__(kBlock) void k_post_kick_rows(const double *__restrict__ f,
+                                                            const double *__restrict__ fd,
+                                                            double *__restrict__ fd_out, unsigned N, unsigned logN,
+                                                            PostStep P)
+{
+    const unsigned m = N - 1, lane = threadIdx.x & 63;
+    const RowWalk w = row_walk(N, gridDim.x, blockIdx.x, E);
+    const unsigned P0 = w.i << (2 * logN), Pp = ((w.i + 1) & m) << (2 * logN), Pm = ((w.i - 1) & m) << (2 * logN);
+    double prev[E], cur[E], nxt[E];
+    ld_row<double, E>(f + P0 + (((w.j0 - 1) & m) << logN) + w.k0, lane, prev);
+    ld_row<double, E>(f + P0 + (w.j0 << logN) + w.k0, lane, cur);
+    for (unsigned r = 0; r < w.R; ++r) {
+        const unsigned j = w.j0 + r, rj = j << logN;
+        double up[E], dn[E], v[E], out[E];
+        ld_row<double, E>(f + P0 + (((j + 1) & m) << logN) + w.k0, lane, nxt);
+        ld_row<double, E>(f + Pp + rj + w.k0, lane, up);
+        ld_row<double, E>(f + Pm + rj + w.k0, lane, dn);
+        ld_row<double, E>(fd + P0 + rj + w.k0, lane, v);
+        double kp_last, km_first;
+        k_neighbours<double, E>(cur, f + P0 + rj, w, N, lane, kp_last, km_first);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const double lap = kplus<double, E>(cur, e, kp_last) + kminus<double, E>(cur, e, km_first) + nxt[e] +
+                               prev[e] + up[e] + dn[e] - 6.0 * cur[e];
+            out[e] = post_kick_value(v[e], lap, P);
+        }
+        st_row<double, E>(fd_out + P0 + rj + w.k0, lane, out);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            prev[e] = cur[e];
+            cur[e] = nxt[e];
+        }
+    }
+}
+
+// Rows of Phi in planes i-1, i, i+1 (index 0, 1, 2) at j-1, j, j+1 (0, 1, 2)
+// slide along j in registers; the kicked Phi' rows of plane i likewise.
+template <int E>
+__global__ __launch_bounds__(kBlock) void k_post_tensor_rows(PostTensorArgs T, unsigned N, unsigned logN, PostStep P)
+{
+    const unsigned m = N - 1, lane = threadIdx.x & 63;
+    const RowWalk w = row_walk(N, gridDim.x, blockIdx.x, E);
+    const unsigned pl[3] = {((w.i - 1) & m) << (2 * logN), w.i << (2 * logN), ((w.i + 1) & m) << (2 * logN)};
+    const double *f = T.f, *fd = T.fd;
+    double F[3][3][E], D[3][E];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ld_row<double, E>(f + pl[a] + (((w.j0 - 1) & m) << logN) + w.k0, lane, F[a][0]);
+        ld_row<double, E>(f + pl[a] + (w.j0 << logN) + w.k0, lane, F[a][1]);
+    }
+    ld_row<double, E>(fd + pl[1] + (((w.j0 - 1) & m) << logN) + w.k0, lane, D[0]);
+    ld_row<double, E>(fd + pl[1] + (w.j0 << logN) + w.k0, lane, D[1]);
+    for (unsigned r = 0; r < w.R; ++r) {
+        const unsigned j = w.j0 + r, rj = j << logN, rjp = ((j + 1) & m) << logN, rjm = ((j - 1) & m) << logN;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ld_row<double, E>(f + pl[a] + rjp + w.k0, lane, F[a][2]);
+        double Dup[E], Ddn[E];
+        ld_row<double, E>(fd + pl[1] + rjp + w.k0, lane, D[2]);
+        ld_row<double, E>(fd + pl[2] + rj + w.k0, lane, Dup);
+        ld_row<double, E>(fd + pl[0] + rj + w.k0, lane, Ddn);
+        // k +- 1 of the rows that need them
+        double c_kp, c_km, xp_kp, xp_km, xm_kp, xm_km, yp_kp, yp_km, ym_kp, ym_km, d_kp, d_km;
+        k_neighbours<double, E>(F[1][1], f + pl[1] + rj, w, N, lane, c_kp, c_km);
+        k_neighbours<double, E>(F[2][1], f + pl[2] + rj, w, N, lane, xp_kp, xp_km);
+        k_neighbours<double, E>(F[0][1], f + pl[0] + rj, w, N, lane, xm_kp, xm_km);
+        k_neighbours<double, E>(F[1][2], f + pl[1] + rjp, w, N, lane, yp_kp, yp_km);
+        k_neighbours<double, E>(F[1][0], f + pl[1] + rjm, w, N, lane, ym_kp, ym_km);
+        k_neighbours<double, E>(D[1], fd + pl[1] + rj, w, N, lane, d_kp, d_km);
+        double S[E][6], fnew[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            SiteStencil q;
+            q.c = F[1][1][e];
+            q.fxp = F[2][1][e];
+            q.fxm = F[0][1][e];
+            q.fyp = F[1][2][e];
+            q.fym = F[1][0][e];
+            q.fzp = kplus<double, E>(F[1][1], e, c_kp);
+            q.fzm = kminus<double, E>(F[1][1], e, c_km);
+            q.fxpyp = F[2][2][e];
+            q.fxpym = F[2][0][e];
+            q.fxmyp = F[0][2][e];
+            q.fxmym = F[0][0][e];
+            q.fxpzp = kplus<double, E>(F[2][1], e, xp_kp);
+            q.fxpzm = kminus<double, E>(F[2][1], e, xp_km);
+            q.fxmzp = kplus<double, E>(F[0][1], e, xm_kp);
+            q.fxmzm = kminus<double, E>(F[0][1], e, xm_km);
+            q.fypzp = kplus<double, E>(F[1][2], e, yp_kp);
+            q.fymzm = kminus<double, E>(F[1][0], e, ym_km);
+            q.dc = D[1][e];
+            q.dxp = Dup[e];
+            q.dxm = Ddn[e];
+            q.dyp = D[2][e];
+            q.dym = D[0][e];
+            q.dzp = kplus<double, E>(D[1], e, d_kp);
+            q.dzm = kminus<double, E>(D[1], e, d_km);
+            post_source(q, P, S[e]);
+            fnew[e] = q.c + P.dd * q.dc;
+        }
+        st_row<double, E>(T.f_out + pl[1] + rj + w.k0, lane, fnew);
+        (void)yp_km;
+        (void)ym_kp;
+#pragma unroll 1
+        for (int c = 0; c < 6; ++c) {
+            const float *__restrict__ h = T.h[c];
+            float hc[E], hjp[E], hjm[E], hip[E], him[E], hv[E];
+            ld_row<float, E>(h + pl[1] + rj + w.k0, lane, hc);
+            ld_row<float, E>(h + pl[1] + rjp + w.k0, lane, hjp);
+            ld_row<float, E>(h + pl[1] + rjm + w.k0, lane, hjm);
+            ld_row<float, E>(h + pl[2] + rj + w.k0, lane, hip);
+            ld_row<float, E>(h + pl[0] + rj + w.k0, lane, him);
+            ld_row<float, E>(T.hd[c] + pl[1] + rj + w.k0, lane, hv);
+            float hkp, hkm;
+            k_neighbours<float, E>(hc, h + pl[1] + rj, w, N, lane, hkp, hkm);
+            float hdn[E], hn[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float hl = kplus<float, E>(hc, e, hkp) + kminus<float, E>(hc, e, hkm) + hjp[e] + hjm[e] +
+                                 hip[e] + him[e] - 6.0f * hc[e];
+                double Sc = S[e][0];
+#pragma unroll
+                for (int q = 1; q < 6; ++q) Sc = c == q ? S[e][q] : Sc;
+                post_tensor_update(hl, hc[e], hv[e], Sc, P, &hdn[e], &hn[e]);
+            }
+            st_row<float, E>(T.hd[c] + pl[1] + rj + w.k0, lane, hdn);
+            st_row<float, E>(T.h_out[c] + pl[1] + rj + w.k0, lane, hn);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                F[a][0][e] = F[a][1][e];
+                F[a][1][e] = F[a][2][e];
+            }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            D[0][e] = D[1][e];
+            D[1][e] = D[2][e];
+        }
+    }
+}
+
+// --------------------------------------------------------- drift and set-up
+
+// evolve_fields on the post-inflation state (src/evolution.cpp:260-284)
+__global__ __launch_bounds__(kBlock) void k_post_drift(double *__restrict__ f, const double *__restrict__ fd,
+                                                        PostTensorArgs T, size_t gs, double d)
+{
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        f[id] += d * fd[id];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) T.h_out[c][id] += (float)(d * (double)T.hd[c][id]);
+    }
+}
+
+// The reference's set-up sums walk the lattice in order with one accumulator
+// each (src/initialize.cpp:261-263, 270-276); the order is part of the result, so
+// one wave streams the arrays through LDS and lane 0 adds in order.
+__global__ __launch_bounds__(64) void k_serial_sums(const double *__restrict__ g1, const double *__restrict__ g2,
+                                                     size_t gs, double *out)
+{
+    __shared__ double c1[1024], c2[1024];
+    double s1 = 0.0, s2 = 0.0;
+    for (size_t base = 0; base < gs; base += 1024) {
+        const size_t len = gs - base < 1024 ? gs - base : 1024;
+        for (size_t e = threadIdx.x; e < len; e += 64) {
+            c1[e] = g1[base + e];
+            if (g2) c2[e] = g2[base + e];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (size_t e = 0; e < len; ++e) s1 += c1[e];
+            if (g2)
+                for (size_t e = 0; e < len; ++e) s2 += c2[e];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s1;
+        out[1] = s2;
+    }
+}
+
+// Phi = P (src - mean) [/ den] (src/initialize.cpp:264-265, 278-281)
+__global__ __launch_bounds__(kBlock) void k_post_phi(double *__restrict__ f, const double *__restrict__ src, size_t gs,
+                                                      double P, double mean, double den, int divide)
+{
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        const double v = P * (src[id] - mean);
+        f[id] = divide ? v / den : v;
+    }
+}
+
+// sums of Phi, Phi' and Phi^2 for meansvars (fixed order: block tree, then
+// the blocks in order by one workgroup)
+__global__ __launch_bounds__(kBlock) void k_post_sums3(const double *__restrict__ f, const double *__restrict__ fd,
+                                                        size_t gs, double *__restrict__ partials)
+{
+    __shared__ double smem[3 * kWaves];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
+        const double v = f[id];
+        acc[0] += v;
+        acc[1] += fd[id];
+        acc[2] += v * v;
+    }
+    block_sum<3>(acc, smem);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) partials[(size_t)blockIdx.x * 3 + q] = acc[q];
+}
+
+__global__ __launch_bounds__(kBlock) void k_post_sums3_final(const double *__restrict__ partials, int nb,
+                                                              double *__restrict__ out)
+{
+    __shared__ double smem[3 * kWaves];
+    double tot[3];
+    sum_partials<3>(partials, nb, tot, smem);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) out[q] = tot[q];
+}
+
+unsigned site_grid(size_t gs)
+{
+    size_t b = (gs + kBlock - 1) / kBlock;
+    if (b > 4096) b = 4096;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+int post_row_elems(int N) { return N < 64 ? 0 : 1; }
+
+// N^2 segments / 32 workgroups: 8 rows per wave (R = 8 divides N >= 64)
+unsigned post_row_blocks(int N, int E)
+{
+    const size_t segs = (size_t)N / (64 * (size_t)E);
+    return (unsigned)((size_t)N * N * segs / 32);
+}
+
+hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s)
+{
+    PostTensorArgs T;
+    T.f = S.f;
+    T.fd = S.fd_out;
+    T.f_out = S.f_out;
+    for (int c = 0; c < 6; ++c) {
+        T.h[c] = S.h[c];
+        T.h_out[c] = S.h_out[c];
+        T.hd[c] = S.hd[c];
+    }
+    const int E = post_row_elems(N);
+    if (E == 0) {
+        const unsigned G = site_grid((size_t)N * N * N);
+        k_post_kick_site<<<G, kBlock, 0, s>>>(S.f, S.fd, S.fd_out, (unsigned)N, (unsigned)logN, P);
+        k_post_tensor_site<<<G, kBlock, 0, s>>>(T, (unsigned)N, (unsigned)logN, P);
+    } else {
+        const unsigned G = post_row_blocks(N, 1);
+        k_post_kick_rows<1><<<G, kBlock, 0, s>>>(S.f, S.fd, S.fd_out, (unsigned)N, (unsigned)logN, P);
+        k_post_tensor_rows<1><<<G, kBlock, 0, s>>>(T, (unsigned)N, (unsigned)logN, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t post_drift(const PostState &S, size_t gs, double d, hipStream_t s)
+{
+    PostTensorArgs T{};
+    for (int c = 0; c < 6; ++c) {
+        T.h_out[c] = S.h[c];
+        T.hd[c] = S.hd[c];
+    }
+    k_post_drift<<<site_grid(gs), kBlock, 0, s>>>(S.f, S.fd, T, gs, d);
+    return hipGetLastError();
+}
+
+hipError_t serial_sums(const double *g1, const double *g2, size_t gs, double *out, hipStream_t s)
+{
+    k_serial_sums<<<1, 64, 0, s>>>(g1, g2, gs, out);
+    return hipGetLastError();
+}
+
+hipError_t post_phi(double *f, const double *src, size_t gs, double P, double mean, double den, int divide,
+                    hipStream_t s)
+{
+    k_post_phi<<<site_grid(gs), kBlock, 0, s>>>(f, src, gs, P, mean, den, divide);
+    return hipGetLastError();
+}
+
+hipError_t post_sums3(const double *f, const double *fd, size_t gs, double *work, double *out, hipStream_t s)
+{
+    const unsigned nb = site_grid(gs) > 1024 ? 1024 : site_grid(gs);
+    k_post_sums3<<<nb, kBlock, 0, s>>>(f, fd, gs, work);
+    k_post_sums3_final<<<1, kBlock, 0, s>>>(work, (int)nb, out);
+    return hipGetLastError();
+}
+
+}  // namespace ie

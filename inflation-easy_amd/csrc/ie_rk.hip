@@ -16,6 +16,10 @@
 
 #include "ie_internal.h"
 
+// no contraction: the stage sums, updates and right-hand sides keep the
+// reference's rounding (the post-inflation right-hand side is bitwise equal)
+#pragma clang fp contract(off)
+
 namespace ie {
 
 namespace {
@@ -158,6 +162,85 @@ __global__ __launch_bounds__(kBlock) void k_rk_rhs(RkState S, Pot P, const RkCoe
     R->kad[stage] = pow(a_s, 3.0 - 2.0 * s) * source - (s + 1.0) * (ad_s * ad_s) / a_s;
 }
 
+// post-inflation right-hand side at the stage state (compute_inflation_rhs's
+// post branch, src/evolution.cpp:520-592): no sums; a'' from (a, a') alone
+__global__ __launch_bounds__(kBlock) void k_rk_rhs_post(RkState S, const RkCoef *C, RkScalars *R, const DevBg *bg,
+                                                        double h, int stage, int use_y, double omega)
+{
+    double a_s = bg->a, ad_s = bg->ad;
+    for (int q = 0; q < stage; ++q) {
+        const double c = C->A[stage][q];
+        if (c == 0.0) continue;
+        a_s += h * c * R->ka[q];
+        ad_s += h * c * R->kad[q];
+    }
+    const double s = bg->rescale_s, dx = bg->dx, w = omega;
+    PostStep P{};
+    P.laplnorm = pow(a_s, -2.0 * s) / (dx * dx);
+    const double scalar_friction = (3.0 * (1.0 + w) + s) * ad_s / a_s;
+    const double tensor_friction = (2.0 + s) * ad_s / a_s;
+    const double Htilde = ad_s / a_s;
+    P.invH = 1.0 / Htilde;
+    P.coeffU = 4.0 / (3.0 * (1.0 + w));
+    P.srcamp = 2.0 * pow(a_s, -2.0 * s);
+    P.hx = 0.5 / dx;
+    P.inv_dx2 = 1.0 / (dx * dx);
+    P.cross_den = 4.0 * dx * dx;
+    const double *fs = use_y ? S.f : S.ftmp;
+    const double *fds = use_y ? S.fd : S.fdtmp;
+    const unsigned gs = gsize(S), N = (unsigned)S.N, logN = (unsigned)S.logN, m = N - 1;
+    for (unsigned id = blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += gridDim.x * blockDim.x) {
+        unsigned n[6];
+        nb(id, N, logN, n);
+        const unsigned k = id & m, j = (id >> logN) & m, i = id >> (2 * logN);
+        auto at = [&](unsigned ii, unsigned jj, unsigned kk) {
+            return ((ii & m) << (2 * logN)) | ((jj & m) << logN) | (kk & m);
+        };
+        SiteStencil q;
+        q.c = fs[id];
+        q.fzp = fs[n[0]];
+        q.fzm = fs[n[1]];
+        q.fyp = fs[n[2]];
+        q.fym = fs[n[3]];
+        q.fxp = fs[n[4]];
+        q.fxm = fs[n[5]];
+        q.fxpyp = fs[at(i + 1, j + 1, k)];
+        q.fxpym = fs[at(i + 1, j - 1, k)];
+        q.fxmyp = fs[at(i - 1, j + 1, k)];
+        q.fxmym = fs[at(i - 1, j - 1, k)];
+        q.fxpzp = fs[at(i + 1, j, k + 1)];
+        q.fxpzm = fs[at(i + 1, j, k - 1)];
+        q.fxmzp = fs[at(i - 1, j, k + 1)];
+        q.fxmzm = fs[at(i - 1, j, k - 1)];
+        q.fypzp = fs[at(i, j + 1, k + 1)];
+        q.fymzm = fs[at(i, j - 1, k - 1)];
+        q.dc = fds[id];
+        q.dzp = fds[n[0]];
+        q.dzm = fds[n[1]];
+        q.dyp = fds[n[2]];
+        q.dym = fds[n[3]];
+        q.dxp = fds[n[4]];
+        q.dxm = fds[n[5]];
+        const double lf = q.fzp + q.fzm + q.fyp + q.fym + q.fxp + q.fxm - 6.0 * q.c;
+        S.kf[stage][id] = q.dc;
+        S.kfd[stage][id] = w * P.laplnorm * lf - scalar_friction * q.dc;
+        double Ssrc[6];
+        post_source(q, P, Ssrc);
+        for (int c6 = 0; c6 < 6; ++c6) {
+            const float *hh = use_y ? S.h[c6] : S.htmp[c6];
+            const float *hdh = use_y ? S.hd[c6] : S.hdtmp[c6];
+            const float lh = hh[n[0]] + hh[n[1]] + hh[n[2]] + hh[n[3]] + hh[n[4]] + hh[n[5]] - 6.0f * hh[id];
+            S.kh[stage][c6][id] = hdh[id];
+            S.khd[stage][c6][id] =
+                (float)(P.laplnorm * (double)lh - tensor_friction * (double)hdh[id] + P.srcamp * Ssrc[c6]);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        R->ka[stage] = ad_s;
+        R->kad[stage] = -(s - 0.5 * (1.0 - 3.0 * w)) * (ad_s * ad_s) / a_s;
+    }
+}
+
 // RK4 weighted update in place (src/evolution.cpp:715-758)
 struct TensorOut {
     float *h[6], *hd[6];
@@ -280,6 +363,8 @@ __global__ __launch_bounds__(kBlock) void k_rk45_final(RkState S, const RkCoef *
 
 struct RkWork {
     int stages = 0, sigw = 0;
+    int post = 0;               // post-inflation right-hand side
+    double omega = 0.0;
     size_t gs = 0;
     std::vector<void *> allocs;
     RkState S{};
@@ -293,6 +378,12 @@ static hipError_t rk_alloc(RkWork *w, void **p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipSuccess) w->allocs.push_back(*p);
     return e;
+}
+
+void rk_set_post(RkWork *w, int post, double omega)
+{
+    w->post = post;
+    w->omega = omega;
 }
 
 void rk_free(RkWork *w)
@@ -391,10 +482,10 @@ static int rk_grid(size_t gs)
 static hipError_t rk_stages(RkWork *w, const Pot &P, DevBg *bg, double h, hipStream_t s)
 {
     const int G = rk_grid(w->gs);
-    k_rk_rhs<<<G, kBlock, 0, s>>>(w->S, P, w->coef, w->scal, bg, h, 0, 1, w->partials);
-    for (int st = 1; st < w->stages; ++st) {
-        k_rk_combine<<<G, kBlock, 0, s>>>(w->S, w->coef, h, st);
-        k_rk_rhs<<<G, kBlock, 0, s>>>(w->S, P, w->coef, w->scal, bg, h, st, 0, w->partials);
+    for (int st = 0; st < w->stages; ++st) {
+        if (st > 0) k_rk_combine<<<G, kBlock, 0, s>>>(w->S, w->coef, h, st);
+        if (w->post) k_rk_rhs_post<<<G, kBlock, 0, s>>>(w->S, w->coef, w->scal, bg, h, st, st == 0, w->omega);
+        else k_rk_rhs<<<G, kBlock, 0, s>>>(w->S, P, w->coef, w->scal, bg, h, st, st == 0, w->partials);
     }
     return hipGetLastError();
 }

@@ -103,6 +103,12 @@ class Tuning(C.Structure):
                 "kick_elems": self.kick_elems, "tensor_code": self.tensor_code_chosen, "steps": self.steps}
 
 
+class PostParams(C.Structure):
+    """Mirror of ``ie_post_params``."""
+    _fields_ = [("horizon_factor", C.c_double), ("omega", C.c_double), ("dt", C.c_double), ("af", C.c_double),
+                ("integrator", C.c_int), ("from_deltaN", C.c_int)]
+
+
 class Background(C.Structure):
     """Mirror of ``ie_background``."""
 
@@ -139,6 +145,10 @@ def lib() -> C.CDLL:
             "ie_upload_fields": (st, [P, dp, dp]),
             "ie_upload_modes": (st, [P, dp, dp, dp, dp, C.c_double, C.c_double]),
             "ie_download_field": (st, [P, C.c_int, dp]),
+            "ie_upload_field": (st, [P, C.c_int, dp]),
+            "ie_post_init": (st, [P, C.POINTER(PostParams), C.POINTER(Background)]),
+            "ie_post_evolve_derivs": (st, [P, C.c_double]),
+            "ie_post_leapfrog_steps": (st, [P, C.c_int, C.POINTER(Background)]),
             "ie_upload_tensor": (st, [P, C.c_int, fp]),
             "ie_download_tensor": (st, [P, C.c_int, fp]),
             "ie_get_background": (st, [P, C.POINTER(Background)]),
@@ -427,6 +437,28 @@ class Run:
 
     def deltaN_center(self):
         self._check(lib().ie_deltaN_center(self.h))
+
+    def upload_field(self, which: int, src: np.ndarray):
+        """FP64 upload: the deltaN map, or f / fd of the deltaN and post-inflation phases."""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        assert src.size == self.gs
+        self._check(lib().ie_upload_field(self.h, which, _dp(src)))
+
+    # ---- post-inflation
+    def post_init(self, omega: float = 1.0 / 3.0, horizon_factor: float = 1.0, dt: float = 0.001,
+                  af: float = 0.0, integrator: int = 0, from_deltaN: int = 1) -> dict:
+        p = PostParams(horizon_factor, omega, dt, af, integrator, from_deltaN)
+        b = Background()
+        self._check(lib().ie_post_init(self.h, C.byref(p), C.byref(b)))
+        return b.as_dict()
+
+    def post_evolve_derivs(self, d: float):
+        self._check(lib().ie_post_evolve_derivs(self.h, d))
+
+    def post_leapfrog_steps(self, n: int) -> dict:
+        b = Background()
+        self._check(lib().ie_post_leapfrog_steps(self.h, n, C.byref(b)))
+        return b.as_dict()
 
 
 class Batch:
