@@ -933,6 +933,18 @@ ie_status ie_initialize(ie_run *r, const ie_ic_params *ic, ie_background *out)
 
 /* ------------------------------------------------------------- deltaN */
 
+// FP64 state of the deltaN and post-inflation phases at the host mirror's parity
+static void dn_buffers(ie_run *r, double **f, double **fd)
+{
+    if (r->cfg.precision == IE_SPLIT) {
+        *f = r->dn_f;
+        *fd = r->dn_fd;
+    } else {
+        *f = (double *)r->F.f[r->bg_host->parity];
+        *fd = (double *)r->F.fd[r->bg_host->parity];
+    }
+}
+
 static ie_status dn_state(ie_run *r, double **f, double **fd)
 {
     ie_status st = pull_bg(r);
@@ -1218,12 +1230,13 @@ ie_status ie_histogram(ie_run *r, int which, int nbins, double *bmin, double *bm
 
 /* ------------------------------------------------------ post-inflation */
 
-// Phi, Phi' (FP64) and h of the current step and the buffers its results go to
+// Phi, Phi' (FP64) and h of the current step and the buffers its results go to,
+// from the host mirror of the background (the callers pull it once, and the
+// leapfrog loop advances it between launches)
 static ie_status post_state(ie_run *r, PostState *S)
 {
     double *f, *fd;
-    ie_status st = dn_state(r, &f, &fd);
-    if (st != IE_OK) return st;
+    dn_buffers(r, &f, &fd);
     const int par = r->bg_host->parity;
     S->f = f;
     S->fd = fd;
