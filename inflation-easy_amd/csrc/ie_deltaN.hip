@@ -103,24 +103,31 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f,
         }
         int ghint = hint;
         bool act = active(D, phi, x, yv, n, &ghint);
-        for (long long s = 0; s < nsteps && act; ++s) {
-            // evolve_derivsN (src/evolution.cpp:1010) with pot_ratio (src/potential.cpp:177-192)
-            double ratio;
-            if (D.P.numerical) {
-                if (!in_table(x, n, phi)) {
-                    err = 1;
-                    break;
+        double ratio = 0.0;
+        bool frozen = false;   // phi of a finished site no longer moves: its ratio is fixed
+        for (long long s = 0; s < nsteps; ++s) {
+            // evolve_derivsN (src/evolution.cpp:1004-1012) on every site, finished
+            // or not, with pot_ratio (src/potential.cpp:177-192)
+            if (!frozen) {
+                if (D.P.numerical) {
+                    if (!in_table(x, n, phi)) {
+                        err = 1;
+                        break;
+                    }
+                    hint = walk(x, n, phi, hint);
+                    ratio = interp(x, ydv, phi, hint) / interp(x, yv, phi, hint);
+                } else {
+                    ratio = hill_dV(D.P, phi) / hill_V(D.P, phi);
                 }
-                hint = walk(x, n, phi, hint);
-                ratio = interp(x, ydv, phi, hint) / interp(x, yv, phi, hint);
-            } else {
-                ratio = hill_dV(D.P, phi) / hill_V(D.P, phi);
+                frozen = !act;
             }
             pi += dN * (-(3.0 - 0.5 * (pi * pi)) * (pi + ratio));
-            // evolve_fieldsN (src/evolution.cpp:1022-1034)
-            N += dN;
-            phi += dN * pi;
-            act = active(D, phi, x, yv, n, &ghint);
+            // evolve_fieldsN (src/evolution.cpp:1015-1037): active sites only
+            if (act) {
+                N += dN;
+                phi += dN * pi;
+                act = active(D, phi, x, yv, n, &ghint);
+            }
         }
         f[id] = phi;
         fd[id] = pi;

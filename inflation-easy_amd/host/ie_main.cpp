@@ -2,11 +2,13 @@
 //
 // Restates the reference program's host side (src/main.cpp:103-146) above the C
 // ABI of include/ie_api.h: the params.txt parser (src/runtime_parameters.cpp),
-// the phase order (initial conditions, inflation loop, deltaN loop), the output
-// cadence and re-synchronisation of save()/saveN() (src/output.cpp:1469-1560) and
-// the results/*.dat formats.  The compile-time toggles of src/parameters.h are
-// run-time keys here (N, numerical_potential, perform_deltaN, calculate_SIGW,
-// monotonic_potential, antimonotonic_potential) plus `precision` (fp64 | split).
+// the phase order (initial conditions, inflation loop, deltaN loop, post-inflation
+// loop), the output cadence and re-synchronisation of save()/saveN()/
+// save_post_inflation() (src/output.cpp:1469-1560,1839-1866) and the results/*.dat
+// and results/post_inflation/*.dat formats.  The compile-time toggles of
+// src/parameters.h are run-time keys here (N, numerical_potential, perform_deltaN,
+// calculate_SIGW, post_inflation, monotonic_potential, antimonotonic_potential) plus
+// `precision` (fp64 | split).
 // Lattice state stays on the GPU; only means, spectra bins, histograms and the
 // requested snapshots come back to the host.
 //
@@ -50,6 +52,9 @@ struct Params {
     int output_bispectrum = 0, output_LOG = 0;
     double eta_log = -0.5;
     int screen_updates = 1, nbins = 256, int_err = 5, int_errN = 5;
+    // post-inflation phase (src/parameters.h:31, src/runtime_parameters.cpp:48-52,79)
+    int post = 1, post_integrator = IE_LEAPFROG;
+    double horizon_factor = 1.0, omega = 1.0 / 3.0, dt_post = 0.001, af_post = 0.0;
 };
 
 std::string trim(const std::string &s)
@@ -145,7 +150,7 @@ Params load_params(const char *path)
         p.dN = 0.0000001;
         p.Nend = 0.001;
     }
-    bool B_set = false, af_set = false, min_set = false, max_set = false;
+    bool B_set = false, af_set = false, min_set = false, max_set = false, af_post_set = false, post_keys = false;
     for (auto &e : kv) {
         const int lineno = e.first;
         const std::string &key = e.second.first, &val = e.second.second;
@@ -203,10 +208,17 @@ Params load_params(const char *path)
             if (val == "split") p.precision = IE_SPLIT;
             else if (val == "fp64") p.precision = IE_FP64;
             else fprintf(stderr, "Ignoring invalid precision '%s' on line %d.\n", val.c_str(), lineno);
-        } else if (key == "horizon_factor" || key == "omega" || key == "dt_post_inflation" ||
-                   key == "af_post_inflation" || key == "post_inflation_integrator" || key == "post_inflation" ||
-                   key == "parallel_calculation") {
-            // post-inflation phase is not part of this engine
+        } else if (key == "post_inflation" && parse_int(val, iv)) p.post = iv != 0;
+        else if (key == "horizon_factor" && parse_double(val, dv)) p.horizon_factor = dv, post_keys = true;
+        else if (key == "omega" && parse_double(val, dv)) p.omega = dv, post_keys = true;
+        else if (key == "dt_post_inflation" && parse_double(val, dv)) p.dt_post = dv, post_keys = true;
+        else if (key == "af_post_inflation" && parse_double(val, dv)) p.af_post = dv, af_post_set = true, post_keys = true;
+        else if (key == "post_inflation_integrator") {
+            post_keys = true;
+            if (!parse_integrator(val, p.post_integrator))
+                fprintf(stderr, "Ignoring invalid post_inflation_integrator '%s' on line %d.\n", val.c_str(), lineno);
+        } else if (key == "parallel_calculation") {
+            // OpenMP toggle of the reference build: nothing to select here
         } else {
             fprintf(stderr, "Ignoring unknown or invalid parameter '%s' on line %d.\n", key.c_str(), lineno);
         }
@@ -217,6 +229,9 @@ Params load_params(const char *path)
     }
     if (!B_set) p.rescale_B = sqrt(p.V0);
     if (!af_set) p.af = 2.0 * p.N;
+    if (!af_post_set) p.af_post = 2.0 * p.N;
+    if (!p.post && post_keys)
+        fprintf(stderr, "Ignoring post-inflation runtime parameters in %s because post_inflation=0.\n", path);
     if (!min_set || p.rk45_min_dt <= 0.0) p.rk45_min_dt = fabs(p.dt) * 1e-6;
     if (!max_set || p.rk45_max_dt <= 0.0) p.rk45_max_dt = fabs(p.dt);
     (void)found;
@@ -631,6 +646,111 @@ struct Driver {
         }
     }
 
+    // save_post_inflation (src/output.cpp:1839-1866): meansvars_post_inflation,
+    // scale_post_inflation, spectraf_post_inflation, the tensor spectra and
+    // histograms_post_inflation into results/post_inflation/
+    void save_post(int infrequent)
+    {
+        const bool staggered = p.post_integrator == IE_LEAPFROG;
+        if (staggered && bg().t > 0.) CHECK(run, ie_evolve_fields(run, -.5 * p.dt_post));
+        ie_background b = bg();
+        double sums[3];
+        CHECK(run, ie_field_sums(run, sums));
+        const double av = sums[0] / (double)gs;
+        double vel = sums[1] / (double)gs;
+        vel = vel * pow(b.a, p.rescale_s - 1) * p.rescale_B;
+        FILE *m = out.get("post_inflation/means"), *v = out.get("post_inflation/variance"),
+             *w = out.get("post_inflation/velocity");
+        fprintf(m, "%f %e %e", b.t, b.a, av);
+        fprintf(w, "%f %e %e", b.t, b.a, vel);
+        fprintf(v, "%f %e %e", b.t, b.a, sums[2] - pw2(av));
+        if (av + FLT_MAX == av || (av != 0. && av / av != 1.)) {   // src/output.cpp:1621-1628
+            printf("Unstable solution developed. Scalar field not numerical at t=%f\n", b.t);
+            fflush(m);
+            fflush(v);
+            exit(1);
+        }
+        fprintf(m, "\n");
+        fprintf(v, "\n");
+        fprintf(w, "\n");
+        if (infrequent) {
+            fflush(m);
+            fflush(v);
+            fflush(w);
+        }
+        FILE *sf = out.get("post_inflation/sf");
+        fprintf(sf, "%f %f %e %e\n", b.t, b.a, b.ad * p.rescale_B * pow(b.a, p.rescale_s - 2.),
+                pw2(p.rescale_B) * pow(b.a, 2. * p.rescale_s - 2.) * (b.ad2 + (p.rescale_s - 1.) * pw2(b.ad) / b.a));
+        if (infrequent) fflush(sf);
+        if (infrequent) {
+            if (p.output_spectra) {
+                const int nb = ie_spectrum_numbins(N);
+                std::vector<double> pp(nb), vv(nb);
+                std::vector<int> cc(nb);
+                const double norm1 = pow(p.L / p.rescale_B, 3) / pow((double)N, 6);
+                CHECK(run, ie_spectrum(run, IE_FIELD_F, norm1, pp.data(), cc.data(), vv.data()));
+                spectra_block(out.get("post_inflation/spectra"), pp, cc, vv);
+                FILE *st = out.get("post_inflation/spectratimes");
+                fprintf(st, "%f %e\n", b.t, b.a);
+                fflush(st);
+                const int ng = (int)(sqrt(3.0) * (N / 2)) + 1;
+                std::vector<double> pg(ng), vg(ng);
+                std::vector<int> cg(ng);
+                CHECK(run, ie_gw_spectrum(run, 0, pg.data(), cg.data(), vg.data()));
+                spectra_block(out.get("post_inflation/spectraGW"), pg, cg, vg);
+                CHECK(run, ie_gw_spectrum(run, 1, pg.data(), cg.data(), vg.data()));
+                spectra_block(out.get("post_inflation/spectraGWdot"), pg, cg, vg);
+            }
+            if (p.output_histogram)
+                histogram(IE_FIELD_F, "post_inflation/histogram", "post_inflation/histogramtimes", b, false);
+        }
+        if (staggered && bg().t > 0.) CHECK(run, ie_evolve_fields(run, .5 * p.dt_post));
+    }
+
+    // run_post_inflation_loop (src/evolution.cpp:1568-1651): the leapfrog on the
+    // device in chunks of output_freq steps, RK4 / RK45 one host-driven step at a time
+    void post_inflation()
+    {
+        ie_post_params pp{p.horizon_factor, p.omega, p.dt_post, p.af_post, p.post_integrator, p.deltaN};
+        CHECK(run, ie_post_init(run, &pp, nullptr));
+        save_post(1);
+        long long n = 0;
+        auto report = [&](long long step, double a) {
+            if (step % p.output_freq == 0) save_post((step % p.output_infrequent_freq == 0) ? 1 : 0);
+            progress(step, a);
+        };
+        if (p.post_integrator == IE_LEAPFROG) {
+            CHECK(run, ie_evolve_fields(run, 0.5 * p.dt_post));
+            const int chunk = p.output_freq > 0 ? p.output_freq : 1;
+            for (;;) {
+                ie_background b;
+                CHECK(run, ie_post_leapfrog_steps(run, chunk, &b));
+                const long long taken = b.steps - n;
+                n = b.steps;
+                if (taken <= 0) break;
+                report(n, b.a);
+                if (b.done) break;
+            }
+        } else if (p.post_integrator == IE_RK4) {
+            while (bg().a <= p.af_post) {
+                CHECK(run, ie_rk4_step(run, p.dt_post));
+                ++n;
+                report(n, bg().a);
+            }
+        } else {
+            double h = clamp_rk45(p.dt_post, p.rk45_max_dt);
+            while (bg().a <= p.af_post) {
+                const double hmax = rk45_hmax(p.dt_post);
+                h = clamp_rk45(h, hmax);
+                h = rk45_drive(h, hmax, false);
+                ++n;
+                report(n, bg().a);
+            }
+        }
+        printf("Saving final inflaton data\n");
+        save_post(1);
+    }
+
     int main(const char *params_path)
     {
         p = load_params(params_path);
@@ -645,6 +765,21 @@ struct Driver {
         } else if (!(info.st_mode & S_IFDIR)) {
             fprintf(stderr, "'results' exists but is not a directory.\n");
             return 1;
+        }
+        if (p.post && !p.sigw) {   // src/parameters.h:33-35
+            fprintf(stderr, "post_inflation requires calculate_SIGW = 1\n");
+            return 1;
+        }
+        if (p.post) {   // ensure_post_inflation_directory (src/main.cpp:68-83)
+            if (stat("results/post_inflation", &info) != 0) {
+                if (mkdir("results/post_inflation", 0777) != 0) {
+                    fprintf(stderr, "Failed to create 'results/post_inflation' directory.\n");
+                    return 1;
+                }
+            } else if (!(info.st_mode & S_IFDIR)) {
+                fprintf(stderr, "'results/post_inflation' exists but is not a directory.\n");
+                return 1;
+            }
         }
         log = fopen("results/output.txt", "w");
         if (!log) {
@@ -717,6 +852,7 @@ struct Driver {
         save(1, 0.0);
         inflation();
         if (p.deltaN) deltaN();
+        if (p.post) post_inflation();
         b = bg();
         info_end(0.0, b.t);
         fprintf(log, "InflationEasy program finished\n");

@@ -67,6 +67,8 @@ def test_deltaN_map_bitwise(case, Nend):
     dn_o, dn_g = st.field(2).copy(), run.field(E.IE_FIELD_DELTAN)
     assert np.array_equal(dn_o, dn_g)
     assert np.array_equal(st.field(0), run.field(E.IE_FIELD_F))
+    # fd of finished sites keeps evolving (evolve_derivsN runs on every site)
+    assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
     finished = dn_o < dn_o.max()
     assert 0 < finished.sum()
 
@@ -118,6 +120,8 @@ def test_deltaN_rk4_map_bitwise(case, Nend):
     run.deltaN_rk4_steps(c.dN, nsteps)
     assert np.array_equal(st.field(2), run.field(E.IE_FIELD_DELTAN))
     assert np.array_equal(st.field(0), run.field(E.IE_FIELD_F))
+    # fd of finished sites keeps evolving (evolve_derivsN runs on every site)
+    assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
     assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
     assert run.background()["a"] == st.background()["a"]
 
@@ -155,6 +159,8 @@ def test_deltaN_rk45_matches_oracle(case, Nend):
     assert accepted > 0
     assert np.array_equal(st.field(2), run.field(E.IE_FIELD_DELTAN))
     assert np.array_equal(st.field(0), run.field(E.IE_FIELD_F))
+    # fd of finished sites keeps evolving (evolve_derivsN runs on every site)
+    assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
     assert np.array_equal(st.field(1), run.field(E.IE_FIELD_FD))
     assert run.background()["a"] == st.background()["a"]
 

@@ -59,7 +59,7 @@ TEXT_FILES = ["means", "velocity", "variance", "sf", "energy", "conservation", "
 # integrator (inflation_integrator, deltaN_integrator in params.txt)
 @pytest.mark.parametrize("case", ["hilltop_n16", "tabulated_n16", "tabulated_n16_rk4", "tabulated_n16_rk45"])
 def test_driver_matches_reference_files(case, tmp_path):
-    g = run_driver(case, tmp_path)
+    g = run_driver(case, tmp_path, "post_inflation = 0\n")
     res = tmp_path / "results"
     N = int(g["N"])
     for key in TEXT_FILES:
@@ -105,11 +105,64 @@ def test_driver_split_precision_close_to_fp64(tmp_path):
     b = tmp_path / "split"
     a.mkdir()
     b.mkdir()
-    run_driver("hilltop_n32", a)
-    run_driver("hilltop_n32", b, "precision = split\n")
+    run_driver("hilltop_n32", a, "post_inflation = 0\n")
+    run_driver("hilltop_n32", b, "precision = split\npost_inflation = 0\n")
     s64 = np.stack(read_blocks(a / "results" / "spectra.dat"))
     s32 = np.stack(read_blocks(b / "results" / "spectra.dat"))
     assert np.allclose(s32, s64, rtol=1e-4)
     m64 = np.concatenate(read_blocks(a / "results" / "sf.dat"))
     m32 = np.concatenate(read_blocks(b / "results" / "sf.dat"))
     assert np.allclose(m32, m64, rtol=1e-6)
+
+
+POST_SERIES = ["sf", "spectratimes", "histogramtimes"]
+
+
+def post_blocks(res: Path, g: dict, key: str) -> tuple[np.ndarray, np.ndarray]:
+    ref = g["post_" + key]
+    return blocks(res / "post_inflation" / f"{key}.dat", key, ref), ref
+
+
+# results/post_inflation/*.dat.  From the deltaN map, Phi inherits the map's
+# one-step differences at a few sites, so the Phi statistics get 1e-2 (spectra,
+# variance, tensor spectra 2e-2) and the means an absolute bound of 1e-2 of the
+# rms of Phi; without deltaN (Phi from f) they follow the inflation tolerances.
+@pytest.mark.parametrize("case", ["tabulated_n16_post", "tabulated_n16_post_rk4", "tabulated_n16_post_nodn",
+                                  "hilltop_n16_post"])
+def test_driver_post_inflation_files(case, tmp_path):
+    g = load_golden(case)
+    extra = "post_inflation = 1\n" + ("perform_deltaN = 0\n" if int(g["perform_deltaN"]) == 0 else "")
+    run_driver(case, tmp_path, extra)
+    res = tmp_path / "results"
+    N = int(g["N"])
+    exact = int(g["perform_deltaN"]) == 0
+    for key in POST_SERIES:
+        got, ref = post_blocks(res, g, key)
+        assert got.shape == ref.shape, key
+        cols = slice(None) if key != "histogramtimes" else slice(0, 2)
+        assert np.allclose(got[:, cols], ref[:, cols], rtol=2e-6, atol=0), key
+    var, var_ref = post_blocks(res, g, "variance")
+    assert np.allclose(var[:, :2], var_ref[:, :2], rtol=2e-6, atol=0)
+    assert np.allclose(var[:, 2], var_ref[:, 2], rtol=2e-6 if exact else 1e-2, atol=0)
+    rms = np.sqrt(var_ref[:, 2] / N ** 3)
+    for key in ("means", "velocity"):
+        got, ref = post_blocks(res, g, key)
+        assert np.allclose(got[:, :2], ref[:, :2], rtol=2e-6, atol=0), key
+        if key == "means":
+            bound = 1e-4 * rms if exact else 1e-2 * rms
+            assert np.all(np.abs(got[:, 2] - ref[:, 2]) <= bound + 1e-6 * np.abs(ref[:, 2])), key
+        else:
+            assert np.allclose(got[:, 2], ref[:, 2], rtol=1e-4 if exact else 5e-2, atol=1e-30), key
+    got, ref = post_blocks(res, g, "spectra")
+    assert np.array_equal(got[..., :2], ref[..., :2])
+    live = ref[..., 2] > 0
+    assert np.allclose(got[..., 2][live], ref[..., 2][live], rtol=2e-6 if exact else 1e-2)
+    for key in ("spectraGW", "spectraGWdot"):
+        got, ref = post_blocks(res, g, key)
+        assert np.array_equal(got[..., :2], ref[..., :2])
+        live = ref[..., 2] > 0
+        assert live.sum() > 0
+        assert np.allclose(got[..., 2][live], ref[..., 2][live], rtol=5e-4 if exact else 2e-2), key
+    got, ref = post_blocks(res, g, "histogram")
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= (2.0 / N ** 3 + 1e-9 if exact else 0.02)
