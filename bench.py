@@ -217,6 +217,40 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
             "batched_ms_per_step": 1e3 * tb / steps, "sequential_ms_per_step": 1e3 * ts / steps}
 
 
+def post_inflation(N: int = 256, steps: int = 200, warmup: int = 10, precision: int | None = None) -> dict:
+    """Post-inflation leapfrog (run_post_inflation_loop, src/evolution.cpp:1596-1604)
+    at N^3: FP64 Phi, Phi' and the six FP32 h_ij, h_ij' from a deltaN map of the
+    hilltop initial field's fluctuations (w = 1/3, horizon_factor 1, dt 0.001).
+    One step = the Phi kick (24 B/site) and the tensor kick with the Hessian source
+    plus both drifts (120 B/site); wall time over `steps` device-resident steps."""
+    import numpy as np
+
+    import ie_engine as E
+    cfg = E.make_config(N, precision=E.IE_SPLIT if precision is None else precision, sigw=1, **PRESET)
+    run = E.Run(cfg)
+    run.initialize(SEED, PHI0, DPHI0)
+    run.deltaN_init()
+    f = run.field(E.IE_FIELD_F)
+    dn = 0.5 + 2e-3 * (f - f.mean()) / (np.abs(f - f.mean()).max() + 1e-300)
+    run.upload_field(E.IE_FIELD_DELTAN, dn)
+    del f, dn
+    run.post_init(omega=1.0 / 3.0, horizon_factor=1.0, dt=0.001, af=1e9)
+    run.evolve_fields(0.0005)
+    run.post_leapfrog_steps(warmup)
+    t0 = time.perf_counter()
+    bg = run.post_leapfrog_steps(steps)
+    secs = time.perf_counter() - t0
+    run.close()
+    if bg["steps"] != steps + warmup:
+        raise RuntimeError(f"unexpected post-inflation loop state {bg}")
+    ms = 1e3 * secs / steps
+    alg = 144 * N ** 3
+    return {"N": N, "steps": steps, "ms_per_step": ms, "site_updates_per_s": N ** 3 * steps / secs,
+            "algorithmic_bytes_per_step": alg, "achieved_GBps": alg / (ms * 1e-3) / 1e9,
+            "kernels": "k_post_kick_rows (24 B/site) + k_post_tensor_rows (120 B/site)",
+            "storage": "FP64 Phi, Phi'; FP32 h, hdot"}
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings (run under the profiler)."""
@@ -239,7 +273,8 @@ def main() -> int:
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["split", "fp64", "sigw"], default=None,
+    ap.add_argument("--post-steps", type=int, default=200, help="post-inflation steps timed (0: skip)")
+    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post"], default=None,
                     help="time a single case (for profiling runs)")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -259,6 +294,9 @@ def main() -> int:
         return 3
     N, K, W = args.N, args.steps, args.warmup
     sites = N ** 3
+    if args.only == "post":
+        print(json.dumps({"post_inflation": post_inflation(N, args.post_steps)}), flush=True)
+        return 0
     cases = {"split": (E.IE_SPLIT, 0), "fp64": (E.IE_FP64, 0), "sigw": (E.IE_SPLIT, 1)}
     todo = [args.only] if args.only else ["split", "fp64", "sigw"]
     res = {}
@@ -316,6 +354,9 @@ def main() -> int:
                            else "k_row_kick (kick + drift)", "kernel_ms": ms,
                            "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
                            "step_ms_events": prof["step_ms"]}
+    if args.post_steps > 0 and not args.only:
+        out["post_inflation"] = post_inflation(N, args.post_steps)
+        print(f"# post-inflation: {out['post_inflation']}", file=sys.stderr, flush=True)
     if args.ensemble_runs > 0 and not args.only:
         out["ensemble"] = ensemble(args.ensemble_runs)
         print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)

@@ -438,22 +438,12 @@ __global__ void k_phiref_final(const double *__restrict__ f, const double *__res
 
 // saveN's mean (src/output.cpp:1527-1537): a float accumulator walked in lattice
 // order over sites with deltaN <= cutoff, then divided by N^3 in float.  The
-// serial order is part of the result, so one wave streams the map through LDS
-// and lane 0 accumulates in order.
+// serial order is part of the result: one wave adds in order (serial_walk).
 __global__ __launch_bounds__(64) void k_float_mean(const double *__restrict__ dn, size_t gs, double cutoff,
                                                     float *out)
 {
-    __shared__ double chunk[1024];
-    float acc = 0.f;
-    for (size_t base = 0; base < gs; base += 1024) {
-        const size_t len = gs - base < 1024 ? gs - base : 1024;
-        for (size_t e = threadIdx.x; e < len; e += 64) chunk[e] = dn[base + e];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (size_t e = 0; e < len; ++e)
-                if (chunk[e] <= cutoff) acc = (float)((double)acc + chunk[e]);
-        __syncthreads();
-    }
+    auto step = [cutoff](float acc, double v) { return v <= cutoff ? (float)((double)acc + v) : acc; };
+    const float acc = serial_walk(dn, gs, 0.f, step);
     if (threadIdx.x == 0) *out = acc / (float)gs;
 }
 

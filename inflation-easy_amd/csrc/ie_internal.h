@@ -374,6 +374,46 @@ __device__ __forceinline__ double first_lane(double x)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The value of lane l of the wave (l a compile-time constant after unrolling),
+// in every lane.
+__device__ __forceinline__ double lane_value(double x, int l)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Serial sums in lattice order on one wave: the lanes load 8 x 64 consecutive
+// values (the next 512 in flight while these are added) and the running sum,
+// kept the same in every lane, takes them lane by lane: the dependent adds are
+// the only serial part.  step(acc, v) applies one element.
+constexpr int kSerialBatch = 8;
+template <typename Acc, typename Step>
+__device__ __forceinline__ Acc serial_walk(const double *__restrict__ g, size_t gs, Acc acc, Step step)
+{
+    const unsigned lane = threadIdx.x & 63;
+    const size_t chunk = 64 * kSerialBatch, full = gs / chunk * chunk;
+    double a[kSerialBatch];
+    if (full)
+#pragma unroll
+        for (int q = 0; q < kSerialBatch; ++q) a[q] = g[q * 64 + lane];
+    for (size_t base = 0; base < full; base += chunk) {
+        const size_t next = base + chunk < full ? base + chunk : base;
+        double na[kSerialBatch];
+#pragma unroll
+        for (int q = 0; q < kSerialBatch; ++q) na[q] = g[next + q * 64 + lane];
+#pragma unroll
+        for (int q = 0; q < kSerialBatch; ++q)
+#pragma unroll
+            for (int l = 0; l < 64; ++l) acc = step(acc, lane_value(a[q], l));
+#pragma unroll
+        for (int q = 0; q < kSerialBatch; ++q) a[q] = na[q];
+    }
+    for (size_t id = full; id < gs; ++id) acc = step(acc, g[id]);
+    return acc;
+}
+
 template <typename T, int E>
 __device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restrict__ row, const RowWalk &w,
                                              unsigned N, unsigned lane, T &kp_last, T &km_first)

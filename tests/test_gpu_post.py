@@ -84,8 +84,10 @@ def start_post(N, precision, from_deltaN=1):
     return st, run
 
 
+# N = 16, 32: site kernels; 64: rows of one segment, 1 site per lane; 128: 2 per
+# lane; 256: 2 per lane over two segments per row
 @pytest.mark.parametrize("N,precision", [(16, E.IE_FP64), (32, E.IE_FP64), (64, E.IE_FP64), (128, E.IE_FP64),
-                                         (16, E.IE_SPLIT), (128, E.IE_SPLIT)])
+                                         (256, E.IE_FP64), (16, E.IE_SPLIT), (128, E.IE_SPLIT)])
 def test_post_leapfrog_bitwise(N, precision):
     st, run = start_post(N, precision)
     assert_same_state(st, run, "set-up")
