@@ -210,6 +210,26 @@ ie_status ie_spectrum(ie_run *run, int which, double norm1, double *p, int *coun
  * numbins = int(sqrt(3)*(N/2))+1; bins >= floor(0.8 numbins) are empty. */
 ie_status ie_gw_spectrum(ie_run *run, int hdot, double *p, int *counts, double *vals);
 
+/* The forcing cutoff of spectraf / spectraf_post_inflation (src/output.cpp:333-404,
+ * 1698-1765): the modes of f and fd with |n| > high or |n| < low (n the integer
+ * lattice momentum) are removed and both fields are transformed back to real space
+ * (the reference applies it after each spectraf when forcing_cutoff is set and
+ * high_cutoff_index > 0).  A low cutoff above 0 removes k = 0, the field mean. */
+ie_status ie_forcing_cutoff(ie_run *run, double high, double low);
+/* bispectraf (src/output.cpp:740-911): the equilateral bispectrum of f (its mean
+ * removed) on numbins = int(sqrt(3)*(N/2))+1 shells: p, counts, re, im as
+ * bispectra.dat's columns (norm1 = (L/B)^6 / N^9 applied).  f is not modified. */
+ie_status ie_bispectrum(ie_run *run, double *p, int *counts, double *re, double *im);
+/* build_deltaN_log and histogramsLOG (src/output.cpp:167-198, 1312-1403): the
+ * log-mapped zeta (1/eta) log(1 - eta (f - <f>) / <fd a^{s-1}/(ad a^{s-2})>) of
+ * the inflation fields into the deltaN buffer (sites with a non-positive
+ * argument: 0), and its histogram over the sites with a positive argument:
+ * freq[nbins] = counts / points, *bmin and *df as histogramtimesLOG.dat writes
+ * them.  *valid = 0 (map and freq 0, *bmin 0, *df 1) when the mean velocity
+ * factor is not finite or below 1e-30.  ie_spectrum(IE_FIELD_DELTAN) then gives
+ * spectraLOG. */
+ie_status ie_deltaN_log(ie_run *run, double eta_log, int nbins, double *bmin, double *df, double *freq, int *valid);
+
 /* --- curvature and tensor power spectra ------------------------------------ */
 /* The analysis notebook's formulas (notebooks/plot.ipynb cells 9, 23) on binned
  * spectra, each row paired with the modes.dat row of the same index (k[b], from

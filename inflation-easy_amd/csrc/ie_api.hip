@@ -865,6 +865,123 @@ ie_status ie_gw_spectrum(ie_run *r, int hdot, double *p, int *counts, double *va
     return IE_OK;
 }
 
+ie_status ie_forcing_cutoff(ie_run *r, double high, double low)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int N = r->N;
+    DevBg &b = *r->bg_host;
+    const int par = b.parity;
+    if (r->in_deltaN) {
+        // FP64 state of the deltaN and post-inflation phases
+        double *f, *fd;
+        if ((st = dn_state(r, &f, &fd)) != IE_OK) return st;
+        IE_HIP(r, fft_forward_double(r->fft, f, 0, r->stream));
+        IE_HIP(r, fft_forward_double(r->fft, fd, 1, r->stream));
+        for (int q = 0; q < 2; ++q) IE_HIP(r, fft_kill_modes(r->fft, q, high, low, r->stream));
+        IE_HIP(r, fft_inverse_slot(r->fft, 0, 0.0, f, r->stream));
+        IE_HIP(r, fft_inverse_slot(r->fft, 1, 0.0, fd, r->stream));
+        return sync(r);
+    }
+    if (r->cfg.precision == IE_FP64) {
+        double *f = (double *)r->F.f[par], *fd = (double *)r->F.fd[par];
+        IE_HIP(r, fft_forward_double(r->fft, f, 0, r->stream));
+        IE_HIP(r, fft_forward_double(r->fft, fd, 1, r->stream));
+        for (int q = 0; q < 2; ++q) IE_HIP(r, fft_kill_modes(r->fft, q, high, low, r->stream));
+        IE_HIP(r, fft_inverse_slot(r->fft, 0, 0.0, f, r->stream));
+        IE_HIP(r, fft_inverse_slot(r->fft, 1, 0.0, fd, r->stream));
+        return sync(r);
+    }
+    // split storage: the fluctuations carry every mode but k = 0's homogeneous
+    // part, which goes with the FP64 mean when the low cutoff removes k = 0
+    float *f = (float *)r->F.f[par], *fd = (float *)r->F.fd[par];
+    IE_HIP(r, fft_forward_float(r->fft, f, 0.0, 0, r->stream));
+    IE_HIP(r, fft_forward_float(r->fft, fd, 0.0, 1, r->stream));
+    for (int q = 0; q < 2; ++q) IE_HIP(r, fft_kill_modes(r->fft, q, high, low, r->stream));
+    IE_HIP(r, fft_inverse_slot(r->fft, 0, 0.0, r->scratch, r->stream));
+    IE_HIP(r, store_float(r->scratch, f, r->gs, r->stream));
+    IE_HIP(r, fft_inverse_slot(r->fft, 1, 0.0, r->scratch, r->stream));
+    IE_HIP(r, store_float(r->scratch, fd, r->gs, r->stream));
+    if (0.0 > high || 0.0 < low) {
+        b.phibar = 0.0;
+        b.pibar = 0.0;
+        return push_bg(r);
+    }
+    (void)N;
+    return sync(r);
+}
+
+ie_status ie_bispectrum(ie_run *r, double *p, int *counts, double *re, double *im)
+{
+    if (!r || !p || !counts || !re || !im) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int par = r->bg_host->parity;
+    if (r->in_deltaN) {
+        double *f, *fd;
+        if ((st = dn_state(r, &f, &fd)) != IE_OK) return st;
+        IE_HIP(r, fft_forward_double(r->fft, f, 0, r->stream));
+    } else if (r->cfg.precision == IE_SPLIT) {
+        IE_HIP(r, fft_forward_float(r->fft, (const float *)r->F.f[par], 0.0, 0, r->stream));
+    } else {
+        IE_HIP(r, fft_forward_double(r->fft, (const double *)r->F.f[par], 0, r->stream));
+    }
+    // the reference transforms f minus its mean (src/output.cpp:766-771)
+    IE_HIP(r, fft_zero_dc(r->fft, 0, r->stream));
+    const int nb = (int)(sqrt(3.0) * (r->N / 2)) + 1;
+    std::vector<double> sums((size_t)nb * 3);
+    IE_HIP(r, bispectrum_sums(r->fft, 0, r->N, sums.data(), r->stream));
+    const double pi = 2.0 * asin(1.0), dp = 2.0 * pi / r->cfg.L;
+    const double norm1 = pow(r->cfg.L / r->cfg.rescale_B, 6) / pow((double)r->N, 9);
+    for (int k = 0; k < nb; ++k) {
+        p[k] = dp * k;
+        counts[k] = (int)sums[3 * k + 2];
+        double a = sums[3 * k], c = sums[3 * k + 1];
+        if (counts[k] > 0) {
+            a = a / counts[k];
+            c = c / counts[k];
+        }
+        re[k] = norm1 * a;
+        im[k] = norm1 * c;
+    }
+    return IE_OK;
+}
+
+ie_status ie_deltaN_log(ie_run *r, double eta_log, int nbins, double *bmin, double *df, double *freq, int *valid)
+{
+    if (!r || !bmin || !df || !freq || !valid || nbins < 1 || !(eta_log != 0.0))
+        return fail(r, IE_ERR_ARG, "bad argument");
+    if (r->in_deltaN) return fail(r, IE_ERR_STATE, "the LOG map is formed from the inflation fields");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const DevBg &b = *r->bg_host;
+    const int par = b.parity, split = r->cfg.precision == IE_SPLIT;
+    if (r->hist_bins < nbins) {
+        if (r->hist) (void)hipFree(r->hist);
+        IE_HIP(r, hipMalloc((void **)&r->hist, (size_t)nbins * 8));
+        r->hist_bins = nbins;
+    }
+    LogMapResult res{};
+    IE_HIP(r, log_map(r->F.f[par], r->F.fd[par], split, r->gs, split ? b.phibar : 0.0, split ? b.pibar : 0.0, b.a,
+                      b.ad, r->cfg.rescale_s, eta_log, nbins, r->F.dn, r->work, r->hist, &res, r->stream));
+    *valid = res.valid;
+    *bmin = res.bmin;
+    *df = res.df;
+    if (!res.valid) {
+        for (int q = 0; q < nbins; ++q) freq[q] = 0.0;
+        return IE_OK;
+    }
+    std::vector<unsigned long long> c(nbins);
+    IE_HIP(r, hipMemcpyAsync(c.data(), r->hist, (size_t)nbins * 8, hipMemcpyDeviceToHost, r->stream));
+    if ((st = sync(r)) != IE_OK) return st;
+    double numpts = 0;
+    for (int q = 0; q < nbins; ++q) numpts += (double)c[q];
+    if (numpts == 0) numpts = 1;
+    for (int q = 0; q < nbins; ++q) freq[q] = (double)c[q] / numpts;
+    return IE_OK;
+}
+
 ie_status ie_upload_modes(ie_run *r, const double *fk, const double *fny, const double *fdk, const double *fdny,
                           double phi0, double dphi0)
 {

@@ -36,6 +36,17 @@ hipError_t fft_bin_tt(FftPlan *p, double dx, double to_phys, double *sums_host, 
 hipError_t fft_inverse_ffteasy(FftPlan *p, const double *fk_host, const double *fny_host, double add,
                                double *out_dev, hipStream_t s);
 
+// The half spectrum of a slot (H[(i N + j) K + kz], K = N/2 + 1), for kernels of
+// other units that read modes.
+const double2 *fft_slot(const FftPlan *p, int slot);
+// Zero the modes of a slot with |n| > hi or |n| < lo (the forcing cutoff).
+hipError_t fft_kill_modes(FftPlan *p, int slot, double hi, double lo, hipStream_t s);
+// Zero the k = 0 mode of a slot (the transform of a mean-subtracted field).
+hipError_t fft_zero_dc(FftPlan *p, int slot, hipStream_t s);
+// Inverse of a slot's half spectrum (1/N^3, the round trip is the identity) into
+// out[] = result + add; the slot's contents are destroyed.
+hipError_t fft_inverse_slot(FftPlan *p, int slot, double add, double *out, hipStream_t s);
+
 // Reference mode counts per shell (closed form, host).
 void fft_counts(int N, int *counts);
 void fft_gw_counts(int N, int *counts);

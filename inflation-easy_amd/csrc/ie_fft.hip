@@ -241,6 +241,22 @@ __global__ __launch_bounds__(kThreads) void k_bin_tt(SixSpectra S, int N, int nb
     }
 }
 
+// The k-space filter of spectraf's forcing cutoff (src/output.cpp:333-404):
+// modes with |n| > hi or |n| < lo set to zero.  The reference zeroes each mode of
+// the kz = 0 and kz = N/2 planes together with its Hermitian mirror, which is the
+// same set: |n| does not change under n -> -n.
+__global__ __launch_bounds__(kThreads) void k_kill_modes(double2 *__restrict__ H, int N, double hi, double lo)
+{
+    const int K = N / 2 + 1;
+    const size_t modes = (size_t)N * N * K;
+    for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < modes; m += (size_t)gridDim.x * blockDim.x) {
+        int px, py, kz;
+        mode_coords(m, N, K, &px, &py, &kz);
+        const double pdisc = sqrt((double)px * px + (double)py * py + (double)kz * kz);
+        if (pdisc > hi || pdisc < lo) H[m] = make_double2(0.0, 0.0);
+    }
+}
+
 // Fixed-order sum of the per-workgroup shell partials.
 __global__ void k_sum_bins(const double *__restrict__ partials, int nblocks, int nb, double *__restrict__ out)
 {
@@ -441,6 +457,34 @@ hipError_t fft_inverse_ffteasy(FftPlan *p, const double *fk, const double *fny, 
                                                                              scale, add);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipStreamSynchronize(s);
+}
+
+const double2 *fft_slot(const FftPlan *p, int slot) { return slot >= 0 && slot < 6 ? p->H[slot] : nullptr; }
+
+hipError_t fft_kill_modes(FftPlan *p, int slot, double hi, double lo, hipStream_t s)
+{
+    if (!p->H[slot]) return hipErrorInvalidValue;
+    k_kill_modes<<<bin_blocks_for(p->modes) * 8, kThreads, 0, s>>>(p->H[slot], p->N, hi, lo);
+    return hipGetLastError();
+}
+
+hipError_t fft_zero_dc(FftPlan *p, int slot, hipStream_t s)
+{
+    if (!p->H[slot]) return hipErrorInvalidValue;
+    return hipMemsetAsync(p->H[slot], 0, sizeof(double2), s);
+}
+
+hipError_t fft_inverse_slot(FftPlan *p, int slot, double add, double *out, hipStream_t s)
+{
+    if (!p->H[slot]) return hipErrorInvalidValue;
+    const int N = p->N;
+    hipError_t e = columns(p, p->H[slot], -1, s);
+    if (e != hipSuccess) return e;
+    const int L = rows_per_block(N);
+    const double scale = 1.0 / ((double)N * N * N);
+    k_fft_z_c2r<<<N * N / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(p->H[slot], out, N, p->logN, L, p->tw,
+                                                                             scale, add);
+    return hipGetLastError();
 }
 
 void fft_counts(int N, int *counts)

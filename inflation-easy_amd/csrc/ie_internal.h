@@ -496,6 +496,23 @@ hipError_t post_phi(double *f, const double *src, size_t gs, double P, double me
 // {sum f, sum fd, sum f^2} in fixed order (work: >= 3072 doubles)
 hipError_t post_sums3(const double *f, const double *fd, size_t gs, double *work, double *out, hipStream_t s);
 
+// Diagnostics (ie_diag.hip).  Bispectrum shell sums {re, im, count} x numbins
+// of the half spectrum in a plan slot (the caller zeroes its k = 0 mode).
+struct FftPlan;
+hipError_t bispectrum_sums(const FftPlan *plan, int slot, int N, double *out_host, hipStream_t s);
+// LOG map of f into dn with its masked histogram counts (counts_dev: nbins
+// uint64, device); f, fd are float fluctuations about fbar, pbar (split) or
+// doubles.  work: >= 4096 doubles.
+struct LogMapResult {
+    double factt, fmean, bmin, df;
+    int valid;
+};
+hipError_t log_map(const void *f, const void *fd, int split, size_t gs, double fbar, double pbar, double a,
+                   double ad, double s_rescale, double eta, int nbins, double *dn, double *work,
+                   unsigned long long *counts_dev, LogMapResult *res, hipStream_t s);
+// float store of a double field (split storage after a k-space filter)
+hipError_t store_float(const double *src, float *dst, size_t gs, hipStream_t s);
+
 // RK4 / RK45 inflation steps on FP64 storage (ie_rk.hip).  Stage buffers are
 // allocated once per run (stages x (2 N^3 doubles + 12 N^3 floats with tensors)).
 struct RkWork;

@@ -404,6 +404,7 @@ struct Driver {
                 FILE *st = out.get("spectratimes");
                 fprintf(st, "%f %e\n", b.t, b.a);
                 fflush(st);
+                forcing_cutoff();
                 if (p.sigw) {
                     const int ng = (int)(sqrt(3.0) * (N / 2)) + 1;
                     std::vector<double> pg(ng), vg(ng);
@@ -417,6 +418,13 @@ struct Driver {
             if (p.output_histogram) histogram(IE_FIELD_F, "histogram", "histogramtimes", b, false);
         }
         if (half != 0.0) CHECK(run, ie_evolve_fields(run, half));
+    }
+
+    // the optional k-space cutoff at the end of spectraf (src/output.cpp:333-404)
+    void forcing_cutoff()
+    {
+        if (p.high_cutoff_index > 0 && p.forcing_cutoff)
+            CHECK(run, ie_forcing_cutoff(run, p.high_cutoff_index, p.low_cutoff_index));
     }
 
     void energy(const ie_background &b)
@@ -545,6 +553,36 @@ struct Driver {
         for (double x : md) fprintf(fp, "%e\n", x);
         fprintf(fp, "\n");
         fflush(fp);
+        if (p.output_bispectrum) {   // bispectraf (src/output.cpp:740-911)
+            std::vector<double> bp(nb), re(nb), im(nb);
+            std::vector<int> bc(nb);
+            CHECK(run, ie_bispectrum(run, bp.data(), bc.data(), re.data(), im.data()));
+            FILE *bf = out.get("bispectra");
+            for (int k = 0; k < nb; ++k) fprintf(bf, "%e %d %e %e\n", bp[k], bc[k], re[k], im[k]);
+            fprintf(bf, "\n");
+            fflush(bf);
+        }
+        if (p.deltaN && p.output_LOG) {   // spectraLOG, histogramsLOG (src/output.cpp:1280-1403)
+            const int nbins = p.nbins > 0 ? p.nbins : 1;
+            std::vector<double> freq(nbins);
+            double bmin = 0.0, df = 1.0;
+            int valid = 0;
+            CHECK(run, ie_deltaN_log(run, p.eta_log, nbins, &bmin, &df, freq.data(), &valid));
+            const int ns = ie_spectrum_numbins(N);
+            std::vector<double> pp(ns), vv(ns);
+            std::vector<int> cc(ns);
+            const double norm1 = pow(p.L / p.rescale_B, 3) / pow((double)N, 6);
+            CHECK(run, ie_spectrum(run, IE_FIELD_DELTAN, norm1, pp.data(), cc.data(), vv.data()));
+            spectra_block(out.get("spectraLOG"), pp, cc, vv);
+            ie_background b = bg();
+            FILE *ht = out.get("histogramtimesLOG"), *h = out.get("histogramLOG");
+            fprintf(ht, "%f %e", b.t, b.a);
+            for (int q = 0; q < p.nbins; ++q) fprintf(h, "%e\n", freq[q]);
+            fprintf(h, "\n");
+            fflush(h);
+            fprintf(ht, " %e %e\n", bmin, df);
+            fflush(ht);
+        }
     }
 
     // run_evolution_loop, leapfrog branch (src/evolution.cpp:909-946,990-997)
@@ -693,6 +731,7 @@ struct Driver {
                 FILE *st = out.get("post_inflation/spectratimes");
                 fprintf(st, "%f %e\n", b.t, b.a);
                 fflush(st);
+                forcing_cutoff();
                 const int ng = (int)(sqrt(3.0) * (N / 2)) + 1;
                 std::vector<double> pg(ng), vg(ng);
                 std::vector<int> cg(ng);

@@ -147,6 +147,9 @@ def lib() -> C.CDLL:
             "ie_download_field": (st, [P, C.c_int, dp]),
             "ie_upload_field": (st, [P, C.c_int, dp]),
             "ie_post_init": (st, [P, C.POINTER(PostParams), C.POINTER(Background)]),
+            "ie_forcing_cutoff": (st, [P, C.c_double, C.c_double]),
+            "ie_bispectrum": (st, [P, dp, ip, dp, dp]),
+            "ie_deltaN_log": (st, [P, C.c_double, C.c_int, dp, dp, dp, ip]),
             "ie_post_evolve_derivs": (st, [P, C.c_double]),
             "ie_post_leapfrog_steps": (st, [P, C.c_int, C.POINTER(Background)]),
             "ie_upload_tensor": (st, [P, C.c_int, fp]),
@@ -443,6 +446,24 @@ class Run:
         src = np.ascontiguousarray(src, dtype=np.float64)
         assert src.size == self.gs
         self._check(lib().ie_upload_field(self.h, which, _dp(src)))
+
+    # ---- diagnostics off by default in the reference
+    def forcing_cutoff(self, high: float, low: float = 0.0):
+        self._check(lib().ie_forcing_cutoff(self.h, high, low))
+
+    def bispectrum(self):
+        nb = int(np.sqrt(3.0) * (self.N // 2)) + 1
+        p, re, im = np.zeros(nb), np.zeros(nb), np.zeros(nb)
+        c = np.zeros(nb, dtype=np.int32)
+        self._check(lib().ie_bispectrum(self.h, _dp(p), _ip(c), _dp(re), _dp(im)))
+        return p, c, re, im
+
+    def deltaN_log(self, eta: float, nbins: int):
+        """LOG map into the deltaN buffer; returns (valid, bmin, df, freq)."""
+        freq = np.zeros(nbins)
+        bmin, df, valid = C.c_double(), C.c_double(), C.c_int()
+        self._check(lib().ie_deltaN_log(self.h, eta, nbins, C.byref(bmin), C.byref(df), _dp(freq), C.byref(valid)))
+        return bool(valid.value), bmin.value, df.value, freq
 
     # ---- post-inflation
     def post_init(self, omega: float = 1.0 / 3.0, horizon_factor: float = 1.0, dt: float = 0.001,

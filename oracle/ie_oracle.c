@@ -131,6 +131,9 @@ void ieo_config_defaults(ieo_config *c, int N, int numerical)
     c->dt_post = 0.001;
     c->af_post = 2.0 * N;
     c->post_integrator = IEO_LEAPFROG;
+    c->output_bispectrum = 0;
+    c->output_LOG = 0;
+    c->eta_log = -0.5;
 }
 
 /* src/runtime_parameters.cpp:342-353 and :184-192 */
@@ -1119,7 +1122,247 @@ void ieo_spectrum_f(ieo_state *st, double *p, int *counts, double *vals)
 {
     ieo_fftrn_d(st->f, st->fny, st->N, 1);
     ieo_bin_spectrum(st->N, st->cfg.L, st->f, st->fny, norm_scalar(st), p, counts, vals);
+    if (st->cfg.high_cutoff_index > 0 && st->cfg.forcing_cutoff) ieo_cutoff_modes(st);
     ieo_fftrn_d(st->f, st->fny, st->N, -1);
+}
+
+static void kill_mode(double *field, double *deriv)
+{
+    field[0] = 0.;
+    field[1] = 0.;
+    deriv[0] = 0.;
+    deriv[1] = 0.;
+}
+
+void ieo_cutoff_modes(ieo_state *st)
+{
+    const int N = st->N;
+    const double hi = st->cfg.high_cutoff_index, lo = st->cfg.low_cutoff_index;
+    double *f = st->f, *fd = st->fd, *fny = st->fny, *fdny = st->fdny;
+    ieo_fftrn_d(fd, fdny, N, 1);
+#define NY(a, i, jj) (a)[(size_t)(i) * 2 * N + (jj)]
+    for (int i = 0; i < N; i++) {
+        const int px = i <= N / 2 ? i : i - N;
+        const int iconj = i == 0 ? 0 : N - i;
+        for (int j = 0; j < N; j++) {
+            const int py = j <= N / 2 ? j : j - N;
+            for (int k = 1; k < N / 2; k++) {
+                const double pdisc = sqrt(sq((double)px) + sq((double)py) + sq((double)k));
+                if (pdisc > hi || pdisc < lo) kill_mode(&f[IX(N, i, j, 2 * k)], &fd[IX(N, i, j, 2 * k)]);
+            }
+            if (j > N / 2 || (i > N / 2 && (j == 0 || j == N / 2))) {
+                const int jconj = j == 0 ? 0 : N - j;
+                double pdisc = sqrt(sq((double)px) + sq((double)py));
+                if (pdisc > hi || pdisc < lo) {
+                    kill_mode(&f[IX(N, i, j, 0)], &fd[IX(N, i, j, 0)]);
+                    f[IX(N, iconj, jconj, 0)] = f[IX(N, i, j, 0)];
+                    f[IX(N, iconj, jconj, 1)] = -f[IX(N, i, j, 1)];
+                    fd[IX(N, iconj, jconj, 0)] = fd[IX(N, i, j, 0)];
+                    fd[IX(N, iconj, jconj, 1)] = -fd[IX(N, i, j, 1)];
+                }
+                pdisc = sqrt(sq((double)px) + sq((double)py) + sq((double)(N / 2.)));
+                if (pdisc > hi || pdisc < lo) {
+                    kill_mode(&NY(fny, i, 2 * j), &NY(fdny, i, 2 * j));
+                    NY(fny, iconj, 2 * jconj) = NY(fny, i, 2 * j);
+                    NY(fny, iconj, 2 * jconj + 1) = -NY(fny, i, 2 * j + 1);
+                    NY(fdny, iconj, 2 * jconj) = NY(fdny, i, 2 * j);
+                    NY(fdny, iconj, 2 * jconj + 1) = -NY(fdny, i, 2 * j + 1);
+                }
+            } else if ((i == 0 || i == N / 2) && (j == 0 || j == N / 2)) {
+                double pdisc = sqrt(sq((double)px) + sq((double)py));
+                if (pdisc > hi || pdisc < lo) kill_mode(&f[IX(N, i, j, 0)], &fd[IX(N, i, j, 0)]);
+                pdisc = sqrt(sq((double)px) + sq((double)py) + sq((double)(N / 2.)));
+                if (pdisc > hi || pdisc < lo) kill_mode(&NY(fny, i, 2 * j), &NY(fdny, i, 2 * j));
+            }
+        }
+    }
+#undef NY
+    ieo_fftrn_d(fd, fdny, N, -1);
+}
+
+/* One FFTEASY-layout mode of f: k < N/2 from the N^3 array, k = N/2 from the
+ * Nyquist plane (the reference reads f at 2k first and then replaces it; the
+ * replaced read is skipped here, it can fall past the array). */
+static void bis_mode(const ieo_state *st, int i, int j, int k, double *re, double *im)
+{
+    const int N = st->N;
+    if (k == N / 2) {
+        *re = st->fny[(size_t)i * 2 * N + 2 * j];
+        *im = st->fny[(size_t)i * 2 * N + 2 * j + 1];
+    } else {
+        *re = st->f[IX(N, i, j, 2 * k)];
+        *im = st->f[IX(N, i, j, 2 * k + 1)];
+    }
+}
+
+void ieo_bispectrum(ieo_state *st, double *p, int *numpoints, double *bisreal, double *bisimag)
+{
+    const int N = st->N;
+    const int numbins = (int)(sqrt(3.0) * (N / 2)) + 1;
+    const double dp = 2.0 * st->pi / st->cfg.L;
+    const double norm1 = pow(st->cfg.L / st->cfg.rescale_B, 6) / pow((double)N, 9);
+    double mean = 0.;
+    for (size_t id = 0; id < st->gs; ++id) mean += st->f[id];
+    mean = mean / (double)st->gs;
+    for (size_t id = 0; id < st->gs; ++id) st->f[id] -= mean;
+    for (int k = 0; k < numbins; k++) {
+        p[k] = dp * k;
+        numpoints[k] = 0;
+        bisreal[k] = 0.;
+        bisimag[k] = 0.;
+    }
+    ieo_fftrn_d(st->f, st->fny, N, 1);
+    for (int k = 0; k < numbins; k++) {
+        const double kf = (double)k;
+        for (int i1 = 0; i1 < N; i1++)
+            for (int j1 = 0; j1 < N; j1++) {
+                const double px1 = i1 <= N / 2 ? i1 : i1 - N;
+                const double py1 = j1 <= N / 2 ? j1 : j1 - N;
+                const double pzaus1 = sq(kf) - sq(px1) - sq(py1);
+                if (!(pzaus1 >= 0)) continue;
+                for (int k1 = (int)round(sqrt(pzaus1)) - 1; k1 < (int)round(sqrt(pzaus1)) + 2; k1++) {
+                    if (!(fabs(sqrt(sq(px1) + sq(py1) + sq((double)k1)) - kf) < 1.5 && k1 <= N / 2 && k1 >= 0))
+                        continue;
+                    for (int i2 = 0; i2 < N; i2++)
+                        for (int j2 = 0; j2 < N; j2++) {
+                            const double px2 = i2 <= N / 2 ? i2 : i2 - N;
+                            const double py2 = j2 <= N / 2 ? j2 : j2 - N;
+                            const double pzaus2 = sq(kf) - sq(px2) - sq(py2);
+                            if (!(pzaus2 >= 0)) continue;
+                            for (int k2 = (int)round(sqrt(pzaus2)) - 1; k2 < (int)round(sqrt(pzaus2)) + 2; k2++) {
+                                if (!(fabs(sqrt(sq(px2) + sq(py2) + sq((double)k2)) - kf) < 1.5 && k2 <= N / 2 &&
+                                      k2 >= 0))
+                                    continue;
+                                const double px3 = px1 + px2, py3 = py1 + py2;
+                                int k3 = k1 + k2;
+                                if (!(px3 <= N / 2 && px3 > -N / 2 && py3 <= N / 2 && py3 > -N / 2)) continue;
+                                const int i3 = (int)(px3 >= 0 ? px3 : px3 + N);
+                                const int j3 = (int)(py3 >= 0 ? py3 : py3 + N);
+                                double f1r, f1i, f2r, f2i, f3r, f3i;
+                                if (fabs(sqrt(sq(px3) + sq(py3) + sq((double)k3)) - kf) < 1.5 && k3 <= N / 2) {
+                                    bis_mode(st, i1, j1, k1, &f1r, &f1i);
+                                    bis_mode(st, i2, j2, k2, &f2r, &f2i);
+                                    bis_mode(st, i3, j3, k3, &f3r, &f3i);
+                                    double counts = 1.;
+                                    if (k1 != N / 2 && k2 != N / 2 && (k1 != 0 || k2 != 0)) counts = 2.;
+                                    numpoints[k] += (int)counts;
+                                    bisreal[k] += counts * (f1r * f2r * f3r - f1i * f2i * f3r + f1i * f2r * f3i
+                                                            + f2i * f1r * f3i);
+                                    bisimag[k] += counts * (-f1r * f2r * f3i + f1i * f2i * f3i + f1i * f2r * f3r
+                                                            + f2i * f1r * f3r);
+                                }
+                                if (k1 != 0 && k2 != 0) {
+                                    k3 = k1 - k2;
+                                    if (k3 >= 0 && fabs(sqrt(sq(px3) + sq(py3) + sq((double)k3)) - kf) < 1.5 &&
+                                        k3 <= N / 2) {
+                                        const int i2n = (int)(-px2 >= 0 ? -px2 : -px2 + N);
+                                        const int j2n = (int)(-py2 >= 0 ? -py2 : -py2 + N);
+                                        bis_mode(st, i1, j1, k1, &f1r, &f1i);
+                                        bis_mode(st, i2n, j2n, k2, &f2r, &f2i);
+                                        f2i = -f2i;
+                                        bis_mode(st, i3, j3, k3, &f3r, &f3i);
+                                        numpoints[k] += 2;
+                                        bisreal[k] += 2.0 * (f1r * f2r * f3r - f1i * f2i * f3r + f1i * f2r * f3i
+                                                             + f2i * f1r * f3i);
+                                        bisimag[k] += 2.0 * (-f1r * f2r * f3i + f1i * f2i * f3i + f1i * f2r * f3r
+                                                             + f2i * f1r * f3r);
+                                    }
+                                }
+                            }
+                        }
+                }
+            }
+    }
+    for (int k = 0; k < numbins; k++) {
+        if (numpoints[k] > 0) {
+            bisreal[k] = bisreal[k] / numpoints[k];
+            bisimag[k] = bisimag[k] / numpoints[k];
+        }
+        bisreal[k] = norm1 * bisreal[k];
+        bisimag[k] = norm1 * bisimag[k];
+    }
+    ieo_fftrn_d(st->f, st->fny, N, -1);
+    for (size_t id = 0; id < st->gs; ++id) st->f[id] += mean;
+}
+
+/* factt = <fd a^{s-1} / (ad a^{s-2})>, fmean = <f> as the LOG outputs form them */
+static int log_factors(const ieo_state *st, double *factt, double *fmean)
+{
+    const double s = st->cfg.rescale_s;
+    double ft = 0.0, fm = 0.0;
+    for (size_t id = 0; id < st->gs; ++id) {
+        ft += st->fd[id] * pow(st->a, s - 1.0) / (st->ad * pow(st->a, s - 2.0));
+        fm += st->f[id];
+    }
+    *factt = ft / (double)st->gs;
+    *fmean = fm / (double)st->gs;
+    return !(!isfinite(*factt) || fabs(*factt) < 1e-30);
+}
+
+int ieo_deltaN_log(ieo_state *st)
+{
+    double factt, fmean;
+    const double eta = st->cfg.eta_log;
+    if (!log_factors(st, &factt, &fmean)) {
+        for (size_t id = 0; id < st->gs; ++id) st->dn[id] = 0.0;
+        return 0;
+    }
+    for (size_t id = 0; id < st->gs; ++id) {
+        st->dn[id] = 0.0;
+        const double arg = 1.0 - eta * (st->f[id] - fmean) / factt;
+        if (arg > 0.0) st->dn[id] = (1.0 / eta) * log(arg);
+    }
+    return 1;
+}
+
+void ieo_spectrum_log(ieo_state *st, double *p, int *counts, double *vals)
+{
+    ieo_deltaN_log(st);
+    ieo_fftrn_d(st->dn, st->fny, st->N, 1);
+    ieo_bin_spectrum(st->N, st->cfg.L, st->dn, st->fny, norm_scalar(st), p, counts, vals);
+    ieo_fftrn_d(st->dn, st->fny, st->N, -1);
+}
+
+int ieo_histogram_log(ieo_state *st, double *bmin_out, double *df_out, double *freq)
+{
+    const int nbins = st->cfg.nbins;
+    const double eta = st->cfg.eta_log;
+    double factt, fmean;
+    if (!log_factors(st, &factt, &fmean)) {
+        for (int b = 0; b < nbins; ++b) freq[b] = 0.0;
+        *bmin_out = 0.0;
+        *df_out = 1.0;
+        return 0;
+    }
+    for (size_t id = 0; id < st->gs; ++id) {
+        st->dn[id] = 0;
+        if ((1 - eta * (st->f[id] - fmean) / factt) > 0)
+            st->dn[id] = 1. / eta * log(1 - eta * (st->f[id] - fmean) / factt);
+    }
+    double bmin = st->dn[0], bmax = bmin;
+    for (size_t id = 0; id < st->gs; ++id)
+        if (1 - eta * (st->f[id] - fmean) / factt > 0) {
+            bmin = st->dn[id] < bmin ? st->dn[id] : bmin;
+            bmax = st->dn[id] > bmax ? st->dn[id] : bmax;
+        }
+    double df = (bmax - bmin) / (double)nbins;
+    if (!isfinite(df) || df <= 0.0) df = 1.0;
+    for (int b = 0; b < nbins; ++b) freq[b] = 0.0;
+    int numpts = 0;
+    for (size_t id = 0; id < st->gs; ++id)
+        if (1 - eta * (st->f[id] - fmean) / factt > 0) {
+            int b = (int)((st->dn[id] - bmin) / df);
+            if (st->dn[id] == bmax) b = nbins - 1;
+            if (b >= 0 && b < nbins) {
+                freq[b]++;
+                numpts++;
+            }
+        }
+    if (numpts == 0) numpts = 1;
+    for (int b = 0; b < nbins; ++b) freq[b] = freq[b] / (double)numpts;
+    *bmin_out = bmin;
+    *df_out = df;
+    return 1;
 }
 
 /* Symmetric tensor packing (src/main.h:43-65). */
@@ -1605,6 +1848,7 @@ typedef struct {
     char dir[400];
     FILE *means, *vars, *vel, *sf, *energy, *cons, *spectra, *spectratimes, *gw, *gwdot;
     FILE *hist, *histtimes, *box, *box2d, *box2dot, *modes, *spectraN, *histN, *histtimesN, *box2dN;
+    FILE *bis, *spectraLOG, *histLOG, *histtimesLOG;
     double *hist_buf;
 } ieo_out;
 
@@ -1626,7 +1870,8 @@ static void out_close(ieo_out *o)
 {
     FILE **all[] = {&o->means, &o->vars, &o->vel, &o->sf, &o->energy, &o->cons, &o->spectra,
                     &o->spectratimes, &o->gw, &o->gwdot, &o->hist, &o->histtimes, &o->box, &o->box2d,
-                    &o->box2dot, &o->modes, &o->spectraN, &o->histN, &o->histtimesN, &o->box2dN};
+                    &o->box2dot, &o->modes, &o->spectraN, &o->histN, &o->histtimesN, &o->box2dN,
+                    &o->bis, &o->spectraLOG, &o->histLOG, &o->histtimesLOG};
     for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q)
         if (*all[q]) { fclose(*all[q]); *all[q] = NULL; }
     free(o->hist_buf);
@@ -1861,6 +2106,34 @@ static int program_inflation(ieo_state *st, ieo_out *o)
         for (int b = 0; b < nb; ++b) fprintf(fp, "%e\n", m[b]);
         fprintf(fp, "\n");
         free(m);
+    }
+    /* the rest of save_last (src/output.cpp:1505-1519) */
+    if (c->output_bispectrum) {
+        const int nb = (int)(sqrt(3.0) * (st->N / 2)) + 1;
+        double *p = calloc(nb, sizeof(double)), *re = calloc(nb, sizeof(double)), *im = calloc(nb, sizeof(double));
+        int *cnt = calloc(nb, sizeof(int));
+        ieo_bispectrum(st, p, cnt, re, im);
+        FILE *fp = out_open(o, &o->bis, "bispectra");
+        for (int k = 0; k < nb; ++k) fprintf(fp, "%e %d %e %e\n", p[k], cnt[k], re[k], im[k]);
+        fprintf(fp, "\n");
+        free(p); free(re); free(im); free(cnt);
+    }
+    if (c->perform_deltaN && c->output_LOG) {
+        const int nb = ieo_spectrum_numbins(st->N);
+        double *p = calloc(nb, sizeof(double)), *v = calloc(nb, sizeof(double));
+        int *cnt = calloc(nb, sizeof(int));
+        ieo_spectrum_log(st, p, cnt, v);
+        out_bins(out_open(o, &o->spectraLOG, "spectraLOG"), nb, p, cnt, v);
+        free(p); free(v); free(cnt);
+        double *freq = calloc(c->nbins > 0 ? c->nbins : 1, sizeof(double)), bmin, df;
+        FILE *ht = out_open(o, &o->histtimesLOG, "histogramtimesLOG");
+        FILE *h = out_open(o, &o->histLOG, "histogramLOG");
+        fprintf(ht, "%f %e", st->t, st->a);
+        ieo_histogram_log(st, &bmin, &df, freq);
+        for (int b = 0; b < c->nbins; ++b) fprintf(h, "%e\n", freq[b]);
+        fprintf(h, "\n");
+        fprintf(ht, " %e %e\n", bmin, df);
+        free(freq);
     }
     return 0;
 }

@@ -63,6 +63,9 @@ typedef struct ieo_config {
     int post_inflation;
     double horizon_factor, omega, dt_post, af_post;
     int post_integrator;
+    /* outputs off by default (src/runtime_parameters.cpp:97-101) */
+    int output_bispectrum, output_LOG;
+    double eta_log;
 } ieo_config;
 
 typedef struct ieo_state ieo_state;
@@ -128,6 +131,24 @@ void ieo_spectrum_f(ieo_state *st, double *p, int *counts, double *vals);
 /* GW TT spectrum (src/output.cpp:466-625). hdot=0: h_ij, 1: hdot_ij (DoubleThenFloat),
  * 2: hdot_ij with the post-inflation float factor (FloatMultiply, :662-669). */
 void ieo_gw_spectrum(ieo_state *st, int hdot, double *p, int *counts, double *vals);
+/* The k-space forcing cutoff of spectraf (src/output.cpp:333-404): with f already
+ * forward-transformed in place (FFTEASY layout, f and fny), transform fd, zero the
+ * modes with |n| > high_cutoff_index or < low_cutoff_index with the reference's
+ * mirror rule, transform fd back.  The caller transforms f back. */
+void ieo_cutoff_modes(ieo_state *st);
+/* bispectraf (src/output.cpp:740-911): equilateral bispectrum of f, numbins =
+ * int(sqrt(3)*(N/2))+1 entries of p, counts, re, im (norm1 applied); f is
+ * mean-subtracted, transformed, transformed back and the mean added, in place. */
+void ieo_bispectrum(ieo_state *st, double *p, int *counts, double *re, double *im);
+/* build_deltaN_log (src/output.cpp:167-198) into the deltaN array; returns 0 when
+ * the mean velocity factor is not usable (map set to 0). */
+int ieo_deltaN_log(ieo_state *st);
+/* spectraLOG (src/output.cpp:1280-1308): the map, then its spectrum (in place). */
+void ieo_spectrum_log(ieo_state *st, double *p, int *counts, double *vals);
+/* histogramsLOG (src/output.cpp:1312-1403): bins of the map over the sites with
+ * a positive logarithm argument; freq[nbins] = counts / points, *bmin, *df as
+ * written to histogramtimesLOG.dat.  Returns 0 when the factor is not usable. */
+int ieo_histogram_log(ieo_state *st, double *bmin, double *df, double *freq);
 /* modes.dat (src/output.cpp:674-738) */
 void ieo_modes(int N, double L, double rescale_B, double *out);
 

@@ -77,6 +77,9 @@ class Config(C.Structure):
         ("dt_post", C.c_double),
         ("af_post", C.c_double),
         ("post_integrator", C.c_int),
+        ("output_bispectrum", C.c_int),
+        ("output_LOG", C.c_int),
+        ("eta_log", C.c_double),
     ]
 
 
@@ -139,6 +142,11 @@ def lib() -> C.CDLL:
             "ieo_post_evolve_derivs": (None, [P, C.c_double]),
             "ieo_post_rk4_step": (None, [P, C.c_double]),
             "ieo_post_rk45_step": (C.c_int, [P, C.c_double, C.c_double, dp]),
+            "ieo_cutoff_modes": (None, [P]),
+            "ieo_bispectrum": (None, [P, dp, ip, dp, dp]),
+            "ieo_deltaN_log": (C.c_int, [P]),
+            "ieo_spectrum_log": (None, [P, dp, ip, dp]),
+            "ieo_histogram_log": (C.c_int, [P, dp, dp, dp]),
             "ieo_program": (C.c_int, [C.POINTER(Config), C.c_char_p]),
             "ieo_num_threads": (C.c_int, []),
         }
@@ -210,7 +218,7 @@ def make_config(N: int, numerical: int, params: dict | None = None, table: Table
              "af_post_inflation": "af_post", "post_inflation_integrator": "post_integrator"}
     for k, v in params.items():
         k = remap.get(k, k)
-        if k in ("output_bispectrum", "output_LOG", "eta_log", "screen_updates"):
+        if k == "screen_updates":
             continue
         if k in _POST_KEYS and not post:
             continue
@@ -281,6 +289,13 @@ class State:
         v = np.zeros(nb)
         lib().ieo_spectrum_f(self.ptr, _dp(p), _ip(c), _dp(v))
         return p, c, v
+
+    def bispectrum(self):
+        nb = int(np.sqrt(3.0) * (self.N // 2)) + 1
+        p, re, im = np.zeros(nb), np.zeros(nb), np.zeros(nb)
+        c = np.zeros(nb, dtype=np.int32)
+        lib().ieo_bispectrum(self.ptr, _dp(p), _ip(c), _dp(re), _dp(im))
+        return p, c, re, im
 
     def gw_spectrum(self, hdot: int):
         nb = int(np.sqrt(3.0) * (self.N // 2)) + 1

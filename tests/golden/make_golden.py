@@ -57,6 +57,22 @@ CASES = {
     "tabulated_n16_post_nodn": (16, 1, "params.numerical.txt", {"af_post_inflation": 8},
                                 {"post": 1, "deltaN": 0}),
     "hilltop_n16_post": (16, 0, "params.analytic.txt", {"af_post_inflation": 8}, {"post": 1}),
+    # the equilateral bispectrum of save_last (off by default).  output_LOG = 1 is
+    # not run: save_last writes the log map into the deltaN vector before
+    # initializeN allocates it (src/output.cpp:1511-1516, src/initialize.cpp:221)
+    # and the program dies with a segmentation fault.
+    "tabulated_n16_extras": (16, 1, "params.numerical.txt", {"output_bispectrum": 1}),
+    "hilltop_n16_extras": (16, 0, "params.analytic.txt", {"output_bispectrum": 1}),
+    "tabulated_n32_extras": (32, 1, "params.numerical.txt", {"output_bispectrum": 1, "af": 4, "Nend": 0.3}),
+    # the k-space forcing cutoff of spectraf and spectraf_post_inflation.  A low
+    # cutoff above 0 removes the k = 0 mode too (the field mean), which takes the
+    # tabulated potential out of its table, so it is run on the hilltop only.
+    "tabulated_n16_cutoff": (16, 1, "params.numerical.txt", {"high_cutoff_index": 5.5, "forcing_cutoff": 1}),
+    "hilltop_n16_cutoff": (16, 0, "params.analytic.txt",
+                           {"high_cutoff_index": 6.5, "low_cutoff_index": 1.2, "forcing_cutoff": 1}),
+    "tabulated_n16_cutoff_post": (16, 1, "params.numerical.txt",
+                                  {"high_cutoff_index": 5.5, "forcing_cutoff": 1, "af_post_inflation": 8},
+                                  {"post": 1}),
 }
 
 # files whose first block only is kept (large text dumps)

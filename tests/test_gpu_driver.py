@@ -128,7 +128,7 @@ def post_blocks(res: Path, g: dict, key: str) -> tuple[np.ndarray, np.ndarray]:
 # variance, tensor spectra 2e-2) and the means an absolute bound of 1e-2 of the
 # rms of Phi; without deltaN (Phi from f) they follow the inflation tolerances.
 @pytest.mark.parametrize("case", ["tabulated_n16_post", "tabulated_n16_post_rk4", "tabulated_n16_post_nodn",
-                                  "hilltop_n16_post"])
+                                  "hilltop_n16_post", "tabulated_n16_cutoff_post"])
 def test_driver_post_inflation_files(case, tmp_path):
     g = load_golden(case)
     extra = "post_inflation = 1\n" + ("perform_deltaN = 0\n" if int(g["perform_deltaN"]) == 0 else "")
@@ -166,3 +166,74 @@ def test_driver_post_inflation_files(case, tmp_path):
     got, ref = post_blocks(res, g, "histogram")
     assert got.shape == ref.shape
     assert np.max(np.abs(got - ref)) <= (2.0 / N ** 3 + 1e-9 if exact else 0.02)
+
+
+# the k-space forcing cutoff after each spectraf (inflation): the standard files;
+# with the low cutoff (hilltop) k = 0 is removed, the mean is round-off, and the
+# mean-like columns are left out
+@pytest.mark.parametrize("case", ["tabulated_n16_cutoff", "hilltop_n16_cutoff"])
+def test_driver_forcing_cutoff(case, tmp_path):
+    g = run_driver(case, tmp_path, "post_inflation = 0\n")
+    res = tmp_path / "results"
+    keys = ["sf", "spectra", "spectratimes", "variance", "modes"]
+    if case.startswith("tabulated"):
+        keys += ["means", "velocity"]
+    for key in keys:
+        ref = g[key]
+        got = blocks(res / f"{key}.dat", key, ref)
+        assert got.shape == ref.shape, key
+        if key == "spectra":
+            # shells above the cutoff hold FFT round-off only (1e-35 of the peak)
+            peak = ref[..., 2].max(axis=-1, keepdims=True)
+            live = ref[..., 2] > 1e-12 * peak
+            assert np.array_equal(got[..., :2], ref[..., :2])
+            assert np.allclose(got[..., 2][live], ref[..., 2][live], rtol=2e-6, atol=0), key
+            assert np.all(got[..., 2] <= 1e-12 * peak + np.where(live, np.inf, 0.0))
+        else:
+            assert np.allclose(got, ref, rtol=2e-6, atol=0), (key, np.max(np.abs(got - ref) / np.abs(ref + 1e-300)))
+
+
+# bispectra.dat of save_last (output_bispectrum = 1): the pair sets are the
+# reference's (counts exact); the sums differ in order and in the field's last bits
+@pytest.mark.parametrize("case", ["tabulated_n16_extras", "hilltop_n16_extras", "tabulated_n32_extras"])
+def test_driver_bispectrum(case, tmp_path):
+    g = run_driver(case, tmp_path, "post_inflation = 0\n")
+    ref = g["bispectra"]
+    got = blocks(tmp_path / "results" / "bispectra.dat", "bispectra", ref)
+    assert got.shape == ref.shape
+    assert np.array_equal(got[..., 1], ref[..., 1])
+    assert np.allclose(got[..., 0], ref[..., 0], rtol=2e-6, atol=0)
+    scale = np.abs(ref[0, 1:, 2:]).max()
+    assert np.allclose(got[0, 1:, 2:], ref[0, 1:, 2:], rtol=1e-4, atol=1e-5 * scale)
+    assert np.all(np.abs(got[0, 0, 2:] - ref[0, 0, 2:]) <= 1e-5 * scale)
+
+
+# output_LOG = 1: the reference program dies before writing these files (it
+# fills the deltaN vector before allocating it), so the driver is compared with the
+# oracle's program, whose log map is pinned by tests/test_oracle_log.py
+@pytest.mark.parametrize("eta", [-0.5, 0.7])
+def test_driver_log_outputs_match_oracle_program(eta, tmp_path):
+    from conftest import golden_config
+    from oracle import oracle as O
+    extra = f"output_LOG = 1\neta_log = {eta}\naf = 3\nNend = 0.4\npost_inflation = 0\n"
+    (tmp_path / "gpu").mkdir()
+    run_driver("tabulated_n16", tmp_path / "gpu", extra)
+    cfg, params, table = golden_config("tabulated_n16")
+    cfg.output_LOG = 1
+    cfg.eta_log = eta
+    cfg.af = 3.0
+    cfg.Nend = 0.4
+    assert O.run_program(cfg, tmp_path / "oracle") == 0
+    for key in ("spectraLOG", "histogramtimesLOG", "histogramLOG"):
+        ref = np.stack(read_blocks(tmp_path / "oracle" / f"{key}.dat"))
+        got = np.stack(read_blocks(tmp_path / "gpu" / "results" / f"{key}.dat"))
+        assert got.shape == ref.shape, key
+        if key == "histogramLOG":
+            assert np.max(np.abs(got - ref)) <= 2.0 / 16 ** 3 + 1e-9
+        elif key == "spectraLOG":
+            # shell 0 is the squared sum of the map, which cancels: 1e-3
+            assert np.array_equal(got[..., :2], ref[..., :2])
+            assert np.allclose(got[..., 1:, 2], ref[..., 1:, 2], rtol=1e-5, atol=0)
+            assert np.allclose(got[..., 0, 2], ref[..., 0, 2], rtol=1e-3, atol=0)
+        else:
+            assert np.allclose(got, ref, rtol=2e-6, atol=0), key

@@ -19,10 +19,14 @@ CASES = ["hilltop_n16", "tabulated_n16", "tabulated_n16_rk4", "tabulated_n16_rk4
 # from f without deltaN, and the analytic potential
 POST_CASES = ["tabulated_n16_post", "tabulated_n16_post_rk4", "tabulated_n16_post_rk45",
               "tabulated_n16_post_nodn", "hilltop_n16_post"]
+# outputs off by default: the equilateral bispectrum, the forcing cutoff (inflation
+# and post-inflation)
+EXTRA_CASES = ["tabulated_n16_extras", "hilltop_n16_extras", "tabulated_n32_extras", "tabulated_n16_cutoff",
+               "hilltop_n16_cutoff", "tabulated_n16_cutoff_post"]
 META = ("params_text", "N", "numerical_potential", "perform_deltaN", "post_inflation")
 
 
-@pytest.mark.parametrize("case", CASES + POST_CASES)
+@pytest.mark.parametrize("case", CASES + POST_CASES + EXTRA_CASES)
 def test_oracle_reproduces_reference(case, tmp_path):
     g = load_golden(case)
     N = int(g["N"])
@@ -46,4 +50,4 @@ def test_oracle_reproduces_reference(case, tmp_path):
         assert got.shape == ref.shape, key
         assert np.array_equal(got, ref), f"{case}/{key}: max |diff| {np.max(np.abs(got - ref))}"
         checked += 1
-    assert checked >= (10 if golden_toggles(g)["post"] else 18)
+    assert checked >= (10 if golden_toggles(g)["post"] else 14)
