@@ -91,6 +91,10 @@ ie_status ie_upload_modes(ie_run *run, const double *fk, const double *fny,
                           const double *fdk, const double *fdny, double phi0, double dphi0);
 /* Download a field as FP64 (N^3); IE_FIELD_F reconstitutes phibar + dphi in split mode. */
 ie_status ie_download_field(ie_run *run, int which, double *out);
+/* One i-plane (N^2 doubles, j-major) of a field as ie_download_field would give
+ * it: the 2-D snapshots (box2d, box2dot, box2dN; src/output.cpp:933-981,
+ * 1195-1218) without moving the whole lattice. */
+ie_status ie_download_plane(ie_run *run, int which, int i, double *out);
 /* Upload a field as FP64 (N^3): IE_FIELD_DELTAN the deltaN map; IE_FIELD_F and
  * IE_FIELD_FD the FP64 state of the deltaN and post-inflation phases. */
 ie_status ie_upload_field(ie_run *run, int which, const double *src);

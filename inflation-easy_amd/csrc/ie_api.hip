@@ -378,6 +378,41 @@ ie_status ie_download_field(ie_run *r, int which, double *out)
 
 static ie_status dn_state(ie_run *r, double **f, double **fd);
 
+ie_status ie_download_plane(ie_run *r, int which, int i, double *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    if (i < 0 || i >= r->N) return fail(r, IE_ERR_ARG, "plane index out of range");
+    const size_t n = (size_t)r->N * r->N, off = (size_t)i * n;
+    if (which == IE_FIELD_DELTAN) {
+        IE_HIP(r, hipMemcpyAsync(out, r->F.dn + off, n * 8, hipMemcpyDeviceToHost, r->stream));
+        return sync(r);
+    }
+    if (which != IE_FIELD_F && which != IE_FIELD_FD) return fail(r, IE_ERR_ARG, "bad field");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const DevBg &b = *r->bg_host;
+    if (r->in_deltaN || r->cfg.precision == IE_FP64) {
+        double *f, *fd;
+        if (r->in_deltaN) {
+            if ((st = dn_state(r, &f, &fd)) != IE_OK) return st;
+        } else {
+            f = (double *)r->F.f[b.parity];
+            fd = (double *)r->F.fd[b.parity];
+        }
+        IE_HIP(r, hipMemcpyAsync(out, (which == IE_FIELD_F ? f : fd) + off, n * 8, hipMemcpyDeviceToHost, r->stream));
+        return sync(r);
+    }
+    // split storage: the plane's FP32 fluctuations plus the FP64 mean, as
+    // ie_download_field reconstitutes them
+    std::vector<float> v(n);
+    const float *src = (const float *)(which == IE_FIELD_F ? r->F.f[b.parity] : r->F.fd[b.parity]);
+    IE_HIP(r, hipMemcpyAsync(v.data(), src + off, n * 4, hipMemcpyDeviceToHost, r->stream));
+    if ((st = sync(r)) != IE_OK) return st;
+    const double base = which == IE_FIELD_F ? b.phibar : b.pibar;
+    for (size_t q = 0; q < n; ++q) out[q] = base + (double)v[q];
+    return IE_OK;
+}
+
 ie_status ie_upload_field(ie_run *r, int which, const double *src)
 {
     if (!r || !src) return fail(r, IE_ERR_ARG, "null argument");

@@ -444,11 +444,13 @@ struct Driver {
         fflush(c);
     }
 
+    // box (src/output.cpp:913-931) downloads the lattice; box2d / box2dot
+    // (:933-981) one plane each
     void snapshots()
     {
-        std::vector<double> f(gs);
-        CHECK(run, ie_download_field(run, IE_FIELD_F, f.data()));
         if (p.output_box3D) {
+            std::vector<double> f(gs);
+            CHECK(run, ie_download_field(run, IE_FIELD_F, f.data()));
             FILE *fp = out.get("box");
             for (size_t id = 0; id < gs; ++id) fprintf(fp, "%.17g\n", f[id]);
             fprintf(fp, "\n");
@@ -456,14 +458,13 @@ struct Driver {
         }
         if (p.output_box2D) {
             const int i = N < 64 ? 5 : 50;
-            std::vector<double> fd(gs);
-            CHECK(run, ie_download_field(run, IE_FIELD_FD, fd.data()));
+            std::vector<double> f((size_t)N * N), fd((size_t)N * N);
+            CHECK(run, ie_download_plane(run, IE_FIELD_F, i, f.data()));
+            CHECK(run, ie_download_plane(run, IE_FIELD_FD, i, fd.data()));
             FILE *a = out.get("snapshots_2d_phi"), *b = out.get("snapshots_2d_phidot");
-            for (int j = 0; j < N; ++j)
-                for (int k = 0; k < N; ++k) fprintf(a, "%.17g\n", f[((size_t)i * N + j) * N + k]);
+            for (size_t q = 0; q < (size_t)N * N; ++q) fprintf(a, "%.17g\n", f[q]);
             fprintf(a, "\n");
-            for (int j = 0; j < N; ++j)
-                for (int k = 0; k < N; ++k) fprintf(b, "%.17g\n", fd[((size_t)i * N + j) * N + k] * p.rescale_B);
+            for (size_t q = 0; q < (size_t)N * N; ++q) fprintf(b, "%.17g\n", fd[q] * p.rescale_B);
             fprintf(b, "\n");
             fflush(a);
             fflush(b);
@@ -658,13 +659,11 @@ struct Driver {
         CHECK(run, ie_deltaN_zeta(run, cutoff, &nmean));
         b = bg();
         histogram(IE_FIELD_DELTAN, "histogramN", "histogramtimesN", b, true);
-        if (p.output_box2D) {
-            std::vector<double> dn(gs);
-            CHECK(run, ie_download_field(run, IE_FIELD_DELTAN, dn.data()));
-            const int i = N < 64 ? 5 : 50;
+        if (p.output_box2D) {   // box2dN (src/output.cpp:1195-1218)
+            std::vector<double> dn((size_t)N * N);
+            CHECK(run, ie_download_plane(run, IE_FIELD_DELTAN, N < 64 ? 5 : 50, dn.data()));
             FILE *fp = out.get("snapshots_2d_deltaN");
-            for (int j = 0; j < N; ++j)
-                for (int k = 0; k < N; ++k) fprintf(fp, "%.17g\n", dn[((size_t)i * N + j) * N + k]);
+            for (double x : dn) fprintf(fp, "%.17g\n", x);
             fprintf(fp, "\n");
             fflush(fp);
         }

@@ -146,6 +146,7 @@ def lib() -> C.CDLL:
             "ie_upload_modes": (st, [P, dp, dp, dp, dp, C.c_double, C.c_double]),
             "ie_download_field": (st, [P, C.c_int, dp]),
             "ie_upload_field": (st, [P, C.c_int, dp]),
+            "ie_download_plane": (st, [P, C.c_int, C.c_int, dp]),
             "ie_post_init": (st, [P, C.POINTER(PostParams), C.POINTER(Background)]),
             "ie_forcing_cutoff": (st, [P, C.c_double, C.c_double]),
             "ie_bispectrum": (st, [P, dp, ip, dp, dp]),
@@ -311,6 +312,11 @@ class Run:
     def field(self, which: int) -> np.ndarray:
         out = np.empty(self.gs)
         self._check(lib().ie_download_field(self.h, which, _dp(out)))
+        return out
+
+    def plane(self, which: int, i: int) -> np.ndarray:
+        out = np.empty(self.N * self.N)
+        self._check(lib().ie_download_plane(self.h, which, i, _dp(out)))
         return out
 
     def tensor(self, comp: int) -> np.ndarray:

@@ -135,3 +135,19 @@ def test_log_map_invalid_factor():
     valid, bmin, df, freq = run.deltaN_log(-0.5, 16)
     assert not valid and bmin == 0.0 and df == 1.0 and np.all(freq == 0)
     assert np.all(run.field(E.IE_FIELD_DELTAN) == 0.0)
+
+
+@pytest.mark.parametrize("precision", [E.IE_FP64, E.IE_SPLIT])
+def test_plane_download_is_a_slice_of_the_field(precision):
+    st, run = start("tabulated_n16", precision, steps=5)
+    N = st.N
+    for phase in ("inflation", "deltaN"):
+        if phase == "deltaN":
+            run.deltaN_init()
+            run.upload_field(E.IE_FIELD_DELTAN, np.arange(N ** 3, dtype=np.float64) * 1e-3)
+        for which in (E.IE_FIELD_F, E.IE_FIELD_FD) + ((E.IE_FIELD_DELTAN,) if phase == "deltaN" else ()):
+            full = run.field(which).reshape(N, N * N)
+            for i in (0, 5, N - 1):
+                assert np.array_equal(run.plane(which, i), full[i]), (phase, which, i)
+    lib = E.lib()
+    assert lib.ie_download_plane(run.h, E.IE_FIELD_F, N, E._dp(np.zeros(N * N))) == E.IE_ERR_ARG
