@@ -174,6 +174,49 @@ def serve_stream(nreq: int, rate: float, seed: int = 20261203) -> dict:
                    "each run: ICs, inflation to af=2N, deltaN, P_zeta"}
 
 
+def pzeta_check(N: int = 256) -> dict:
+    """P_zeta of split-storage runs against FP64 runs from the same initial
+    conditions at N^3: one whole run per potential and storage (initial
+    conditions, inflation to af = 2N, the deltaN map, P_zeta = k^3/(2 pi^2) P_dN),
+    served together.  Reported per potential: the largest relative difference
+    over the shells 1 <= k <= N/2 (complete spherical shells) and over all shells
+    (the corner shells k > N/2 sqrt(2) hold a handful of modes each)."""
+    import numpy as np
+
+    import ie_engine as E
+    with np.load(REPO / "tests" / "golden" / "potential_table.npz") as z:
+        class T:  # table kept alive for the request pointers
+            x = np.ascontiguousarray(z["field"])
+            v = np.ascontiguousarray(z["potential"])
+            dv = np.ascontiguousarray(z["potential_derivative"])
+    srv = E.Server(240 << 30, 500, 4)
+    t0 = srv.clock()
+    ids = {}
+    for prec in (E.IE_SPLIT, E.IE_FP64):
+        cfg = E.make_config(N, precision=prec, V0=3.338e-13, ns=0.97, dt=0.0005, dN=1e-7, Nend=0.001)
+        ids[srv.submit(cfg, 8, 0.0935, 0.000796, 1, 0)] = ("hilltop", prec)
+        cfg = E.make_config(N, precision=prec, numerical=1, table=T, V0=3e-9, dt=0.001, dN=1e-4, Nend=2.0)
+        ids[srv.submit(cfg, 8, 2.9181235049318586, -0.06727651095116181, 1, 0, T)] = ("tabulated_usr", prec)
+    got = {}
+    while len(got) < len(ids):
+        srv.pump()
+        for r in srv.results():
+            got[ids[r["id"]]] = r
+    secs = srv.clock() - t0
+    srv.close()
+    res = {"N": N, "wall_s": secs,
+           "storage": "split (FP32 fluctuation about FP64 mode) vs FP64, same initial conditions"}
+    for name in ("hilltop", "tabulated_usr"):
+        a, b = got[(name, E.IE_SPLIT)], got[(name, E.IE_FP64)]
+        n = min(len(a["pzeta"]), len(b["pzeta"]))
+        pa, pb = a["pzeta"][:n], b["pzeta"][:n]
+        rel = np.where(pb > 0, np.abs(pa - pb) / np.where(pb > 0, pb, 1.0), 0.0)
+        res[name] = {"status": [a["status"], b["status"]], "steps": a["steps"],
+                     "max_rel_err_shells_1_to_N_over_2": float(rel[1:N // 2 + 1].max()),
+                     "max_rel_err_all_shells": float(rel[1:].max()), "shells": n}
+    return res
+
+
 def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203) -> dict:
     """Batched ensemble: nruns hilltop lattices of N^3, seeds
     1000 + r, V0 log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split
@@ -274,7 +317,8 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--post-steps", type=int, default=200, help="post-inflation steps timed (0: skip)")
-    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post"], default=None,
+    ap.add_argument("--pzeta-check", type=int, default=1, help="split vs FP64 P_zeta of whole runs (0: skip)")
+    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "pzeta"], default=None,
                     help="time a single case (for profiling runs)")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -296,6 +340,9 @@ def main() -> int:
     sites = N ** 3
     if args.only == "post":
         print(json.dumps({"post_inflation": post_inflation(N, args.post_steps)}), flush=True)
+        return 0
+    if args.only == "pzeta":
+        print(json.dumps({"pzeta_split_vs_fp64": pzeta_check(N)}), flush=True)
         return 0
     cases = {"split": (E.IE_SPLIT, 0), "fp64": (E.IE_FP64, 0), "sigw": (E.IE_SPLIT, 1)}
     todo = [args.only] if args.only else ["split", "fp64", "sigw"]
@@ -354,6 +401,9 @@ def main() -> int:
                            else "k_row_kick (kick + drift)", "kernel_ms": ms,
                            "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
                            "step_ms_events": prof["step_ms"]}
+    if args.pzeta_check and not args.only:
+        out["pzeta_split_vs_fp64"] = pzeta_check(N)
+        print(f"# pzeta check: {out['pzeta_split_vs_fp64']}", file=sys.stderr, flush=True)
     if args.post_steps > 0 and not args.only:
         out["post_inflation"] = post_inflation(N, args.post_steps)
         print(f"# post-inflation: {out['post_inflation']}", file=sys.stderr, flush=True)
