@@ -394,11 +394,13 @@ def main() -> int:
         ms = prof["kick_drift_ms"]
         achieved = alg / (ms * 1e-3) / 1e9
         fused = 128 <= N <= 256
+        variant = head["tuning"].get("march_variant", 0)
+        kname = ("k_march_pf<float,true,4>" if variant == 1 else "k_march_w5<float,true,4>") + \
+            " (fused kick + drift + next-step sums)"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
                            "traffic": read_traffic(f"{'march' if fused else 'kick_drift'}_split_{N}"),
-                           "kernel": "k_march_w5<float,true,4> (fused kick + drift + next-step sums)" if fused
-                           else "k_row_kick (kick + drift)", "kernel_ms": ms,
+                           "kernel": kname if fused else "k_row_kick (kick + drift)", "kernel_ms": ms,
                            "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
                            "step_ms_events": prof["step_ms"]}
     if args.pzeta_check and not args.only:
