@@ -16,6 +16,15 @@ constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxRed = 8;       // reduction slots per block
 
+// Scalars of one loop step as a pure function of the background between steps
+// and the sums of the current f in DevBg::red (run_evolution_loop's step
+// dt astep^{s-1}, the implicit Friedmann half-kick of evolve_derivs, the kick
+// coefficients, and for split storage the homogeneous mode's kick and drift;
+// src/evolution.cpp:153-181, 256-283, 938).
+struct StepScalars {
+    double d, ad2, ad_half, ad_next, friction, potnorm, laplnorm, srcamp, vpmean, pibar_next, phibar_next;
+};
+
 // Background of one run, resident on the device.  Kernels of a step read it at
 // their start; one-workgroup finish kernels (or the last workgroup of an output
 // reduction, by arrival counter) update it, so a whole step sequence runs without
@@ -28,6 +37,9 @@ struct DevBg {
     double vpmean;                              // <V'(phi)> over the lattice (split)
     double red[kMaxRed];                        // results of the last reduction
     double dt, af, rescale_s, dx, inv_gs;       // constants of the run
+    StepScalars nx;                             // scalars of the next fused loop step: set by the
+                                                // kernel that closes the previous step (or sums the
+                                                // f a loop starts from), read by the step's kernels
     long long steps;
     int done, error, parity, pad0;
     unsigned int counter;

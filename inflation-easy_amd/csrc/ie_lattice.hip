@@ -213,16 +213,9 @@ __device__ void friedmann_finish_body(DevBg *bg, const double *partials, unsigne
     bg->vpmean = tot[2] / gsd;
 }
 
-// Scalars of one loop step as a pure function of the background between steps
-// and the sums of the current f in bg->red (run_evolution_loop's step
-// dt astep^{s-1}, the implicit Friedmann half-kick of evolve_derivs, the kick
-// coefficients, and for split storage the homogeneous mode's kick and drift;
-// src/evolution.cpp:153-181, 256-283, 938).  Every kernel of a fused step
-// evaluates it and gets the same bits: no contraction, same operations.
-struct StepScalars {
-    double d, ad2, ad_half, ad_next, friction, potnorm, laplnorm, srcamp, vpmean, pibar_next, phibar_next;
-};
-
+// The StepScalars of a fused loop step, evaluated once per step (no contraction)
+// by the one-workgroup kernel before it and stored in bg->nx, so the step's
+// kernels start loading at once instead of evaluating four FP64 pows each.
 #pragma clang fp contract(off)
 __device__ __forceinline__ StepScalars step_scalars(const DevBg *bg, double gsd)
 {
@@ -258,7 +251,7 @@ __device__ void step_close_body(DevBg *bg, const double *partials, unsigned G, s
     double tot[3];
     sum_partials<3>(partials, (int)G, tot, smem);
     if (threadIdx.x != 0) return;
-    const StepScalars S = step_scalars(bg, (double)gs);
+    const StepScalars S = bg->nx;
     bg->d = S.d;
     bg->ad2 = S.ad2;
     bg->friction = S.friction;
@@ -280,10 +273,12 @@ __device__ void step_close_body(DevBg *bg, const double *partials, unsigned G, s
     bg->red[0] = tot[0];
     bg->red[1] = tot[1];
     bg->red[2] = tot[2];
+    bg->nx = step_scalars(bg, (double)gs);
 }
 
-// Sums of the current f into bg->red at the start of a fused loop (from a reduce).
-__device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, double *smem)
+// Sums of the current f into bg->red at the start of a fused loop (from a reduce),
+// and the scalars of the loop's first step.
+__device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, size_t gs, double *smem)
 {
     if (bg->done) return;
     double tot[3];
@@ -292,6 +287,7 @@ __device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, 
     bg->red[0] = tot[0];
     bg->red[1] = tot[1];
     bg->red[2] = tot[2];
+    bg->nx = step_scalars(bg, (double)gs);
 }
 
 __global__ __launch_bounds__(kBlock) void k_friedmann_finish(DevBg *bg, const double *partials, unsigned G,
@@ -462,7 +458,7 @@ __device__ __forceinline__ void row_tensor_body(const Fields &F, DevBg *bg, int 
     // fused loop steps: the step's scalars from the background (as the march kernel)
     double d = bg->d, laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
     if (fresh) {
-        const StepScalars S = step_scalars(bg, (double)N * N * N);
+        const StepScalars S = bg->nx;
         d = S.d;
         laplnorm = S.laplnorm;
         friction = S.friction;
@@ -637,7 +633,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
     const double phibar = SPLIT ? bg->phibar : 0.0;
     double phibar_next = phibar;
     if constexpr (!SUMS) {
-        const StepScalars S = step_scalars(bg, (double)N * N * N);
+        const StepScalars S = bg->nx;
         d = uniform(S.d);
         laplnorm = uniform(S.laplnorm);
         friction = uniform(S.friction);
@@ -836,10 +832,11 @@ __global__ __launch_bounds__(kBlock) void k_step_close(DevBg *bg, const double *
     step_close_body(bg, partials, G, (size_t)gs, split, smem);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sums_finish(DevBg *bg, const double *partials, unsigned G)
+__global__ __launch_bounds__(kBlock) void k_sums_finish(DevBg *bg, const double *partials, unsigned G,
+                                                         unsigned long long gs)
 {
     __shared__ double smem[3 * kWaves];
-    sums_finish_body(bg, partials, G, smem);
+    sums_finish_body(bg, partials, G, (size_t)gs, smem);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -878,7 +875,8 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
     __shared__ double smem[3 * kWaves];
     const RunDesc &R = runs[blockIdx.x];
     if (!R.march) return;
-    sums_finish_body(R.bg, R.partials, R.Gm, smem);
+    const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
+    sums_finish_body(R.bg, R.partials, R.Gm, gs, smem);
 }
 
 // ----------------------------------------------------- tensor march kernel
@@ -914,7 +912,7 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
     const T *__restrict__ f = (const T *)F.f[par];
     double d = bg->d, laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
     if (fresh) {
-        const StepScalars S = step_scalars(bg, (double)N * N * N);
+        const StepScalars S = bg->nx;
         d = S.d;
         laplnorm = S.laplnorm;
         friction = S.friction;
@@ -1220,7 +1218,7 @@ __device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d
     double d = (MODE == DRIFT && !loop) ? d_arg : bg->d;
     double laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
     if (fresh) {
-        const StepScalars S = step_scalars(bg, (double)gs);
+        const StepScalars S = bg->nx;
         d = S.d;
         laplnorm = S.laplnorm;
         friction = S.friction;
@@ -1560,7 +1558,7 @@ hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *b
         if (E == 2) k_march_sums<double, false, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
         else k_march_sums<double, false, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
     }
-    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G);
+    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G, (unsigned long long)F.N * F.N * F.N);
     return hipGetLastError();
 }
 
