@@ -183,8 +183,10 @@ typedef struct {
     int elems[3];              /* sites per lane of each kick candidate */
     double kick_ms[3];         /* mean kick launch time per candidate */
     int ntensor;               /* tensor candidates timed (0 without SIGW) */
-    int tensor_code[5];        /* 1, 2, 4: row kernel; 17, 18: march kernel with 1, 2 */
-    double tensor_ms[5];       /* mean tensor launch time per candidate */
+    int tensor_code[8];        /* 1, 2, 4: row kernel with that many sites per lane; 16 + E: march
+                                  kernel with E = 1, 2, 4 (4: split storage); 24 + E: march kernel
+                                  loading the next plane early (E = 1, 2) */
+    double tensor_ms[8];       /* mean tensor launch time per candidate */
     int nmarch;                /* fused step kernel variants timed (128 <= N <= 256) */
     double march_ms[2];        /* mean fused step kernel time: plain, early loads */
     int kick_elems;            /* chosen kick shape */

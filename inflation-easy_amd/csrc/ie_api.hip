@@ -708,10 +708,14 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     for (int e = 1; e <= 4 && def > 0; e *= 2)
         if (e <= def) out->elems[out->ncand++] = e;
     if (r->F.sigw && def > 0) {
+        // row kernel shapes, then march kernel shapes (E = 4 with FP32 rows only)
+        // without and with the early loads of the next plane
         for (int e = 1; e <= 4; e *= 2)
             if (e <= def) out->tensor_code[out->ntensor++] = e;
+        for (int e = 1; e <= 4; e *= 2)
+            if (e <= def && (e < 4 || split)) out->tensor_code[out->ntensor++] = kTensorMarch | e;
         for (int e = 1; e <= 2; e *= 2)
-            if (e <= def) out->tensor_code[out->ntensor++] = kTensorMarch | e;
+            if (e <= def) out->tensor_code[out->ntensor++] = kTensorMarch | kTensorPf | e;
     }
     // fused steps have one kick shape and two kernel variants; unfused steps have
     // kick shapes
@@ -753,8 +757,8 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
         }
     }
     if ((st = sync(r)) != IE_OK) return st;
-    std::vector<double> ksum(3, 0.0), tsum(5, 0.0);
-    std::vector<int> kcnt(3, 0), tcnt(5, 0);
+    std::vector<double> ksum(3, 0.0), tsum(8, 0.0);
+    std::vector<int> kcnt(3, 0), tcnt(8, 0);
     for (int c = 0; c < rounds; ++c)
         for (int q = 0; q < K; ++q) {
             hipEvent_t *e = &ev[((size_t)c * K + q) * 4];

@@ -230,10 +230,12 @@ unsigned step_blocks(int N);
 int row_elems(int N);
 int shape_elems(int N, int want);
 // Tensor step of loop steps (N >= 64): Fields::tensor_e is a shape code, 0 or
-// 1/2/4 for the row kernel, kTensorMarch | E (E = 1, 2) for the march kernel.
+// 1/2/4 for the row kernel, kTensorMarch | E (E = 1, 2, 4) for the march kernel,
+// plus kTensorPf for its variant that loads the next plane early.
 constexpr int kTensorMarch = 16;
+constexpr int kTensorPf = 8;
 bool tensor_is_march(int code);
-int tensor_march_elems(int N, int code);
+int tensor_march_elems(int N, int code, int split);
 unsigned tensor_march_blocks(int N, int E);
 // Partial Friedmann sums of the current f, one triple per workgroup.
 hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, int loop,

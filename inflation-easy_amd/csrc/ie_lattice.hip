@@ -183,6 +183,18 @@ __device__ void finalize_scalars(DevBg *bg, int mode, int split)
     }
 }
 
+// std::pow of the scale factor in the step's coefficients.  The exponents of the
+// default rescale_s = 0 (-1, 0, 2) are evaluated exactly as a correctly rounded
+// pow gives them (1/x, 1, x*x: one IEEE operation each), skipping the FP64 pow
+// routine in the one-thread chain between two steps.
+__device__ __forceinline__ double scale_pow(double x, double y)
+{
+    if (y == 0.0) return 1.0;
+    if (y == 2.0) return x * x;
+    if (y == -1.0) return 1.0 / x;
+    return pow(x, y);
+}
+
 // Fixed-order total of the G partials and the implicit Friedmann half-kick
 // (evolve_derivs, src/evolution.cpp:153-167).  One workgroup.
 __device__ void friedmann_finish_body(DevBg *bg, const double *partials, unsigned G, size_t gs, double d_arg,
@@ -197,19 +209,19 @@ __device__ void friedmann_finish_body(DevBg *bg, const double *partials, unsigne
     bg->red[2] = tot[2];
     const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
     const double gsd = (double)gs;
-    const double d = loop ? bg->dt * pow(bg->astep, s - 1.0) : d_arg;
+    const double d = loop ? bg->dt * scale_pow(bg->astep, s - 1.0) : d_arg;
     const double Ge = 0.5 * tot[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
     const double Pm = tot[1] / gsd;
     const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
     bg->d = d;
     bg->ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
                (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a +
-                           d * d * sfev1 * pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) / d;
+                           d * d * sfev1 * scale_pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) / d;
     bg->ad = ad + 0.5 * d * bg->ad2;
     bg->friction = (2.0 + s) * bg->ad / a;
-    bg->potnorm = pow(a, 2.0 - 2.0 * s);
-    bg->laplnorm = pow(a, -2.0 * s) / (dx * dx);
-    bg->srcamp = 2.0 * pow(a, -2.0 * s);
+    bg->potnorm = scale_pow(a, 2.0 - 2.0 * s);
+    bg->laplnorm = scale_pow(a, -2.0 * s) / (dx * dx);
+    bg->srcamp = 2.0 * scale_pow(a, -2.0 * s);
     bg->vpmean = tot[2] / gsd;
 }
 
@@ -221,20 +233,21 @@ __device__ __forceinline__ StepScalars step_scalars(const DevBg *bg, double gsd)
 {
     StepScalars S;
     const double a = bg->a, ad = bg->ad, s = bg->rescale_s, dx = bg->dx;
-    const double d = bg->dt * pow(bg->astep, s - 1.0);
+    const double d = bg->dt * scale_pow(bg->astep, s - 1.0);
     const double Ge = 0.5 * bg->red[0] * ((1.0 / (a * dx)) * (1.0 / (a * dx))) / gsd;
     const double Pm = bg->red[1] / gsd;
     const double sfev1 = s + 1.0, sfev2 = -2.0 * s + 2.0;
     S.d = d;
     S.ad2 = (-2.0 * ad - 2.0 * a / d / sfev1 *
-             (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a + d * d * sfev1 * pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) /
+             (1.0 - sqrt(1.0 + 2.0 * d * sfev1 * ad / a +
+                         d * d * sfev1 * scale_pow(a, sfev2) * (2.0 * Ge / 3.0 + Pm)))) /
             d;
     S.ad_half = ad + 0.5 * d * S.ad2;
     S.ad_next = S.ad_half + 0.5 * d * S.ad2;
     S.friction = (2.0 + s) * S.ad_half / a;
-    S.potnorm = pow(a, 2.0 - 2.0 * s);
-    S.laplnorm = pow(a, -2.0 * s) / (dx * dx);
-    S.srcamp = 2.0 * pow(a, -2.0 * s);
+    S.potnorm = scale_pow(a, 2.0 - 2.0 * s);
+    S.laplnorm = scale_pow(a, -2.0 * s) / (dx * dx);
+    S.srcamp = 2.0 * scale_pow(a, -2.0 * s);
     S.vpmean = bg->red[2] / gsd;
     S.pibar_next = bg->pibar + d * (-S.friction * bg->pibar - S.potnorm * S.vpmean);
     S.phibar_next = bg->phibar + d * S.pibar_next;
@@ -889,10 +902,18 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
 // every h row comes from HBM about once (the j +- 1 rows streamed from L2 instead
 // were evicted between the two uses, doubling the reads).  hdot is updated in
 // place (pointwise), h goes to the other buffer.  Same arithmetic as the row
-// tensor kernel; no cross-site sums, so the shape (E sites per lane) is tunable.
+// tensor kernel; no cross-site sums, so the shape (E sites per lane, PF) is
+// tunable.  PF: the rows of plane i+2, the hdot rows and the halo rows of plane
+// i+1 and the segment-edge values of the plane-(i+1) rows are loaded before the
+// plane-i barrier, so they arrive while plane i is computed.
 constexpr int kTensorWaves = 8;
 
-__device__ __forceinline__ unsigned tmarch_ti(unsigned N) { return N >= 128 ? 16u : 8u; }
+// Planes per tile: with E = 4 one workgroup per CU (LDS of 143 KB) marching the
+// XCD's whole slab at N = 256; otherwise 16 (8 below N = 128).
+__device__ __host__ __forceinline__ unsigned tmarch_ti(unsigned N, int E)
+{
+    return (E == 4 && N >= 256) ? N / kXcd : (N >= 128 ? 16u : 8u);
+}
 
 // LDS tile of one plane: rows j0-1 .. j0+8 of the six h components and of f.
 template <typename T, int E>
@@ -901,7 +922,15 @@ struct TensorTile {
     T f[2][kTensorWaves + 2][64 * E];
 };
 
+// The two segment-edge values of a row (sites k0 - 1 and k0 + 64 E), wave-uniform.
 template <typename T, int E>
+__device__ __forceinline__ void row_edges(const T *__restrict__ row, unsigned k0, unsigned m, T &lo, T &hi)
+{
+    lo = first_lane(row[(k0 - 1) & m]);
+    hi = first_lane(row[(k0 + 64u * E) & m]);
+}
+
+template <typename T, int E, bool PF>
 __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, int fresh, unsigned blk,
                                                   TensorTile<T, E> &L)
 {
@@ -924,7 +953,7 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
     srcamp = uniform(srcamp);
     const double hx = 0.5 / bg->dx;
     // tile: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
-    const unsigned TI = tmarch_ti(N), jblocks = N / kTensorWaves, segs = N / (64u * E);
+    const unsigned TI = tmarch_ti(N, E), jblocks = N / kTensorWaves, segs = N / (64u * E);
     const unsigned xcd = blk % kXcd, lb = blk / kXcd;
     const unsigned jb = lb % jblocks, t2 = lb / jblocks;
     RowWalk wk{};
@@ -938,10 +967,14 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
     const bool halo = w == 0 || w == kTensorWaves - 1;
     const unsigned rh = ((w == 0 ? j0 - 1 : j0 + kTensorWaves) & m) << logN;
     const unsigned sh = w == 0 ? 0u : (unsigned)kTensorWaves + 1;
-    T fdn[E], fcur[E];
-    float hdn[6][E], hcur[6][E];
+    const unsigned P2 = 2 * logN;
+    T fdn[E], fcur[E], fup[E], fh[E];
+    float hdn[6][E], hcur[6][E], hup[6][E], hv[6][E], hh[6][E];
+    // PF: edge values of the plane-i rows (f and the six h), loaded a plane ahead
+    T felo = (T)0, fehi = (T)0;
+    float helo[6], hehi[6];
     {
-        const unsigned pm = ((i0 - 1) & m) << (2 * logN), p0 = (i0 & m) << (2 * logN);
+        const unsigned pm = ((i0 - 1) & m) << P2, p0 = (i0 & m) << P2;
         ld_row<T, E>(f + pm + rj, lane, fdn);
         ld_row<T, E>(f + p0 + rj, lane, fcur);
 #pragma unroll
@@ -949,10 +982,37 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
             ld_row<float, E>(F.h[par][c] + pm + rj, lane, hdn[c]);
             ld_row<float, E>(F.h[par][c] + p0 + rj, lane, hcur[c]);
         }
+        if constexpr (PF) {
+            const unsigned p1 = ((i0 + 1) & m) << P2;
+            ld_row<T, E>(f + p1 + rj, lane, fup);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                ld_row<float, E>(F.h[par][c] + p1 + rj, lane, hup[c]);
+                ld_row<float, E>(F.hd[c] + p0 + rj, lane, hv[c]);
+            }
+            if (halo) {
+                ld_row<T, E>(f + p0 + rh + wk.k0, lane, fh);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) ld_row<float, E>(F.h[par][c] + p0 + rh + wk.k0, lane, hh[c]);
+            }
+            if (segs > 1) {
+                row_edges<T, E>(f + p0 + row0, wk.k0, m, felo, fehi);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) row_edges<float, E>(F.h[par][c] + p0 + row0, wk.k0, m, helo[c], hehi[c]);
+            }
+        }
     }
     for (unsigned t = 0; t < TI; ++t) {
         const unsigned b = t & 1;
-        const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
+        const unsigned pi = ((i0 + t) & m) << P2, pu = ((i0 + t + 1) & m) << P2;
+        if constexpr (!PF) {
+            if (halo) {
+                ld_row<T, E>(f + pi + rh + wk.k0, lane, fh);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) ld_row<float, E>(F.h[par][c] + pi + rh + wk.k0, lane, hh[c]);
+            }
+            ld_row<T, E>(f + pu + rj, lane, fup);
+        }
         // plane i rows into LDS: the wave's own, and the halo rows
 #pragma unroll
         for (int e = 0; e < E; ++e) L.f[b][w + 1][lane * E + e] = fcur[e];
@@ -961,23 +1021,52 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
 #pragma unroll
             for (int e = 0; e < E; ++e) L.h[b][c][w + 1][lane * E + e] = hcur[c][e];
         if (halo) {
-            T fh[E];
-            ld_row<T, E>(f + pi + rh + wk.k0, lane, fh);
 #pragma unroll
             for (int e = 0; e < E; ++e) L.f[b][sh][lane * E + e] = fh[e];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                float hh[E];
-                ld_row<float, E>(F.h[par][c] + pi + rh + wk.k0, lane, hh);
+            for (int c = 0; c < 6; ++c)
 #pragma unroll
-                for (int e = 0; e < E; ++e) L.h[b][c][sh][lane * E + e] = hh[e];
+                for (int e = 0; e < E; ++e) L.h[b][c][sh][lane * E + e] = hh[c][e];
+        }
+        // PF: the next plane's loads go out before the barrier
+        T fnx[E], fhn[E];
+        float hnx[6][E], hvnx[6][E], hhn[6][E];
+        T felon = (T)0, fehin = (T)0;
+        float helon[6], hehin[6];
+        if constexpr (PF) {
+            if (t + 1 < TI) {
+                const unsigned pn = ((i0 + t + 2) & m) << P2;
+                ld_row<T, E>(f + pn + rj, lane, fnx);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    ld_row<float, E>(F.h[par][c] + pn + rj, lane, hnx[c]);
+                    ld_row<float, E>(F.hd[c] + pu + rj, lane, hvnx[c]);
+                }
+                if (halo) {
+                    ld_row<T, E>(f + pu + rh + wk.k0, lane, fhn);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) ld_row<float, E>(F.h[par][c] + pu + rh + wk.k0, lane, hhn[c]);
+                }
+                if (segs > 1) {
+                    row_edges<T, E>(f + pu + row0, wk.k0, m, felon, fehin);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        row_edges<float, E>(F.h[par][c] + pu + row0, wk.k0, m, helon[c], hehin[c]);
+                }
             }
         }
-        T fup[E];
-        ld_row<T, E>(f + pu + rj, lane, fup);
         __syncthreads();
         T fkp, fkm;
-        k_neighbours<T, E>(fcur, f + pi + row0, wk, N, lane, fkp, fkm);
+        if constexpr (PF) {
+            fkp = __shfl(fcur[0], (int)((lane + 1) & 63));
+            fkm = __shfl(fcur[E - 1], (int)((lane + 63) & 63));
+            if (segs > 1) {
+                fkm = lane == 0 ? felo : fkm;
+                fkp = lane == 63 ? fehi : fkp;
+            }
+        } else {
+            k_neighbours<T, E>(fcur, f + pi + row0, wk, N, lane, fkp, fkm);
+        }
         double gx[E], gy[E], gz[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -991,60 +1080,94 @@ __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, in
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             const float *__restrict__ h = F.h[par][c];
-            float hup[E], hv[E];
-            ld_row<float, E>(h + pu + rj, lane, hup);
-            ld_row<float, E>(F.hd[c] + pi + rj, lane, hv);
+            if constexpr (!PF) {
+                ld_row<float, E>(h + pu + rj, lane, hup[c]);
+                ld_row<float, E>(F.hd[c] + pi + rj, lane, hv[c]);
+            }
             float hkp, hkm;
-            k_neighbours<float, E>(hcur[c], h + pi + row0, wk, N, lane, hkp, hkm);
+            if constexpr (PF) {
+                hkp = __shfl(hcur[c][0], (int)((lane + 1) & 63));
+                hkm = __shfl(hcur[c][E - 1], (int)((lane + 63) & 63));
+                if (segs > 1) {
+                    hkm = lane == 0 ? helo[c] : hkm;
+                    hkp = lane == 63 ? hehi[c] : hkp;
+                }
+            } else {
+                k_neighbours<float, E>(hcur[c], h + pi + row0, wk, N, lane, hkp, hkm);
+            }
             float hdn_new[E], hn[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const float kp = e < E - 1 ? hcur[c][e + 1 < E ? e + 1 : e] : hkp;
                 const float km = e > 0 ? hcur[c][e > 0 ? e - 1 : 0] : hkm;
                 const float jp = L.h[b][c][w + 2][lane * E + e], jm = L.h[b][c][w][lane * E + e];
-                const float hl = kp + km + jp + jm + hup[e] + hdn[c][e] - 6.0f * hcur[c][e];
+                const float hl = kp + km + jp + jm + hup[c][e] + hdn[c][e] - 6.0f * hcur[c][e];
                 const double src = c == 0 ? gx[e] * gx[e]
                                  : c == 1 ? gy[e] * gy[e]
                                  : c == 2 ? gz[e] * gz[e]
                                  : c == 3 ? gx[e] * gy[e]
                                  : c == 4 ? gx[e] * gz[e]
                                           : gy[e] * gz[e];
-                const double rhs = d * (laplnorm * (double)hl - friction * (double)hv[e] + srcamp * src);
-                hdn_new[e] = hv[e] + (float)rhs;
+                const double rhs = d * (laplnorm * (double)hl - friction * (double)hv[c][e] + srcamp * src);
+                hdn_new[e] = hv[c][e] + (float)rhs;
                 hn[e] = hcur[c][e] + (float)(d * (double)hdn_new[e]);
             }
             st_row<float, E>(F.hd[c] + pi + rj, lane, hdn_new);
             st_row<float, E>(F.h[par ^ 1][c] + pi + rj, lane, hn);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 hdn[c][e] = hcur[c][e];
-                hcur[c][e] = hup[e];
+                hcur[c][e] = hup[c][e];
             }
-        }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             fdn[e] = fcur[e];
             fcur[e] = fup[e];
         }
+        if constexpr (PF) {
+            if (t + 1 < TI) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    fup[e] = fnx[e];
+                    fh[e] = fhn[e];
+                }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        hup[c][e] = hnx[c][e];
+                        hv[c][e] = hvnx[c][e];
+                        hh[c][e] = hhn[c][e];
+                    }
+                    helo[c] = helon[c];
+                    hehi[c] = hehin[c];
+                }
+                felo = felon;
+                fehi = fehin;
+            }
+        }
     }
 }
 
-template <typename T, int E>
+template <typename T, int E, bool PF>
 __global__ __launch_bounds__(64 * kTensorWaves)
 void k_tensor_march(Fields F, DevBg *bg, int fresh)
 {
     __shared__ TensorTile<T, E> L;
-    tensor_march_body<T, E>(F, bg, fresh, blockIdx.x, L);
+    tensor_march_body<T, E, PF>(F, bg, fresh, blockIdx.x, L);
 }
 
-template <typename T, int E>
+template <typename T, int E, bool PF>
 __global__ __launch_bounds__(64 * kTensorWaves)
 void kb_tensor_march(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
 {
     __shared__ TensorTile<T, E> L;
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    tensor_march_body<T, E>(R.F, R.bg, R.march, mp.y, L);
+    tensor_march_body<T, E, PF>(R.F, R.bg, R.march, mp.y, L);
 }
 
 // ------------------------------------------------- small-lattice site kernels
@@ -1415,10 +1538,18 @@ void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loo
 {
     if (mode == DRIFT) k_tensor<T, DRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, 0);
     else if (mode == KICKDRIFT && loop && row_elems(F.N) > 0 && tensor_is_march(F.tensor_e)) {
-        const int E = tensor_march_elems(F.N, F.tensor_e);
+        const int E = tensor_march_elems(F.N, F.tensor_e, sizeof(T) == 4);
         const unsigned Gt = tensor_march_blocks(F.N, E);
-        if (E == 1) k_tensor_march<T, 1><<<Gt, 64 * kTensorWaves, 0, s>>>(F, bg, fresh);
-        else k_tensor_march<T, 2><<<Gt, 64 * kTensorWaves, 0, s>>>(F, bg, fresh);
+        const unsigned nt = 64 * kTensorWaves;
+        if (F.tensor_e & kTensorPf) {
+            if (E == 1) k_tensor_march<T, 1, true><<<Gt, nt, 0, s>>>(F, bg, fresh);
+            else if (E == 2) k_tensor_march<T, 2, true><<<Gt, nt, 0, s>>>(F, bg, fresh);
+            else if constexpr (sizeof(T) == 4) k_tensor_march<T, 4, true><<<Gt, nt, 0, s>>>(F, bg, fresh);
+        } else {
+            if (E == 1) k_tensor_march<T, 1, false><<<Gt, nt, 0, s>>>(F, bg, fresh);
+            else if (E == 2) k_tensor_march<T, 2, false><<<Gt, nt, 0, s>>>(F, bg, fresh);
+            else if constexpr (sizeof(T) == 4) k_tensor_march<T, 4, false><<<Gt, nt, 0, s>>>(F, bg, fresh);
+        }
     } else if (row_elems(F.N) == 0) {
         if (mode == KICK) k_tensor<T, KICK><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, fresh);
         else k_tensor<T, KICKDRIFT><<<G, kBlock, 0, s>>>(F, bg, d_arg, loop, fresh);
@@ -1465,21 +1596,20 @@ int shape_elems(int N, int want)
 }
 
 // Tensor step shape codes: 0 default (row kernel), 1/2/4 row kernel with that many
-// sites per lane, kTensorMarch | E the march kernel with E (1 or 2) sites per lane
-// (six components' three-plane windows must stay in registers).
+// sites per lane, kTensorMarch | [kTensorPf] | E the march kernel with E sites per
+// lane (E = 4 with FP32 f only: the LDS tile of FP64 rows would not fit).
 bool tensor_is_march(int code) { return (code & kTensorMarch) != 0; }
 
-int tensor_march_elems(int N, int code)
+int tensor_march_elems(int N, int code, int split)
 {
     const int e = shape_elems(N, code & 7);
-    return e > 2 ? 2 : e;
+    return e > 2 && !split ? 2 : e;
 }
 
 unsigned tensor_march_blocks(int N, int E)
 {
     const unsigned segs = (unsigned)N / (64u * (unsigned)E);
-    const unsigned TI = N >= 128 ? 16u : 8u;
-    return (unsigned)(N / kTensorWaves) * segs * ((unsigned)N / TI);
+    return (unsigned)(N / kTensorWaves) * segs * ((unsigned)N / tmarch_ti((unsigned)N, E));
 }
 
 int reduce_blocks(size_t gs)
@@ -1682,7 +1812,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         const int kE = d[r].march ? row_elems(N) : shape_elems(N, runs[r].F.kick_e);
         const int tc = runs[r].F.tensor_e;
         const int tE = row_elems(N) == 0 ? 0
-                       : tensor_is_march(tc) ? kTensorMarch | tensor_march_elems(N, tc) : shape_elems(N, tc);
+                       : tensor_is_march(tc) ? kTensorMarch | (tc & kTensorPf) | tensor_march_elems(N, tc, split)
+                                             : shape_elems(N, tc);
         const int pf = d[r].march && runs[r].F.march_pf ? 1 : 0;
         const int key = pf * 1000000 + d[r].march * 100000 + row_elems(N) * 10000 + kE * 100 + tE;
         size_t g = 0;
@@ -1696,8 +1827,9 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         for (unsigned b = 0; b < d[r].G; ++b) rmaps[g].push_back(make_uint2((unsigned)r, b));
         if (runs[r].F.sigw) {
             // tensor step: march kernel (N >= 64) or site kernel
-            const unsigned Gt = (row_elems(N) > 0 && tensor_is_march(tc)) ? tensor_march_blocks(N, tensor_march_elems(N, tc))
-                                                                          : d[r].G;
+            const unsigned Gt = (row_elems(N) > 0 && tensor_is_march(tc))
+                                    ? tensor_march_blocks(N, tensor_march_elems(N, tc, split))
+                                    : d[r].G;
             for (unsigned b = 0; b < Gt; ++b) tmaps[g].push_back(make_uint2((unsigned)r, b));
         }
         const unsigned Gk = d[r].march ? d[r].Gm : d[r].G;
@@ -1803,10 +1935,19 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t
     }
     if (!G.ntblocks) return;
     if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
-    else if (G.tE == (kTensorMarch | 1))
-        kb_tensor_march<T, 1><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
-    else if (G.tE == (kTensorMarch | 2))
-        kb_tensor_march<T, 2><<<G.ntblocks, 64 * kTensorWaves, 0, s>>>(B->runs, G.tmap);
+    else if (tensor_is_march(G.tE)) {
+        const int E = G.tE & 7;
+        const unsigned nt = 64 * kTensorWaves;
+        if (G.tE & kTensorPf) {
+            if (E == 1) kb_tensor_march<T, 1, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+            else if (E == 2) kb_tensor_march<T, 2, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+            else if constexpr (sizeof(T) == 4) kb_tensor_march<T, 4, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+        } else {
+            if (E == 1) kb_tensor_march<T, 1, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+            else if (E == 2) kb_tensor_march<T, 2, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+            else if constexpr (sizeof(T) == 4) kb_tensor_march<T, 4, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+        }
+    }
     else if (G.tE == 1) kb_row_tensor<T, 1><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
     else if (G.tE == 2) kb_row_tensor<T, 2><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
     else kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);

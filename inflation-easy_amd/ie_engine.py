@@ -90,7 +90,7 @@ class Tuning(C.Structure):
     """Mirror of ``ie_tuning``."""
 
     _fields_ = [("ncand", C.c_int), ("elems", C.c_int * 3), ("kick_ms", C.c_double * 3),
-                ("ntensor", C.c_int), ("tensor_code", C.c_int * 5), ("tensor_ms", C.c_double * 5),
+                ("ntensor", C.c_int), ("tensor_code", C.c_int * 8), ("tensor_ms", C.c_double * 8),
                 ("nmarch", C.c_int), ("march_ms", C.c_double * 2),
                 ("kick_elems", C.c_int), ("tensor_code_chosen", C.c_int), ("march_variant", C.c_int),
                 ("steps", C.c_int64)]

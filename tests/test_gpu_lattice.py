@@ -234,24 +234,32 @@ def test_segmented_rows_match_oracle_n512(precision):
     run.close()
 
 
-def test_tuned_shapes_leave_results_unchanged():
-    """The tuner times real loop steps with the tensor step as the row kernel (1 and
-    2 sites per lane) and as the plane-marching kernel (1 and 2), and both variants
-    of the fused step kernel, N = 128; none changes an operation or a sum order, so
-    a run that changes shape between steps matches one that never does, bit for
-    bit."""
-    st, _, table = oracle_state("hilltop_n32", N=128, calculate_sigw=1)
+# tensor shape codes the tuner tries: row kernel 1/2/4 sites per lane, plane
+# march 16 + E (E = 4: one row per wave, FP32 rows), early-load march 24 + E
+TUNER_TENSOR_CODES = {128: [1, 2, 17, 18, 25, 26], 256: [1, 2, 4, 17, 18, 20, 25, 26]}
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_tuned_shapes_leave_results_unchanged(N):
+    """The tuner times real loop steps with the tensor step as the row kernel and
+    as the plane-marching kernels (with and without early loads), and both
+    variants of the fused step kernel; none changes an operation or a sum order,
+    so a run that changes shape between steps matches one that never does, bit
+    for bit."""
+    print(f"tuned shapes N={N}", flush=True)
+    st, _, table = oracle_state("hilltop_n32", N=N, calculate_sigw=1)
     ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
     rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=1)
     dt = st.cfg.dt
     for r in (ra, rb):
         r.evolve_fields(0.5 * dt)
     t = ra.tune(4)
-    assert t["tensor_codes"] == [1, 2, 17, 18] and t["steps"] == 16
-    assert t["tensor_code"] in (1, 2, 17, 18) and t["march_variant"] in (0, 1)
+    codes = TUNER_TENSOR_CODES[N]
+    assert t["tensor_codes"] == codes and t["steps"] == 4 * len(codes)
+    assert t["tensor_code"] in codes and t["march_variant"] in (0, 1)
     assert all(x > 0 for x in t["tensor_ms"] + t["march_ms"]) and len(t["march_ms"]) == 2
     ra.leapfrog_steps(12)
-    rb.leapfrog_steps(28)
+    rb.leapfrog_steps(12 + 4 * len(codes))
     assert ra.background()["steps"] == rb.background()["steps"]
     assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
     assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
