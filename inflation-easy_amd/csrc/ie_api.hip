@@ -260,6 +260,7 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     if ((st = dev_alloc(r, (void **)&r->scratch, r->gs * 8)) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, (void **)&r->partials, 2048 * 8 * 8)) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, (void **)&r->bg, sizeof(DevBg))) != IE_OK) return bail(st);
+    if ((st = dev_alloc(r, (void **)&r->F.rec, 2 * sizeof(DevBg))) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, (void **)&r->work, 4096 * 8)) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, (void **)&r->nmean, 256)) != IE_OK) return bail(st);
     if (hipHostMalloc((void **)&r->bg_host, sizeof(DevBg), hipHostMallocDefault) != hipSuccess)
@@ -616,7 +617,8 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
         // fused steps: one march launch per step computes the next step's sums
         if (max_steps > 0) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
         for (int n = 0; n < max_steps; ++n)
-            IE_HIP(r, launch_fused_step(r->F, split, r->P, r->bg, r->partials, r->stream));
+            IE_HIP(r, launch_fused_step(r->F, split, r->P, r->bg, r->partials, n, r->stream));
+        if (max_steps > 0) IE_HIP(r, launch_loop_end(r->F, split, r->bg, r->partials, max_steps, r->stream));
     } else {
         for (int n = 0; n < max_steps; ++n) {
             IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
@@ -644,14 +646,15 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
     for (int n = 0; n < nsteps; ++n) {
         hipEvent_t *e = &ev[(size_t)n * nev];
         if (fused) {
-            // the march kernel is the kick, the drift and the next step's sums;
-            // e[1], e[2] stay unrecorded (no reduce or Friedmann launch)
+            // the march kernel is the previous step's close, the kick, the drift
+            // and the next step's sums; e[1], e[2] stay unrecorded (no reduce or
+            // Friedmann launch), e[5] follows e[4] (no close launch)
             IE_HIP(r, hipEventRecord(e[0], r->stream));
-            IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, r->stream));
+            IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, n, r->stream));
             IE_HIP(r, hipEventRecord(e[3], r->stream));
-            if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, 1));
+            if (r->F.sigw)
+                IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, step_record(n)));
             IE_HIP(r, hipEventRecord(e[4], r->stream));
-            IE_HIP(r, launch_step_close(r->F, split, r->bg, r->partials, r->stream));
             IE_HIP(r, hipEventRecord(e[5], r->stream));
             continue;
         }
@@ -667,6 +670,7 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
         IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
         IE_HIP(r, hipEventRecord(e[5], r->stream));
     }
+    if (fused) IE_HIP(r, launch_loop_end(r->F, split, r->bg, r->partials, nsteps, r->stream));
     ie_status st = sync(r);
     if (st != IE_OK) return st;
     double red = 0, kd = 0, ten = 0, step = 0;
@@ -742,20 +746,22 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
         r->F.march_pf = fused ? c % nk : 0;
         r->F.tensor_e = nt ? out->tensor_code[c % nt] : 0;
         for (int q = 0; q < K; ++q) {
-            hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
+            const long long n = (long long)c * K + q;
+            hipEvent_t *e = &ev[(size_t)n * 4];
             if (!fused) IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[0], r->stream));
-            if (fused) IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, r->stream));
+            if (fused) IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, n, r->stream));
             else IE_HIP(r, launch_scalar_step(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[1], r->stream));
             if (r->F.sigw)
-                IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, fused ? 1 : 0));
+                IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream,
+                                             fused ? step_record(n) : 0));
             IE_HIP(r, hipEventRecord(e[2], r->stream));
-            if (fused) IE_HIP(r, launch_step_close(r->F, split, r->bg, r->partials, r->stream));
-            else IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
+            if (!fused) IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[3], r->stream));
         }
     }
+    if (fused) IE_HIP(r, launch_loop_end(r->F, split, r->bg, r->partials, (long long)rounds * K, r->stream));
     if ((st = sync(r)) != IE_OK) return st;
     std::vector<double> ksum(3, 0.0), tsum(8, 0.0);
     std::vector<int> kcnt(3, 0), tcnt(8, 0);
@@ -1574,8 +1580,10 @@ struct ie_batch {
     std::vector<ie_run *> runs;
     BatchDev *dev = nullptr;
     hipStream_t stream = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // captured runs of graph_steps loop steps, by the parity of their first step
+    // (the steps' kernels take the step records' slots from it)
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     int graph_steps = 0;
     std::string err;
 };
@@ -1629,33 +1637,46 @@ ie_status ie_batch_create(ie_run *const *runs, int nruns, ie_batch **out)
     return IE_OK;
 }
 
+static void drop_graphs(ie_batch *b)
+{
+    for (int p = 0; p < 2; ++p) {
+        if (b->exec[p]) (void)hipGraphExecDestroy(b->exec[p]);
+        if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+        b->exec[p] = nullptr;
+        b->graph[p] = nullptr;
+    }
+}
+
 ie_status ie_batch_steps(ie_batch *b, int nsteps, int graph_steps)
 {
     if (!b || nsteps < 0 || graph_steps < 0) return bfail(b, IE_ERR_ARG, "bad argument");
     IE_BHIP(b, batch_loop_begin(b->dev, b->stream));
     int done = 0;
     if (graph_steps > 0 && nsteps >= graph_steps) {
-        if (!b->exec || b->graph_steps != graph_steps) {
-            if (b->exec) (void)hipGraphExecDestroy(b->exec);
-            if (b->graph) (void)hipGraphDestroy(b->graph);
-            b->exec = nullptr;
-            b->graph = nullptr;
-            IE_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            for (int n = 0; n < graph_steps; ++n) {
-                hipError_t e = batch_step(b->dev, b->stream);
-                if (e != hipSuccess) {
-                    hipGraph_t g;
-                    (void)hipStreamEndCapture(b->stream, &g);
-                    return bfail(b, IE_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
-                }
-            }
-            IE_BHIP(b, hipStreamEndCapture(b->stream, &b->graph));
-            IE_BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        if (b->graph_steps != graph_steps) {
+            drop_graphs(b);
             b->graph_steps = graph_steps;
         }
-        for (; done + graph_steps <= nsteps; done += graph_steps) IE_BHIP(b, hipGraphLaunch(b->exec, b->stream));
+        for (; done + graph_steps <= nsteps; done += graph_steps) {
+            const int p = done & 1;
+            if (!b->exec[p]) {
+                IE_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                for (int n = 0; n < graph_steps; ++n) {
+                    hipError_t e = batch_step(b->dev, p + n, b->stream);
+                    if (e != hipSuccess) {
+                        hipGraph_t g;
+                        (void)hipStreamEndCapture(b->stream, &g);
+                        return bfail(b, IE_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+                    }
+                }
+                IE_BHIP(b, hipStreamEndCapture(b->stream, &b->graph[p]));
+                IE_BHIP(b, hipGraphInstantiate(&b->exec[p], b->graph[p], nullptr, nullptr, 0));
+            }
+            IE_BHIP(b, hipGraphLaunch(b->exec[p], b->stream));
+        }
     }
-    for (; done < nsteps; ++done) IE_BHIP(b, batch_step(b->dev, b->stream));
+    for (; done < nsteps; ++done) IE_BHIP(b, batch_step(b->dev, done, b->stream));
+    if (nsteps > 0) IE_BHIP(b, batch_loop_end(b->dev, nsteps, b->stream));
     IE_BHIP(b, hipStreamSynchronize(b->stream));
     for (ie_run *r : b->runs) {
         if (pull_bg(r) != IE_OK) return bfail(b, IE_ERR_HIP, r->err);
@@ -1668,8 +1689,7 @@ ie_status ie_batch_destroy(ie_batch *b)
 {
     if (!b) return IE_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    drop_graphs(b);
     batch_free(b->dev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -44,6 +44,7 @@ struct DevBg {
     int done, error, parity, pad0;
     unsigned int counter;
     unsigned int pad;
+    int marched, pad1;                          // step record of a fused loop: its step has run
 };
 
 // Potential parameters (src/potential.cpp:77-83 analytic hilltop; tabulated
@@ -69,6 +70,7 @@ struct Fields {
     int sigw;
     int kick_e, tensor_e;   // kick sites per lane / tensor shape code (0: default)
     int march_pf;           // fused step kernel variant: 1 loads the next plane early
+    DevBg *rec;             // the two step records of a fused loop (device)
 };
 
 // ---------------------------------------------------------------------- helpers
@@ -80,14 +82,16 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Deterministic sum of NV values over the workgroup; result valid in thread 0.
+// Deterministic sum of NV values over the first kBlock threads of the workgroup
+// (all of it but in the fused step kernel, whose larger workgroup passes zeros
+// from its other waves); result valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double *smem)
 {
 #pragma unroll
     for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
+    if (lane == 0 && wave < kWaves) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) smem[q * kWaves + wave] = v[q];
     }
@@ -128,7 +132,7 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblocks
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+    for (int b = threadIdx.x; b < nblocks && threadIdx.x < kBlock; b += kBlock) {
 #pragma unroll
         for (int q = 0; q < NV; ++q)
             acc[q] += partials[(size_t)b * NV + q];
@@ -245,22 +249,31 @@ hipError_t launch_reduce(const Fields &F, int split, const Pot &P, DevBg *bg, do
 // loop = 1: loop step d = dt astep^{s-1}, skipped once done.
 hipError_t launch_friedmann_finish(const Fields &F, DevBg *bg, const double *partials, double d_arg, int loop,
                                    hipStream_t s);
-// Kick / drift / fused kick+drift of f, fd; of h, hd (tensor step).
+// Kick / drift / fused kick+drift of f, fd; of h, hd (tensor step).  The tensor
+// step's scalars come from bg (fresh = 0) or, in a fused loop, from the step
+// record in slot fresh - 1 (step_record).
 hipError_t launch_scalar_step(const Fields &F, int split, const Pot &P, DevBg *bg, int mode, double d_arg, int loop,
                               hipStream_t s);
 hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, double d_arg, int loop, hipStream_t s,
                               int fresh = 0);
-// Fused loop steps for 128 <= N <= 256: the march kernel kicks, drifts and sums the
-// new f for the next step's Friedmann update; the close kernel applies the
-// step's scalar updates.  A loop starts with launch_march_begin (a reduce of the
-// current f into bg->red); then launch_fused_step per step (march, tensor step
-// with the same scalars when sigw, close).
+// Fused loop steps for 128 <= N <= 256: the march kernel closes the previous step
+// (the scalar updates from its partial sums) into the loop's step records
+// (Fields::rec), then kicks, drifts and sums the new f for the next step's
+// Friedmann update.  A loop starts with launch_march_begin (a reduce of the
+// current f into bg->red and the first record); then launch_fused_step per step
+// (march, and the tensor step with the same scalars when sigw); then
+// launch_loop_end, which closes the last step into bg.
 bool march_ok(int N);
 unsigned march_blocks(int N);
 hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s);
-hipError_t launch_march(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s);
-hipError_t launch_step_close(const Fields &F, int split, DevBg *bg, const double *partials, hipStream_t s);
-hipError_t launch_fused_step(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s);
+hipError_t launch_march(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, long long n,
+                        hipStream_t s);
+// launch_tensor_step's `fresh` for step n of a fused loop: that step's record
+int step_record(long long n);
+hipError_t launch_loop_end(const Fields &F, int split, DevBg *bg, const double *partials, long long steps,
+                           hipStream_t s);
+hipError_t launch_fused_step(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, long long n,
+                             hipStream_t s);
 // One thread: the scalar updates that close a kick, a drift or a loop step.
 hipError_t launch_step_finish(DevBg *bg, int split, int mode, double d_arg, int loop, hipStream_t s);
 // launch_reduce + launch_friedmann_finish.
@@ -294,7 +307,8 @@ struct BatchDev;
 hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out);
 void batch_free(BatchDev *B);
 hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s);
-hipError_t batch_step(const BatchDev *B, hipStream_t s);
+hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s);        // step n of the loop
+hipError_t batch_loop_end(const BatchDev *B, long long steps, hipStream_t s);  // after its steps
 
 // deltaN phase and field utilities (ie_deltaN.hip).  `work` is a device buffer of
 // at least 4096 doubles.

@@ -255,52 +255,89 @@ __device__ __forceinline__ StepScalars step_scalars(const DevBg *bg, double gsd)
 }
 #pragma clang fp contract(fast)
 
-// Closes a fused loop step: applies the step's scalar updates (the same values
-// its kernels used) and stores the fixed-order sums of the new f for the next
-// step.  One workgroup.
-__device__ void step_close_body(DevBg *bg, const double *partials, unsigned G, size_t gs, int split, double *smem)
+// Closes a fused loop step on a background record: applies the step's scalar
+// updates (b.nx, the values its kernels used), stores the fixed-order sums of the
+// new f for the next step and evaluates that step's scalars.
+__device__ void close_state(DevBg &b, const double (&tot)[3], double gsd, int split)
 {
-    if (bg->done) return;
-    double tot[3];
-    sum_partials<3>(partials, (int)G, tot, smem);
-    if (threadIdx.x != 0) return;
-    const StepScalars S = bg->nx;
-    bg->d = S.d;
-    bg->ad2 = S.ad2;
-    bg->friction = S.friction;
-    bg->potnorm = S.potnorm;
-    bg->laplnorm = S.laplnorm;
-    bg->srcamp = S.srcamp;
-    bg->vpmean = S.vpmean;
-    bg->ad = S.ad_next;
-    bg->t += S.d;
+    const StepScalars S = b.nx;
+    b.d = S.d;
+    b.ad2 = S.ad2;
+    b.friction = S.friction;
+    b.potnorm = S.potnorm;
+    b.laplnorm = S.laplnorm;
+    b.srcamp = S.srcamp;
+    b.vpmean = S.vpmean;
+    b.ad = S.ad_next;
+    b.t += S.d;
     if (split) {
-        bg->pibar = S.pibar_next;
-        bg->phibar = S.phibar_next;
+        b.pibar = S.pibar_next;
+        b.phibar = S.phibar_next;
     }
-    bg->a += S.d * bg->ad;
-    bg->parity ^= 1;
-    bg->steps += 1;
-    bg->astep = bg->a;
-    if (!(bg->a <= bg->af)) bg->done = 1;
-    bg->red[0] = tot[0];
-    bg->red[1] = tot[1];
-    bg->red[2] = tot[2];
-    bg->nx = step_scalars(bg, (double)gs);
+    b.a += S.d * b.ad;
+    b.parity ^= 1;
+    b.steps += 1;
+    b.astep = b.a;
+    if (!(b.a <= b.af)) b.done = 1;
+    b.red[0] = tot[0];
+    b.red[1] = tot[1];
+    b.red[2] = tot[2];
+    b.nx = step_scalars(&b, gsd);
+}
+
+// A fused loop keeps the background of its current step in two records
+// (Fields::rec).  Step n of a loop (counted by the host from the loop's start)
+// finds the previous step's record in slot n & 1: its kernel closes it with the
+// partial sums that step wrote to partials slot n & 1 (every workgroup alike, the
+// same bits), marches with the result, writes that record to slot (n + 1) & 1
+// (workgroup 0) and its own partials to slot (n + 1) & 1.  No launch writes what
+// it reads; bg itself is brought up to date once, after the loop.
+// Word-wise copy of a background record by the lanes of one wave.
+static_assert(sizeof(DevBg) % 8 == 0 && sizeof(DevBg) / 8 <= 64, "DevBg is copied by one wave");
+__device__ __forceinline__ void copy_bg_wave(DevBg *dst, const DevBg *src, unsigned lane)
+{
+    if (lane < sizeof(DevBg) / 8)
+        reinterpret_cast<unsigned long long *>(dst)[lane] = reinterpret_cast<const unsigned long long *>(src)[lane];
+}
+
+// After a fused loop: closes its last step (if one ran and the loop had not
+// ended) into bg, keeping the error flag that the step kernels raised in bg.
+// One workgroup.
+__device__ void loop_end_body(DevBg *bg, const DevBg *rec, unsigned sp, const double *partials, unsigned G, size_t gs,
+                              int split, double *smem, DevBg &sb)
+{
+    const DevBg *prev = rec + sp;
+    const bool close = prev->marched && !prev->done;
+    double tot[3] = {0.0, 0.0, 0.0};
+    if (close) sum_partials<3>(partials + (size_t)sp * G * 3, (int)G, tot, smem);
+    if (threadIdx.x < 64) copy_bg_wave(&sb, prev, threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (close) close_state(sb, tot, (double)gs, split);
+    const int err = bg->error;
+    const unsigned cnt = bg->counter;
+    *bg = sb;
+    bg->error = err;
+    bg->counter = cnt;
 }
 
 // Sums of the current f into bg->red at the start of a fused loop (from a reduce),
-// and the scalars of the loop's first step.
-__device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, size_t gs, double *smem)
+// the scalars of the loop's first step, and the loop's first record.
+__device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, size_t gs, DevBg *rec, double *smem)
 {
-    if (bg->done) return;
-    double tot[3];
-    sum_partials<3>(partials, (int)G, tot, smem);
+    if (!bg->done) {
+        double tot[3];
+        sum_partials<3>(partials, (int)G, tot, smem);
+        if (threadIdx.x == 0) {
+            bg->red[0] = tot[0];
+            bg->red[1] = tot[1];
+            bg->red[2] = tot[2];
+            bg->nx = step_scalars(bg, (double)gs);
+        }
+    }
     if (threadIdx.x != 0) return;
-    bg->red[0] = tot[0];
-    bg->red[1] = tot[1];
-    bg->red[2] = tot[2];
-    bg->nx = step_scalars(bg, (double)gs);
+    rec[0] = *bg;
+    rec[0].marched = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_friedmann_finish(DevBg *bg, const double *partials, unsigned G,
@@ -464,11 +501,12 @@ template <typename T, int E, int MODE>
 __device__ __forceinline__ void row_tensor_body(const Fields &F, DevBg *bg, int loop, int fresh, unsigned G,
                                                 unsigned blk)
 {
+    // fused loop steps: the step's record (its scalars, as the march kernel's)
+    if (fresh) bg = F.rec + (fresh - 1);
     if (loop && bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
-    // fused loop steps: the step's scalars from the background (as the march kernel)
     double d = bg->d, laplnorm = bg->laplnorm, friction = bg->friction, srcamp = bg->srcamp;
     if (fresh) {
         const StepScalars S = bg->nx;
@@ -552,11 +590,11 @@ __global__ __launch_bounds__(kBlock) void k_row_tensor(Fields F, DevBg *bg, int 
 
 template <typename T, int E>
 __global__ __launch_bounds__(kBlock) void kb_row_tensor(const RunDesc *__restrict__ runs,
-                                                         const uint2 *__restrict__ map)
+                                                         const uint2 *__restrict__ map, unsigned rslot)
 {
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.march, R.G, mp.y);
+    row_tensor_body<T, E, KICKDRIFT>(R.F, R.bg, 1, R.march ? 1 + (int)rslot : 0, R.G, mp.y);
 }
 
 
@@ -628,32 +666,41 @@ __device__ __forceinline__ double uniform(double x)
 // several calls gives the same bits as one call.
 // PF: the next plane's rows are loaded before the current plane's arithmetic
 // (more registers, loads in flight across the barrier); the same results.
+// A step launch first closes the previous step from its record in slot sp (its
+// first rows already in flight) and writes the result to the other slot.
 template <typename T, bool SPLIT, int E, bool SUMS = false, bool PF = false>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned blk,
-                                           T (*lds)[kMarchWaves][64 * E], double *smem)
+                                           T (*lds)[kMarchWaves][64 * E], double *smem, unsigned sp = 0)
 {
-    if (bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const unsigned w = threadIdx.x >> 6;
-    const int par = bg->parity;
+    const unsigned G = (N / kTileJ) * (N / march_ti(N));
+    int par;
+    bool pmarched = false;
+    __shared__ DevBg sb;        // step launches: this step's record
+    double tot[3] = {0.0, 0.0, 0.0};
+    if constexpr (SUMS) {
+        if (bg->done) return;
+        par = bg->parity;
+    } else {
+        // one round of loads: the previous record (flags, and whole into LDS) and
+        // its step's partials, in sum_partials' order
+        const DevBg *prev = F.rec + sp;
+        if (w == 0) copy_bg_wave(&sb, prev, lane);
+        if (threadIdx.x < kBlock)
+            for (unsigned b = threadIdx.x; b < G; b += kBlock)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) tot[q] += partials[((size_t)sp * G + b) * 3 + q];
+        if (prev->done) return;
+        pmarched = prev->marched != 0;
+        par = prev->parity ^ (pmarched ? 1 : 0);
+    }
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
     // fd is read from [par] and written to [par ^ 1]: the halo row and the extra
     // plane read fd rows that their owning workgroups update in the same launch
     const T *__restrict__ fd = (const T *)F.fd[par];
     T *__restrict__ fdout = (T *)F.fd[par ^ 1];
-    double d = 0.0, laplnorm = 0.0, friction = 0.0, potnorm = 0.0, vpmean = 0.0;
-    const double phibar = SPLIT ? bg->phibar : 0.0;
-    double phibar_next = phibar;
-    if constexpr (!SUMS) {
-        const StepScalars S = bg->nx;
-        d = uniform(S.d);
-        laplnorm = uniform(S.laplnorm);
-        friction = uniform(S.friction);
-        potnorm = uniform(S.potnorm);
-        vpmean = SPLIT ? uniform(S.vpmean) : 0.0;
-        phibar_next = SPLIT ? uniform(S.phibar_next) : 0.0;
-    }
     // tile of this workgroup: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
     const unsigned TI = march_ti(N), jblocks = N / kTileJ;
     const unsigned xcd = blk % kXcd, lb = blk / kXcd;
@@ -678,6 +725,32 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
         ld_row<T, E>(f + p0 + rjp, lane, njp);
         ld_row<T, E>(f + p0 + rjm, lane, njm);
         ld_row<T, E>(fd + p0 + rj, lane, nvin);
+    }
+    // the step's scalars: the loop's first record as it stands, later ones closed
+    // here with the previous step's partials (every workgroup, the same bits)
+    double d = 0.0, laplnorm = 0.0, friction = 0.0, potnorm = 0.0, vpmean = 0.0, phibar = 0.0, phibar_next = 0.0;
+    double *pout = partials;
+    if constexpr (SUMS) {
+        phibar = SPLIT ? bg->phibar : 0.0;
+        phibar_next = phibar;
+    } else {
+        if (pmarched) block_sum<3>(tot, smem);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (pmarched) close_state(sb, tot, (double)N * N * N, SPLIT);
+            sb.marched = 1;
+        }
+        __syncthreads();
+        if (blk == 0 && w == 0) copy_bg_wave(F.rec + (sp ^ 1), &sb, lane);
+        if (sb.done) return;
+        d = uniform(sb.nx.d);
+        laplnorm = uniform(sb.nx.laplnorm);
+        friction = uniform(sb.nx.friction);
+        potnorm = uniform(sb.nx.potnorm);
+        vpmean = SPLIT ? uniform(sb.nx.vpmean) : 0.0;
+        phibar = SPLIT ? uniform(sb.phibar) : 0.0;
+        phibar_next = SPLIT ? uniform(sb.nx.phibar_next) : 0.0;
+        pout = partials + (size_t)(sp ^ 1) * G * 3;
     }
     for (unsigned t = 0; t <= TI; ++t) {
         const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
@@ -777,7 +850,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
         for (int q = 0; q < 3; ++q) {
             double tot = 0.0;
             for (int ww = 0; ww < kMarchWaves; ++ww) tot += smem[q * kMarchWaves + ww];
-            partials[(size_t)blk * 3 + q] = tot;
+            pout[(size_t)blk * 3 + q] = tot;
         }
 }
 
@@ -785,38 +858,40 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
 // FP64 rows need more registers and run one.
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kMarchThreads) __attribute__((amdgpu_waves_per_eu(5)))
-void k_march_w5(Fields F, Pot P, DevBg *bg, double *partials)
+void k_march_w5(Fields F, Pot P, DevBg *bg, double *partials, unsigned sp)
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem);
+    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
-__global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg *bg, double *partials)
+__global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg *bg, double *partials,
+                                                         unsigned sp)
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem);
+    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
-__global__ __launch_bounds__(kMarchThreads) void k_march_pf(Fields F, Pot P, DevBg *bg, double *partials)
+__global__ __launch_bounds__(kMarchThreads) void k_march_pf(Fields F, Pot P, DevBg *bg, double *partials,
+                                                            unsigned sp)
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E, false, true>(F, P, bg, partials, blockIdx.x, lds, smem);
+    march_body<T, SPLIT, E, false, true>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kMarchThreads) void kb_march_pf(const RunDesc *__restrict__ runs,
-                                                              const uint2 *__restrict__ map)
+                                                              const uint2 *__restrict__ map, unsigned sp)
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E, false, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
+    march_body<T, SPLIT, E, false, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -838,49 +913,51 @@ __global__ __launch_bounds__(kMarchThreads) void kb_march_sums(const RunDesc *__
     march_body<T, SPLIT, E, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
 }
 
-__global__ __launch_bounds__(kBlock) void k_step_close(DevBg *bg, const double *partials, unsigned G,
-                                                        unsigned long long gs, int split)
+__global__ __launch_bounds__(kBlock) void k_loop_end(DevBg *bg, const DevBg *rec, unsigned sp, const double *partials,
+                                                      unsigned G, unsigned long long gs, int split)
 {
     __shared__ double smem[3 * kWaves];
-    step_close_body(bg, partials, G, (size_t)gs, split, smem);
+    __shared__ DevBg sb;
+    loop_end_body(bg, rec, sp, partials, G, (size_t)gs, split, smem, sb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sums_finish(DevBg *bg, const double *partials, unsigned G,
-                                                         unsigned long long gs)
+                                                         unsigned long long gs, DevBg *rec)
 {
     __shared__ double smem[3 * kWaves];
-    sums_finish_body(bg, partials, G, (size_t)gs, smem);
+    sums_finish_body(bg, partials, G, (size_t)gs, rec, smem);
 }
 
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kMarchThreads) __attribute__((amdgpu_waves_per_eu(5)))
-void kb_march_w5(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
+void kb_march_w5(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map, unsigned sp)
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
+    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
 __global__ __launch_bounds__(kMarchThreads) void kb_march(const RunDesc *__restrict__ runs,
-                                                           const uint2 *__restrict__ map)
+                                                           const uint2 *__restrict__ map, unsigned sp)
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
+    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
 }
 
-__global__ __launch_bounds__(kBlock) void kb_step_close(const RunDesc *__restrict__ runs)
+__global__ __launch_bounds__(kBlock) void kb_loop_end(const RunDesc *__restrict__ runs, unsigned sp)
 {
     __shared__ double smem[3 * kWaves];
+    __shared__ DevBg sb;
     const RunDesc &R = runs[blockIdx.x];
     if (!R.march) return;
     const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
-    step_close_body(R.bg, R.partials, R.Gm, gs, R.split, smem);
+    loop_end_body(R.bg, R.F.rec, sp, R.partials, R.Gm, gs, R.split, smem, sb);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restrict__ runs)
@@ -889,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
     const RunDesc &R = runs[blockIdx.x];
     if (!R.march) return;
     const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
-    sums_finish_body(R.bg, R.partials, R.Gm, gs, smem);
+    sums_finish_body(R.bg, R.partials, R.Gm, gs, R.F.rec, smem);
 }
 
 // ----------------------------------------------------- tensor march kernel
@@ -934,6 +1011,7 @@ template <typename T, int E, bool PF>
 __device__ __forceinline__ void tensor_march_body(const Fields &F, DevBg *bg, int fresh, unsigned blk,
                                                   TensorTile<T, E> &L)
 {
+    if (fresh) bg = F.rec + (fresh - 1);
     if (bg->done) return;
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const unsigned w = threadIdx.x >> 6;
@@ -1162,12 +1240,12 @@ void k_tensor_march(Fields F, DevBg *bg, int fresh)
 
 template <typename T, int E, bool PF>
 __global__ __launch_bounds__(64 * kTensorWaves)
-void kb_tensor_march(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
+void kb_tensor_march(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map, unsigned rslot)
 {
     __shared__ TensorTile<T, E> L;
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    tensor_march_body<T, E, PF>(R.F, R.bg, R.march, mp.y, L);
+    tensor_march_body<T, E, PF>(R.F, R.bg, R.march ? 1 + (int)rslot : 0, mp.y, L);
 }
 
 // ------------------------------------------------- small-lattice site kernels
@@ -1334,6 +1412,7 @@ template <typename T, int MODE>
 __device__ __forceinline__ void tensor_body(const Fields &F, DevBg *bg, double d_arg, int loop, int fresh, unsigned G,
                                             unsigned blk)
 {
+    if (fresh) bg = F.rec + (fresh - 1);
     if (loop && bg->done) return;
     const size_t gs = (size_t)F.N * F.N * F.N;
     const int par = bg->parity;
@@ -1392,11 +1471,12 @@ __global__ __launch_bounds__(kBlock) void k_tensor(Fields F, DevBg *bg, double d
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void kb_tensor(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map)
+__global__ __launch_bounds__(kBlock) void kb_tensor(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map,
+                                                     unsigned rslot)
 {
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.march, R.G, mp.y);
+    tensor_body<T, KICKDRIFT>(R.F, R.bg, 0.0, 1, R.march ? 1 + (int)rslot : 0, R.G, mp.y);
 }
 
 // ------------------------------------------------------- output utilities
@@ -1560,22 +1640,20 @@ void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loo
 }
 
 template <typename T, bool SPLIT>
-void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
+void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, unsigned sp, hipStream_t s)
 {
     const int E = row_elems(F.N);
     if (F.march_pf) {
-        if (E == 2) k_march_pf<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else k_march_pf<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        if (E == 2) k_march_pf<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        else k_march_pf<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
         return;
     }
     if constexpr (sizeof(T) == 4) {
-        if (E == 1) k_march_w5<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else k_march_w5<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        else k_march_w5<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
     } else {
-        if (E == 1) k_march<T, SPLIT, 1><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else if (E == 2) k_march<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else k_march<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        if (E == 2) k_march<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        else k_march<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
     }
 }
 
@@ -1688,30 +1766,35 @@ hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *b
         if (E == 2) k_march_sums<double, false, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
         else k_march_sums<double, false, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
     }
-    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G, (unsigned long long)F.N * F.N * F.N);
+    k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G, (unsigned long long)F.N * F.N * F.N, F.rec);
     return hipGetLastError();
 }
 
-hipError_t launch_march(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
+hipError_t launch_march(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, long long n,
+                        hipStream_t s)
 {
     const unsigned G = march_blocks(F.N);
-    if (split) march_dispatch<float, true>(F, P, bg, partials, G, s);
-    else march_dispatch<double, false>(F, P, bg, partials, G, s);
+    if (split) march_dispatch<float, true>(F, P, bg, partials, G, (unsigned)(n & 1), s);
+    else march_dispatch<double, false>(F, P, bg, partials, G, (unsigned)(n & 1), s);
     return hipGetLastError();
 }
 
-hipError_t launch_step_close(const Fields &F, int split, DevBg *bg, const double *partials, hipStream_t s)
+int step_record(long long n) { return 1 + (int)((n + 1) & 1); }
+
+hipError_t launch_loop_end(const Fields &F, int split, DevBg *bg, const double *partials, long long steps,
+                           hipStream_t s)
 {
     const size_t gs = (size_t)F.N * F.N * F.N;
-    k_step_close<<<1, kBlock, 0, s>>>(bg, partials, march_blocks(F.N), (unsigned long long)gs, split);
+    k_loop_end<<<1, kBlock, 0, s>>>(bg, F.rec, (unsigned)(steps & 1), partials, march_blocks(F.N),
+                                    (unsigned long long)gs, split);
     return hipGetLastError();
 }
 
-hipError_t launch_fused_step(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
+hipError_t launch_fused_step(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, long long n,
+                             hipStream_t s)
 {
-    hipError_t e = launch_march(F, split, P, bg, partials, s);
-    if (e == hipSuccess && F.sigw) e = launch_tensor_step(F, split, bg, KICKDRIFT, 0.0, 1, s, 1);
-    if (e == hipSuccess) e = launch_step_close(F, split, bg, partials, s);
+    hipError_t e = launch_march(F, split, P, bg, partials, n, s);
+    if (e == hipSuccess && F.sigw) e = launch_tensor_step(F, split, bg, KICKDRIFT, 0.0, 1, s, step_record(n));
     return e;
 }
 
@@ -1912,20 +1995,22 @@ static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream
 }
 
 template <typename T, bool SPLIT>
-static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t s)
+static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n, hipStream_t s)
 {
+    // records: the previous step's in slot n & 1, this step's in the other
+    const unsigned sp = (unsigned)(n & 1), rs = sp ^ 1u;
     if (G.march && G.pf) {
-        if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-        else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+        if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+        else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
     } else if (G.march) {
         if constexpr (sizeof(T) == 4) {
-            if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-            else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-            else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+            else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+            else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
         } else {
-            if (G.E == 1) kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-            else if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-            else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+            if (G.E == 1) kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+            else if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+            else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
         }
     } else switch (G.kE) {
     case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
@@ -1934,26 +2019,26 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, hipStream_t
     default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     }
     if (!G.ntblocks) return;
-    if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+    if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);
     else if (tensor_is_march(G.tE)) {
         const int E = G.tE & 7;
         const unsigned nt = 64 * kTensorWaves;
         if (G.tE & kTensorPf) {
-            if (E == 1) kb_tensor_march<T, 1, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
-            else if (E == 2) kb_tensor_march<T, 2, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
-            else if constexpr (sizeof(T) == 4) kb_tensor_march<T, 4, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+            if (E == 1) kb_tensor_march<T, 1, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap, rs);
+            else if (E == 2) kb_tensor_march<T, 2, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap, rs);
+            else if constexpr (sizeof(T) == 4) kb_tensor_march<T, 4, true><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap, rs);
         } else {
-            if (E == 1) kb_tensor_march<T, 1, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
-            else if (E == 2) kb_tensor_march<T, 2, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
-            else if constexpr (sizeof(T) == 4) kb_tensor_march<T, 4, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap);
+            if (E == 1) kb_tensor_march<T, 1, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap, rs);
+            else if (E == 2) kb_tensor_march<T, 2, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap, rs);
+            else if constexpr (sizeof(T) == 4) kb_tensor_march<T, 4, false><<<G.ntblocks, nt, 0, s>>>(B->runs, G.tmap, rs);
         }
     }
-    else if (G.tE == 1) kb_row_tensor<T, 1><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
-    else if (G.tE == 2) kb_row_tensor<T, 2><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
-    else kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap);
+    else if (G.tE == 1) kb_row_tensor<T, 1><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);
+    else if (G.tE == 2) kb_row_tensor<T, 2><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);
+    else kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);
 }
 
-hipError_t batch_step(const BatchDev *B, hipStream_t s)
+hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s)
 {
     if (B->has_plain) {
         for (const BatchGroup &G : B->groups) {
@@ -1964,11 +2049,16 @@ hipError_t batch_step(const BatchDev *B, hipStream_t s)
         kb_friedmann_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
     }
     for (const BatchGroup &G : B->groups) {
-        if (B->split) batch_group_kick<float, true>(B, G, s);
-        else batch_group_kick<double, false>(B, G, s);
+        if (B->split) batch_group_kick<float, true>(B, G, n, s);
+        else batch_group_kick<double, false>(B, G, n, s);
     }
     if (B->has_plain) kb_step_finish<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
-    if (B->has_march) kb_step_close<<<B->nruns, kBlock, 0, s>>>(B->runs);
+    return hipGetLastError();
+}
+
+hipError_t batch_loop_end(const BatchDev *B, long long steps, hipStream_t s)
+{
+    if (B->has_march) kb_loop_end<<<B->nruns, kBlock, 0, s>>>(B->runs, (unsigned)(steps & 1));
     return hipGetLastError();
 }
 

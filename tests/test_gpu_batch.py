@@ -54,13 +54,16 @@ def test_alone_equals_in_mix_bitwise():
         assert ba["a"] == bb["a"] and ba["ad"] == bb["ad"] and ba["steps"] == bb["steps"] == 60
 
 
-def test_graph_replay_equals_direct_launches():
+@pytest.mark.parametrize("nsteps,graph_steps", [(25, 25), (23, 7)])
+def test_graph_replay_equals_direct_launches(nsteps, graph_steps):
+    """(23, 7): three replays per call starting at even and odd steps of the loop
+    (graphs captured for both step parities), then two direct steps."""
     r1 = [new_run(32, 21), new_run(16, 22), new_run(64, 24), new_run(128, 23, sigw=1)]
     r2 = [new_run(32, 21), new_run(16, 22), new_run(64, 24), new_run(128, 23, sigw=1)]
     b1, b2 = E.Batch(r1), E.Batch(r2)
     for _ in range(4):   # several replays with host work in between
-        b1.steps(25, graph_steps=25)
-        b2.steps(25, graph_steps=0)
+        b1.steps(nsteps, graph_steps=graph_steps)
+        b2.steps(nsteps, graph_steps=0)
         for a, b in zip(r1, r2):
             assert np.array_equal(a.field(E.IE_FIELD_F), b.field(E.IE_FIELD_F))
             assert a.background()["a"] == b.background()["a"]
