@@ -385,10 +385,11 @@ def main() -> int:
     out["autotune"] = {k: v["tuning"] for k, v in res.items()}
     prof = head["profile"]
     if prof:
-        # dominant kernel: the fused step kernel (k_march: kick, drift and the next
-        # step's Friedmann sums, 128 <= N <= 256) or the row kick; algorithmic bytes
-        # per site: read dphi and dpi, write dpi and the next dphi (4 B each in split
-        # storage, 8 B in FP64); the sums read nothing more
+        # dominant kernel: the fused step kernel (k_march: the previous step's
+        # close, kick, drift and the next step's Friedmann sums, 128 <= N <= 256)
+        # or the row kick; algorithmic bytes per site: read dphi and dpi, write dpi
+        # and the next dphi (4 B each in split storage, 8 B in FP64); the sums and
+        # the close read nothing more per site
         per_site = 16 if head is res.get("split") else 32
         alg = per_site * sites
         ms = prof["kick_drift_ms"]
@@ -396,7 +397,7 @@ def main() -> int:
         fused = 128 <= N <= 256
         variant = head["tuning"].get("march_variant", 0)
         kname = ("k_march_pf<float,true,4>" if variant == 1 else "k_march_w5<float,true,4>") + \
-            " (fused kick + drift + next-step sums)"
+            " (previous step's close + kick + drift + next-step sums: the whole loop step)"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
                            "traffic": read_traffic(f"{'march' if fused else 'kick_drift'}_split_{N}"),

@@ -191,7 +191,7 @@ size_t ie_footprint_bytes(const ie_config *c)
     if (c->calculate_sigw) b += 18 * gs * 4;       // h ping-pong + hd
     b += gs * 8;                                   // scratch (downloads, deltaN)
     b += ie::fft_workspace_bytes(c->N);            // spectra
-    b += 2048 * 8 * 8 + sizeof(DevBg) + 3 * (size_t)std::max(c->tab_n, 0) * 8;
+    b += 2048 * 8 * 8 + 3 * sizeof(DevBg) + 3 * (size_t)std::max(c->tab_n, 0) * 8;
     return b;
 }
 
