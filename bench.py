@@ -318,7 +318,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--post-steps", type=int, default=200, help="post-inflation steps timed (0: skip)")
     ap.add_argument("--pzeta-check", type=int, default=1, help="split vs FP64 P_zeta of whole runs (0: skip)")
-    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "pzeta"], default=None,
+    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "pzeta", "serve"], default=None,
                     help="time a single case (for profiling runs)")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -343,6 +343,9 @@ def main() -> int:
         return 0
     if args.only == "pzeta":
         print(json.dumps({"pzeta_split_vs_fp64": pzeta_check(N)}), flush=True)
+        return 0
+    if args.only == "serve":
+        print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_rate)}), flush=True)
         return 0
     cases = {"split": (E.IE_SPLIT, 0), "fp64": (E.IE_FP64, 0), "sigw": (E.IE_SPLIT, 1)}
     todo = [args.only] if args.only else ["split", "fp64", "sigw"]

@@ -267,21 +267,24 @@ def test_tuned_shapes_leave_results_unchanged(N):
         assert np.array_equal(ra.tensor(comp), rb.tensor(comp)), comp
 
 
-@pytest.mark.parametrize("N", [64, 128])
-def test_loop_in_chunks_equals_one_call(N):
+@pytest.mark.parametrize("N,chunks", [(64, (10, 10, 10)), (128, (10, 10, 10)), (128, (7, 1, 6, 5)),
+                                      (256, (3, 4))])
+def test_loop_in_chunks_equals_one_call(N, chunks):
     """A loop split into several ie_leapfrog_steps calls gives the same bits as one
-    call: fused steps (N = 128) start each call with the sums of the current f in
-    a step's own partition and order."""
+    call: fused steps (N >= 128) start each call with the sums of the current f in
+    a step's own partition and order, and end it by closing the last step record
+    (odd and even chunk lengths end in either record slot)."""
     st, _, table = oracle_state("hilltop_n32", N=N, calculate_sigw=0)
     ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=0)
     rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=0)
     for r in (ra, rb):
         r.evolve_fields(0.5 * st.cfg.dt)
-    for _ in range(3):
-        ra.leapfrog_steps(10)
-    rb.leapfrog_steps(30)
+    for n in chunks:
+        ra.leapfrog_steps(n)
+    total = sum(chunks)
+    rb.leapfrog_steps(total)
     ba, bb = ra.background(), rb.background()
-    assert ba["steps"] == bb["steps"] == 30
+    assert ba["steps"] == bb["steps"] == total
     assert ba["a"] == bb["a"] and ba["ad"] == bb["ad"]
     assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
     assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
