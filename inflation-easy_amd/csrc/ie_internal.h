@@ -387,6 +387,25 @@ __device__ __forceinline__ RowWalk row_walk(unsigned N, unsigned G, unsigned blk
     return w;
 }
 
+// As row_walk, but the 4 waves of a workgroup take the same rows of 4 consecutive
+// planes, so the i +- 1 rows a wave reads are the rows its neighbours load at the
+// same time (kernels whose i +- 1 rows L2 would otherwise evict between uses).
+__device__ __forceinline__ RowWalk row_walk_stacked(unsigned N, unsigned G, unsigned blk, int E)
+{
+    const unsigned segs = N / (64u * (unsigned)E);
+    const unsigned R = (N * N * segs) / (4u * G);
+    const unsigned jblocks = N / R;
+    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
+    const unsigned t = lb / jblocks;
+    RowWalk w;
+    w.j0 = (lb % jblocks) * R;
+    w.k0 = (t % segs) * (64u * (unsigned)E);
+    w.i = xcd * (N / kXcd) + (t / segs) * 4u + (threadIdx.x >> 6);
+    w.R = R;
+    w.segs = segs;
+    return w;
+}
+
 // k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
 // `row`): in-lane, then across lanes, then across segments for N > 64 E.
 // The value of the first active lane, in scalar registers (the same in every lane).

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_post_kick_rows(const double *__restr
                                                             PostStep P)
 {
     const unsigned m = N - 1, lane = threadIdx.x & 63;
-    const RowWalk w = row_walk(N, gridDim.x, blockIdx.x, E);
+    const RowWalk w = row_walk_stacked(N, gridDim.x, blockIdx.x, E);
     const unsigned P0 = w.i << (2 * logN), Pp = ((w.i + 1) & m) << (2 * logN), Pm = ((w.i - 1) & m) << (2 * logN);
     double prev[E], cur[E], nxt[E];
     ld_row<double, E>(f + P0 + (((w.j0 - 1) & m) << logN) + w.k0, lane, prev);
@@ -190,12 +190,13 @@ __global__ __launch_bounds__(kBlock) void k_post_kick_rows(const double *__restr
 }
 
 // Rows of Phi in planes i-1, i, i+1 (index 0, 1, 2) at j-1, j, j+1 (0, 1, 2)
-// slide along j in registers; the kicked Phi' rows of plane i likewise.
+// slide along j in registers; the kicked Phi' rows of plane i and the rows of
+// the six h_ij of plane i likewise.
 template <int E>
 __global__ __launch_bounds__(kBlock) void k_post_tensor_rows(PostTensorArgs T, unsigned N, unsigned logN, PostStep P)
 {
     const unsigned m = N - 1, lane = threadIdx.x & 63;
-    const RowWalk w = row_walk(N, gridDim.x, blockIdx.x, E);
+    const RowWalk w = row_walk_stacked(N, gridDim.x, blockIdx.x, E);
     const unsigned pl[3] = {((w.i - 1) & m) << (2 * logN), w.i << (2 * logN), ((w.i + 1) & m) << (2 * logN)};
     const double *f = T.f, *fd = T.fd;
     double F[3][3][E], D[3][E];
@@ -206,6 +207,12 @@ __global__ __launch_bounds__(kBlock) void k_post_tensor_rows(PostTensorArgs T, u
     }
     ld_row<double, E>(fd + pl[1] + (((w.j0 - 1) & m) << logN) + w.k0, lane, D[0]);
     ld_row<double, E>(fd + pl[1] + (w.j0 << logN) + w.k0, lane, D[1]);
+    float HJ[6][2][E];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        ld_row<float, E>(T.h[c] + pl[1] + (((w.j0 - 1) & m) << logN) + w.k0, lane, HJ[c][0]);
+        ld_row<float, E>(T.h[c] + pl[1] + (w.j0 << logN) + w.k0, lane, HJ[c][1]);
+    }
     for (unsigned r = 0; r < w.R; ++r) {
         const unsigned j = w.j0 + r, rj = j << logN, rjp = ((j + 1) & m) << logN, rjm = ((j - 1) & m) << logN;
 #pragma unroll
@@ -256,47 +263,61 @@ __global__ __launch_bounds__(kBlock) void k_post_tensor_rows(PostTensorArgs T, u
         st_row<double, E>(T.f_out + pl[1] + rj + w.k0, lane, fnew);
         (void)yp_km;
         (void)ym_kp;
-        // the six components one after the other, the next one's rows loaded
-        // while the current one is computed
-        float hc[E], hjp[E], hjm[E], hip[E], him[E], hv[E];
-        ld_row<float, E>(T.h[0] + pl[1] + rj + w.k0, lane, hc);
-        ld_row<float, E>(T.h[0] + pl[1] + rjp + w.k0, lane, hjp);
-        ld_row<float, E>(T.h[0] + pl[1] + rjm + w.k0, lane, hjm);
-        ld_row<float, E>(T.h[0] + pl[2] + rj + w.k0, lane, hip);
-        ld_row<float, E>(T.h[0] + pl[0] + rj + w.k0, lane, him);
-        ld_row<float, E>(T.hd[0] + pl[1] + rj + w.k0, lane, hv);
+        // the six components one after the other, each with its j-1 and j rows
+        // held from the previous row (HJ[0] is the current component's: the
+        // windows rotate by one per component, so the loop indexes no register
+        // array); the next component's rows are loaded while one is computed
+        float nj[E], ni[E], nm[E], nv[E];
+        ld_row<float, E>(T.h[0] + pl[1] + rjp + w.k0, lane, nj);
+        ld_row<float, E>(T.h[0] + pl[2] + rj + w.k0, lane, ni);
+        ld_row<float, E>(T.h[0] + pl[0] + rj + w.k0, lane, nm);
+        ld_row<float, E>(T.hd[0] + pl[1] + rj + w.k0, lane, nv);
 #pragma unroll 1
         for (int c = 0; c < 6; ++c) {
+            float hjp[E], hip[E], him[E], hv[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                hjp[e] = nj[e];
+                hip[e] = ni[e];
+                him[e] = nm[e];
+                hv[e] = nv[e];
+            }
             const int cn = c < 5 ? c + 1 : 5;
-            float nc[E], njp[E], njm[E], nip[E], nim[E], nv[E];
-            ld_row<float, E>(T.h[cn] + pl[1] + rj + w.k0, lane, nc);
-            ld_row<float, E>(T.h[cn] + pl[1] + rjp + w.k0, lane, njp);
-            ld_row<float, E>(T.h[cn] + pl[1] + rjm + w.k0, lane, njm);
-            ld_row<float, E>(T.h[cn] + pl[2] + rj + w.k0, lane, nip);
-            ld_row<float, E>(T.h[cn] + pl[0] + rj + w.k0, lane, nim);
+            ld_row<float, E>(T.h[cn] + pl[1] + rjp + w.k0, lane, nj);
+            ld_row<float, E>(T.h[cn] + pl[2] + rj + w.k0, lane, ni);
+            ld_row<float, E>(T.h[cn] + pl[0] + rj + w.k0, lane, nm);
             ld_row<float, E>(T.hd[cn] + pl[1] + rj + w.k0, lane, nv);
             float hkp, hkm;
-            k_neighbours<float, E>(hc, T.h[c] + pl[1] + rj, w, N, lane, hkp, hkm);
+            k_neighbours<float, E>(HJ[0][1], T.h[c] + pl[1] + rj, w, N, lane, hkp, hkm);
             float hdn[E], hn[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const float hl = kplus<float, E>(hc, e, hkp) + kminus<float, E>(hc, e, hkm) + hjp[e] + hjm[e] +
-                                 hip[e] + him[e] - 6.0f * hc[e];
+                const float hl = kplus<float, E>(HJ[0][1], e, hkp) + kminus<float, E>(HJ[0][1], e, hkm) + hjp[e] +
+                                 HJ[0][0][e] + hip[e] + him[e] - 6.0f * HJ[0][1][e];
                 double Sc = S[e][0];
 #pragma unroll
                 for (int q = 1; q < 6; ++q) Sc = c == q ? S[e][q] : Sc;
-                post_tensor_update(hl, hc[e], hv[e], Sc, P, &hdn[e], &hn[e]);
+                post_tensor_update(hl, HJ[0][1][e], hv[e], Sc, P, &hdn[e], &hn[e]);
             }
             st_row<float, E>(T.hd[c] + pl[1] + rj + w.k0, lane, hdn);
             st_row<float, E>(T.h_out[c] + pl[1] + rj + w.k0, lane, hn);
+            float keep[2][E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                hc[e] = nc[e];
-                hjp[e] = njp[e];
-                hjm[e] = njm[e];
-                hip[e] = nip[e];
-                him[e] = nim[e];
-                hv[e] = nv[e];
+                keep[0][e] = HJ[0][1][e];
+                keep[1][e] = hjp[e];
+            }
+#pragma unroll
+            for (int q = 0; q < 5; ++q)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    HJ[q][0][e] = HJ[q + 1][0][e];
+                    HJ[q][1][e] = HJ[q + 1][1][e];
+                }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                HJ[5][0][e] = keep[0][e];
+                HJ[5][1][e] = keep[1][e];
             }
         }
 #pragma unroll
