@@ -148,7 +148,9 @@ ie_status ie_evolve_derivs(ie_run *run, double d);
 ie_status ie_evolve_fields(ie_run *run, double d);
 /* Up to max_steps iterations of the leapfrog loop body of run_evolution_loop
  * (src/evolution.cpp:937-945) with the loop test a <= af evaluated on the device:
- * no host synchronisation between steps.  *out receives the background after. */
+ * no host synchronisation between steps.  Steps are enqueued 1024 at a time and
+ * the host reads the loop test between those chunks, so a call with a large
+ * max_steps returns soon after af is reached.  *out receives the background after. */
 ie_status ie_leapfrog_steps(ie_run *run, int max_steps, ie_background *out);
 /* The same loop as ie_leapfrog_steps, with HIP events recorded on the run's
  * stream around every launch; *out receives the mean device time per launch of

@@ -191,7 +191,7 @@ size_t ie_footprint_bytes(const ie_config *c)
     if (c->calculate_sigw) b += 18 * gs * 4;       // h ping-pong + hd
     b += gs * 8;                                   // scratch (downloads, deltaN)
     b += ie::fft_workspace_bytes(c->N);            // spectra
-    b += 2048 * 8 * 8 + 3 * sizeof(DevBg) + 3 * (size_t)std::max(c->tab_n, 0) * 8;
+    b += 2048 * 8 * 8 + 3 * sizeof(DevBg) + 7 * (size_t)std::max(c->tab_n, 0) * 8;
     return b;
 }
 
@@ -288,15 +288,32 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->P.n = 0;
     if (c.numerical_potential) {
         const size_t n = (size_t)c.tab_n;
-        if ((st = dev_alloc(r, (void **)&r->table, 3 * n * 8)) != IE_OK) return bail(st);
-        if (hipMemcpy(r->table, c.tab_x, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(r->table + n, c.tab_v, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(r->table + 2 * n, c.tab_dv, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        // x, V, V' as given; then the lattice kernels' form: V/B^2, V'/B^2 and
+        // the slopes of the intervals [l-1, l]
+        std::vector<double> t(7 * n, 0.0);
+        for (size_t l = 0; l < n; ++l) {
+            t[l] = c.tab_x[l];
+            t[n + l] = c.tab_v[l];
+            t[2 * n + l] = c.tab_dv[l];
+            t[3 * n + l] = c.tab_v[l] * r->P.invB2;
+            t[4 * n + l] = c.tab_dv[l] * r->P.invB2;
+            if (l > 0) {
+                const double dx = c.tab_x[l - 1] - c.tab_x[l];
+                t[5 * n + l] = (c.tab_v[l - 1] - c.tab_v[l]) / dx * r->P.invB2;
+                t[6 * n + l] = (c.tab_dv[l - 1] - c.tab_dv[l]) / dx * r->P.invB2;
+            }
+        }
+        if ((st = dev_alloc(r, (void **)&r->table, 7 * n * 8)) != IE_OK) return bail(st);
+        if (hipMemcpy(r->table, t.data(), 7 * n * 8, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(nullptr, IE_ERR_HIP, "table upload failed"));
         r->P.n = c.tab_n;
         r->P.x = r->table;
         r->P.v = r->table + n;
         r->P.dv = r->table + 2 * n;
+        r->P.vB = r->table + 3 * n;
+        r->P.dvB = r->table + 4 * n;
+        r->P.svB = r->table + 5 * n;
+        r->P.sdvB = r->table + 6 * n;
     }
     DevBg &b = *r->bg_host;
     memset(&b, 0, sizeof(b));
@@ -611,19 +628,36 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
         if (st != IE_OK) return st;
         max_steps -= (int)std::min<int64_t>(tun.steps, max_steps);
     }
-    k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
-    IE_HIP(r, hipGetLastError());
-    if (march_ok(r->N)) {
-        // fused steps: one march launch per step computes the next step's sums
-        if (max_steps > 0) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
-        for (int n = 0; n < max_steps; ++n)
-            IE_HIP(r, launch_fused_step(r->F, split, r->P, r->bg, r->partials, n, r->stream));
-        if (max_steps > 0) IE_HIP(r, launch_loop_end(r->F, split, r->bg, r->partials, max_steps, r->stream));
-    } else {
-        for (int n = 0; n < max_steps; ++n) {
-            IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
-            IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+    // the steps go out in chunks with the loop test read back in between, so a
+    // call with a large max_steps returns soon after a > af (the step kernels of a
+    // finished loop return at once, but each launch still costs its dispatch);
+    // a loop in chunks gives the same bits as one loop
+    constexpr int kChunk = 1024;
+    for (int done = 0; done < max_steps;) {
+        const int n_chunk = std::min(kChunk, max_steps - done);
+        k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+        IE_HIP(r, hipGetLastError());
+        if (march_ok(r->N)) {
+            // fused steps: the march launch of a step closes the previous one and
+            // computes the next step's sums
+            IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
+            for (int n = 0; n < n_chunk; ++n)
+                IE_HIP(r, launch_fused_step(r->F, split, r->P, r->bg, r->partials, n, r->stream));
+            IE_HIP(r, launch_loop_end(r->F, split, r->bg, r->partials, n_chunk, r->stream));
+        } else {
+            for (int n = 0; n < n_chunk; ++n) {
+                IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
+                IE_HIP(r, launch_kick_drift(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+            }
         }
+        done += n_chunk;
+        ie_status st = check_kernel_error(r);
+        if (st != IE_OK) return st;
+        if (r->bg_host->done) break;
+    }
+    if (max_steps == 0) {
+        k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+        IE_HIP(r, hipGetLastError());
     }
     ie_status st = check_kernel_error(r);
     if (st != IE_OK) return st;

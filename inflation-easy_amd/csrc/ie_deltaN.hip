@@ -6,9 +6,11 @@
 // short of the phiref surface (src/evolution.cpp:1004-1037).  There is no
 // neighbour coupling, so k_deltaN_steps keeps (phi, pi, deltaN) of a site in
 // registers for all steps: one read and one write of the state per launch instead
-// of two lattice sweeps per step.  A site that reaches the surface is frozen: its
-// phi and deltaN never change again in the reference either; its pi, which the
-// reference keeps integrating but never reads again, is left as it was.
+// of two lattice sweeps per step.  A site that reaches the surface stops: its phi
+// and deltaN never change again in the reference either, while its pi is still
+// kicked every step with the ratio V'/V frozen at its final phi, as the reference
+// does (evolve_derivsN runs over every site); the post-inflation phase starts
+// from those velocities.
 //
 // The site arithmetic keeps the reference's operation order with FP contraction
 // off, so the map equals the reference's bit for bit given the same input state.

@@ -56,6 +56,10 @@ struct Pot {
     const double *x, *v, *dv;   // device table, descending x
     // constants of the lattice kernels: V0/B^2, V0 (1-ns)/4 /B^2, V0 (1-ns)/2 /B^2, 1/B^2
     double hV, hV4, hV2, invB2;
+    // the table in the lattice kernels' form: V/B^2, V'/B^2 and the slopes of
+    // their intervals ([l-1, l] at index l), so an interpolation is one
+    // multiply-add instead of a division
+    const double *vB, *dvB, *svB, *sdvB;
 };
 
 // Device pointers of one lattice's fields.  Storage of f, fd is double (FP64) or
@@ -144,6 +148,17 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblocks
 
 // ----------------------------------------------------------------- potential
 
+// A tabulated potential of up to kPotLds rows is staged by the lattice kernels in
+// their workgroup's dynamic LDS (5 n doubles: x, V/B^2, V'/B^2 and their
+// slopes), so the bracket search and interpolation of every site read LDS
+// instead of L2.  Every launch of a staging kernel passes pot_lds_bytes of its
+// potential (the largest of its runs' in a batch).
+constexpr int kPotLds = 1536;
+inline size_t pot_lds_bytes(const Pot &P)
+{
+    return P.numerical && P.n <= kPotLds ? (size_t)5 * P.n * sizeof(double) : 0;
+}
+
 __device__ __forceinline__ int pot_bracket(const Pot &P, double v, int *err)
 {
     if (v > P.x[0] || v < P.x[P.n - 1]) {
@@ -219,6 +234,130 @@ __device__ __forceinline__ double pot_dV_fast(const Pot &P, double v, int *err)
         return pot_interp(P, v, l, P.dv) * P.invB2;
     }
     return -P.hV2 * v;
+}
+
+// Bracket search on a table x (descending) of n rows: the unique l with
+// x[l-1] >= v > x[l] (n - 1 at v = x[n-1]); *err set when v leaves the table.
+// tab_bisect bisects; tab_walk walks from a hint l (the bracket of a nearby
+// value, as the previous site of the same lane) and ends at the same l, in a
+// step or two instead of log2 n dependent reads.
+__device__ __forceinline__ int tab_bisect(const double *x, int n, double v, int *err)
+{
+    if (v > x[0] || v < x[n - 1]) {
+        *err = 1;
+        return 1;
+    }
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (x[mid] >= v) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+__device__ __forceinline__ int tab_walk(const double *x, int n, double xfirst, double xlast, double v, int l,
+                                        int *err)
+{
+    if (v > xfirst || v < xlast) {
+        *err = 1;
+        return 1;
+    }
+    l = l < 1 ? 1 : (l > n - 1 ? n - 1 : l);
+    // the hint usually brackets v already; the walk runs only for the lanes (of
+    // a wave with any) whose value left it
+    const bool ok = (l == n - 1 || x[l] < v) && x[l - 1] >= v;
+    if (!__all(ok)) {
+        while (l < n - 1 && x[l] >= v) ++l;
+        while (l > 1 && x[l - 1] < v) --l;
+    }
+    return l;
+}
+
+__device__ __forceinline__ double tab_interp(const double *x, const double *y, double v, int l)
+{
+    const double x0 = x[l - 1], x1 = x[l], y0 = y[l - 1], y1 = y[l];
+    return y1 + (v - x1) * (y0 - y1) / (x0 - x1);
+}
+
+// The potential as the lattice kernels evaluate it: the division by B^2 folded
+// into constants and the interpolation through precomputed interval slopes
+// (y_l + (v - x_l) s_l; a rounding or two from the reference's
+// y_l + (v - x_l)(y_{l-1} - y_l)/(x_{l-1} - x_l), below the lattice sums' own
+// order dependence).  A table of up to kPotLds rows is read from the workgroup's
+// copy in dynamic LDS (stage_pot), whose pointers the methods take straight from
+// the LDS symbol so that the reads are LDS reads; a longer table from global
+// memory.  Walks start from the lane's previous bracket.
+struct PotView {
+    Pot P;
+    bool lds;
+    double xfirst, xlast;   // the table's ends (numerical potentials)
+
+    __device__ __forceinline__ int bracket(double v, int l, bool walk, int *err) const
+    {
+        extern __shared__ double pot_lds[];
+        if (lds) return walk ? tab_walk(pot_lds, P.n, xfirst, xlast, v, l, err) : tab_bisect(pot_lds, P.n, v, err);
+        return walk ? tab_walk(P.x, P.n, xfirst, xlast, v, l, err) : tab_bisect(P.x, P.n, v, err);
+    }
+    // V/B^2 (q = 1) or V'/B^2 (q = 2) at v in bracket l
+    __device__ __forceinline__ double interp(double v, int l, int q) const
+    {
+        extern __shared__ double pot_lds[];
+        if (lds) {
+            const double *y = pot_lds + q * P.n, *sl = pot_lds + (q + 2) * P.n;
+            return y[l] + (v - pot_lds[l]) * sl[l];
+        }
+        const double *y = q == 1 ? P.vB : P.dvB, *sl = q == 1 ? P.svB : P.sdvB;
+        return y[l] + (v - P.x[l]) * sl[l];
+    }
+    __device__ __forceinline__ double dV(double v, int &hint, bool walk, int *err) const
+    {
+        if (!P.numerical) return -P.hV2 * v;
+        hint = bracket(v, hint, walk, err);
+        return interp(v, hint, 2);
+    }
+    __device__ __forceinline__ void eval(double v, double *V, double *dV, int &hint, bool walk, int *err) const
+    {
+        if (!P.numerical) {
+            *V = P.hV - P.hV4 * (v * v);
+            *dV = -P.hV2 * v;
+            return;
+        }
+        hint = bracket(v, hint, walk, err);
+        *V = interp(v, hint, 1);
+        *dV = interp(v, hint, 2);
+    }
+    // a first hint for the lane's walks
+    __device__ __forceinline__ int start(double v) const
+    {
+        int unused = 0;
+        return P.numerical ? bracket(v, 0, false, &unused) : 0;
+    }
+};
+
+// Copies a table of up to kPotLds rows (x, V/B^2, V'/B^2, slopes) into the workgroup's dynamic
+// LDS (a uniform branch: every thread of the workgroup reaches the barrier or
+// none does).
+__device__ __forceinline__ PotView stage_pot(const Pot &P)
+{
+    extern __shared__ double pot_lds[];
+    PotView Q{P, false, 0.0, 0.0};
+    if (P.numerical) {
+        Q.xfirst = P.x[0];
+        Q.xlast = P.x[P.n - 1];
+    }
+    if (P.numerical && P.n <= kPotLds) {
+        for (int e = threadIdx.x; e < P.n; e += blockDim.x) {
+            pot_lds[e] = P.x[e];
+            pot_lds[P.n + e] = P.vB[e];
+            pot_lds[2 * P.n + e] = P.dvB[e];
+            pot_lds[3 * P.n + e] = P.svB[e];
+            pot_lds[4 * P.n + e] = P.sdvB[e];
+        }
+        __syncthreads();
+        Q.lds = true;
+    }
+    return Q;
 }
 
 // ---------------------------------------------------------------- launchers

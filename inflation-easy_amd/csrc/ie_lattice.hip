@@ -385,10 +385,11 @@ __global__ void kb_loop_begin(const RunDesc *__restrict__ runs, int nruns)
 // ------------------------------------------------------------ row kernels
 
 template <typename T, bool SPLIT, int E>
-__device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop,
+__device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P0, DevBg *bg, double *partials, int loop,
                                                 unsigned G, unsigned blk, double *smem)
 {
     if (loop && bg->done) return;
+    const PotView Pv = stage_pot(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
     const double phibar = SPLIT ? bg->phibar : 0.0;
@@ -400,7 +401,9 @@ __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P, D
     ld_row<T, E>(plane + (((w.j0 - 1) & m) << logN), lane, prev);
     ld_row<T, E>(plane + (w.j0 << logN), lane, cur);
     double acc[3] = {0.0, 0.0, 0.0};
-    int err = 0;
+    int err = 0, hint[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) hint[e] = Pv.start(phibar + (double)cur[e]);
     for (unsigned r = 0; r < w.R; ++r) {
         const unsigned j = w.j0 + r;
         ld_row<T, E>(plane + (((j + 1) & m) << logN), lane, nxt);
@@ -415,14 +418,10 @@ __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P, D
             const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
             const double lap = lapl_sites(kp, km, (double)nxt[e], (double)prev[e], (double)up[e], (double)dn[e], c);
             acc[0] -= c * lap;
-            if (SPLIT) {
-                double V, dV;
-                pot_eval_fast(P, phibar + c, &V, &dV, &err);
-                acc[1] += V;
-                acc[2] += dV;
-            } else {
-                acc[1] += pot_V_fast(P, c, &err);
-            }
+            double V, dV;
+            Pv.eval(phibar + c, &V, &dV, hint[e], true, &err);
+            acc[1] += V;
+            if (SPLIT) acc[2] += dV;
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -437,10 +436,11 @@ __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P, D
 }
 
 template <typename T, bool SPLIT, int E, int MODE>
-__device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, DevBg *bg, int loop, unsigned G,
+__device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P0, DevBg *bg, int loop, unsigned G,
                                               unsigned blk)
 {
     if (loop && bg->done) return;
+    const PotView Pv = stage_pot(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
@@ -458,7 +458,9 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, Dev
     T prev[E], cur[E], nxt[E], up[E], dn[E], vin[E];
     ld_row<T, E>(plane + (((w.j0 - 1) & m) << logN), lane, prev);
     ld_row<T, E>(plane + (w.j0 << logN), lane, cur);
-    int err = 0;
+    int err = 0, hint[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) hint[e] = Pv.start((SPLIT ? phibar : 0.0) + (double)cur[e]);
     for (unsigned r = 0; r < w.R; ++r) {
         const unsigned j = w.j0 + r;
         const unsigned roff = poff + (j << logN);
@@ -475,7 +477,7 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P, Dev
             const double kp = (double)(e < E - 1 ? cur[e + 1 < E ? e + 1 : e] : kp_last);
             const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
             const double lap = lapl_sites(kp, km, (double)nxt[e], (double)prev[e], (double)up[e], (double)dn[e], c);
-            double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+            double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
             if (SPLIT) dV -= vpmean;
             double v = (double)vin[e];
             v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
@@ -669,7 +671,7 @@ __device__ __forceinline__ double uniform(double x)
 // A step launch first closes the previous step from its record in slot sp (its
 // first rows already in flight) and writes the result to the other slot.
 template <typename T, bool SPLIT, int E, bool SUMS = false, bool PF = false>
-__device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned blk,
+__device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg *bg, double *partials, unsigned blk,
                                            T (*lds)[kMarchWaves][64 * E], double *smem, unsigned sp = 0)
 {
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
@@ -695,6 +697,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
         pmarched = prev->marched != 0;
         par = prev->parity ^ (pmarched ? 1 : 0);
     }
+    const PotView Pv = stage_pot(P0);
     const T *__restrict__ f = (const T *)F.f[par];
     T *__restrict__ fout = (T *)F.f[par ^ 1];
     // fd is read from [par] and written to [par ^ 1]: the halo row and the extra
@@ -752,6 +755,10 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
         phibar_next = SPLIT ? uniform(sb.nx.phibar_next) : 0.0;
         pout = partials + (size_t)(sp ^ 1) * G * 3;
     }
+    // brackets of the tabulated potential, walked from each lane's previous site
+    int hint[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) hint[e] = Pv.start((SPLIT ? phibar : 0.0) + (double)fcur[e]);
     for (unsigned t = 0; t <= TI; ++t) {
         const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
         if constexpr (PF && !SUMS) {
@@ -792,7 +799,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
                 const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : km_first);
                 const double lap =
                     lapl_sites(kp, km, (double)fjp[e], (double)fjm[e], (double)fup[e], (double)fdn[e], c);
-                double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+                double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
                 if (SPLIT) dV -= vpmean;
                 double v = (double)vin[e];
                 v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
@@ -818,7 +825,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P, DevBg 
                     const double dj = (double)buf[w + 1][lane * E + e] - c;
                     acc0 += dk * dk + dj * dj;
                     double V, dV;
-                    pot_eval_fast(P, SPLIT ? phibar_next + c : c, &V, &dV, &err);
+                    Pv.eval(SPLIT ? phibar_next + c : c, &V, &dV, hint[e], true, &err);
                     acc1 += V;
                     acc2 += dV;
                 }
@@ -1291,10 +1298,11 @@ __device__ __forceinline__ double stencil_lapl(const Stencil<T, U> &s, int u)
 constexpr int kUnroll = 4;
 
 template <typename T, bool SPLIT>
-__device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop,
+__device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P0, DevBg *bg, double *partials, int loop,
                                                  unsigned G, unsigned blk, double *smem)
 {
     if (loop && bg->done) return;
+    const PotView Pv = stage_pot(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
     const unsigned gs = N * N * N;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
@@ -1311,14 +1319,11 @@ __device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P, 
             const double c = (double)s.c[u];
             const double lap = stencil_lapl<T, kUnroll>(s, u);
             acc[0] -= c * lap;
-            if (SPLIT) {
-                double V, dV;
-                pot_eval_fast(P, phibar + c, &V, &dV, &err);
-                acc[1] += V;
-                acc[2] += dV;
-            } else {
-                acc[1] += pot_V_fast(P, c, &err);
-            }
+            double V, dV;
+            int h = 0;
+            Pv.eval(phibar + c, &V, &dV, h, false, &err);
+            acc[1] += V;
+            if (SPLIT) acc[2] += dV;
         }
     }
     flag_error(bg, err);
@@ -1328,10 +1333,11 @@ __device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P, 
 }
 
 template <typename T, bool SPLIT, int MODE>
-__device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P, DevBg *bg, double d_arg, int loop,
+__device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P0, DevBg *bg, double d_arg, int loop,
                                                unsigned G, unsigned blk)
 {
     if (loop && bg->done) return;
+    const PotView Pv = stage_pot(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
     const unsigned gs = N * N * N;
     const int par = bg->parity;
@@ -1356,7 +1362,8 @@ __device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P, De
                 if (!s.ok[u]) continue;
                 const double c = (double)s.c[u];
                 const double lap = stencil_lapl<T, kUnroll>(s, u);
-                double dV = pot_dV_fast(P, SPLIT ? phibar + c : c, &err);
+                int h = 0;
+                double dV = Pv.dV(SPLIT ? phibar + c : c, h, false, &err);
                 if (SPLIT) dV -= vpmean;
                 double v = (double)vin[u];
                 v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
@@ -1570,10 +1577,10 @@ template <typename T, bool SPLIT>
 void reduce_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop, unsigned G, hipStream_t s)
 {
     switch (row_elems(F.N)) {
-    case 1: k_row_reduce<T, SPLIT, 1><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
-    case 2: k_row_reduce<T, SPLIT, 2><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
-    case 4: k_row_reduce<T, SPLIT, 4><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
-    default: k_site_reduce<T, SPLIT><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    case 1: k_row_reduce<T, SPLIT, 1><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
+    case 2: k_row_reduce<T, SPLIT, 2><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
+    case 4: k_row_reduce<T, SPLIT, 4><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
+    default: k_site_reduce<T, SPLIT><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
     }
 }
 
@@ -1581,9 +1588,9 @@ template <typename T, bool SPLIT, int MODE>
 void row_kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int loop, unsigned G, hipStream_t s)
 {
     switch (shape_elems(F.N, F.kick_e)) {
-    case 1: k_row_kick<T, SPLIT, 1, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
-    case 2: k_row_kick<T, SPLIT, 2, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
-    default: k_row_kick<T, SPLIT, 4, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
+    case 1: k_row_kick<T, SPLIT, 1, MODE><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, loop); break;
+    case 2: k_row_kick<T, SPLIT, 2, MODE><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, loop); break;
+    default: k_row_kick<T, SPLIT, 4, MODE><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, loop); break;
     }
 }
 
@@ -1592,10 +1599,10 @@ void kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_
                    hipStream_t s)
 {
     if (mode == DRIFT) {
-        k_site_kick<T, SPLIT, DRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
+        k_site_kick<T, SPLIT, DRIFT><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, d_arg, loop);
     } else if (row_elems(F.N) == 0) {
-        if (mode == KICK) k_site_kick<T, SPLIT, KICK><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
-        else k_site_kick<T, SPLIT, KICKDRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
+        if (mode == KICK) k_site_kick<T, SPLIT, KICK><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, d_arg, loop);
+        else k_site_kick<T, SPLIT, KICKDRIFT><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, d_arg, loop);
     } else {
         if (mode == KICK) row_kick_dispatch<T, SPLIT, KICK>(F, P, bg, loop, G, s);
         else row_kick_dispatch<T, SPLIT, KICKDRIFT>(F, P, bg, loop, G, s);
@@ -1644,16 +1651,16 @@ void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, 
 {
     const int E = row_elems(F.N);
     if (F.march_pf) {
-        if (E == 2) k_march_pf<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
-        else k_march_pf<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march_pf<T, SPLIT, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        else k_march_pf<T, SPLIT, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
         return;
     }
     if constexpr (sizeof(T) == 4) {
-        if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
-        else k_march_w5<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        else k_march_w5<T, SPLIT, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
     } else {
-        if (E == 2) k_march<T, SPLIT, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
-        else k_march<T, SPLIT, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march<T, SPLIT, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        else k_march<T, SPLIT, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
     }
 }
 
@@ -1760,11 +1767,11 @@ hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *b
     const unsigned G = march_blocks(F.N);
     const int E = row_elems(F.N);
     if (split) {
-        if (E == 2) k_march_sums<float, true, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else k_march_sums<float, true, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        if (E == 2) k_march_sums<float, true, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
+        else k_march_sums<float, true, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
     } else {
-        if (E == 2) k_march_sums<double, false, 2><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
-        else k_march_sums<double, false, 4><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+        if (E == 2) k_march_sums<double, false, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
+        else k_march_sums<double, false, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
     }
     k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G, (unsigned long long)F.N * F.N * F.N, F.rec);
     return hipGetLastError();
@@ -1863,6 +1870,7 @@ struct BatchGroup {
     uint2 *rmap = nullptr;       // reduce workgroups (march groups: loop start only)
     uint2 *tmap = nullptr;       // tensor workgroups
     int nblocks = 0, nrblocks = 0, ntblocks = 0;
+    size_t pot_lds = 0;          // dynamic LDS of the potential tables (largest of the group)
 };
 
 struct BatchDev {
@@ -1881,6 +1889,7 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     // groups of runs launched together: same step structure and kernel shapes
     std::vector<int> keys;
     std::vector<std::vector<uint2>> maps, rmaps, tmaps;
+    std::vector<size_t> lds;
     for (int r = 0; r < n; ++r) {
         const int N = runs[r].F.N;
         d[r].F = runs[r].F;
@@ -1906,7 +1915,9 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
             maps.emplace_back();
             rmaps.emplace_back();
             tmaps.emplace_back();
+            lds.push_back(0);
         }
+        lds[g] = std::max(lds[g], pot_lds_bytes(runs[r].P));
         for (unsigned b = 0; b < d[r].G; ++b) rmaps[g].push_back(make_uint2((unsigned)r, b));
         if (runs[r].F.sigw) {
             // tensor step: march kernel (N >= 64) or site kernel
@@ -1935,6 +1946,7 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         G.nblocks = (int)maps[g].size();
         G.nrblocks = (int)rmaps[g].size();
         G.ntblocks = (int)tmaps[g].size();
+        G.pot_lds = lds[g];
         upload(maps[g], &G.map);
         upload(rmaps[g], &G.rmap);
         upload(tmaps[g], &G.tmap);
@@ -1971,11 +1983,11 @@ hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
         for (const BatchGroup &G : B->groups) {
             if (!G.march) continue;
             if (B->split) {
-                if (G.E == 2) kb_march_sums<float, true, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-                else kb_march_sums<float, true, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+                if (G.E == 2) kb_march_sums<float, true, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
+                else kb_march_sums<float, true, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
             } else {
-                if (G.E == 2) kb_march_sums<double, false, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
-                else kb_march_sums<double, false, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+                if (G.E == 2) kb_march_sums<double, false, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
+                else kb_march_sums<double, false, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
             }
         }
         kb_sums_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
@@ -1987,10 +1999,10 @@ template <typename T, bool SPLIT>
 static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
     switch (G.E) {
-    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
-    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
-    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
-    default: kb_site_reduce<T, SPLIT><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
+    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
+    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
+    default: kb_site_reduce<T, SPLIT><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
     }
 }
 
@@ -2000,23 +2012,23 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n
     // records: the previous step's in slot n & 1, this step's in the other
     const unsigned sp = (unsigned)(n & 1), rs = sp ^ 1u;
     if (G.march && G.pf) {
-        if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
-        else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+        if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+        else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
     } else if (G.march) {
         if constexpr (sizeof(T) == 4) {
-            if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
-            else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
-            else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+            if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+            else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+            else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
         } else {
-            if (G.E == 1) kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
-            else if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
-            else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+            if (G.E == 1) kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+            else if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+            else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
         }
     } else switch (G.kE) {
-    case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
-    case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
-    case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
-    default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
+    case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
+    case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
+    default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
     }
     if (!G.ntblocks) return;
     if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);

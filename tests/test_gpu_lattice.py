@@ -73,8 +73,9 @@ def test_energies_match_oracle(case):
 # N < 64: site kernels; N = 64: row kernels (one site per lane); 128 <= N <= 256:
 # fused steps (march kernel with the next step's sums, two and four sites per lane)
 @pytest.mark.parametrize("case,N,nsteps", [("hilltop_n32", 32, 200), ("tabulated_n16", 16, 200),
-                                           ("hilltop_n32", 64, 100), ("tabulated_n16", 128, 20),
-                                           ("hilltop_n32", 128, 40), ("hilltop_n32", 256, 6)])
+                                           ("hilltop_n32", 64, 100), ("tabulated_n16", 64, 40),
+                                           ("tabulated_n16", 128, 20), ("hilltop_n32", 128, 40),
+                                           ("hilltop_n32", 256, 6), ("tabulated_n16", 256, 4)])
 def test_leapfrog_fp64_matches_oracle(case, N, nsteps):
     print(f"leapfrog parity {case} N={N} steps={nsteps}", flush=True)
     st, _, table = oracle_state(case, N=N, calculate_sigw=0)
@@ -107,7 +108,7 @@ def test_explicit_kick_and_drift_match_oracle():
     assert fluct_err(st.field(1), run.field(E.IE_FIELD_FD)) < 1e-10
 
 
-# N = 128: fused steps (march + close kernels) stop in the close kernel
+# N = 128: fused steps stop when a step kernel closes the record past af
 @pytest.mark.parametrize("N", [16, 128])
 def test_loop_stops_at_af(N):
     st, _, table = oracle_state("hilltop_n16", N=N, calculate_sigw=0, af=1.01)
