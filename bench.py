@@ -66,6 +66,35 @@ def bench_case(N: int, precision: int, sigw: int, steps: int, warmup: int, profi
     return secs, prof, tuning
 
 
+def tabulated_case(N: int = 128, steps: int = 1000, warmup: int = 50) -> dict:
+    """Split against FP64 storage with the tabulated SR -> USR -> SR potential
+    (inputs/ tables, linear interpolation), N^3, the USR preset's initial field:
+    loop steps per second of each, after the tuner and an untimed warmup."""
+    import numpy as np
+
+    import ie_engine as E
+    with np.load(REPO / "tests" / "golden" / "potential_table.npz") as z:
+        class T:  # table kept alive for the run
+            x = np.ascontiguousarray(z["field"])
+            v = np.ascontiguousarray(z["potential"])
+            dv = np.ascontiguousarray(z["potential_derivative"])
+    out = {"N": N, "steps": steps, "potential": "tabulated SR-USR-SR (inputs/ tables)"}
+    for name, prec in (("split", E.IE_SPLIT), ("fp64", E.IE_FP64)):
+        cfg = E.make_config(N, precision=prec, numerical=1, table=T, V0=3e-9, dt=0.001)
+        run = E.Run(cfg, T)
+        run.initialize(SEED, 2.9181235049318586, -0.06727651095116181)
+        run.evolve_fields(0.0005)
+        run.tune(8)
+        secs, bg = time_steps(run, steps, warmup)
+        if bg["done"]:
+            raise RuntimeError(f"unexpected loop state {bg}")
+        run.close()
+        out[f"{name}_ms_per_step"] = secs / steps * 1e3
+        out[f"{name}_site_updates_per_s"] = N ** 3 * steps / secs
+    out["split_over_fp64"] = out["split_site_updates_per_s"] / out["fp64_site_updates_per_s"]
+    return out
+
+
 def read_traffic(name: str):
     """HBM bytes per launch measured by rocprofv3 --pmc (profiles/traffic.json)."""
     p = REPO / "profiles" / "traffic.json"
@@ -318,7 +347,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--post-steps", type=int, default=200, help="post-inflation steps timed (0: skip)")
     ap.add_argument("--pzeta-check", type=int, default=1, help="split vs FP64 P_zeta of whole runs (0: skip)")
-    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "pzeta", "serve"], default=None,
+    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "pzeta", "serve", "tab"], default=None,
                     help="time a single case (for profiling runs)")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -346,6 +375,9 @@ def main() -> int:
         return 0
     if args.only == "serve":
         print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_rate)}), flush=True)
+        return 0
+    if args.only == "tab":
+        print(json.dumps({"tabulated_128": tabulated_case(128)}), flush=True)
         return 0
     cases = {"split": (E.IE_SPLIT, 0), "fp64": (E.IE_FP64, 0), "sigw": (E.IE_SPLIT, 1)}
     todo = [args.only] if args.only else ["split", "fp64", "sigw"]
@@ -407,6 +439,9 @@ def main() -> int:
                            "kernel": kname if fused else "k_row_kick (kick + drift)", "kernel_ms": ms,
                            "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
                            "step_ms_events": prof["step_ms"]}
+    if not args.only:
+        out["tabulated_128"] = tabulated_case(128)
+        print(f"# tabulated 128: {out['tabulated_128']}", file=sys.stderr, flush=True)
     if args.pzeta_check and not args.only:
         out["pzeta_split_vs_fp64"] = pzeta_check(N)
         print(f"# pzeta check: {out['pzeta_split_vs_fp64']}", file=sys.stderr, flush=True)
