@@ -169,7 +169,8 @@ ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
 /* Shape autotuner of the step kernels (no counterpart in the reference: the
  * reference has no kernel shapes).  The unfused kick kernel of N >= 128 lattices
  * can hold 1, 2 or 4 sites per lane of a row; the tensor step can run as the row
- * kernel (1, 2 or 4 sites per lane) or as the plane-marching kernel (1 or 2); the
+ * kernel (1, 2 or 4 sites per lane) or as the plane-marching kernel (1, 2, or 4
+ * for split storage); the
  * fused step kernel can load the next plane before or after the current one's
  * arithmetic (variants 0, 1; the same sums in the same order).
  * None of these changes a result (no cross-site sums), so the tuner times real
