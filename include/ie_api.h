@@ -154,7 +154,9 @@ ie_status ie_evolve_fields(ie_run *run, double d);
 ie_status ie_leapfrog_steps(ie_run *run, int max_steps, ie_background *out);
 /* The same loop as ie_leapfrog_steps, with HIP events recorded on the run's
  * stream around every launch; *out receives the mean device time per launch of
- * each kernel of the step and per whole step (milliseconds).  No reference
+ * each kernel of the step and per whole step (milliseconds).  Then nsteps more
+ * steps with events around the loop alone (loop_ms): events between launches
+ * hold each launch apart, so the per-launch figures run above the loop's own.  No reference
  * counterpart: the reference has wall-clock timing only (src/output.cpp:1451-1461). */
 typedef struct ie_profile {
     double step_ms;            /* whole step, first to last event */
@@ -163,6 +165,10 @@ typedef struct ie_profile {
     double kick_drift_ms;      /* scalar kick + drift (fused steps: with the sums) */
     double tensor_ms;          /* tensor kick + drift (0 without SIGW) */
     int64_t steps;
+    double loop_ms;            /* nsteps further steps with events around the whole loop
+                                  only, per step: for a fused step without SIGW (one
+                                  launch per step) the step kernel's mean launch
+                                  duration, gaps between launches included */
 } ie_profile;
 ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
 

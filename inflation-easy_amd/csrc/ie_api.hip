@@ -724,8 +724,31 @@ ie_status ie_profile_steps(ie_run *r, int nsteps, ie_profile *out)
         IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[5]));
         step += ms;
     }
+    // the loop alone, events around it only
+    k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
+    if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
+    IE_HIP(r, hipEventRecord(ev[0], r->stream));
+    for (int n = 0; n < nsteps; ++n) {
+        if (fused) {
+            IE_HIP(r, launch_march(r->F, split, r->P, r->bg, r->partials, n, r->stream));
+            if (r->F.sigw)
+                IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream, step_record(n)));
+            continue;
+        }
+        IE_HIP(r, launch_reduce(r->F, split, r->P, r->bg, r->partials, 1, r->stream));
+        IE_HIP(r, launch_friedmann_finish(r->F, r->bg, r->partials, 0.0, 1, r->stream));
+        IE_HIP(r, launch_scalar_step(r->F, split, r->P, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+        if (r->F.sigw) IE_HIP(r, launch_tensor_step(r->F, split, r->bg, KICKDRIFT, 0.0, 1, r->stream));
+        IE_HIP(r, launch_step_finish(r->bg, split, KICKDRIFT, 0.0, 1, r->stream));
+    }
+    IE_HIP(r, hipEventRecord(ev[1], r->stream));
+    if (fused) IE_HIP(r, launch_loop_end(r->F, split, r->bg, r->partials, nsteps, r->stream));
+    if ((st = sync(r)) != IE_OK) return st;
+    float loop = 0.f;
+    IE_HIP(r, hipEventElapsedTime(&loop, ev[0], ev[1]));
     for (auto &e : ev) (void)hipEventDestroy(e);
     if ((st = check_kernel_error(r)) != IE_OK) return st;
+    out->loop_ms = loop / nsteps;
     out->reduce_ms = red / nsteps;
     out->kick_drift_ms = kd / nsteps;
     out->tensor_ms = ten / nsteps;

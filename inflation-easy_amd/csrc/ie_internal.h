@@ -545,8 +545,6 @@ __device__ __forceinline__ RowWalk row_walk_stacked(unsigned N, unsigned G, unsi
     return w;
 }
 
-// k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
-// `row`): in-lane, then across lanes, then across segments for N > 64 E.
 // The value of the first active lane, in scalar registers (the same in every lane).
 __device__ __forceinline__ float first_lane(float x)
 {
@@ -600,6 +598,8 @@ __device__ __forceinline__ Acc serial_walk(const double *__restrict__ g, size_t 
     return acc;
 }
 
+// k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
+// `row`): in-lane, then across lanes, then across segments for N > 64 E.
 template <typename T, int E>
 __device__ __forceinline__ void k_neighbours(const T (&c)[E], const T *__restrict__ row, const RowWalk &w,
                                              unsigned N, unsigned lane, T &kp_last, T &km_first)

@@ -666,11 +666,13 @@ __device__ __forceinline__ double uniform(double x)
 // SUMS: the sums alone, of the current f taken as the new f (loop start), with
 // the same partition and operation order as a step's, so that a loop split into
 // several calls gives the same bits as one call.
-// PF: the next plane's rows are loaded before the current plane's arithmetic
-// (more registers, loads in flight across the barrier); the same results.
+// PF: the rows of plane i + PF are loaded before plane i's arithmetic (PF = 1, 2;
+// more registers, loads in flight across the barriers); the same results.  The
+// plane loop is unrolled by 3 over three row sets, so a row set in flight is never
+// moved between registers (a move waits for its load).
 // A step launch first closes the previous step from its record in slot sp (its
 // first rows already in flight) and writes the result to the other slot.
-template <typename T, bool SPLIT, int E, bool SUMS = false, bool PF = false>
+template <typename T, bool SPLIT, int E, bool SUMS = false, int PF = 0>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg *bg, double *partials, unsigned blk,
                                            T (*lds)[kMarchWaves][64 * E], double *smem, unsigned sp = 0)
 {
@@ -711,24 +713,41 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
     const unsigned j = ((lb % jblocks) * kTileJ + w) & m;
     const bool owner = w < (unsigned)kTileJ;
     const unsigned rj = j << logN, rjp = ((j + 1) & m) << logN, rjm = ((j - 1) & m) << logN;
-    T fdn[E], fcur[E], fup[E], fjp[E], fjm[E], vin[E], fprev[E];
+    // the rolling rows: f of planes i - 1 and i, and the new f of plane i - 1, in
+    // FP64 (each value converted once), plus plane i's stored values for the shuffles
+    T fdn[E], fcur[E];
     ld_row<T, E>(f + (((i0 - 1) & m) << (2 * logN)) + rj, lane, fdn);
     ld_row<T, E>(f + ((i0 & m) << (2 * logN)) + rj, lane, fcur);
+    double ddn[E], dcur[E], dprev[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) fprev[e] = (T)0;
+    for (int e = 0; e < E; ++e) {
+        ddn[e] = (double)fdn[e];
+        dcur[e] = (double)fcur[e];
+        dprev[e] = 0.0;
+    }
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     int err = 0;
     RowWalk wk{};   // k neighbours within the whole row (one segment)
     wk.segs = 1;
     wk.k0 = 0;
-    T nup[E], njp[E], njm[E], nvin[E];
-    if constexpr (PF && !SUMS) {
-        const unsigned p0 = (i0 & m) << (2 * logN), p1 = ((i0 + 1) & m) << (2 * logN);
-        ld_row<T, E>(f + p1 + rj, lane, nup);
-        ld_row<T, E>(f + p0 + rjp, lane, njp);
-        ld_row<T, E>(f + p0 + rjm, lane, njm);
-        ld_row<T, E>(fd + p0 + rj, lane, nvin);
-    }
+    // rows plane step t reads besides the wave's own: f(i0+t+1) row j, and for a
+    // step f(i0+t) rows j +- 1 and fd(i0+t) row j
+    struct Rows {
+        T up[E], jp[E], jm[E], vin[E];
+    };
+    Rows S0, S1, S2;
+    auto load_rows = [&](unsigned t, Rows &r) __attribute__((always_inline)) {
+        const unsigned pt = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
+        ld_row<T, E>(f + pu + rj, lane, r.up);
+        if constexpr (!SUMS) {
+            ld_row<T, E>(f + pt + rjp, lane, r.jp);
+            ld_row<T, E>(f + pt + rjm, lane, r.jm);
+            ld_row<T, E>(fd + pt + rj, lane, r.vin);
+        }
+    };
+    constexpr int kAhead = SUMS ? 0 : PF;
+    if constexpr (kAhead >= 1) load_rows(0, S0);
+    if constexpr (kAhead >= 2) load_rows(1, S1);
     // the step's scalars: the loop's first record as it stands, later ones closed
     // here with the previous step's partials (every workgroup, the same bits)
     double d = 0.0, laplnorm = 0.0, friction = 0.0, potnorm = 0.0, vpmean = 0.0, phibar = 0.0, phibar_next = 0.0;
@@ -759,52 +778,38 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
     int hint[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) hint[e] = Pv.start((SPLIT ? phibar : 0.0) + (double)fcur[e]);
-    for (unsigned t = 0; t <= TI; ++t) {
-        const unsigned pi = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
-        if constexpr (PF && !SUMS) {
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                fup[e] = nup[e];
-                fjp[e] = njp[e];
-                fjm[e] = njm[e];
-                vin[e] = nvin[e];
-            }
-            if (t < TI) {
-                const unsigned pn = ((i0 + t + 2) & m) << (2 * logN);
-                ld_row<T, E>(f + pn + rj, lane, nup);
-                ld_row<T, E>(f + pu + rjp, lane, njp);
-                ld_row<T, E>(f + pu + rjm, lane, njm);
-                ld_row<T, E>(fd + pu + rj, lane, nvin);
-            }
-        } else {
-            ld_row<T, E>(f + pu + rj, lane, fup);
-        }
+    // plane step t with its rows in `cur`; the rows of step t + kAhead go to `fill`
+    // (the last steps reload plane TI's rows, unused, so that every step loads)
+    auto plane_step = [&](unsigned t, Rows &cur, Rows &fill) __attribute__((always_inline)) {
+        if constexpr (kAhead == 0) load_rows(t, cur);
+        else load_rows(t + kAhead <= TI ? t + kAhead : TI, fill);
+        const unsigned pi = ((i0 + t) & m) << (2 * logN);
         T fnew[E];
+        double dnew[E];
         if constexpr (SUMS) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) fnew[e] = fcur[e];
-        } else {
-            if constexpr (!PF) {
-                ld_row<T, E>(f + pi + rjp, lane, fjp);
-                ld_row<T, E>(f + pi + rjm, lane, fjm);
-                ld_row<T, E>(fd + pi + rj, lane, vin);
+            for (int e = 0; e < E; ++e) {
+                fnew[e] = fcur[e];
+                dnew[e] = dcur[e];
             }
+        } else {
             T kp_last, km_first;
             k_neighbours<T, E>(fcur, f + pi + rj, wk, N, lane, kp_last, km_first);
             T vout[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const double c = (double)fcur[e];
-                const double kp = (double)(e < E - 1 ? fcur[e + 1 < E ? e + 1 : e] : kp_last);
-                const double km = (double)(e > 0 ? fcur[e > 0 ? e - 1 : 0] : km_first);
+                const double c = dcur[e];
+                const double kp = e < E - 1 ? dcur[e + 1 < E ? e + 1 : e] : (double)kp_last;
+                const double km = e > 0 ? dcur[e > 0 ? e - 1 : 0] : (double)km_first;
                 const double lap =
-                    lapl_sites(kp, km, (double)fjp[e], (double)fjm[e], (double)fup[e], (double)fdn[e], c);
+                    lapl_sites(kp, km, (double)cur.jp[e], (double)cur.jm[e], (double)cur.up[e], ddn[e], c);
                 double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
                 if (SPLIT) dV -= vpmean;
-                double v = (double)vin[e];
+                double v = (double)cur.vin[e];
                 v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
                 vout[e] = (T)v;
                 fnew[e] = (T)(c + d * (double)vout[e]);
+                dnew[e] = (double)fnew[e];
             }
             if (owner && t < TI) {
                 st_row<T, E>(fdout + pi + rj, lane, vout);
@@ -820,8 +825,8 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
                 const T knext = __shfl(fnew[0], (int)((lane + 1) & 63));
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const double c = (double)fnew[e];
-                    const double dk = (double)(e < E - 1 ? fnew[e + 1 < E ? e + 1 : e] : knext) - c;
+                    const double c = dnew[e];
+                    const double dk = (e < E - 1 ? dnew[e + 1 < E ? e + 1 : e] : (double)knext) - c;
                     const double dj = (double)buf[w + 1][lane * E + e] - c;
                     acc0 += dk * dk + dj * dj;
                     double V, dV;
@@ -833,16 +838,30 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
             if (t >= 1) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const double di = (double)fnew[e] - (double)fprev[e];
+                    const double di = dnew[e] - dprev[e];
                     acc0 += di * di;
                 }
             }
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            fprev[e] = fnew[e];
-            fdn[e] = fcur[e];
-            fcur[e] = fup[e];
+            dprev[e] = dnew[e];
+            ddn[e] = dcur[e];
+            fcur[e] = cur.up[e];
+            dcur[e] = (double)cur.up[e];
+        }
+    };
+    // set of step t: S[t mod 3]; step t fills S[(t + kAhead) mod 3], last read by
+    // step t + kAhead - 3.  TI + 1 steps, a multiple of 3 (march_ok).
+    for (unsigned t = 0; t <= TI; t += 3) {
+        if constexpr (kAhead == 2) {
+            plane_step(t, S0, S2);
+            plane_step(t + 1, S1, S0);
+            plane_step(t + 2, S2, S1);
+        } else {
+            plane_step(t, S0, S1);
+            plane_step(t + 1, S1, S2);
+            plane_step(t + 2, S2, S0);
         }
     }
     flag_error(bg, err);
@@ -860,6 +879,10 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
             pout[(size_t)blk * 3 + q] = tot;
         }
 }
+
+// Planes the early-load march kernels load ahead (1 measured faster than 2 at
+// 256^3: 0.0475 against 0.0481 ms per split step).
+constexpr int kMarchAhead = 1;
 
 // Split storage fits two workgroups per CU (5 waves per SIMD) without spills;
 // FP64 rows need more registers and run one.
@@ -887,7 +910,7 @@ __global__ __launch_bounds__(kMarchThreads) void k_march_pf(Fields F, Pot P, Dev
 {
     __shared__ T lds[2][kMarchWaves][64 * E];
     __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E, false, true>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
+    march_body<T, SPLIT, E, false, kMarchAhead>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -898,7 +921,7 @@ __global__ __launch_bounds__(kMarchThreads) void kb_march_pf(const RunDesc *__re
     __shared__ double smem[3 * kMarchWaves];
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E, false, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
+    march_body<T, SPLIT, E, false, kMarchAhead>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
 }
 
 template <typename T, bool SPLIT, int E>
@@ -1758,7 +1781,8 @@ hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, d
 // tile's halo row and per-plane barrier cost more than the reduce they save (a
 // batch of 256 lattices of 64^3 steps faster unfused); above 256 a row spans
 // several segments, whose k pairs would cross waves.
-bool march_ok(int N) { return N >= 128 && N <= 256; }
+// (the march kernels step TI + 1 planes three at a time)
+bool march_ok(int N) { return N >= 128 && N <= 256 && (march_ti((unsigned)N) + 1) % 3 == 0; }
 
 unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (int)march_ti((unsigned)N))); }
 

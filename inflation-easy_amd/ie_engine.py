@@ -80,7 +80,7 @@ class Profile(C.Structure):
     """Mirror of ``ie_profile``."""
 
     _fields_ = [("step_ms", C.c_double), ("reduce_ms", C.c_double), ("kick_drift_ms", C.c_double),
-                ("tensor_ms", C.c_double), ("steps", C.c_int64)]
+                ("tensor_ms", C.c_double), ("steps", C.c_int64), ("loop_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

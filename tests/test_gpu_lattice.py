@@ -289,3 +289,26 @@ def test_loop_in_chunks_equals_one_call(N, chunks):
     assert ba["a"] == bb["a"] and ba["ad"] == bb["ad"]
     assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
     assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
+
+
+@pytest.mark.parametrize("N,sigw", [(64, 0), (128, 0), (128, 1)])
+def test_profile_steps_are_loop_steps(N, sigw):
+    """ie_profile_steps runs the loop's own launches, twice n steps (events around
+    each launch, then around the loop alone), so it leaves the state of 2n loop
+    steps, bit for bit, and reports positive times."""
+    print(f"profile steps N={N} sigw={sigw}", flush=True)
+    st, _, table = oracle_state("hilltop_n32", N=N, calculate_sigw=sigw)
+    ra = engine_for(st, table, precision=E.IE_SPLIT, sigw=sigw)
+    rb = engine_for(st, table, precision=E.IE_SPLIT, sigw=sigw)
+    for r in (ra, rb):
+        r.evolve_fields(0.5 * st.cfg.dt)
+    p = ra.profile_steps(5)
+    rb.leapfrog_steps(10)
+    assert p["steps"] == 10 and ra.background()["steps"] == 10
+    assert p["step_ms"] > 0 and p["kick_drift_ms"] > 0 and p["loop_ms"] > 0
+    assert (p["tensor_ms"] > 0) == bool(sigw)
+    assert (p["reduce_ms"] > 0) == (N < 128)
+    assert np.array_equal(ra.field(E.IE_FIELD_F), rb.field(E.IE_FIELD_F))
+    assert np.array_equal(ra.field(E.IE_FIELD_FD), rb.field(E.IE_FIELD_FD))
+    for comp in range(12 if sigw else 0):
+        assert np.array_equal(ra.tensor(comp), rb.tensor(comp)), comp
