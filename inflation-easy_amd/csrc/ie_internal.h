@@ -239,8 +239,8 @@ __device__ __forceinline__ double pot_dV_fast(const Pot &P, double v, int *err)
 // Bracket search on a table x (descending) of n rows: the unique l with
 // x[l-1] >= v > x[l] (n - 1 at v = x[n-1]); *err set when v leaves the table.
 // tab_bisect bisects; tab_walk walks from a hint l (the bracket of a nearby
-// value, as the previous site of the same lane) and ends at the same l, in a
-// step or two instead of log2 n dependent reads.
+// value, as the previous site of the same lane) and ends at the same l (the
+// bracket is unique), in a step or two instead of log2 n dependent reads.
 __device__ __forceinline__ int tab_bisect(const double *x, int n, double v, int *err)
 {
     if (v > x[0] || v < x[n - 1]) {
@@ -264,8 +264,11 @@ __device__ __forceinline__ int tab_walk(const double *x, int n, double xfirst, d
         return 1;
     }
     l = l < 1 ? 1 : (l > n - 1 ? n - 1 : l);
-    // the hint usually brackets v already; the walk runs only for the lanes (of
-    // a wave with any) whose value left it
+    // v usually lies in the hint's bracket or a neighbour of it (a field whose
+    // sites straddle a grid point moves lanes by one bracket all the time): one
+    // step toward v from the bracket's two ends, without a branch, then the walk
+    // only for the lanes (of a wave with any) that are still outside
+    l += (x[l] >= v && l < n - 1 ? 1 : 0) - (x[l - 1] < v && l > 1 ? 1 : 0);
     const bool ok = (l == n - 1 || x[l] < v) && x[l - 1] >= v;
     if (!__all(ok)) {
         while (l < n - 1 && x[l] >= v) ++l;

@@ -2039,13 +2039,12 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n
         if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
         else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
     } else if (G.march) {
+        // (march groups have N = 128 or 256: E = 2 or 4)
         if constexpr (sizeof(T) == 4) {
-            if (G.E == 1) kb_march_w5<T, SPLIT, 1><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-            else if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+            if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
             else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
         } else {
-            if (G.E == 1) kb_march<T, SPLIT, 1><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-            else if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+            if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
             else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
         }
     } else switch (G.kE) {
