@@ -7,9 +7,11 @@
 // share the GPU:
 //   - admission: FIFO; a request is admitted when its device footprint fits the
 //     budget still free; its initial conditions are generated on a worker thread;
-//   - stepping: all inflating runs of one storage kind form one ie_batch; the
-//     batch is rebuilt only when membership changes and advanced by a quantum of
-//     steps per round (one launch per kernel per step, replayed from a graph);
+//   - stepping: the inflating runs of one storage kind and lattice size form one
+//     ie_batch, rebuilt only when its membership changes; each round advances a
+//     batch of N^3 lattices by quantum * max(1, 256 / N) steps (one launch per
+//     kernel per step, replayed from a graph), so that small runs, whose steps
+//     cost little, do not wait for the rounds of large ones;
 //   - finishing: a run whose loop test ends leaves the batch and is completed on
 //     a worker thread (final leapfrog sync, deltaN phase, zeta spectrum), on its
 //     own stream, while the remaining runs keep stepping; its memory is released
@@ -119,8 +121,12 @@ struct ie_server {
     std::vector<Job *> live;              // admitted, not yet reported
     std::deque<ie_result> results;
     std::mutex mu;
-    ie_batch *batch[2] = {nullptr, nullptr};   // [0] FP64 storage, [1] split
-    std::vector<ie_run *> members[2];
+    struct Group {
+        int kind, N;                  // storage (IE_FP64 / IE_SPLIT), lattice size
+        ie_batch *batch;
+        std::vector<ie_run *> members;
+    };
+    std::vector<Group> groups;
 };
 
 extern "C" {
@@ -179,11 +185,9 @@ ie_status ie_server_submit(ie_server *s, const ie_request *req, int64_t *id)
 
 static void drop_batches(ie_server *s)
 {
-    for (int k = 0; k < 2; ++k) {
-        if (s->batch[k]) ie_batch_destroy(s->batch[k]);
-        s->batch[k] = nullptr;
-        s->members[k].clear();
-    }
+    for (auto &g : s->groups)
+        if (g.batch) ie_batch_destroy(g.batch);
+    s->groups.clear();
 }
 
 static void fail_job(ie_server *s, Job *j, ie_status st)
@@ -230,7 +234,6 @@ ie_status ie_server_pump(ie_server *s, int *active, int *pending)
         });
     }
     // 2. runs whose initial conditions are ready join the batches
-    bool changed = false;
     for (Job *j : s->live) {
         if (j->ready || j->finishing || !j->init.valid()) continue;
         if (j->init.wait_for(std::chrono::seconds(0)) != std::future_status::ready) continue;
@@ -242,30 +245,41 @@ ie_status ie_server_pump(ie_server *s, int *active, int *pending)
             continue;
         }
         j->inflating = true;
-        changed = true;
     }
-    // 3. one quantum of steps for every inflating run (one batch per storage kind)
-    std::vector<ie_run *> want[2];
+    // 3. one round of steps for every inflating run: one batch per storage kind
+    //    and lattice size; batches whose members are unchanged are kept
+    std::vector<ie_server::Group> want;
+    auto add = [&want](int kind, int N, ie_run *r) {
+        for (auto &g : want)
+            if (g.kind == kind && g.N == N) {
+                g.members.push_back(r);
+                return;
+            }
+        want.push_back(ie_server::Group{kind, N, nullptr, {r}});
+    };
     for (Job *j : s->live) {
         if (!j->inflating) continue;
-        want[j->req.cfg.precision == IE_SPLIT ? 1 : 0].push_back(j->run);
-        if (j->twin) want[0].push_back(j->twin);
+        add(j->req.cfg.precision == IE_SPLIT ? IE_SPLIT : IE_FP64, j->req.cfg.N, j->run);
+        if (j->twin) add(IE_FP64, j->req.cfg.N, j->twin);
     }
-    for (int k = 0; k < 2; ++k)
-        if (want[k] != s->members[k]) changed = true;
-    if (changed) {
-        drop_batches(s);
-        for (int k = 0; k < 2; ++k) {
-            if (want[k].empty()) continue;
-            if (ie_batch_create(want[k].data(), (int)want[k].size(), &s->batch[k]) != IE_OK) return IE_ERR_HIP;
-            s->members[k] = want[k];
+    for (auto &w : want)
+        for (auto &g : s->groups)
+            if (g.batch && g.kind == w.kind && g.N == w.N && g.members == w.members) {
+                w.batch = g.batch;
+                g.batch = nullptr;
+            }
+    drop_batches(s);
+    s->groups = std::move(want);
+    for (auto &g : s->groups)
+        if (!g.batch && ie_batch_create(g.members.data(), (int)g.members.size(), &g.batch) != IE_OK) {
+            drop_batches(s);
+            return IE_ERR_HIP;
         }
+    for (auto &g : s->groups) {
+        const int q = s->quantum * std::max(1, 256 / g.N);
+        ie_status st = ie_batch_steps(g.batch, q, s->quantum);
+        if (st != IE_OK) return st;
     }
-    for (int k = 0; k < 2; ++k)
-        if (s->batch[k]) {
-            ie_status st = ie_batch_steps(s->batch[k], s->quantum, s->quantum);
-            if (st != IE_OK) return st;
-        }
     // 4. runs past af leave the batch and are finished on a worker thread
     int finishing_now = 0;
     for (Job *j : s->live)
