@@ -366,8 +366,9 @@ typedef struct ie_result {
 ie_status ie_server_create(size_t hbm_budget_bytes, int quantum_steps, int finish_threads, ie_server **out);
 /* Copy a request in (table pointers are copied too); *id identifies it. */
 ie_status ie_server_submit(ie_server *s, const ie_request *req, int64_t *id);
-/* One scheduling round: admit what fits, advance the resident runs by one
- * quantum, hand finished runs to the finishing threads, collect results.
+/* One scheduling round: admit what fits, advance the resident runs (one batch
+ * per storage kind and lattice size; N^3 runs by quantum_steps * max(1, 256/N)
+ * steps), hand finished runs to the finishing threads, collect results.
  * *active and *pending receive the counts after the round. */
 ie_status ie_server_pump(ie_server *s, int *active, int *pending);
 /* Pop one completed result (returns IE_ERR_STATE when none is ready). */
