@@ -334,6 +334,11 @@ ie_status ie_batch_create(ie_run *const *runs, int nruns, ie_batch **out);
  * graph_steps > 0 captures that many steps per HIP graph launch, 0 launches
  * directly. */
 ie_status ie_batch_steps(ie_batch *batch, int nsteps, int graph_steps);
+/* ie_batch_steps in two halves, so that several batches (each on its own stream)
+ * run at once: ie_batch_launch enqueues the steps and returns; ie_batch_wait
+ * waits for them and refreshes the runs' backgrounds and error flags. */
+ie_status ie_batch_launch(ie_batch *batch, int nsteps, int graph_steps);
+ie_status ie_batch_wait(ie_batch *batch);
 ie_status ie_batch_destroy(ie_batch *batch);
 const char *ie_batch_last_error(const ie_batch *batch);
 

@@ -1704,7 +1704,7 @@ static void drop_graphs(ie_batch *b)
     }
 }
 
-ie_status ie_batch_steps(ie_batch *b, int nsteps, int graph_steps)
+ie_status ie_batch_launch(ie_batch *b, int nsteps, int graph_steps)
 {
     if (!b || nsteps < 0 || graph_steps < 0) return bfail(b, IE_ERR_ARG, "bad argument");
     IE_BHIP(b, batch_loop_begin(b->dev, b->stream));
@@ -1734,12 +1734,24 @@ ie_status ie_batch_steps(ie_batch *b, int nsteps, int graph_steps)
     }
     for (; done < nsteps; ++done) IE_BHIP(b, batch_step(b->dev, done, b->stream));
     if (nsteps > 0) IE_BHIP(b, batch_loop_end(b->dev, nsteps, b->stream));
+    return IE_OK;
+}
+
+ie_status ie_batch_wait(ie_batch *b)
+{
+    if (!b) return IE_ERR_ARG;
     IE_BHIP(b, hipStreamSynchronize(b->stream));
     for (ie_run *r : b->runs) {
         if (pull_bg(r) != IE_OK) return bfail(b, IE_ERR_HIP, r->err);
         if (r->bg_host->error) return bfail(b, IE_ERR_TABLE_RANGE, "field value outside the numerical potential range");
     }
     return IE_OK;
+}
+
+ie_status ie_batch_steps(ie_batch *b, int nsteps, int graph_steps)
+{
+    const ie_status st = ie_batch_launch(b, nsteps, graph_steps);
+    return st != IE_OK ? st : ie_batch_wait(b);
 }
 
 ie_status ie_batch_destroy(ie_batch *b)

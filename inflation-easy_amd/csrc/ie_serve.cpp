@@ -275,11 +275,17 @@ ie_status ie_server_pump(ie_server *s, int *active, int *pending)
             drop_batches(s);
             return IE_ERR_HIP;
         }
+    // all batches enqueued first (one stream each), so that they run together
+    ie_status st = IE_OK;
     for (auto &g : s->groups) {
         const int q = s->quantum * std::max(1, 256 / g.N);
-        ie_status st = ie_batch_steps(g.batch, q, s->quantum);
-        if (st != IE_OK) return st;
+        if (st == IE_OK) st = ie_batch_launch(g.batch, q, s->quantum);
     }
+    for (auto &g : s->groups) {
+        const ie_status w = ie_batch_wait(g.batch);
+        if (st == IE_OK) st = w;
+    }
+    if (st != IE_OK) return st;
     // 4. runs past af leave the batch and are finished on a worker thread
     int finishing_now = 0;
     for (Job *j : s->live)
