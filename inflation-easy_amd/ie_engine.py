@@ -184,6 +184,8 @@ def lib() -> C.CDLL:
             "ie_modes": (st, [C.c_int, C.c_double, C.c_double, dp]),
             "ie_batch_create": (st, [C.POINTER(P), C.c_int, C.POINTER(P)]),
             "ie_batch_steps": (st, [P, C.c_int, C.c_int]),
+            "ie_batch_launch": (st, [P, C.c_int, C.c_int]),
+            "ie_batch_wait": (st, [P]),
             "ie_batch_destroy": (st, [P]),
             "ie_batch_last_error": (C.c_char_p, [P]),
             "ie_server_create": (st, [C.c_size_t, C.c_int, C.c_int, C.POINTER(P)]),
@@ -502,6 +504,17 @@ class Batch:
 
     def steps(self, n: int, graph_steps: int = 0):
         st = lib().ie_batch_steps(self.h, n, graph_steps)
+        if st:
+            raise EngineError(st, lib().ie_batch_last_error(self.h).decode())
+
+    def launch(self, n: int, graph_steps: int = 0):
+        """Enqueue n steps on the batch's stream and return (ie_batch_launch)."""
+        st = lib().ie_batch_launch(self.h, n, graph_steps)
+        if st:
+            raise EngineError(st, lib().ie_batch_last_error(self.h).decode())
+
+    def wait(self):
+        st = lib().ie_batch_wait(self.h)
         if st:
             raise EngineError(st, lib().ie_batch_last_error(self.h).decode())
 

@@ -54,6 +54,30 @@ def test_alone_equals_in_mix_bitwise():
         assert ba["a"] == bb["a"] and ba["ad"] == bb["ad"] and ba["steps"] == bb["steps"] == 60
 
 
+def test_batches_running_together_equal_batches_alone():
+    """Two batches enqueued on their own streams before either is waited for
+    (ie_batch_launch, ie_batch_wait: the serving rounds) give each run the bits
+    it gets with the batches stepped one after the other."""
+    table = potential_table()
+    specs = [[(32, 21, 0, 0), (32, 22, 1, 0)], [(128, 23, 0, 1)]]
+    together = [[new_run(N, s, numerical=p, sigw=g, table=table) for N, s, p, g in grp] for grp in specs]
+    apart = [[new_run(N, s, numerical=p, sigw=g, table=table) for N, s, p, g in grp] for grp in specs]
+    bt = [E.Batch(g) for g in together]
+    for q, b in zip((40, 10), bt):
+        b.launch(q, graph_steps=10)
+    for b in bt:
+        b.wait()
+    for q, g in zip((40, 10), apart):
+        E.Batch(g).steps(q, graph_steps=10)
+    for ga, gb in zip(together, apart):
+        for a, b in zip(ga, gb):
+            fa, fda, ba = state(a)
+            fb, fdb, bb = state(b)
+            assert np.array_equal(fa, fb) and np.array_equal(fda, fdb)
+            assert ba["a"] == bb["a"] and ba["steps"] == bb["steps"]
+    assert together[0][0].background()["steps"] == 40 and together[1][0].background()["steps"] == 10
+
+
 @pytest.mark.parametrize("nsteps,graph_steps", [(25, 25), (23, 7)])
 def test_graph_replay_equals_direct_launches(nsteps, graph_steps):
     """(23, 7): three replays per call starting at even and odd steps of the loop
