@@ -106,14 +106,34 @@ def read_traffic(name: str):
         return None
 
 
+def host_cpu() -> dict:
+    """The host's CPU model, logical CPU count and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "allowed_cpus": allowed}
+
+
 def cpu_baseline(N: int = 256, target_steps_af: float = 1.25) -> dict | None:
     """The reference program (built from its sources by oracle/build_ref.sh, OpenMP)
-    timed on this host: loop time = wall(af = target) - wall(af = 1, no step), the
-    initial conditions and outputs being identical in both runs."""
+    timed on this host with one OpenMP thread per CPU this process may use
+    (OMP_NUM_THREADS when set, which caps it to the host share given to this
+    job): loop time = wall(af = target) - wall(af = 1, no step), the initial
+    conditions and outputs being identical in both runs."""
     binary = REPO / "oracle" / "_ref" / f"ie_ref_n{N}_p0_g0_d0_o1"
     if not binary.exists():
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cpu["allowed_cpus"]
     base = (f"seed = {SEED}\ndt = {PRESET['dt']}\nV0 = {PRESET['V0']}\nns = {PRESET['ns']}\n"
             f"initial_field = {PHI0}\ninitial_derivative = {DPHI0}\nL = {PRESET['L']}\n"
             "output_freq = 100000000\noutput_infrequent_freq = 100000000\noutput_spectra = 0\n"
@@ -144,121 +164,187 @@ def cpu_baseline(N: int = 256, target_steps_af: float = 1.25) -> dict | None:
         return None
     rate = N ** 3 * steps / (w1 - w0)
     return {"value": rate, "unit": "site-updates/s", "cores": threads, "kind": "reference",
+            "host_cpu": cpu,
             "sample": f"{N}^3 hilltop, FP64, {steps} leapfrog steps (af={target_steps_af}) of the reference "
-                      f"OpenMP build; loop time = wall(af={target_steps_af}) - wall(af=1), "
+                      f"OpenMP build on {threads} threads; loop time = wall(af={target_steps_af}) - wall(af=1), "
                       f"{w1 - w0:.1f} s of CPU loop time"}
 
 
-def serve_stream(nreq: int, rate: float, seed: int = 20261203) -> dict:
-    """Serving stream: Poisson arrivals, N in {32, 64, 128,
-    256} with weights 0.4/0.3/0.2/0.1, half hilltop / half tabulated USR, 20% with
-    SIGW, every 20th with an FP64 twin; each request is a whole run (ICs, inflation to
-    af = 2N, deltaN map, P_zeta)."""
+def load_table():
     import numpy as np
-
-    import ie_engine as E
-    sys.path.insert(0, str(REPO / "tests"))
     with np.load(REPO / "tests" / "golden" / "potential_table.npz") as z:
         class T:  # table kept alive for the request pointers
             x = np.ascontiguousarray(z["field"])
             v = np.ascontiguousarray(z["potential"])
             dv = np.ascontiguousarray(z["potential_derivative"])
+    return T
+
+
+USR = dict(V0=3e-9, dt=0.001, dN=1e-4, Nend=2.0)           # params.numerical.txt
+USR_PHI0, USR_DPHI0 = 2.9181235049318586, -0.06727651095116181
+HILL = dict(V0=3.338e-13, ns=0.97, dt=0.001, dN=1e-7, Nend=0.001)   # params.analytic.txt
+
+
+def request_stream(nreq: int, seed: int):
+    """The serving mix: N in {32, 64, 128, 256} with weights 0.4/0.3/0.2/0.1,
+    half hilltop / half tabulated USR, 20% with SIGW, every 20th with an FP64
+    twin (5%)."""
+    import numpy as np
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
     sizes = rng.choice([32, 64, 128, 256], size=nreq, p=[0.4, 0.3, 0.2, 0.1])
     tab = rng.random(nreq) < 0.5
     sigw = rng.random(nreq) < 0.2
-    check = np.arange(nreq) % 20 == 0   # every 20th request (5 %) with an FP64 twin
-    srv = E.Server(240 << 30, 100, 8)
-    submitted, results, n = {}, [], 0
-    t0 = srv.clock()
-    while len(results) < nreq:
-        now = srv.clock() - t0
-        while n < nreq and arrivals[n] <= now:
-            N = int(sizes[n])
-            if tab[n]:
-                cfg = E.make_config(N, precision=E.IE_SPLIT, numerical=1, sigw=int(sigw[n]), table=T,
-                                    V0=3e-9, dt=0.001, dN=1e-4, Nend=2.0)
-                rid = srv.submit(cfg, 1000 + n, 2.9181235049318586, -0.06727651095116181, 1, int(check[n]), T)
-            else:
-                cfg = E.make_config(N, precision=E.IE_SPLIT, sigw=int(sigw[n]), V0=3.338e-13, ns=0.97, dt=0.001,
-                                    dN=1e-7, Nend=0.001)
-                rid = srv.submit(cfg, 1000 + n, 0.0935, 0.000796, 1, int(check[n]))
-            submitted[rid] = n
-            n += 1
-        srv.pump()
-        for r in srv.results():
-            results.append(r)
-            print(f"# serve: done N={r['N']} status={r['status']} latency={r['latency_s']:.3f}s "
-                  f"({len(results)}/{nreq})", file=sys.stderr, flush=True)
-    makespan = max(r["done_s"] for r in results) - t0
-    srv.close()
-    lat = np.array([r["latency_s"] for r in results])
-    errs = [r["fp64_max_rel_err"] for r in results if r["fp64_max_rel_err"] >= 0]
-    return {"requests": nreq, "arrival_rate_per_s": rate, "runs_per_s": nreq / makespan,
-            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "failed": int(sum(r["status"] != 0 for r in results)),
-            "fp64_checks": len(errs), "fp64_max_rel_err_pzeta": max(errs) if errs else None,
-            "mix": "N 32/64/128/256 (0.4/0.3/0.2/0.1), 50% tabulated USR, 20% SIGW, every 20th with an FP64 twin; "
-                   "each run: ICs, inflation to af=2N, deltaN, P_zeta"}
+    check = np.arange(nreq) % 20 == 0
+    return [(int(sizes[n]), bool(tab[n]), int(sigw[n]), int(check[n])) for n in range(nreq)], rng
 
 
-def pzeta_check(N: int = 256) -> dict:
-    """P_zeta of split-storage runs against FP64 runs from the same initial
-    conditions at N^3: one whole run per potential and storage (initial
-    conditions, inflation to af = 2N, the deltaN map, P_zeta = k^3/(2 pi^2) P_dN),
-    served together.  Reported per potential: the largest relative difference
-    over the shells 1 <= k <= N/2 (complete spherical shells), over all shells
-    (the corner shells k > N/2 sqrt(2) hold a handful of modes each), and over 64
-    log-spaced k bins (shell values averaged per bin)."""
+def submit_request(srv, E, T, n, spec):
+    N, tab, sigw, check = spec
+    if tab:
+        cfg = E.make_config(N, precision=E.IE_SPLIT, numerical=1, sigw=sigw, table=T, **USR)
+        return srv.submit(cfg, 1000 + n, USR_PHI0, USR_DPHI0, 1, check, T)
+    cfg = E.make_config(N, precision=E.IE_SPLIT, sigw=sigw, **HILL)
+    return srv.submit(cfg, 1000 + n, PHI0, DPHI0, 1, check)
+
+
+def serve_stream(nreq: int, load: float = 0.8, burst: int = 24, seed: int = 20261203) -> dict:
+    """Serving: each request is a whole run (initial conditions, inflation to
+    af = 2N, Omega_GW for SIGW runs, the deltaN map, P_zeta).  Capacity is
+    measured first: `burst` requests of the mix submitted at once, runs/s over
+    their makespan.  Then `nreq` requests of the same mix (another draw) arrive
+    as a Poisson stream at load x capacity while the server keeps admitting by
+    footprint (240 GiB budget) and stepping whatever is resident."""
     import numpy as np
 
     import ie_engine as E
-    with np.load(REPO / "tests" / "golden" / "potential_table.npz") as z:
-        class T:  # table kept alive for the request pointers
-            x = np.ascontiguousarray(z["field"])
-            v = np.ascontiguousarray(z["potential"])
-            dv = np.ascontiguousarray(z["potential_derivative"])
-    srv = E.Server(240 << 30, 500, 4)
-    t0 = srv.clock()
-    ids = {}
-    for prec in (E.IE_SPLIT, E.IE_FP64):
-        cfg = E.make_config(N, precision=prec, V0=3.338e-13, ns=0.97, dt=0.0005, dN=1e-7, Nend=0.001)
-        ids[srv.submit(cfg, 8, 0.0935, 0.000796, 1, 0)] = ("hilltop", prec)
-        cfg = E.make_config(N, precision=prec, numerical=1, table=T, V0=3e-9, dt=0.001, dN=1e-4, Nend=2.0)
-        ids[srv.submit(cfg, 8, 2.9181235049318586, -0.06727651095116181, 1, 0, T)] = ("tabulated_usr", prec)
-    got = {}
-    while len(got) < len(ids):
-        srv.pump()
-        for r in srv.results():
-            got[ids[r["id"]]] = r
-    secs = srv.clock() - t0
-    srv.close()
-    res = {"N": N, "wall_s": secs,
-           "storage": "split (FP32 fluctuation about FP64 mode) vs FP64, same initial conditions"}
-    for name in ("hilltop", "tabulated_usr"):
-        a, b = got[(name, E.IE_SPLIT)], got[(name, E.IE_FP64)]
-        n = min(len(a["pzeta"]), len(b["pzeta"]))
-        pa, pb = a["pzeta"][:n], b["pzeta"][:n]
-        rel = np.where(pb > 0, np.abs(pa - pb) / np.where(pb > 0, pb, 1.0), 0.0)
-        # 64 log-spaced bins in k over the shells 1 .. n-1, each the mean of the
-        # shell values whose k falls in it (bins narrower than a shell stay empty)
-        shell = np.arange(1, n)
-        which = np.clip(np.digitize(shell, np.logspace(0.0, np.log10(n), 65)) - 1, 0, 63)
-        lrel = [abs(pa[shell[which == q]].mean() - pb[shell[which == q]].mean()) / pb[shell[which == q]].mean()
-                for q in range(64) if np.any(which == q) and pb[shell[which == q]].mean() > 0]
-        res[name] = {"status": [a["status"], b["status"]], "steps": a["steps"],
-                     "max_rel_err_shells_1_to_N_over_2": float(rel[1:N // 2 + 1].max()),
-                     "max_rel_err_all_shells": float(rel[1:].max()), "shells": n,
-                     "max_rel_err_64_log_bins": float(max(lrel)), "log_bins_filled": len(lrel)}
+    T = load_table()
+
+    def run_stream(specs, arrivals, label):
+        srv = E.Server(240 << 30, 100, 8)
+        results, n = [], 0
+        t0 = srv.clock()
+        while len(results) < len(specs):
+            now = srv.clock() - t0
+            while n < len(specs) and arrivals[n] <= now:
+                submit_request(srv, E, T, n, specs[n])
+                n += 1
+            srv.pump()
+            for r in srv.results():
+                results.append(r)
+                if len(results) % 10 == 0 or len(results) == len(specs):
+                    print(f"# serve {label}: {len(results)}/{len(specs)} done, last N={r['N']} "
+                          f"latency={r['latency_s']:.2f}s", file=sys.stderr, flush=True)
+        makespan = max(r["done_s"] for r in results) - t0
+        srv.close()
+        return results, makespan
+
+    bspecs, _ = request_stream(burst, seed + 1)
+    bres, bspan = run_stream(bspecs, np.zeros(burst), "capacity burst")
+    capacity = burst / bspan
+    rate = load * capacity
+    specs, rng = request_stream(nreq, seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    results, makespan = run_stream(specs, arrivals, f"poisson {rate:.3f}/s")
+    lat = np.array([r["latency_s"] for r in results])
+    errs = [r["fp64_max_rel_err"] for r in results if r["fp64_max_rel_err"] >= 0]
+    gerrs = [r["gw_fp64_max_rel_err"] for r in results if r["gw_fp64_max_rel_err"] >= 0]
+    return {"requests": nreq, "capacity_runs_per_s": capacity, "capacity_burst": burst,
+            "offered_load": load, "arrival_rate_per_s": rate, "runs_per_s": nreq / makespan,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "latency_samples": int(lat.size), "max_latency_s": float(lat.max()),
+            "failed": int(sum(r["status"] != 0 for r in results) + sum(r["status"] != 0 for r in bres)),
+            "sigw_runs_with_omega_gw": int(sum(r["gw_nbins"] > 0 for r in results)),
+            "fp64_checks": len(errs), "fp64_max_rel_err_pzeta": max(errs) if errs else None,
+            "fp64_max_rel_err_omega_gw": max(gerrs) if gerrs else None,
+            "mix": "N 32/64/128/256 (0.4/0.3/0.2/0.1), 50% tabulated USR, 20% SIGW, every 20th with an FP64 twin; "
+                   "each run: ICs, inflation to af=2N, Omega_GW (SIGW), deltaN, P_zeta"}
+
+
+def compare_spectra(pa, pb, nlog: int) -> dict:
+    """Split (pa) against FP64 (pb) spectra on the reference shells: the largest
+    relative difference over shells 1 .. n-1 with pb > 0, over the complete
+    spherical shells 1 .. N/2 (when n > N/2) and over nlog log-spaced bins in k
+    (each the mean of the shell values whose index falls in it; bins narrower
+    than a shell stay empty)."""
+    import numpy as np
+    n = min(len(pa), len(pb))
+    pa, pb = np.asarray(pa[:n]), np.asarray(pb[:n])
+    rel = np.where(pb > 0, np.abs(pa - pb) / np.where(pb > 0, pb, 1.0), 0.0)
+    shell = np.arange(1, n)
+    which = np.clip(np.digitize(shell, np.logspace(0.0, np.log10(n), nlog + 1)) - 1, 0, nlog - 1)
+    lrel = []
+    for q in range(nlog):
+        sel = shell[which == q]
+        if sel.size and pb[sel].mean() > 0:
+            lrel.append(abs(pa[sel].mean() - pb[sel].mean()) / pb[sel].mean())
+    return {"shells": n, "max_rel_err_shells": float(rel[1:].max()), "log_bins": nlog,
+            "log_bins_filled": len(lrel), "max_rel_err_log_bins": float(max(lrel)) if lrel else None}
+
+
+def whole_run(E, N, precision, sigw, numerical, T=None, seed=SEED):
+    """One whole run on its own: initial conditions, the leapfrog loop to
+    af = 2N, then ie_run_outputs (final sync, Omega_GW, deltaN, P_zeta)."""
+    if numerical:
+        cfg = E.make_config(N, precision=precision, numerical=1, sigw=sigw, table=T, **USR)
+        run = E.Run(cfg, T)
+        run.initialize(seed, USR_PHI0, USR_DPHI0)
+    else:
+        cfg = E.make_config(N, precision=precision, sigw=sigw, **HILL)
+        run = E.Run(cfg)
+        run.initialize(seed, PHI0, DPHI0)
+    run.evolve_fields(0.5 * cfg.dt)
+    t0 = time.perf_counter()
+    bg = run.leapfrog_steps(10 ** 8)
+    loop_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = run.outputs(1)
+    out_s = time.perf_counter() - t0
+    run.close()
+    if not bg["done"]:
+        raise RuntimeError(f"loop did not reach af: {bg}")
+    out["loop_s"], out["outputs_s"] = loop_s, out_s
+    return out
+
+
+def c2_tabulated(N: int = 128) -> dict:
+    """The tabulated SR -> USR -> SR potential at N^3 (params.numerical.txt:
+    dt = 0.001, dN = 1e-4, Nend = 2.0, seed 8): whole runs in split and FP64
+    storage from the same initial conditions; P_zeta in the reference shells and
+    in 64 log bins, split against FP64."""
+    import ie_engine as E
+    T = load_table()
+    a = whole_run(E, N, E.IE_SPLIT, 0, 1, T)
+    b = whole_run(E, N, E.IE_FP64, 0, 1, T)
+    res = {"N": N, "potential": "tabulated SR-USR-SR (inputs/ tables)", "steps": a["steps"],
+           "split_loop_s": a["loop_s"], "fp64_loop_s": b["loop_s"], "outputs_s": a["outputs_s"]}
+    res["pzeta"] = compare_spectra(a["pzeta"], b["pzeta"], 64)
+    return res
+
+
+def c3_sigw(N: int = 256) -> dict:
+    """The hilltop preset at N^3 with SIGW and deltaN (params.analytic.txt:
+    dt = 0.001, dN = 1e-7, Nend = 1e-3, seed 8): a whole split run and its FP64
+    twin from the same initial conditions; P_zeta in the 222 reference shells and
+    128 log bins, Omega_GW over the kept shells (floor(0.8 numbins)), each against
+    the twin."""
+    import ie_engine as E
+    a = whole_run(E, N, E.IE_SPLIT, 1, 0)
+    b = whole_run(E, N, E.IE_FP64, 1, 0)
+    res = {"N": N, "potential": "hilltop", "sigw": True, "steps": a["steps"], "a_end": a["a_end"],
+           "split_loop_s": a["loop_s"], "fp64_loop_s": b["loop_s"], "outputs_s": a["outputs_s"]}
+    res["pzeta"] = compare_spectra(a["pzeta"], b["pzeta"], 128)
+    res["omega_gw"] = compare_spectra(a["omega_gw"], b["omega_gw"], 128)
+    res["omega_gw"]["kept_shells"] = int(a["gw_nbins"])
     return res
 
 
 def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203) -> dict:
-    """Batched ensemble: nruns hilltop lattices of N^3, seeds
-    1000 + r, V0 log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split
-    storage.  The same number of steps is timed once as one batch (one launch per
-    kernel per step, HIP graph replay) and once as sequential per-run launches."""
+    """Batched ensemble: nruns hilltop lattices of N^3, seeds 1000 + r, V0
+    log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split storage,
+    shapes tuned before the batch is built.  The same number of steps is timed
+    once as one batch (one launch per kernel per step, HIP graph replay) and once
+    as sequential per-run launches; then the batch runs every lattice to
+    af = 2N and each run's P_zeta is computed (ie_run_outputs)."""
     from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
@@ -269,32 +355,62 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
     ns = rng.uniform(0.95, 0.98, nruns)
 
     def make(r):
-        cfg = E.make_config(N, precision=E.IE_SPLIT, V0=float(V0[r]), ns=float(ns[r]), dt=0.001, L=10.0)
+        cfg = E.make_config(N, precision=E.IE_SPLIT, V0=float(V0[r]), ns=float(ns[r]), dt=0.001, L=10.0,
+                            dN=1e-7, Nend=0.001)
         run = E.Run(cfg)
         run.initialize(1000 + r, PHI0, DPHI0)
         run.evolve_fields(0.0005)
         return run
 
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+    probe = make(0)
+    tuning = probe.tune(8)    # shapes of this (N, storage), taken by the batch
+    probe.close()
+    workers = min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
         runs = list(ex.map(make, range(nruns)))
     batch = E.Batch(runs)
     batch.steps(20, graph_steps=20)                # warm-up, graph instantiation
     t0 = time.perf_counter()
     batch.steps(steps, graph_steps=50)
     tb = time.perf_counter() - t0
-    batch.close()
     for r in runs[:4]:
         r.leapfrog_steps(5)                          # warm-up of the per-run path
     t0 = time.perf_counter()
-    for r in runs:
+    for r in runs[4:]:
         r.leapfrog_steps(steps)
-    ts = time.perf_counter() - t0
-    sites = nruns * N ** 3 * steps
+    ts = (time.perf_counter() - t0) * nruns / (nruns - 4)
+    batch.close()
+    # whole runs: a fresh batch of the same lattices to af, then per-run P_zeta
     for r in runs:
         r.close()
-    return {"runs": nruns, "N": N, "steps": steps, "batched_site_updates_per_s": sites / tb,
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        runs = list(ex.map(make, range(nruns)))
+    batch = E.Batch(runs)
+    t0 = time.perf_counter()
+    rounds = 0
+    while not all(r.background()["done"] for r in runs):
+        batch.steps(500, graph_steps=50)
+        rounds += 1
+        if rounds > 100:
+            raise RuntimeError("ensemble did not reach af")
+    t_whole = time.perf_counter() - t0
+    batch.close()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        outs = list(ex.map(lambda r: r.outputs(1), runs))
+    t_out = time.perf_counter() - t0
+    for r in runs:
+        r.close()
+    pz = np.array([o["pzeta"] for o in outs])
+    sites = nruns * N ** 3 * steps
+    return {"runs": nruns, "N": N, "steps": steps, "tuning": tuning,
+            "batched_site_updates_per_s": sites / tb,
             "sequential_site_updates_per_s": sites / ts, "batched_over_sequential": ts / tb,
-            "batched_ms_per_step": 1e3 * tb / steps, "sequential_ms_per_step": 1e3 * ts / steps}
+            "batched_ms_per_step": 1e3 * tb / steps, "sequential_ms_per_step": 1e3 * ts / steps,
+            "whole_runs": {"steps_per_run": int(outs[0]["steps"]), "loop_s": t_whole, "outputs_s": t_out,
+                           "pzeta_returned": int(sum(o["nbins"] > 0 for o in outs)), "pzeta_shells": int(pz.shape[1]),
+                           "pzeta_shell8_min": float(pz[:, 8].min()), "pzeta_shell8_max": float(pz[:, 8].max()),
+                           "pzeta_run0": [float(x) for x in pz[0]]}}
 
 
 def post_inflation(N: int = 256, steps: int = 200, warmup: int = 10, precision: int | None = None) -> dict:
@@ -354,12 +470,14 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--post-steps", type=int, default=200, help="post-inflation steps timed (0: skip)")
-    ap.add_argument("--pzeta-check", type=int, default=1, help="split vs FP64 P_zeta of whole runs (0: skip)")
-    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "pzeta", "serve", "tab"], default=None,
-                    help="time a single case (for profiling runs)")
+    ap.add_argument("--pzeta-check", type=int, default=1,
+                    help="whole runs, split vs FP64: 128^3 tabulated and 256^3 SIGW + deltaN (0: skip)")
+    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "c2", "c3", "serve", "tab", "ensemble"],
+                    default=None, help="time a single case (for profiling runs)")
     ap.add_argument("--ensemble-runs", type=int, default=256)
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="Poisson arrival rate as a fraction of the capacity measured first")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -378,11 +496,17 @@ def main() -> int:
     if args.only == "post":
         print(json.dumps({"post_inflation": post_inflation(N, args.post_steps)}), flush=True)
         return 0
-    if args.only == "pzeta":
-        print(json.dumps({"pzeta_split_vs_fp64": pzeta_check(N)}), flush=True)
+    if args.only == "c2":
+        print(json.dumps({"c2_tabulated_128": c2_tabulated(128)}), flush=True)
+        return 0
+    if args.only == "c3":
+        print(json.dumps({"c3_sigw_256": c3_sigw(N)}), flush=True)
+        return 0
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble(args.ensemble_runs)}), flush=True)
         return 0
     if args.only == "serve":
-        print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_rate)}), flush=True)
+        print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_load)}), flush=True)
         return 0
     if args.only == "tab":
         print(json.dumps({"tabulated_128": tabulated_case(128)}), flush=True)
@@ -457,8 +581,10 @@ def main() -> int:
         out["tabulated_128"] = tabulated_case(128)
         print(f"# tabulated 128: {out['tabulated_128']}", file=sys.stderr, flush=True)
     if args.pzeta_check and not args.only:
-        out["pzeta_split_vs_fp64"] = pzeta_check(N)
-        print(f"# pzeta check: {out['pzeta_split_vs_fp64']}", file=sys.stderr, flush=True)
+        out["c2_tabulated_128"] = c2_tabulated(128)
+        print(f"# c2: {out['c2_tabulated_128']}", file=sys.stderr, flush=True)
+        out["c3_sigw_256"] = c3_sigw(N)
+        print(f"# c3: {out['c3_sigw_256']}", file=sys.stderr, flush=True)
     if args.post_steps > 0 and not args.only:
         out["post_inflation"] = post_inflation(N, args.post_steps)
         print(f"# post-inflation: {out['post_inflation']}", file=sys.stderr, flush=True)
@@ -466,7 +592,7 @@ def main() -> int:
         out["ensemble"] = ensemble(args.ensemble_runs)
         print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)
     if args.serve_requests > 0 and not args.only:
-        out["serving"] = serve_stream(args.serve_requests, args.serve_rate)
+        out["serving"] = serve_stream(args.serve_requests, args.serve_load)
         out["p50_latency_s"] = out["serving"]["p50_latency_s"]
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     if not args.no_cpu_baseline:

@@ -103,6 +103,9 @@ ie_status ie_upload_tensor(ie_run *run, int comp, const float *h);
 ie_status ie_download_tensor(ie_run *run, int comp, float *out);
 ie_status ie_get_background(ie_run *run, ie_background *out);
 ie_status ie_set_background(ie_run *run, const ie_background *in);
+/* The run's configuration as ie_create took it (table pointers: the run's own
+ * host copies, valid while the run lives). */
+ie_status ie_get_config(const ie_run *run, ie_config *out);
 
 /* --- initial conditions ----------------------------------------------------- */
 /* Inputs of initializef() (src/initialize.cpp:118-214) that are not in ie_config. */
@@ -336,7 +339,9 @@ ie_status ie_batch_create(ie_run *const *runs, int nruns, ie_batch **out);
 ie_status ie_batch_steps(ie_batch *batch, int nsteps, int graph_steps);
 /* ie_batch_steps in two halves, so that several batches (each on its own stream)
  * run at once: ie_batch_launch enqueues the steps and returns; ie_batch_wait
- * waits for them and refreshes the runs' backgrounds and error flags. */
+ * waits for them and refreshes every run's background and error flag, then
+ * returns IE_ERR_TABLE_RANGE if any run raised one (the flag stays set in that
+ * run's background, so the caller can tell which). */
 ie_status ie_batch_launch(ie_batch *batch, int nsteps, int graph_steps);
 ie_status ie_batch_wait(ie_batch *batch);
 ie_status ie_batch_destroy(ie_batch *batch);
@@ -344,7 +349,7 @@ const char *ie_batch_last_error(const ie_batch *batch);
 
 /* --- serving ---------------------------------------------------------------- */
 /* A request loop over whole runs (initial conditions, inflation loop to af, then
- * the deltaN map and P_zeta).  Requests wait in FIFO order until their device
+ * the GW spectra and Omega_GW of SIGW runs and the deltaN map and P_zeta).  Requests wait in FIFO order until their device
  * footprint (ie_footprint_bytes) fits the budget; admitted runs stay resident
  * and are advanced together by batches (continuous batching: runs join and
  * leave between quanta); finished runs are completed on worker threads while
@@ -367,7 +372,21 @@ typedef struct ie_result {
     int32_t nbins;             /* P_zeta shells written to pzeta (<= 512) */
     double pzeta[512];         /* P_zeta(k) = k^3/(2 pi^2) spectraN on the modes.dat k (ie_pzeta_deltaN) */
     double fp64_max_rel_err;   /* max over shells of |P_split - P_fp64| / P_fp64, -1 if not checked */
+    int32_t gw_nbins;          /* SIGW runs: Omega_GW shells written to omega_gw, floor(0.8 numbins); else 0 */
+    double omega_gw[512];      /* Omega_GW(k) at the end of inflation (ie_omega_gw of spectraGW, spectraGWdot) */
+    double gw_fp64_max_rel_err;/* max over kept shells of the FP64 twin's Omega_GW difference, -1 if not checked */
 } ie_result;
+/* The end of one run as the reference's main() does after the inflation loop
+ * (src/main.cpp:129-146): the final leapfrog synchronisation
+ * (src/evolution.cpp:993-995); for a SIGW run spectraGW and spectraGWdot
+ * (src/output.cpp:631-648) and Omega_GW on the modes.dat k with that output's a
+ * and H (notebooks/plot.ipynb cell 23); with deltaN = 1 the deltaN phase
+ * (src/evolution.cpp:1291-1375) and saveN's zeta spectrum (src/output.cpp:
+ * 1522-1560) as P_zeta (notebooks/plot.ipynb cell 9).  Fills steps, a_end,
+ * nbins, pzeta, gw_nbins and omega_gw of *out and leaves its other fields.  The
+ * run must be in its inflation phase with its loop ended; the serving loop
+ * finishes every run with it. */
+ie_status ie_run_outputs(ie_run *run, int deltaN, ie_result *out);
 ie_status ie_server_create(size_t hbm_budget_bytes, int quantum_steps, int finish_threads, ie_server **out);
 /* Copy a request in (table pointers are copied too); *id identifies it. */
 ie_status ie_server_submit(ie_server *s, const ie_request *req, int64_t *id);

@@ -190,6 +190,7 @@ size_t ie_footprint_bytes(const ie_config *c)
     size_t b = 4 * gs * es;                        // f and fd ping-pong
     if (c->calculate_sigw) b += 18 * gs * 4;       // h ping-pong + hd
     b += gs * 8;                                   // scratch (downloads, deltaN)
+    if (c->precision == IE_SPLIT) b += 2 * gs * 8; // FP64 state of the deltaN phase
     b += ie::fft_workspace_bytes(c->N);            // spectra
     b += 2048 * 8 * 8 + 3 * sizeof(DevBg) + 7 * (size_t)std::max(c->tab_n, 0) * 8;
     return b;
@@ -258,6 +259,12 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
         }
     }
     if ((st = dev_alloc(r, (void **)&r->scratch, r->gs * 8)) != IE_OK) return bail(st);
+    // split runs: the FP64 state of the deltaN phase, allocated here so that no
+    // later phase allocates (ie_footprint_bytes counts it)
+    if (c.precision == IE_SPLIT) {
+        if ((st = dev_alloc(r, (void **)&r->dn_f, r->gs * 8)) != IE_OK) return bail(st);
+        if ((st = dev_alloc(r, (void **)&r->dn_fd, r->gs * 8)) != IE_OK) return bail(st);
+    }
     if ((st = dev_alloc(r, (void **)&r->partials, 2048 * 8 * 8)) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, (void **)&r->bg, sizeof(DevBg))) != IE_OK) return bail(st);
     if ((st = dev_alloc(r, (void **)&r->F.rec, 2 * sizeof(DevBg))) != IE_OK) return bail(st);
@@ -327,6 +334,13 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->fft = fft_create(c.N);
     if (!r->fft) return bail(fail(nullptr, IE_ERR_OOM, "FFT workspace allocation failed"));
     *out = r;
+    return IE_OK;
+}
+
+ie_status ie_get_config(const ie_run *r, ie_config *out)
+{
+    if (!r || !out) return IE_ERR_ARG;
+    *out = r->cfg;
     return IE_OK;
 }
 
@@ -1222,10 +1236,6 @@ ie_status ie_deltaN_init(ie_run *r)
     ie_status st = pull_bg(r);
     if (st != IE_OK) return st;
     if (r->cfg.precision == IE_SPLIT) {
-        if (!r->dn_f) {
-            if ((st = dev_alloc(r, (void **)&r->dn_f, r->gs * 8)) != IE_OK) return st;
-            if ((st = dev_alloc(r, (void **)&r->dn_fd, r->gs * 8)) != IE_OK) return st;
-        }
         IE_HIP(r, launch_reconstitute(r->F, 1, 0, r->bg, r->dn_f, r->stream));
         IE_HIP(r, launch_reconstitute(r->F, 1, 1, r->bg, r->dn_fd, r->stream));
     }
@@ -1537,10 +1547,6 @@ ie_status ie_post_init(ie_run *r, const ie_post_params *p, ie_background *out)
     if (st != IE_OK) return st;
     const int split = r->cfg.precision == IE_SPLIT;
     if (split) {
-        if (!r->dn_f) {
-            if ((st = dev_alloc(r, (void **)&r->dn_f, r->gs * 8)) != IE_OK) return st;
-            if ((st = dev_alloc(r, (void **)&r->dn_fd, r->gs * 8)) != IE_OK) return st;
-        }
         if (!r->in_deltaN) {
             IE_HIP(r, launch_reconstitute(r->F, 1, 0, r->bg, r->dn_f, r->stream));
             IE_HIP(r, launch_reconstitute(r->F, 1, 1, r->bg, r->dn_fd, r->stream));
@@ -1741,10 +1747,12 @@ ie_status ie_batch_wait(ie_batch *b)
 {
     if (!b) return IE_ERR_ARG;
     IE_BHIP(b, hipStreamSynchronize(b->stream));
+    int raised = 0;
     for (ie_run *r : b->runs) {
         if (pull_bg(r) != IE_OK) return bfail(b, IE_ERR_HIP, r->err);
-        if (r->bg_host->error) return bfail(b, IE_ERR_TABLE_RANGE, "field value outside the numerical potential range");
+        if (r->bg_host->error) raised = 1;
     }
+    if (raised) return bfail(b, IE_ERR_TABLE_RANGE, "field value outside the numerical potential range");
     return IE_OK;
 }
 

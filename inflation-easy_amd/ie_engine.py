@@ -67,11 +67,13 @@ class Result(C.Structure):
 
     _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("N", C.c_int32), ("submit_s", C.c_double),
                 ("admit_s", C.c_double), ("done_s", C.c_double), ("steps", C.c_int64), ("a_end", C.c_double),
-                ("nbins", C.c_int32), ("pzeta", C.c_double * 512), ("fp64_max_rel_err", C.c_double)]
+                ("nbins", C.c_int32), ("pzeta", C.c_double * 512), ("fp64_max_rel_err", C.c_double),
+                ("gw_nbins", C.c_int32), ("omega_gw", C.c_double * 512), ("gw_fp64_max_rel_err", C.c_double)]
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pzeta"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pzeta", "omega_gw")}
         d["pzeta"] = np.array(self.pzeta[: self.nbins])
+        d["omega_gw"] = np.array(self.omega_gw[: self.gw_nbins])
         d["latency_s"] = self.done_s - self.submit_s
         return d
 
@@ -157,6 +159,8 @@ def lib() -> C.CDLL:
             "ie_download_tensor": (st, [P, C.c_int, fp]),
             "ie_get_background": (st, [P, C.POINTER(Background)]),
             "ie_set_background": (st, [P, C.POINTER(Background)]),
+            "ie_get_config": (st, [P, C.POINTER(Config)]),
+            "ie_run_outputs": (st, [P, C.c_int, C.POINTER(Result)]),
             "ie_gradient_energy": (st, [P, dp]),
             "ie_kin_energy": (st, [P, dp]),
             "ie_potential_energy": (st, [P, dp]),
@@ -415,6 +419,14 @@ class Run:
         v = np.zeros(nb)
         self._check(lib().ie_gw_spectrum(self.h, hdot, _dp(p), _ip(c), _dp(v)))
         return p, c, v
+
+    def outputs(self, deltaN: int = 1) -> dict:
+        """End of the run (ie_run_outputs): final sync, Omega_GW of a SIGW run,
+        the deltaN phase and P_zeta."""
+        r = Result()
+        self._check(lib().ie_run_outputs(self.h, deltaN, C.byref(r)))
+        d = r.as_dict()
+        return {k: d[k] for k in ("steps", "a_end", "nbins", "pzeta", "gw_nbins", "omega_gw")}
 
     # ---- deltaN
     def deltaN_init(self):

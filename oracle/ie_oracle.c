@@ -2138,6 +2138,38 @@ static int program_inflation(ieo_state *st, ieo_out *o)
     return 0;
 }
 
+int ieo_deltaN_zeta_map(ieo_state *st, float *nmean_out)
+{
+    const ieo_config *c = &st->cfg;
+    st->Ne = 0.0;
+    ieo_initializeN(st);
+    ieo_evolve_fieldsN(st, 0.5 * c->dN);
+    st->phiref = c->use_phiref_manual ? c->phiref_manual_value : ieo_get_phiref(st);
+    st->phiref_potential = ieo_potential(st, st->phiref);
+    int n = 0;
+    while (st->Ne <= c->Nend) {
+        ieo_evolve_derivsN(st, c->dN);
+        ieo_evolve_fieldsN(st, c->dN);
+        st->Ne += c->dN;
+        ++n;
+    }
+    ieo_evolve_fieldsN(st, -.5 * c->dN);
+    float Nmean = 0.;
+    const double cutoff = c->Nend - c->dN;
+    for (size_t id = 0; id < st->gs; ++id)
+        if (st->dn[id] <= cutoff) Nmean += st->dn[id];
+    Nmean = Nmean / (int)st->gs;
+    for (size_t id = 0; id < st->gs; ++id) st->dn[id] -= Nmean;
+    for (size_t id = 0; id < st->gs; ++id)
+        if (st->dn[id] > (cutoff - Nmean)) st->dn[id] = 0;
+    double mean = 0.0;
+    for (size_t id = 0; id < st->gs; ++id) mean += st->dn[id];
+    mean /= (double)st->gs;
+    for (size_t id = 0; id < st->gs; ++id) st->dn[id] -= mean;
+    if (nmean_out) *nmean_out = Nmean;
+    return n;
+}
+
 /* saveN (src/output.cpp:1522-1560) with spectraN (:1140-1169) */
 static void program_saveN(ieo_state *st, ieo_out *o)
 {

@@ -157,6 +157,13 @@ void ieo_initializeN(ieo_state *st);
 void ieo_evolve_derivsN(ieo_state *st, double d);
 void ieo_evolve_fieldsN(ieo_state *st, double d);
 double ieo_get_phiref(ieo_state *st);
+/* run_deltaN_loop with the leapfrog integrator (src/evolution.cpp:1296-1328) and
+ * saveN's zeta map without its outputs (src/output.cpp:1523-1553): the float
+ * mean over finished sites subtracted, unfinished sites zeroed, then the FP64
+ * mean removed as spectraN's build_deltaN_linear does (src/output.cpp:156-164);
+ * st->dn holds the map spectraN transforms.  Returns the deltaN steps taken,
+ * *nmean the float mean. */
+int ieo_deltaN_zeta_map(ieo_state *st, float *nmean);
 void ieo_rk4_stepN(ieo_state *st, double h);
 int ieo_rk45_stepN(ieo_state *st, double h, double hmax, double *h_next);
 

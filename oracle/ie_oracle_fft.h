@@ -60,16 +60,20 @@ static void IEO_FN(ieo_fftc1_)(IEO_R *data, int n, int skip, int forward)
 }
 
 /* 3-D complex transform over sizes (n0, n1, n2) complex points, last dimension
- * first (src/ffteasy.hpp:74-88). */
+ * first (src/ffteasy.hpp:74-88).  The 1-D transforms of one dimension touch
+ * disjoint elements, so they run on OpenMP threads with each transform's
+ * operations unchanged (the same bits as the serial loop). */
 static void IEO_FN(ieo_fftc3_)(IEO_R *data, const int size[3], int forward)
 {
     int plane = 1, skip = 1;
     const int total = size[0] * size[1] * size[2];
     for (int dim = 2; dim >= 0; --dim) {
         plane *= size[dim];
-        for (int i = 0; i < total; i += plane)
-            for (int j = 0; j < skip; ++j)
-                IEO_FN(ieo_fftc1_)(data + 2 * (i + j), size[dim], skip, forward);
+        const int nlines = total / plane, n = size[dim], sk = skip, pl = plane;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 0; i < nlines; ++i)
+            for (int j = 0; j < sk; ++j)
+                IEO_FN(ieo_fftc1_)(data + 2 * ((size_t)i * pl + j), n, sk, forward);
         skip *= size[dim];
     }
 }

@@ -136,6 +136,7 @@ def lib() -> C.CDLL:
             "ieo_evolve_derivsN": (None, [P, C.c_double]),
             "ieo_evolve_fieldsN": (None, [P, C.c_double]),
             "ieo_get_phiref": (C.c_double, [P]),
+            "ieo_deltaN_zeta_map": (C.c_int, [P, C.POINTER(C.c_float)]),
             "ieo_rk4_stepN": (None, [P, C.c_double]),
             "ieo_rk45_stepN": (C.c_int, [P, C.c_double, C.c_double, dp]),
             "ieo_post_initialize": (None, [P]),

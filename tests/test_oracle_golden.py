@@ -51,3 +51,31 @@ def test_oracle_reproduces_reference(case, tmp_path):
         assert np.array_equal(got, ref), f"{case}/{key}: max |diff| {np.max(np.abs(got - ref))}"
         checked += 1
     assert checked >= (10 if golden_toggles(g)["post"] else 14)
+
+
+@pytest.mark.parametrize("case", ["hilltop_n16", "tabulated_n16", "hilltop_n32", "tabulated_n32"])
+def test_oracle_deltaN_zeta_map_gives_reference_spectraN(case):
+    """ieo_deltaN_zeta_map (the deltaN phase and saveN's map, no files) after the
+    inflation loop and its final sync, then the FFT and shell binning, reproduces
+    the reference's spectraN.dat as printed (%e: 7 significant digits)."""
+    g = load_golden(case)
+    N = int(g["N"])
+    numerical = int(g["numerical_potential"])
+    params = O.parse_params_text(str(g["params_text"]))
+    table = potential_table() if numerical else None
+    cfg = O.make_config(N, numerical, params, table, **golden_toggles(g))
+    st = O.State(cfg, table)
+    st.initialize()
+    O.lib().ieo_evolve_fields(st.ptr, 0.5 * cfg.dt)
+    O.lib().ieo_leapfrog_steps(st.ptr, 10 ** 8)
+    O.lib().ieo_evolve_fields(st.ptr, -0.5 * cfg.dt * st.background()["astep"] ** (cfg.rescale_s - 1.0))
+    n = O.lib().ieo_deltaN_zeta_map(st.ptr, None)
+    assert n > 0
+    norm1 = (cfg.L / cfg.rescale_B) ** 3 / N ** 6
+    F, ny = O.fftrn(st.field(2), +1)
+    p, c, v = O.bin_spectrum(N, cfg.L, F, ny, norm1)
+    ref = g["spectraN"]
+    ref = ref[-1] if ref.ndim == 3 else ref
+    assert np.array_equal(c, ref[:, 1].astype(int))
+    got = np.array([float(f"{x:e}") for x in v])
+    assert np.array_equal(got, ref[:, 2])
