@@ -16,7 +16,8 @@ initializer restated, seed 8 of the presets, dt = 0.001) unless stated.
     zeta map and P_zeta in the 111 shells and in 64 log-spaced bins; P_zeta of
     the split run's own deltaN phase against the oracle's;
   - 256 runs of 64^3 in one batch (seeds 1000 + r, V0 log-uniform in
-    [1e-11, 1e-9], ns uniform in [0.95, 0.98]), whole runs with per-run P_zeta:
+    [1e-11, 1e-9], ns uniform in [0.95, 0.98]; the preset's deltaN window scaled
+    by sqrt(V0 / V0_preset)), whole runs with per-run P_zeta:
     8 sampled runs bitwise equal to the same runs alone, one against the oracle;
   - rescale_s = 1 (the step's scale-factor powers through the device pow).
 Each test prints the errors it measures.
@@ -227,9 +228,19 @@ def test_tabulated_128_loop_and_deltaN_match_oracle():
 PHI0, DPHI0 = 0.0935, 0.000796
 
 
-def ensemble_run(r, V0, ns, af, Nend):
+def deltaN_window(V0):
+    """dN and Nend of the hilltop preset (1e-7, 1e-3 at V0 = 3.338e-13) scaled by
+    sqrt(V0 / 3.338e-13): V0 only rescales the fluctuations (V / B^2 does not
+    depend on it), so the spread of the sites' e-folds, and with it the window
+    the deltaN phase needs, grows as sqrt(V0)."""
+    s = float(np.sqrt(V0 / 3.338e-13))
+    return 1e-7 * s, 1e-3 * s
+
+
+def ensemble_run(r, V0, ns, af):
+    dN, Nend = deltaN_window(V0[r])
     cfg = E.make_config(64, precision=E.IE_SPLIT, V0=float(V0[r]), ns=float(ns[r]), dt=0.001, L=10.0, af=af,
-                        dN=1e-7, Nend=Nend)
+                        dN=dN, Nend=Nend)
     run = E.Run(cfg)
     run.initialize(1000 + r, PHI0, DPHI0)
     run.evolve_fields(0.0005)
@@ -245,12 +256,12 @@ def step_until_done(batch, runs, chunk=256):
 
 
 def test_ensemble_256x64_per_run_pzeta():
-    nruns, af, Nend = 256, 4.0, 1e-3
+    nruns, af = 256, 4.0
     print(f"ensemble {nruns} x 64^3 to af={af}", flush=True)
     rng = np.random.default_rng(20261203)
     V0 = 10.0 ** rng.uniform(-11, -9, nruns)
     ns = rng.uniform(0.95, 0.98, nruns)
-    runs = [ensemble_run(r, V0, ns, af, Nend) for r in range(nruns)]
+    runs = [ensemble_run(r, V0, ns, af) for r in range(nruns)]
     batch = E.Batch(runs)
     step_until_done(batch, runs)
     batch.close()
@@ -266,7 +277,7 @@ def test_ensemble_256x64_per_run_pzeta():
         r.close()
     sample = [0, 37, 64, 101, 128, 170, 211, 255]
     for q in sample:
-        alone = ensemble_run(q, V0, ns, af, Nend)
+        alone = ensemble_run(q, V0, ns, af)
         b1 = E.Batch([alone])
         step_until_done(b1, [alone])
         b1.close()
@@ -275,9 +286,10 @@ def test_ensemble_256x64_per_run_pzeta():
         assert o["steps"] == outs[q]["steps"] and o["a_end"] == outs[q]["a_end"], q
         assert np.array_equal(o["pzeta"], outs[q]["pzeta"]), q
     # run 0 against the oracle (FP64) from the same initial conditions
+    dN0, Nend0 = deltaN_window(V0[0])
     cfg = O.make_config(64, 0, {"dt": 0.001, "V0": float(V0[0]), "ns": float(ns[0]), "seed": 1000,
                                 "initial_field": PHI0, "initial_derivative": DPHI0, "L": 10.0, "af": af,
-                                "dN": 1e-7, "Nend": Nend}, None, sigw=0)
+                                "dN": dN0, "Nend": Nend0}, None, sigw=0)
     st = O.State(cfg, None)
     st.initialize()
     O.lib().ieo_evolve_fields(st.ptr, 0.0005)
