@@ -74,6 +74,8 @@ typedef struct ie_run ie_run;
  * src/main.cpp:15-37 and the checks of initialize(), src/initialize.cpp:93-114). */
 ie_status ie_create(const ie_config *cfg, ie_run **out);
 ie_status ie_destroy(ie_run *run);
+/* The message of the last failed call on run, or with run = NULL of the last
+ * failed call on this thread that had no run (ie_create, ie_spectrum_noalloc). */
 const char *ie_last_error(const ie_run *run);
 /* Bytes of device memory the run holds (admission by HBM footprint). */
 size_t ie_footprint_bytes(const ie_config *cfg);
@@ -223,6 +225,22 @@ ie_status ie_rk45_step(ie_run *run, double h, double hmax, int *accepted, double
  * forward FFT is out of place; the field is not modified. */
 int ie_spectrum_numbins(int N);
 ie_status ie_spectrum(ie_run *run, int which, double norm1, double *p, int *counts, double *vals);
+/* ie_spectrum without a run handle and without allocating: the isotropic power
+ * spectrum, in the same shells and with the same kernels and numbers as
+ * ie_spectrum, of an FP64 field that the caller already holds in device memory
+ * (f_dev: N^3 doubles, z-fastest as idx(i,j,k) = (i N + j) N + k; the field is
+ * not modified).  N must be a power of two with 8 <= N <= 128: the kernels'
+ * workspace (half spectrum, binning partials, shell sums, twiddles: about
+ * 17.5 MB) is a fixed device array of the library that each GPU using this
+ * entry point holds until the process ends; larger N is refused with IE_ERR_ARG
+ * (message in ie_last_error(NULL)).  stream: a hipStream_t, NULL for the null
+ * stream; the call returns when p, counts and vals are written.  Calls on one GPU
+ * share its workspace, so calls that overlap on one GPU race.
+ * No counterpart in the reference: this entry point is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+ie_status ie_spectrum_noalloc(const double *f_dev, int N, double L, double norm1, double *p, int *counts,
+                              double *vals, void *stream);
 /* TT-projected tensor spectrum (write_gw_spectrum_impl, src/output.cpp:466-625):
  * hdot = 0 for spectraGW, 1 for spectraGWdot (to_phys = B a^{s-1}).
  * numbins = int(sqrt(3)*(N/2))+1; bins >= floor(0.8 numbins) are empty. */

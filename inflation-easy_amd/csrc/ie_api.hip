@@ -169,6 +169,10 @@ __global__ void k_loop_begin(DevBg *bg)
 
 }  // namespace
 
+namespace ie {
+void set_unbound_error(const std::string &msg) { g_create_error = msg; }
+}  // namespace ie
+
 extern "C" {
 
 const char *ie_version(void) { return "inflation-easy-amd 0.1 (gfx950)"; }

@@ -11,6 +11,15 @@ struct FftPlan;
 size_t fft_workspace_bytes(int N);
 FftPlan *fft_create(int N);
 void fft_destroy(FftPlan *p);
+// The twiddle table of a plan of size N (N complex: cos, sin of 2 pi k / N).
+void fft_twiddles(int N, double2 *tw);
+// Scalar spectrum of an FP64 field over caller memory, with no allocation:
+// `work` holds fft_view_bytes(N) bytes of device memory, `tw` the N twiddles in
+// device memory.  The same kernels and shell sums as fft_forward_double followed
+// by fft_bin_scalar; sums_host receives fft_numbins(N) values.
+size_t fft_view_bytes(int N);
+hipError_t fft_spectrum_in(int N, void *work, const double2 *tw, const double *in, double *sums_host,
+                           hipStream_t s);
 
 // Number of reference shells: int(1.73205*(N/2))+1 (src/output.cpp:69).
 int fft_numbins(int N);

@@ -305,6 +305,24 @@ size_t fft_workspace_bytes(int N)
            (size_t)bin_blocks_for(modes) * nb * 8 + (size_t)nb * 8;
 }
 
+void fft_twiddles(int N, double2 *tw)
+{
+    const double two_pi = 4.0 * asin(1.0);
+    for (int k = 0; k < N; ++k) tw[k] = make_double2(cos(two_pi * k / N), sin(two_pi * k / N));
+}
+
+// Parts of a one-slot plan over caller memory, each 256-byte aligned: the half
+// spectrum, the binning partials, the shell sums.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t fft_view_bytes(int N)
+{
+    const size_t modes = (size_t)N * N * (N / 2 + 1);
+    const int nb = fft_gw_numbins(N);
+    return align256(modes * sizeof(double2)) + align256((size_t)bin_blocks_for(modes) * nb * 8) +
+           align256((size_t)nb * 8);
+}
+
 FftPlan *fft_create(int N)
 {
     FftPlan *p = new FftPlan();
@@ -323,8 +341,7 @@ FftPlan *fft_create(int N)
         return nullptr;
     }
     std::vector<double2> tw(N);
-    const double two_pi = 4.0 * asin(1.0);
-    for (int k = 0; k < N; ++k) tw[k] = make_double2(cos(two_pi * k / N), sin(two_pi * k / N));
+    fft_twiddles(N, tw.data());
     if (hipMemcpy(p->tw, tw.data(), N * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
         fft_destroy(p);
         return nullptr;
@@ -418,6 +435,29 @@ hipError_t fft_bin_scalar_dc(FftPlan *p, int slot, double dc_add, double *sums_h
 hipError_t fft_bin_scalar(FftPlan *p, int slot, double *sums_host, hipStream_t s)
 {
     return fft_bin_scalar_dc(p, slot, 0.0, sums_host, s);
+}
+
+hipError_t fft_spectrum_in(int N, void *work, const double2 *tw, const double *in, double *sums_host,
+                           hipStream_t s)
+{
+    FftPlan p;
+    p.N = N;
+    p.logN = ilog2(N);
+    p.K = N / 2 + 1;
+    p.modes = (size_t)N * N * p.K;
+    p.bin_blocks = bin_blocks_for(p.modes);
+    char *w = (char *)work;
+    p.H[0] = (double2 *)w;
+    w += align256(p.modes * sizeof(double2));
+    p.partials = (double *)w;
+    w += align256((size_t)p.bin_blocks * fft_gw_numbins(N) * 8);
+    p.sums = (double *)w;
+    p.tw = (double2 *)tw;
+    hipError_t e = forward<double>(&p, in, 0, s);
+    if (e != hipSuccess) return e;
+    e = fft_bin_scalar_dc(&p, 0, 0.0, sums_host, s);
+    p.H[0] = nullptr;   // caller memory: nothing for the destructor-free plan to own
+    return e;
 }
 
 hipError_t fft_bin_tt(FftPlan *p, double dx, double to_phys, double *sums_host, hipStream_t s)

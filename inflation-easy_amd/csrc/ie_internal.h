@@ -716,4 +716,8 @@ hipError_t rk45_step(RkWork *w, const Pot &P, DevBg *bg, DevBg *bg_host, double 
                      float *const *hd, double step, double abs_tol, double rel_tol, double safety, int *accepted,
                      double *factor, double *err_norm, hipStream_t s);
 
+// The message ie_last_error(NULL) returns on the calling thread (entry points
+// without a run handle).
+void set_unbound_error(const std::string &msg);
+
 }  // namespace ie

@@ -175,6 +175,7 @@ def lib() -> C.CDLL:
             "ie_rk45_step": (st, [P, C.c_double, C.c_double, ip, dp]),
             "ie_spectrum_numbins": (C.c_int, [C.c_int]),
             "ie_spectrum": (st, [P, C.c_int, C.c_double, dp, ip, dp]),
+            "ie_spectrum_noalloc": (st, [C.c_void_p, C.c_int, C.c_double, C.c_double, dp, ip, dp, C.c_void_p]),
             "ie_gw_spectrum": (st, [P, C.c_int, dp, ip, dp]),
             "ie_deltaN_init": (st, [P]),
             "ie_evolve_fieldsN": (st, [P, C.c_double]),
