@@ -334,6 +334,12 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     b.af = c.af;
     b.rescale_s = c.rescale_s;
     b.dx = dx;
+    // fused loops of the hilltop sum f and f^2 for the Friedmann step (V is
+    // quadratic in f); the close turns them into sum V and sum V'
+    b.hill_sums = c.numerical_potential ? 0 : (c.precision == IE_SPLIT ? 2 : 1);
+    b.hV = r->P.hV;
+    b.hV4 = r->P.hV4;
+    b.hV2 = r->P.hV2;
     if (push_bg(r) != IE_OK) return bail(IE_ERR_HIP);
     r->fft = fft_create(c.N);
     if (!r->fft) return bail(fail(nullptr, IE_ERR_OOM, "FFT workspace allocation failed"));

@@ -44,7 +44,11 @@ struct DevBg {
     int done, error, parity, pad0;
     unsigned int counter;
     unsigned int pad;
-    int marched, pad1;                          // step record of a fused loop: its step has run
+    int marched;                                // step record of a fused loop: its step has run
+    int hill_sums;                              // fused loops of the hilltop potential sum f and f^2 in
+                                                // place of V and V' (1: FP64 storage, 2: split storage,
+                                                // f the fluctuation about phibar); 0: V and V'
+    double hV, hV4, hV2;                        // the hilltop's V0/B^2, V0 (1-ns)/4 /B^2, V0 (1-ns)/2 /B^2
 };
 
 // Potential parameters (src/potential.cpp:77-83 analytic hilltop; tabulated
@@ -502,6 +506,68 @@ __device__ __forceinline__ void st_row(T *__restrict__ row, unsigned lane, const
 #pragma unroll
     for (int e = 0; e < E; ++e) x.v[e] = v[e];
     reinterpret_cast<RowVec<T, E> *>(row)[lane] = x;
+}
+
+// Buffer resource over `bytes` bytes of a field (raw dword access; an access past
+// the end reads 0 and writes nothing).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+typedef unsigned ie_v2u __attribute__((ext_vector_type(2)));
+typedef unsigned ie_v4u __attribute__((ext_vector_type(4)));
+
+// A lane's E values of the row at byte offset `soff` (wave-uniform, a scalar
+// register) through a buffer resource: no per-lane address arithmetic.
+template <typename T, int E>
+__device__ __forceinline__ void bld_row(__amdgpu_buffer_rsrc_t r, unsigned soff, unsigned lane, T (&o)[E])
+{
+    constexpr unsigned B = (unsigned)sizeof(T) * E;
+    const unsigned vo = lane * B;
+    RowVec<T, E> x;
+    if constexpr (B == 4) {
+        x = __builtin_bit_cast(RowVec<T, E>, __builtin_amdgcn_raw_buffer_load_b32(r, vo, soff, 0));
+    } else if constexpr (B == 8) {
+        x = __builtin_bit_cast(RowVec<T, E>, __builtin_amdgcn_raw_buffer_load_b64(r, vo, soff, 0));
+    } else if constexpr (B == 16) {
+        x = __builtin_bit_cast(RowVec<T, E>, __builtin_amdgcn_raw_buffer_load_b128(r, vo, soff, 0));
+    } else {
+        static_assert(B == 32, "rows of 4, 8, 16 or 32 bytes per lane");
+        struct alignas(32) Two {
+            ie_v4u a, b;
+        } t;
+        t.a = __builtin_amdgcn_raw_buffer_load_b128(r, vo, soff, 0);
+        t.b = __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16, soff, 0);
+        x = __builtin_bit_cast(RowVec<T, E>, t);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) o[e] = x.v[e];
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void bst_row(__amdgpu_buffer_rsrc_t r, unsigned soff, unsigned lane, const T (&v)[E])
+{
+    constexpr unsigned B = (unsigned)sizeof(T) * E;
+    const unsigned vo = lane * B;
+    RowVec<T, E> x;
+#pragma unroll
+    for (int e = 0; e < E; ++e) x.v[e] = v[e];
+    if constexpr (B == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, soff, 0);
+    } else if constexpr (B == 8) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ie_v2u, x), r, vo, soff, 0);
+    } else if constexpr (B == 16) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ie_v4u, x), r, vo, soff, 0);
+    } else {
+        static_assert(B == 32, "rows of 4, 8, 16 or 32 bytes per lane");
+        struct alignas(32) Two {
+            ie_v4u a, b;
+        };
+        const Two t = __builtin_bit_cast(Two, x);
+        __builtin_amdgcn_raw_buffer_store_b128(t.a, r, vo, soff, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(t.b, r, vo + 16, soff, 0);
+    }
 }
 
 // Work of one wave: rows j0 .. j0+R-1 of plane i, k-segment [k0, k0 + 64 E).

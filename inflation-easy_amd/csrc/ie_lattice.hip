@@ -255,11 +255,25 @@ __device__ __forceinline__ StepScalars step_scalars(const DevBg *bg, double gsd)
 }
 #pragma clang fp contract(fast)
 
+// The fused loop's sums of a hilltop lattice are those of f and f^2 (f the
+// stored value: the fluctuation about phibar in split storage); with
+// V = hV - hV4 (phibar + f)^2 and V' = -hV2 (phibar + f) they give sum V and
+// sum V' over the gsd sites.
+__device__ __forceinline__ void hill_sums(const DevBg &b, double gsd, double (&tot)[3])
+{
+    if (!b.hill_sums) return;
+    const double pb = b.hill_sums == 2 ? b.phibar : 0.0;
+    const double s1 = tot[1], s2 = tot[2];
+    tot[1] = gsd * (b.hV - b.hV4 * (pb * pb)) - b.hV4 * (2.0 * pb * s1 + s2);
+    tot[2] = -b.hV2 * (gsd * pb + s1);
+}
+
 // Closes a fused loop step on a background record: applies the step's scalar
 // updates (b.nx, the values its kernels used), stores the fixed-order sums of the
 // new f for the next step and evaluates that step's scalars.
-__device__ void close_state(DevBg &b, const double (&tot)[3], double gsd, int split)
+__device__ void close_state(DevBg &b, const double (&tot_in)[3], double gsd, int split)
 {
+    double tot[3] = {tot_in[0], tot_in[1], tot_in[2]};
     const StepScalars S = b.nx;
     b.d = S.d;
     b.ad2 = S.ad2;
@@ -279,6 +293,7 @@ __device__ void close_state(DevBg &b, const double (&tot)[3], double gsd, int sp
     b.steps += 1;
     b.astep = b.a;
     if (!(b.a <= b.af)) b.done = 1;
+    hill_sums(b, gsd, tot);   // of the new f, about the new phibar
     b.red[0] = tot[0];
     b.red[1] = tot[1];
     b.red[2] = tot[2];
@@ -329,6 +344,7 @@ __device__ void sums_finish_body(DevBg *bg, const double *partials, unsigned G, 
         double tot[3];
         sum_partials<3>(partials, (int)G, tot, smem);
         if (threadIdx.x == 0) {
+            hill_sums(*bg, (double)gs, tot);
             bg->red[0] = tot[0];
             bg->red[1] = tot[1];
             bg->red[2] = tot[2];
@@ -666,19 +682,49 @@ __device__ __forceinline__ double uniform(double x)
 // SUMS: the sums alone, of the current f taken as the new f (loop start), with
 // the same partition and operation order as a step's, so that a loop split into
 // several calls gives the same bits as one call.
-// PF: the rows of plane i + PF are loaded before plane i's arithmetic (PF = 1, 2;
-// more registers, loads in flight across the barriers); the same results.  The
-// plane loop is unrolled by 3 over three row sets, so a row set in flight is never
-// moved between registers (a move waits for its load).
+// The rows a plane step streams from HBM (f of plane i + 1, fd of plane i) are
+// loaded one plane step ahead; the plane loop is unrolled by 3 over three row
+// sets, so a row set in flight is never moved between registers (a move waits
+// for its load).
+// JL: where a kick finds its j +- 1 rows of old f.  false: loaded from L2 with
+// the plane's other rows (the neighbouring waves streamed them from HBM one step
+// earlier); true: the neighbouring waves' own rows, passed through LDS in FP64
+// (the tile's halo rows j0 - 1 and j0 + 9 by waves 0 and 8) and read right after
+// the previous plane's barrier, so that the sums' arithmetic covers their
+// latency.  The same values either way, so the same results.
+// POT: the hilltop potential (kPotHill: V' linear in f, so its Friedmann sums are
+// those of f and f^2, turned into sum V and sum V' by the step's close) or the
+// tabulated one (kPotTab: V and V' interpolated per site).
+// Loads and stores go through buffer resources with the row offset in a scalar
+// register, so a row costs no per-lane address arithmetic.
 // A step launch first closes the previous step from its record in slot sp (its
 // first rows already in flight) and writes the result to the other slot.
-template <typename T, bool SPLIT, int E, bool SUMS = false, int PF = 0>
+constexpr int kPotHill = 0, kPotTab = 1;
+
+// LDS rows of E values per lane stored as two halves of E/2 (8 or 16 bytes per
+// lane in each), so that each LDS access of a wave is free of bank conflicts.
+template <typename U, int E>
+struct LdsRow {
+    U v[2][64][E / 2];
+    __device__ __forceinline__ U &at(unsigned lane, int e) { return v[e / (E / 2)][lane][e % (E / 2)]; }
+};
+
+template <typename T, int E, bool JL>
+struct MarchLds {
+    LdsRow<double, E> oldf[JL ? 2 : 1][JL ? kTileJ + 3 : 1];   // old f of plane t, rows j0 - 1 .. j0 + 9
+    LdsRow<T, E> newf[2][kMarchWaves];                          // new f of plane t, rows j0 .. j0 + 8
+    double smem[3 * kMarchWaves];
+};
+
+template <typename T, bool SPLIT, int E, int POT, bool JL, bool SUMS = false>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg *bg, double *partials, unsigned blk,
-                                           T (*lds)[kMarchWaves][64 * E], double *smem, unsigned sp = 0)
+                                           MarchLds<T, E, JL> &L, unsigned sp = 0)
 {
+    static_assert(E == 2 || E == 4, "fused steps hold 2 or 4 sites per lane");
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
-    const unsigned w = threadIdx.x >> 6;
+    const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned G = (N / kTileJ) * (N / march_ti(N));
+    constexpr bool kLds = JL && !SUMS;
     int par;
     bool pmarched = false;
     __shared__ DevBg sb;        // step launches: this step's record
@@ -700,24 +746,31 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         par = prev->parity ^ (pmarched ? 1 : 0);
     }
     const PotView Pv = stage_pot(P0);
-    const T *__restrict__ f = (const T *)F.f[par];
-    T *__restrict__ fout = (T *)F.f[par ^ 1];
     // fd is read from [par] and written to [par ^ 1]: the halo row and the extra
     // plane read fd rows that their owning workgroups update in the same launch
-    const T *__restrict__ fd = (const T *)F.fd[par];
-    T *__restrict__ fdout = (T *)F.fd[par ^ 1];
+    const unsigned bytes = N * N * N * (unsigned)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rf = field_rsrc(F.f[par], bytes), rfo = field_rsrc(F.f[par ^ 1], bytes);
+    const __amdgpu_buffer_rsrc_t rv = field_rsrc(F.fd[par], bytes), rvo = field_rsrc(F.fd[par ^ 1], bytes);
     // tile of this workgroup: XCD blk % 8 owns planes [x N/8, (x+1) N/8)
     const unsigned TI = march_ti(N), jblocks = N / kTileJ;
     const unsigned xcd = blk % kXcd, lb = blk / kXcd;
     const unsigned i0 = xcd * (N / kXcd) + (lb / jblocks) * TI;
-    const unsigned j = ((lb % jblocks) * kTileJ + w) & m;
+    const unsigned j0 = (lb % jblocks) * kTileJ;
+    const unsigned j = (j0 + w) & m, jp = (j + 1) & m, jm = (j - 1) & m;
     const bool owner = w < (unsigned)kTileJ;
-    const unsigned rj = j << logN, rjp = ((j + 1) & m) << logN, rjm = ((j - 1) & m) << logN;
+    // LDS rows: the tile's halo rows of old f are j0 - 1 (loaded by wave 0) and
+    // j0 + 9 (wave 8, whose own row j0 + 8 is the next tile's first, recomputed)
+    const bool halo = kLds && (w == 0 || w == (unsigned)kTileJ);
+    const unsigned jh = w == 0 ? jm : jp;
+    const unsigned hslot = w == 0 ? 0u : (unsigned)kTileJ + 2u;
+    auto roff = [&](unsigned i, unsigned jj) -> unsigned {
+        return ((((i & m) << logN) + jj) << logN) * (unsigned)sizeof(T);
+    };
     // the rolling rows: f of planes i - 1 and i, and the new f of plane i - 1, in
-    // FP64 (each value converted once), plus plane i's stored values for the shuffles
+    // FP64 (each value converted once)
     T fdn[E], fcur[E];
-    ld_row<T, E>(f + (((i0 - 1) & m) << (2 * logN)) + rj, lane, fdn);
-    ld_row<T, E>(f + ((i0 & m) << (2 * logN)) + rj, lane, fcur);
+    bld_row<T, E>(rf, roff(i0 - 1, j), lane, fdn);
+    bld_row<T, E>(rf, roff(i0, j), lane, fcur);
     double ddn[E], dcur[E], dprev[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -727,36 +780,46 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
     }
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     int err = 0;
-    RowWalk wk{};   // k neighbours within the whole row (one segment)
-    wk.segs = 1;
-    wk.k0 = 0;
-    // rows plane step t reads besides the wave's own: f(i0+t+1) row j, and for a
-    // step f(i0+t) rows j +- 1 and fd(i0+t) row j
+    // rows plane step t streams: f(i0+t+1) row j, fd(i0+t) row j, and f(i0+t)
+    // rows j +- 1 (JL false) or the halo row f(i0+t+1) (JL true)
     struct Rows {
-        T up[E], jp[E], jm[E], vin[E];
+        T up[E], vin[E], a[E], b[E];
     };
     Rows S0, S1, S2;
     auto load_rows = [&](unsigned t, Rows &r) __attribute__((always_inline)) {
-        const unsigned pt = ((i0 + t) & m) << (2 * logN), pu = ((i0 + t + 1) & m) << (2 * logN);
-        ld_row<T, E>(f + pu + rj, lane, r.up);
+        bld_row<T, E>(rf, roff(i0 + t + 1, j), lane, r.up);
         if constexpr (!SUMS) {
-            ld_row<T, E>(f + pt + rjp, lane, r.jp);
-            ld_row<T, E>(f + pt + rjm, lane, r.jm);
-            ld_row<T, E>(fd + pt + rj, lane, r.vin);
+            bld_row<T, E>(rv, roff(i0 + t, j), lane, r.vin);
+            if constexpr (!JL) {
+                bld_row<T, E>(rf, roff(i0 + t, jp), lane, r.a);
+                bld_row<T, E>(rf, roff(i0 + t, jm), lane, r.b);
+            } else if (halo) {
+                bld_row<T, E>(rf, roff(i0 + t + 1, jh), lane, r.a);
+            }
         }
     };
-    constexpr int kAhead = SUMS ? 0 : PF;
-    if constexpr (kAhead >= 1) load_rows(0, S0);
-    if constexpr (kAhead >= 2) load_rows(1, S1);
+    if constexpr (!SUMS) load_rows(0, S0);
+    if constexpr (kLds) {
+        // plane i0's old f rows for the j +- 1 neighbours of the first kick
+#pragma unroll
+        for (int e = 0; e < E; ++e) L.oldf[0][w + 1].at(lane, e) = dcur[e];
+        if (halo) {
+            T hr[E];
+            bld_row<T, E>(rf, roff(i0, jh), lane, hr);
+#pragma unroll
+            for (int e = 0; e < E; ++e) L.oldf[0][hslot].at(lane, e) = (double)hr[e];
+        }
+    }
     // the step's scalars: the loop's first record as it stands, later ones closed
     // here with the previous step's partials (every workgroup, the same bits)
-    double d = 0.0, laplnorm = 0.0, friction = 0.0, potnorm = 0.0, vpmean = 0.0, phibar = 0.0, phibar_next = 0.0;
+    double d = 0.0, dl = 0.0, cf = 0.0, dpot = 0.0, ch = 0.0, ck = 0.0, vpmean = 0.0, phibar = 0.0,
+           phibar_next = 0.0;
     double *pout = partials;
     if constexpr (SUMS) {
         phibar = SPLIT ? bg->phibar : 0.0;
         phibar_next = phibar;
     } else {
-        if (pmarched) block_sum<3>(tot, smem);
+        if (pmarched) block_sum<3>(tot, L.smem);
         __syncthreads();
         if (threadIdx.x == 0) {
             if (pmarched) close_state(sb, tot, (double)N * N * N, SPLIT);
@@ -766,26 +829,45 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         if (blk == 0 && w == 0) copy_bg_wave(F.rec + (sp ^ 1), &sb, lane);
         if (sb.done) return;
         d = uniform(sb.nx.d);
-        laplnorm = uniform(sb.nx.laplnorm);
-        friction = uniform(sb.nx.friction);
-        potnorm = uniform(sb.nx.potnorm);
+        const double laplnorm = uniform(sb.nx.laplnorm), friction = uniform(sb.nx.friction);
+        const double potnorm = uniform(sb.nx.potnorm);
         vpmean = SPLIT ? uniform(sb.nx.vpmean) : 0.0;
         phibar = SPLIT ? uniform(sb.phibar) : 0.0;
         phibar_next = SPLIT ? uniform(sb.nx.phibar_next) : 0.0;
+        // fd += d (laplnorm lapl f - friction fd - potnorm (V'(f) - <V'>)) as
+        // fd (1 - d friction) + d laplnorm lapl f + pt, with pt = -d potnorm (V' - <V'>):
+        // for the hilltop V' = -hV2 (phibar + f), pt = ch f + ck
+        dl = d * laplnorm;
+        cf = 1.0 - d * friction;
+        dpot = d * potnorm;
+        if constexpr (POT == kPotHill) {
+            ch = dpot * P0.hV2;
+            ck = SPLIT ? dpot * (P0.hV2 * phibar + vpmean) : 0.0;
+        }
         pout = partials + (size_t)(sp ^ 1) * G * 3;
+    }
+    // the first kick's j +- 1 rows (JL: from LDS, after the barriers above)
+    double njp[E], njm[E];
+    if constexpr (kLds) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            njp[e] = L.oldf[0][w + 2].at(lane, e);
+            njm[e] = L.oldf[0][w].at(lane, e);
+        }
     }
     // brackets of the tabulated potential, walked from each lane's previous site
     int hint[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) hint[e] = Pv.start((SPLIT ? phibar : 0.0) + (double)fcur[e]);
-    // plane step t with its rows in `cur`; the rows of step t + kAhead go to `fill`
-    // (the last steps reload plane TI's rows, unused, so that every step loads)
+    for (int e = 0; e < E; ++e) hint[e] = POT == kPotTab ? Pv.start((SPLIT ? phibar : 0.0) + dcur[e]) : 0;
+    // plane step t with its rows in `cur`; the rows of step t + 1 go to `fill`
+    // (the last step reloads plane TI's rows, unused, so that every step loads)
     auto plane_step = [&](unsigned t, Rows &cur, Rows &fill) __attribute__((always_inline)) {
-        if constexpr (kAhead == 0) load_rows(t, cur);
-        else load_rows(t + kAhead <= TI ? t + kAhead : TI, fill);
-        const unsigned pi = ((i0 + t) & m) << (2 * logN);
+        if constexpr (SUMS) load_rows(t, cur);
+        else load_rows(t + 1 <= TI ? t + 1 : TI, fill);
         T fnew[E];
-        double dnew[E];
+        double dnew[E], dup[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) dup[e] = (double)cur.up[e];
         if constexpr (SUMS) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -793,46 +875,71 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
                 dnew[e] = dcur[e];
             }
         } else {
-            T kp_last, km_first;
-            k_neighbours<T, E>(fcur, f + pi + rj, wk, N, lane, kp_last, km_first);
+            const double kp_last = __shfl(dcur[0], (int)((lane + 1) & 63));
+            const double km_first = __shfl(dcur[E - 1], (int)((lane + 63) & 63));
             T vout[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const double c = dcur[e];
-                const double kp = e < E - 1 ? dcur[e + 1 < E ? e + 1 : e] : (double)kp_last;
-                const double km = e > 0 ? dcur[e > 0 ? e - 1 : 0] : (double)km_first;
-                const double lap =
-                    lapl_sites(kp, km, (double)cur.jp[e], (double)cur.jm[e], (double)cur.up[e], ddn[e], c);
-                double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
-                if (SPLIT) dV -= vpmean;
-                double v = (double)cur.vin[e];
-                v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
+                const double kp = e < E - 1 ? dcur[e + 1 < E ? e + 1 : e] : kp_last;
+                const double km = e > 0 ? dcur[e > 0 ? e - 1 : 0] : km_first;
+                const double yp = JL ? njp[e] : (double)cur.a[e], ym = JL ? njm[e] : (double)cur.b[e];
+                const double lap = lapl_sites(kp, km, yp, ym, dup[e], ddn[e], c);
+                double pt;
+                if constexpr (POT == kPotHill) {
+                    pt = SPLIT ? fma(ch, c, ck) : ch * c;
+                } else {
+                    double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
+                    if (SPLIT) dV -= vpmean;
+                    pt = -dpot * dV;
+                }
+                const double v = fma(dl, lap, fma(cf, (double)cur.vin[e], pt));
                 vout[e] = (T)v;
-                fnew[e] = (T)(c + d * (double)vout[e]);
+                fnew[e] = (T)fma(d, v, c);
                 dnew[e] = (double)fnew[e];
             }
             if (owner && t < TI) {
-                st_row<T, E>(fdout + pi + rj, lane, vout);
-                st_row<T, E>(fout + pi + rj, lane, fnew);
+                bst_row<T, E>(rvo, roff(i0 + t, j), lane, vout);
+                bst_row<T, E>(rfo, roff(i0 + t, j), lane, fnew);
+            }
+            if constexpr (JL) {
+                // plane t + 1's old f rows for the next kick's j +- 1 neighbours
+#pragma unroll
+                for (int e = 0; e < E; ++e) L.oldf[(t + 1) & 1][w + 1].at(lane, e) = dup[e];
+                if (halo)
+#pragma unroll
+                    for (int e = 0; e < E; ++e) L.oldf[(t + 1) & 1][hslot].at(lane, e) = (double)cur.a[e];
             }
         }
-        T(*buf)[64 * E] = lds[t & 1];
 #pragma unroll
-        for (int e = 0; e < E; ++e) buf[w][lane * E + e] = fnew[e];
+        for (int e = 0; e < E; ++e) L.newf[t & 1][w].at(lane, e) = fnew[e];
         __syncthreads();
+        if constexpr (kLds) {
+            // the next kick's j +- 1 rows, in flight while the sums below are formed
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                njp[e] = L.oldf[(t + 1) & 1][w + 2].at(lane, e);
+                njm[e] = L.oldf[(t + 1) & 1][w].at(lane, e);
+            }
+        }
         if (owner) {
             if (t < TI) {
-                const T knext = __shfl(fnew[0], (int)((lane + 1) & 63));
+                const double knext = __shfl(dnew[0], (int)((lane + 1) & 63));
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const double c = dnew[e];
-                    const double dk = (e < E - 1 ? dnew[e + 1 < E ? e + 1 : e] : (double)knext) - c;
-                    const double dj = (double)buf[w + 1][lane * E + e] - c;
+                    const double dk = (e < E - 1 ? dnew[e + 1 < E ? e + 1 : e] : knext) - c;
+                    const double dj = (double)L.newf[t & 1][w + 1].at(lane, e) - c;
                     acc0 += dk * dk + dj * dj;
-                    double V, dV;
-                    Pv.eval(SPLIT ? phibar_next + c : c, &V, &dV, hint[e], true, &err);
-                    acc1 += V;
-                    acc2 += dV;
+                    if constexpr (POT == kPotHill) {
+                        acc1 += c;
+                        acc2 += c * c;
+                    } else {
+                        double V, dV;
+                        Pv.eval(SPLIT ? phibar_next + c : c, &V, &dV, hint[e], true, &err);
+                        acc1 += V;
+                        acc2 += dV;
+                    }
                 }
             }
             if (t >= 1) {
@@ -848,21 +955,15 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
             dprev[e] = dnew[e];
             ddn[e] = dcur[e];
             fcur[e] = cur.up[e];
-            dcur[e] = (double)cur.up[e];
+            dcur[e] = dup[e];
         }
     };
-    // set of step t: S[t mod 3]; step t fills S[(t + kAhead) mod 3], last read by
-    // step t + kAhead - 3.  TI + 1 steps, a multiple of 3 (march_ok).
+    // set of step t: S[t mod 3]; step t fills S[(t + 1) mod 3], last read by
+    // step t - 2.  TI + 1 steps, a multiple of 3 (march_ok).
     for (unsigned t = 0; t <= TI; t += 3) {
-        if constexpr (kAhead == 2) {
-            plane_step(t, S0, S2);
-            plane_step(t + 1, S1, S0);
-            plane_step(t + 2, S2, S1);
-        } else {
-            plane_step(t, S0, S1);
-            plane_step(t + 1, S1, S2);
-            plane_step(t + 2, S2, S0);
-        }
+        plane_step(t, S0, S1);
+        plane_step(t + 1, S1, S2);
+        plane_step(t + 2, S2, S0);
     }
     flag_error(bg, err);
     // fixed-order workgroup sum over the 9 waves
@@ -870,77 +971,68 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
 #pragma unroll
     for (int q = 0; q < 3; ++q) acc[q] = wave_sum(acc[q]);
     if (lane == 0)
-        for (int q = 0; q < 3; ++q) smem[q * kMarchWaves + w] = acc[q];
+        for (int q = 0; q < 3; ++q) L.smem[q * kMarchWaves + w] = acc[q];
     __syncthreads();
     if (threadIdx.x == 0)
         for (int q = 0; q < 3; ++q) {
-            double tot = 0.0;
-            for (int ww = 0; ww < kMarchWaves; ++ww) tot += smem[q * kMarchWaves + ww];
-            pout[(size_t)blk * 3 + q] = tot;
+            double s = 0.0;
+            for (int ww = 0; ww < kMarchWaves; ++ww) s += L.smem[q * kMarchWaves + ww];
+            pout[(size_t)blk * 3 + q] = s;
         }
 }
 
-// Planes the early-load march kernels load ahead (1 measured faster than 2 at
-// 256^3: 0.0475 against 0.0481 ms per split step).
-constexpr int kMarchAhead = 1;
-
-// Split storage fits two workgroups per CU (5 waves per SIMD) without spills;
-// FP64 rows need more registers and run one.
-template <typename T, bool SPLIT, int E>
-__global__ __launch_bounds__(kMarchThreads) __attribute__((amdgpu_waves_per_eu(5)))
-void k_march_w5(Fields F, Pot P, DevBg *bg, double *partials, unsigned sp)
+// Two variants with the same results, picked by the tuner: the j +- 1 rows of
+// old f from L2 (k_march_pf) or through LDS (k_march).
+template <typename T, bool SPLIT, int E, int POT>
+__global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg *bg, double *partials, unsigned sp)
 {
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
+    __shared__ MarchLds<T, E, true> L;
+    march_body<T, SPLIT, E, POT, true>(F, P, bg, partials, blockIdx.x, L, sp);
 }
 
-template <typename T, bool SPLIT, int E>
-__global__ __launch_bounds__(kMarchThreads) void k_march(Fields F, Pot P, DevBg *bg, double *partials,
-                                                         unsigned sp)
-{
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
-}
-
-template <typename T, bool SPLIT, int E>
+template <typename T, bool SPLIT, int E, int POT>
 __global__ __launch_bounds__(kMarchThreads) void k_march_pf(Fields F, Pot P, DevBg *bg, double *partials,
                                                             unsigned sp)
 {
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E, false, kMarchAhead>(F, P, bg, partials, blockIdx.x, lds, smem, sp);
+    __shared__ MarchLds<T, E, false> L;
+    march_body<T, SPLIT, E, POT, false>(F, P, bg, partials, blockIdx.x, L, sp);
 }
 
-template <typename T, bool SPLIT, int E>
+template <typename T, bool SPLIT, int E, int POT>
 __global__ __launch_bounds__(kMarchThreads) void kb_march_pf(const RunDesc *__restrict__ runs,
                                                               const uint2 *__restrict__ map, unsigned sp)
 {
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
+    __shared__ MarchLds<T, E, false> L;
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E, false, kMarchAhead>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
+    march_body<T, SPLIT, E, POT, false>(R.F, R.P, R.bg, R.partials, mp.y, L, sp);
 }
 
-template <typename T, bool SPLIT, int E>
+template <typename T, bool SPLIT, int E, int POT>
+__global__ __launch_bounds__(kMarchThreads) void kb_march(const RunDesc *__restrict__ runs,
+                                                           const uint2 *__restrict__ map, unsigned sp)
+{
+    __shared__ MarchLds<T, E, true> L;
+    const uint2 mp = map[blockIdx.x];
+    const RunDesc &R = runs[mp.x];
+    march_body<T, SPLIT, E, POT, true>(R.F, R.P, R.bg, R.partials, mp.y, L, sp);
+}
+
+template <typename T, bool SPLIT, int E, int POT>
 __global__ __launch_bounds__(kMarchThreads) void k_march_sums(Fields F, Pot P, DevBg *bg, double *partials)
 {
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
-    march_body<T, SPLIT, E, true>(F, P, bg, partials, blockIdx.x, lds, smem);
+    __shared__ MarchLds<T, E, false> L;
+    march_body<T, SPLIT, E, POT, false, true>(F, P, bg, partials, blockIdx.x, L);
 }
 
-template <typename T, bool SPLIT, int E>
+template <typename T, bool SPLIT, int E, int POT>
 __global__ __launch_bounds__(kMarchThreads) void kb_march_sums(const RunDesc *__restrict__ runs,
                                                                 const uint2 *__restrict__ map)
 {
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
+    __shared__ MarchLds<T, E, false> L;
     const uint2 mp = map[blockIdx.x];
     const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E, true>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem);
+    march_body<T, SPLIT, E, POT, false, true>(R.F, R.P, R.bg, R.partials, mp.y, L);
 }
 
 __global__ __launch_bounds__(kBlock) void k_loop_end(DevBg *bg, const DevBg *rec, unsigned sp, const double *partials,
@@ -956,28 +1048,6 @@ __global__ __launch_bounds__(kBlock) void k_sums_finish(DevBg *bg, const double 
 {
     __shared__ double smem[3 * kWaves];
     sums_finish_body(bg, partials, G, (size_t)gs, rec, smem);
-}
-
-template <typename T, bool SPLIT, int E>
-__global__ __launch_bounds__(kMarchThreads) __attribute__((amdgpu_waves_per_eu(5)))
-void kb_march_w5(const RunDesc *__restrict__ runs, const uint2 *__restrict__ map, unsigned sp)
-{
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
-    const uint2 mp = map[blockIdx.x];
-    const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
-}
-
-template <typename T, bool SPLIT, int E>
-__global__ __launch_bounds__(kMarchThreads) void kb_march(const RunDesc *__restrict__ runs,
-                                                           const uint2 *__restrict__ map, unsigned sp)
-{
-    __shared__ T lds[2][kMarchWaves][64 * E];
-    __shared__ double smem[3 * kMarchWaves];
-    const uint2 mp = map[blockIdx.x];
-    const RunDesc &R = runs[mp.x];
-    march_body<T, SPLIT, E>(R.F, R.P, R.bg, R.partials, mp.y, lds, smem, sp);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_loop_end(const RunDesc *__restrict__ runs, unsigned sp)
@@ -1669,22 +1739,32 @@ void tensor_dispatch(const Fields &F, DevBg *bg, int mode, double d_arg, int loo
     }
 }
 
-template <typename T, bool SPLIT>
-void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, unsigned sp, hipStream_t s)
+template <typename T, bool SPLIT, int POT>
+void march_dispatch_pot(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, unsigned sp,
+                        hipStream_t s)
 {
     const int E = row_elems(F.N);
     if (F.march_pf) {
-        if (E == 2) k_march_pf<T, SPLIT, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
-        else k_march_pf<T, SPLIT, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
-        return;
-    }
-    if constexpr (sizeof(T) == 4) {
-        if (E == 2) k_march_w5<T, SPLIT, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
-        else k_march_w5<T, SPLIT, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march_pf<T, SPLIT, 2, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        else k_march_pf<T, SPLIT, 4, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
     } else {
-        if (E == 2) k_march<T, SPLIT, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
-        else k_march<T, SPLIT, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march<T, SPLIT, 2, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        else k_march<T, SPLIT, 4, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
     }
+}
+
+template <typename T, bool SPLIT>
+void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, unsigned sp, hipStream_t s)
+{
+    if (P.numerical) march_dispatch_pot<T, SPLIT, kPotTab>(F, P, bg, partials, G, sp, s);
+    else march_dispatch_pot<T, SPLIT, kPotHill>(F, P, bg, partials, G, sp, s);
+}
+
+template <typename T, bool SPLIT, int POT>
+void march_sums_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
+{
+    if (row_elems(F.N) == 2) k_march_sums<T, SPLIT, 2, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
+    else k_march_sums<T, SPLIT, 4, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
 }
 
 }  // namespace
@@ -1789,14 +1869,10 @@ unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (int)march_
 hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
 {
     const unsigned G = march_blocks(F.N);
-    const int E = row_elems(F.N);
-    if (split) {
-        if (E == 2) k_march_sums<float, true, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
-        else k_march_sums<float, true, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
-    } else {
-        if (E == 2) k_march_sums<double, false, 2><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
-        else k_march_sums<double, false, 4><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
-    }
+    if (split && P.numerical) march_sums_dispatch<float, true, kPotTab>(F, P, bg, partials, G, s);
+    else if (split) march_sums_dispatch<float, true, kPotHill>(F, P, bg, partials, G, s);
+    else if (P.numerical) march_sums_dispatch<double, false, kPotTab>(F, P, bg, partials, G, s);
+    else march_sums_dispatch<double, false, kPotHill>(F, P, bg, partials, G, s);
     k_sums_finish<<<1, kBlock, 0, s>>>(bg, partials, G, (unsigned long long)F.N * F.N * F.N, F.rec);
     return hipGetLastError();
 }
@@ -1889,7 +1965,8 @@ hipError_t launch_reconstitute(const Fields &F, int split, int which, const DevB
 struct BatchGroup {
     int E = 0, kE = 0, tE = 0;   // elements per lane: reduce, kick, tensor
     int march = 0;               // fused steps (march kernel) for this group
-    int pf = 0;                  // the march kernel variant with early loads
+    int pf = 0;                  // the march kernel variant loading one plane step ahead
+    int pot = 0;                 // fused groups: 1 tabulated potential, 0 hilltop
     uint2 *map = nullptr;        // kick / march workgroups
     uint2 *rmap = nullptr;       // reduce workgroups (march groups: loop start only)
     uint2 *tmap = nullptr;       // tensor workgroups
@@ -1931,7 +2008,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
                        : tensor_is_march(tc) ? kTensorMarch | (tc & kTensorPf) | tensor_march_elems(N, tc, split)
                                              : shape_elems(N, tc);
         const int pf = d[r].march && runs[r].F.march_pf ? 1 : 0;
-        const int key = pf * 1000000 + d[r].march * 100000 + row_elems(N) * 10000 + kE * 100 + tE;
+        const int pot = d[r].march && runs[r].P.numerical ? 1 : 0;   // fused kernels are per potential kind
+        const int key = pot * 10000000 + pf * 1000000 + d[r].march * 100000 + row_elems(N) * 10000 + kE * 100 + tE;
         size_t g = 0;
         while (g < keys.size() && keys[g] != key) ++g;
         if (g == keys.size()) {
@@ -1962,7 +2040,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     };
     for (size_t g = 0; g < keys.size() && e == hipSuccess; ++g) {
         BatchGroup G;
-        G.pf = keys[g] / 1000000;
+        G.pot = keys[g] / 10000000;
+        G.pf = keys[g] / 1000000 % 10;
         G.march = keys[g] / 100000 % 10;
         G.E = keys[g] / 10000 % 10;
         G.kE = keys[g] / 100 % 100;
@@ -1999,6 +2078,26 @@ void batch_free(BatchDev *B)
 template <typename T, bool SPLIT>
 static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream_t s);
 
+template <typename T, bool SPLIT, int POT>
+static void batch_march_sums(const BatchDev *B, const BatchGroup &G, hipStream_t s)
+{
+    if (G.E == 2) kb_march_sums<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
+    else kb_march_sums<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
+}
+
+template <typename T, bool SPLIT, int POT>
+static void batch_march(const BatchDev *B, const BatchGroup &G, unsigned sp, hipStream_t s)
+{
+    // (march groups have N = 128 or 256: E = 2 or 4)
+    if (G.pf) {
+        if (G.E == 2) kb_march_pf<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+        else kb_march_pf<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+    } else {
+        if (G.E == 2) kb_march<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+        else kb_march<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+    }
+}
+
 hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
 {
     kb_loop_begin<<<(B->nruns + 63) / 64, 64, 0, s>>>(B->runs, B->nruns);
@@ -2006,13 +2105,10 @@ hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s)
         // fused groups start from the sums of the current f, in a step's order
         for (const BatchGroup &G : B->groups) {
             if (!G.march) continue;
-            if (B->split) {
-                if (G.E == 2) kb_march_sums<float, true, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
-                else kb_march_sums<float, true, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
-            } else {
-                if (G.E == 2) kb_march_sums<double, false, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
-                else kb_march_sums<double, false, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
-            }
+            if (B->split && G.pot) batch_march_sums<float, true, kPotTab>(B, G, s);
+            else if (B->split) batch_march_sums<float, true, kPotHill>(B, G, s);
+            else if (G.pot) batch_march_sums<double, false, kPotTab>(B, G, s);
+            else batch_march_sums<double, false, kPotHill>(B, G, s);
         }
         kb_sums_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
     }
@@ -2035,18 +2131,9 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n
 {
     // records: the previous step's in slot n & 1, this step's in the other
     const unsigned sp = (unsigned)(n & 1), rs = sp ^ 1u;
-    if (G.march && G.pf) {
-        if (G.E == 2) kb_march_pf<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-        else kb_march_pf<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-    } else if (G.march) {
-        // (march groups have N = 128 or 256: E = 2 or 4)
-        if constexpr (sizeof(T) == 4) {
-            if (G.E == 2) kb_march_w5<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-            else kb_march_w5<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-        } else {
-            if (G.E == 2) kb_march<T, SPLIT, 2><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-            else kb_march<T, SPLIT, 4><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-        }
+    if (G.march) {
+        if (G.pot) batch_march<T, SPLIT, kPotTab>(B, G, sp, s);
+        else batch_march<T, SPLIT, kPotHill>(B, G, sp, s);
     } else switch (G.kE) {
     case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
     case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
