@@ -207,7 +207,8 @@ def submit_request(srv, E, T, n, spec):
     return srv.submit(cfg, 1000 + n, PHI0, DPHI0, 1, check)
 
 
-def serve_stream(nreq: int, load: float = 0.8, burst: int = 24, seed: int = 20261203) -> dict:
+def serve_stream(nreq: int, load: float = 0.8, burst: int = 24, seed: int = 20261203,
+                 graph_steps: int | None = None) -> dict:
     """Serving: each request is a whole run (initial conditions, inflation to
     af = 2N, Omega_GW for SIGW runs, the deltaN map, P_zeta).  Capacity is
     measured first: `burst` requests of the mix submitted at once, runs/s over
@@ -221,6 +222,8 @@ def serve_stream(nreq: int, load: float = 0.8, burst: int = 24, seed: int = 2026
 
     def run_stream(specs, arrivals, label):
         srv = E.Server(240 << 30, 100, 8)
+        if graph_steps is not None:
+            srv.set_graph_steps(graph_steps)
         results, n = [], 0
         t0 = srv.clock()
         while len(results) < len(specs):
@@ -341,7 +344,7 @@ def c3_sigw(N: int = 256) -> dict:
     return res
 
 
-def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203) -> dict:
+def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203, graph_steps: int = 50) -> dict:
     """Batched ensemble: nruns hilltop lattices of N^3, seeds 1000 + r, V0
     log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split storage,
     shapes tuned before the batch is built, the deltaN window scaled with V0.  The same number of steps is timed
@@ -376,9 +379,9 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
     with ThreadPoolExecutor(max_workers=workers) as ex:
         runs = list(ex.map(make, range(nruns)))
     batch = E.Batch(runs)
-    batch.steps(20, graph_steps=20)                # warm-up, graph instantiation
+    batch.steps(20, graph_steps=min(20, graph_steps))   # warm-up, graph instantiation
     t0 = time.perf_counter()
-    batch.steps(steps, graph_steps=50)
+    batch.steps(steps, graph_steps=graph_steps)
     tb = time.perf_counter() - t0
     for r in runs[:4]:
         r.leapfrog_steps(5)                          # warm-up of the per-run path
@@ -396,7 +399,7 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
     t0 = time.perf_counter()
     rounds = 0
     while not all(r.background()["done"] for r in runs):
-        batch.steps(500, graph_steps=50)
+        batch.steps(500, graph_steps=graph_steps)
         rounds += 1
         if rounds > 100:
             raise RuntimeError("ensemble did not reach af")
@@ -414,6 +417,7 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
             "batched_site_updates_per_s": sites / tb,
             "sequential_site_updates_per_s": sites / ts, "batched_over_sequential": ts / tb,
             "batched_ms_per_step": 1e3 * tb / steps, "sequential_ms_per_step": 1e3 * ts / steps,
+            "graph_steps": graph_steps,
             "whole_runs": {"steps_per_run": int(outs[0]["steps"]), "loop_s": t_whole, "outputs_s": t_out,
                            "pzeta_returned": int(sum(o["nbins"] > 0 for o in outs)), "pzeta_shells": int(pz.shape[1]),
                            "pzeta_positive_runs": int(sum(np.all(p[1:] > 0) for p in pz)),
@@ -486,6 +490,9 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson arrival rate as a fraction of the capacity measured first")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="loop steps per HIP graph launch of the batched legs (0: direct launches, as for a "
+                         "kernel-traced run; default: 50 for the ensemble, the quantum for serving)")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -511,10 +518,12 @@ def main() -> int:
         print(json.dumps({"c3_sigw_256": c3_sigw(N)}), flush=True)
         return 0
     if args.only == "ensemble":
-        print(json.dumps({"ensemble": ensemble(args.ensemble_runs)}), flush=True)
+        gs = 50 if args.graph_steps is None else args.graph_steps
+        print(json.dumps({"ensemble": ensemble(args.ensemble_runs, graph_steps=gs)}), flush=True)
         return 0
     if args.only == "serve":
-        print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_load)}), flush=True)
+        print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_load,
+                                                  graph_steps=args.graph_steps)}), flush=True)
         return 0
     if args.only == "tab":
         print(json.dumps({"tabulated_128": tabulated_case(128)}), flush=True)
@@ -597,10 +606,10 @@ def main() -> int:
         out["post_inflation"] = post_inflation(N, args.post_steps)
         print(f"# post-inflation: {out['post_inflation']}", file=sys.stderr, flush=True)
     if args.ensemble_runs > 0 and not args.only:
-        out["ensemble"] = ensemble(args.ensemble_runs)
+        out["ensemble"] = ensemble(args.ensemble_runs, graph_steps=50 if args.graph_steps is None else args.graph_steps)
         print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)
     if args.serve_requests > 0 and not args.only:
-        out["serving"] = serve_stream(args.serve_requests, args.serve_load)
+        out["serving"] = serve_stream(args.serve_requests, args.serve_load, graph_steps=args.graph_steps)
         out["p50_latency_s"] = out["serving"]["p50_latency_s"]
         out["p99_latency_s"] = out["serving"]["p99_latency_s"]
     if not args.no_cpu_baseline:

@@ -416,6 +416,10 @@ ie_status ie_server_pump(ie_server *s, int *active, int *pending);
 /* Pop one completed result (returns IE_ERR_STATE when none is ready). */
 ie_status ie_server_pop_result(ie_server *s, ie_result *out);
 double ie_server_clock(const ie_server *s);
+/* Loop steps per HIP graph launch of the batches (default: quantum_steps; 0:
+ * every step launched directly, e.g. for tools that cannot follow graph
+ * launches). */
+ie_status ie_server_set_graph_steps(ie_server *s, int graph_steps);
 ie_status ie_server_destroy(ie_server *s);
 
 /* --- modes ------------------------------------------------------------------ */

@@ -382,14 +382,12 @@ ie_status ie_upload_fields(ie_run *r, const double *f, const double *fd)
         IE_HIP(r, hipMemcpyAsync(r->F.fd[0], fd, bytes, hipMemcpyHostToDevice, r->stream));
         return sync(r);
     }
-    // split: stage both fields in FP64 (scratch + the unused second f buffer is too
-    // small in FP32), compute the homogeneous mode on the device, store fluctuations
-    double *tmp = nullptr;
-    IE_HIP(r, hipMallocAsync((void **)&tmp, bytes, r->stream));
+    // split: stage both fields in FP64 (scratch and the FP64 state buffer of the
+    // deltaN phase, unused before it), compute the homogeneous mode on the device,
+    // store fluctuations
     IE_HIP(r, hipMemcpyAsync(r->scratch, f, bytes, hipMemcpyHostToDevice, r->stream));
-    IE_HIP(r, hipMemcpyAsync(tmp, fd, bytes, hipMemcpyHostToDevice, r->stream));
-    IE_HIP(r, launch_split_upload(r->scratch, tmp, r->F, r->bg, r->partials, r->stream));
-    IE_HIP(r, hipFreeAsync(tmp, r->stream));
+    IE_HIP(r, hipMemcpyAsync(r->dn_fd, fd, bytes, hipMemcpyHostToDevice, r->stream));
+    IE_HIP(r, launch_split_upload(r->scratch, r->dn_fd, r->F, r->bg, r->partials, r->stream));
     return sync(r);
 }
 
@@ -410,11 +408,18 @@ ie_status ie_download_field(ie_run *r, int which, double *out)
         IE_HIP(r, hipMemcpyAsync(out, src, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
         return sync(r);
     }
-    double *tmp = nullptr;
-    IE_HIP(r, hipMallocAsync((void **)&tmp, r->gs * 8, r->stream));
-    IE_HIP(r, launch_reconstitute(r->F, r->cfg.precision == IE_SPLIT, which, r->bg, tmp, r->stream));
-    IE_HIP(r, hipMemcpyAsync(out, tmp, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
-    IE_HIP(r, hipFreeAsync(tmp, r->stream));
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    const int par = r->bg_host->parity;
+    if (r->cfg.precision == IE_FP64) {
+        const void *src = which == IE_FIELD_F ? r->F.f[par] : r->F.fd[par];
+        IE_HIP(r, hipMemcpyAsync(out, src, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
+        return sync(r);
+    }
+    // split: phibar + the fluctuation in FP64, staged in the deltaN phase's FP64
+    // buffer (unused before that phase)
+    IE_HIP(r, launch_reconstitute(r->F, 1, which, r->bg, r->dn_f, r->stream));
+    IE_HIP(r, hipMemcpyAsync(out, r->dn_f, r->gs * 8, hipMemcpyDeviceToHost, r->stream));
     return sync(r);
 }
 
@@ -1121,12 +1126,11 @@ ie_status ie_upload_modes(ie_run *r, const double *fk, const double *fny, const 
         IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, (double *)r->F.fd[0], r->stream));
         return sync(r);
     }
-    double *tmp = nullptr;
-    IE_HIP(r, hipMallocAsync((void **)&tmp, r->gs * 8, r->stream));
+    // split: both fields in FP64 in scratch and the deltaN phase's FP64 buffer
+    // (unused before that phase), then the homogeneous mode and the fluctuations
     IE_HIP(r, fft_inverse_ffteasy(r->fft, fk, fny, phi0, r->scratch, r->stream));
-    IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, tmp, r->stream));
-    IE_HIP(r, launch_split_upload(r->scratch, tmp, r->F, r->bg, r->partials, r->stream));
-    IE_HIP(r, hipFreeAsync(tmp, r->stream));
+    IE_HIP(r, fft_inverse_ffteasy(r->fft, fdk, fdny, dphi0, r->dn_fd, r->stream));
+    IE_HIP(r, launch_split_upload(r->scratch, r->dn_fd, r->F, r->bg, r->partials, r->stream));
     return sync(r);
 }
 

@@ -220,6 +220,7 @@ double max_rel(const double *a, const double *b, int n)
 struct ie_server {
     size_t budget = 0, used = 0;   // pump thread only
     int quantum = 100;
+    int graph_steps = -1;          // steps per graph launch of the batches (-1: the quantum, 0: direct launches)
     int64_t next_id = 1;
     Clock::time_point t0 = Clock::now();
     std::deque<Job *> pending;
@@ -324,7 +325,7 @@ ie_status step_round(ie_server *s)
     ie_status st = IE_OK;
     for (auto &g : s->groups) {
         const int q = s->quantum * std::max(1, 256 / g.N);
-        if (st == IE_OK) st = ie_batch_launch(g.batch, q, s->quantum);
+        if (st == IE_OK) st = ie_batch_launch(g.batch, q, s->graph_steps < 0 ? s->quantum : s->graph_steps);
     }
     for (auto &g : s->groups) {
         const ie_status w = ie_batch_wait(g.batch);
@@ -410,6 +411,13 @@ ie_status ie_server_create(size_t budget, int quantum, int threads, ie_server **
 }
 
 double ie_server_clock(const ie_server *s) { return s ? since(s->t0) : 0.0; }
+
+ie_status ie_server_set_graph_steps(ie_server *s, int graph_steps)
+{
+    if (!s || graph_steps < 0) return IE_ERR_ARG;
+    s->graph_steps = graph_steps;
+    return IE_OK;
+}
 
 ie_status ie_server_submit(ie_server *s, const ie_request *req, int64_t *id)
 {

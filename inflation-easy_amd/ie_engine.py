@@ -198,6 +198,7 @@ def lib() -> C.CDLL:
             "ie_server_pump": (st, [P, ip, ip]),
             "ie_server_pop_result": (st, [P, C.POINTER(Result)]),
             "ie_server_clock": (C.c_double, [P]),
+            "ie_server_set_graph_steps": (st, [P, C.c_int]),
             "ie_server_destroy": (st, [P]),
             "ie_pzeta_linear": (st, [C.c_int, dp, dp, C.c_double, C.c_double, dp]),
             "ie_pzeta_deltaN": (st, [C.c_int, dp, dp, dp]),
@@ -581,6 +582,12 @@ class Server:
 
     def clock(self) -> float:
         return lib().ie_server_clock(self.h)
+
+    def set_graph_steps(self, n: int):
+        """Steps per HIP graph launch of the batches (0: direct launches)."""
+        st = lib().ie_server_set_graph_steps(self.h, n)
+        if st:
+            raise EngineError(st, "ie_server_set_graph_steps failed")
 
     def close(self):
         if getattr(self, "h", None):
