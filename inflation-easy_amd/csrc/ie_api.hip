@@ -1738,13 +1738,11 @@ ie_status ie_batch_launch(ie_batch *b, int nsteps, int graph_steps)
             const int p = done & 1;
             if (!b->exec[p]) {
                 IE_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                for (int n = 0; n < graph_steps; ++n) {
-                    hipError_t e = batch_step(b->dev, p + n, b->stream);
-                    if (e != hipSuccess) {
-                        hipGraph_t g;
-                        (void)hipStreamEndCapture(b->stream, &g);
-                        return bfail(b, IE_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
-                    }
+                const hipError_t e = batch_steps(b->dev, p, graph_steps, b->stream);
+                if (e != hipSuccess) {
+                    hipGraph_t g;
+                    (void)hipStreamEndCapture(b->stream, &g);
+                    return bfail(b, IE_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
                 }
                 IE_BHIP(b, hipStreamEndCapture(b->stream, &b->graph[p]));
                 IE_BHIP(b, hipGraphInstantiate(&b->exec[p], b->graph[p], nullptr, nullptr, 0));
@@ -1752,7 +1750,7 @@ ie_status ie_batch_launch(ie_batch *b, int nsteps, int graph_steps)
             IE_BHIP(b, hipGraphLaunch(b->exec[p], b->stream));
         }
     }
-    for (; done < nsteps; ++done) IE_BHIP(b, batch_step(b->dev, done, b->stream));
+    if (done < nsteps) IE_BHIP(b, batch_steps(b->dev, done, nsteps - done, b->stream));
     if (nsteps > 0) IE_BHIP(b, batch_loop_end(b->dev, nsteps, b->stream));
     return IE_OK;
 }

@@ -451,6 +451,10 @@ struct BatchRun {
 };
 struct BatchDev;
 hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out);
+// `count` loop steps of every run from loop step n0: resident groups in one
+// launch, the other groups one step at a time
+hipError_t batch_steps(const BatchDev *B, long long n0, int count, hipStream_t s);
+bool resident_ok(int N, int sigw, int split, int numerical);
 void batch_free(BatchDev *B);
 hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s);
 hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s);        // step n of the loop

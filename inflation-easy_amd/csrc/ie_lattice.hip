@@ -34,6 +34,8 @@
 // in FP64; only the stored fluctuation is rounded to FP32.
 #include <vector>
 
+#include <type_traits>
+
 #include "ie_internal.h"
 
 namespace ie {
@@ -117,6 +119,7 @@ struct RunDesc {
     unsigned Gm;       // workgroups of the fused march kernel (0: not fused)
     int split;
     int march;         // loop steps fused (march kernel + close)
+    int res;           // batched 64^3 lattice stepped by one resident workgroup (kb_resident)
 };
 
 struct Nb {
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void kb_friedmann_finish(const RunDesc *__r
 {
     __shared__ double smem[3 * kWaves];
     const RunDesc &R = runs[blockIdx.x];
-    if (R.march) return;
+    if (R.march || R.res) return;
     const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
     friedmann_finish_body(R.bg, R.partials, R.G, gs, 0.0, 1, smem);
 }
@@ -383,7 +386,7 @@ __global__ void k_step_finish(DevBg *bg, int mode, int split, double d_arg, int 
 __global__ void kb_step_finish(const RunDesc *__restrict__ runs, int nruns)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nruns || runs[r].march) return;
+    if (r >= nruns || runs[r].march || runs[r].res) return;
     DevBg *bg = runs[r].bg;
     if (bg->done) return;
     finalize_scalars(bg, KICKDRIFT, runs[r].split);
@@ -1067,6 +1070,247 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
     if (!R.march) return;
     const size_t gs = (size_t)R.F.N * R.F.N * R.F.N;
     sums_finish_body(R.bg, R.partials, R.Gm, gs, R.F.rec, smem);
+}
+
+// ------------------------------------------------ resident small lattices
+//
+// Batched 64^3 lattices without SIGW: one workgroup of 16 waves owns a whole
+// lattice and runs every loop step of a launch itself.  A plane is its 64 rows,
+// 4 consecutive rows per wave with one site per lane, so a kick finds its k +- 1
+// neighbours by lane shuffles, its j +- 1 rows in the wave's own registers (the
+// wave's two edge rows through LDS), its i +- 1 rows in the registers of the
+// planes the wave marches through.  Each site is kicked once, with no halo, and
+// the next step's Friedmann sums (the same pairs as the fused march kernel's,
+// including the i pair of planes N - 1 and 0 from plane 0's new rows kept in
+// registers) never leave the workgroup: after the last plane the workgroup
+// reduces them and closes the step on its own copy of the background
+// (close_state), then starts the next step.  A launch first forms the sums of
+// the current f (the same partition and order), so a loop split into several
+// launches gives the same bits as one.  Reads f, fd and writes them once per
+// step (16 B/site in split storage).
+constexpr int kResN = 64;
+constexpr int kResWaves = 16;
+constexpr int kResRows = kResN / kResWaves;   // rows per wave
+constexpr int kResThreads = 64 * kResWaves;
+
+template <typename T>
+struct ResLds {
+    double oldf[2][kResN][64];   // old f of plane t: the waves' edge rows
+    T newf[2][kResN][64];        // new f of plane t: the waves' first rows
+    double smem[3 * kResWaves];
+};
+
+template <typename T, bool SPLIT, int POT>
+__device__ __forceinline__ void resident_body(const RunDesc &R, int nsteps, ResLds<T> &L, DevBg &sb)
+{
+    constexpr unsigned N = kResN, m = N - 1;
+    const unsigned lane = threadIdx.x & 63;
+    const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned j0 = w * kResRows;
+    const Fields &F = R.F;
+    DevBg *bg = R.bg;
+    if (w == 0) copy_bg_wave(&sb, bg, lane);
+    __syncthreads();
+    if (sb.done) return;
+    const PotView Pv = stage_pot(R.P);
+    const unsigned bytes = N * N * N * (unsigned)sizeof(T);
+    const double gsd = (double)N * N * N;
+    int err = 0;
+    auto roff = [&](unsigned i, unsigned j) -> unsigned { return ((((i & m) << 6) + (j & m)) << 6) * (unsigned)sizeof(T); };
+    // one pass over the planes: SUMS the sums of the current f (launch start),
+    // otherwise a loop step (kick, drift, stores, the new f's sums)
+    auto pass = [&](auto sums_tag) __attribute__((always_inline)) {
+        constexpr bool SUMS = decltype(sums_tag)::value;
+        const int par = sb.parity;
+        const __amdgpu_buffer_rsrc_t rf = field_rsrc(F.f[par], bytes), rfo = field_rsrc(F.f[par ^ 1], bytes);
+        const __amdgpu_buffer_rsrc_t rv = field_rsrc(F.fd[par], bytes), rvo = field_rsrc(F.fd[par ^ 1], bytes);
+        double d = 0.0, dl = 0.0, cf = 0.0, dpot = 0.0, ch = 0.0, ck = 0.0, vpmean = 0.0, phibar = 0.0, phibar_next;
+        if constexpr (SUMS) {
+            phibar = SPLIT ? sb.phibar : 0.0;
+            phibar_next = phibar;
+        } else {
+            d = uniform(sb.nx.d);
+            dl = d * uniform(sb.nx.laplnorm);
+            cf = 1.0 - d * uniform(sb.nx.friction);
+            dpot = d * uniform(sb.nx.potnorm);
+            vpmean = SPLIT ? uniform(sb.nx.vpmean) : 0.0;
+            phibar = SPLIT ? uniform(sb.phibar) : 0.0;
+            phibar_next = SPLIT ? uniform(sb.nx.phibar_next) : 0.0;
+            if constexpr (POT == kPotHill) {
+                ch = dpot * R.P.hV2;
+                ck = SPLIT ? dpot * (R.P.hV2 * phibar + vpmean) : 0.0;
+            }
+        }
+        // rolling rows of the wave: old f of planes t - 1 and t, new f of plane
+        // t - 1 and of plane 0 (the last i pair); only plane t in FP64, the others
+        // in their storage type (each converts exactly)
+        double dcur[kResRows];
+        T ddn[kResRows], dprev[kResRows], dfirst[kResRows];
+        int hint[kResRows];
+#pragma unroll
+        for (int r = 0; r < kResRows; ++r) {
+            T a[1], b[1];
+            bld_row<T, 1>(rf, roff(m, j0 + r), lane, a);
+            bld_row<T, 1>(rf, roff(0, j0 + r), lane, b);
+            ddn[r] = a[0];
+            dcur[r] = (double)b[0];
+            dprev[r] = (T)0;
+            dfirst[r] = (T)0;
+            hint[r] = POT == kPotTab ? Pv.start((SPLIT ? phibar : 0.0) + dcur[r]) : 0;
+        }
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        struct Rows {
+            T up[kResRows], vin[kResRows];
+        };
+        Rows S0, S1;
+        auto load_rows = [&](unsigned t, Rows &q) __attribute__((always_inline)) {
+#pragma unroll
+            for (int r = 0; r < kResRows; ++r) {
+                T a[1];
+                bld_row<T, 1>(rf, roff(t + 1, j0 + r), lane, a);
+                q.up[r] = a[0];
+                if constexpr (!SUMS) {
+                    bld_row<T, 1>(rv, roff(t, j0 + r), lane, a);
+                    q.vin[r] = a[0];
+                }
+            }
+        };
+        load_rows(0, S0);
+        if constexpr (!SUMS) {
+            L.oldf[0][j0][lane] = dcur[0];
+            L.oldf[0][j0 + kResRows - 1][lane] = dcur[kResRows - 1];
+            __syncthreads();
+        }
+        // plane step t with its rows in `cur`; plane t + 1's go to `fill` (the set
+        // step t - 1 used)
+        auto plane_step = [&](unsigned t, Rows &cur, Rows &fill) __attribute__((always_inline)) {
+            load_rows(t + 1 < N ? t + 1 : t, fill);
+            double dnew[kResRows];
+            T fnew[kResRows];
+            if constexpr (SUMS) {
+#pragma unroll
+                for (int r = 0; r < kResRows; ++r) {
+                    fnew[r] = (T)dcur[r];
+                    dnew[r] = dcur[r];
+                }
+            } else {
+                const double below = L.oldf[t & 1][(j0 - 1) & m][lane];
+                const double above = L.oldf[t & 1][(j0 + kResRows) & m][lane];
+#pragma unroll
+                for (int r = 0; r < kResRows; ++r) {
+                    const double c = dcur[r];
+                    const double kp = __shfl(c, (int)((lane + 1) & 63)), km = __shfl(c, (int)((lane + 63) & 63));
+                    const double yp = r < kResRows - 1 ? dcur[r + 1 < kResRows ? r + 1 : r] : above;
+                    const double ym = r > 0 ? dcur[r > 0 ? r - 1 : 0] : below;
+                    const double lap = lapl_sites(kp, km, yp, ym, (double)cur.up[r], (double)ddn[r], c);
+                    double pt;
+                    if constexpr (POT == kPotHill) {
+                        pt = SPLIT ? fma(ch, c, ck) : ch * c;
+                    } else {
+                        double dV = Pv.dV(SPLIT ? phibar + c : c, hint[r], true, &err);
+                        if (SPLIT) dV -= vpmean;
+                        pt = -dpot * dV;
+                    }
+                    const double v = fma(dl, lap, fma(cf, (double)cur.vin[r], pt));
+                    T a[1] = {(T)v};
+                    fnew[r] = (T)fma(d, v, c);
+                    dnew[r] = (double)fnew[r];
+                    T b[1] = {fnew[r]};
+                    bst_row<T, 1>(rvo, roff(t, j0 + r), lane, a);
+                    bst_row<T, 1>(rfo, roff(t, j0 + r), lane, b);
+                }
+                // plane t + 1's edge rows for the neighbouring waves' next kick
+                L.oldf[(t + 1) & 1][j0][lane] = (double)cur.up[0];
+                L.oldf[(t + 1) & 1][j0 + kResRows - 1][lane] = (double)cur.up[kResRows - 1];
+            }
+            L.newf[t & 1][j0][lane] = fnew[0];
+            __syncthreads();
+            const T above_new = L.newf[t & 1][(j0 + kResRows) & m][lane];
+            if (t == 0) {
+#pragma unroll
+                for (int r = 0; r < kResRows; ++r) dfirst[r] = fnew[r];
+            }
+#pragma unroll
+            for (int r = 0; r < kResRows; ++r) {
+                const double c = dnew[r];
+                const double dk = __shfl(c, (int)((lane + 1) & 63)) - c;
+                const double dj = (r < kResRows - 1 ? dnew[r + 1 < kResRows ? r + 1 : r] : (double)above_new) - c;
+                acc0 += dk * dk + dj * dj;
+                if (t >= 1) {
+                    const double di = c - (double)dprev[r];
+                    acc0 += di * di;
+                }
+                if (t == N - 1) {
+                    const double di = (double)dfirst[r] - c;   // the periodic pair (N - 1, 0)
+                    acc0 += di * di;
+                }
+                if constexpr (POT == kPotHill) {
+                    acc1 += c;
+                    acc2 += c * c;
+                } else {
+                    double V, dV;
+                    Pv.eval(SPLIT ? phibar_next + c : c, &V, &dV, hint[r], true, &err);
+                    acc1 += V;
+                    acc2 += dV;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kResRows; ++r) {
+                dprev[r] = fnew[r];
+                ddn[r] = (T)dcur[r];
+                dcur[r] = (double)cur.up[r];
+            }
+        };
+        // N = 64 planes in steps of two over the two row sets
+        for (unsigned t = 0; t < N; t += 2) {
+            plane_step(t, S0, S1);
+            plane_step(t + 1, S1, S0);
+        }
+        // fixed-order workgroup sums, then the close on the workgroup's record
+        double acc[3] = {acc0, acc1, acc2};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q] = wave_sum(acc[q]);
+        if (lane == 0)
+            for (int q = 0; q < 3; ++q) L.smem[q * kResWaves + w] = acc[q];
+        __threadfence_block();   // this step's stores before the next pass reads them
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot[3];
+            for (int q = 0; q < 3; ++q) {
+                double s = 0.0;
+                for (int ww = 0; ww < kResWaves; ++ww) s += L.smem[q * kResWaves + ww];
+                tot[q] = s;
+            }
+            if constexpr (SUMS) {
+                hill_sums(sb, gsd, tot);
+                sb.red[0] = tot[0];
+                sb.red[1] = tot[1];
+                sb.red[2] = tot[2];
+                sb.nx = step_scalars(&sb, gsd);
+            } else {
+                close_state(sb, tot, gsd, SPLIT);
+            }
+        }
+        __syncthreads();
+    };
+    pass(std::true_type{});
+    for (int s = 0; s < nsteps && !sb.done; ++s) pass(std::false_type{});
+    // the closed record back to the run, with the error flag any thread raised
+    flag_error(bg, err);
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x == 0 && bg->error) sb.error = bg->error;
+    __syncthreads();
+    if (w == 0) copy_bg_wave(bg, &sb, lane);
+}
+
+template <typename T, bool SPLIT, int POT>
+__global__ __launch_bounds__(kResThreads) void kb_resident(const RunDesc *__restrict__ runs,
+                                                           const uint2 *__restrict__ map, int nsteps)
+{
+    __shared__ ResLds<T> L;
+    __shared__ DevBg sb;
+    resident_body<T, SPLIT, POT>(runs[map[blockIdx.x].x], nsteps, L, sb);
 }
 
 // ----------------------------------------------------- tensor march kernel
@@ -1864,6 +2108,11 @@ hipError_t launch_tensor_step(const Fields &F, int split, DevBg *bg, int mode, d
 // (the march kernels step TI + 1 planes three at a time)
 bool march_ok(int N) { return N >= 128 && N <= 256 && (march_ti((unsigned)N) + 1) % 3 == 0; }
 
+bool resident_ok(int N, int sigw, int split, int numerical)
+{
+    return N == kResN && !sigw && (split || !numerical);
+}
+
 unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (int)march_ti((unsigned)N))); }
 
 hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
@@ -1967,6 +2216,7 @@ struct BatchGroup {
     int march = 0;               // fused steps (march kernel) for this group
     int pf = 0;                  // the march kernel variant loading one plane step ahead
     int pot = 0;                 // fused groups: 1 tabulated potential, 0 hilltop
+    int res = 0;                 // resident 64^3 lattices (kb_resident: every step of a launch at once)
     uint2 *map = nullptr;        // kick / march workgroups
     uint2 *rmap = nullptr;       // reduce workgroups (march groups: loop start only)
     uint2 *tmap = nullptr;       // tensor workgroups
@@ -1977,7 +2227,7 @@ struct BatchGroup {
 struct BatchDev {
     RunDesc *runs = nullptr;
     std::vector<BatchGroup> groups;
-    int nruns = 0, split = 0, has_march = 0, has_plain = 0;
+    int nruns = 0, split = 0, has_march = 0, has_plain = 0, has_res = 0;
 };
 
 hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
@@ -1985,6 +2235,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     *out = nullptr;
     std::vector<RunDesc> d(n);
     BatchDev *B = new BatchDev();
+    // one resident workgroup per 64^3 lattice without tensors (FP64 tabulated
+    // lattices excepted: their rows and table exceed the workgroup's LDS)
     B->nruns = n;
     B->split = split;
     // groups of runs launched together: same step structure and kernel shapes
@@ -2001,6 +2253,23 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         d[r].march = march_ok(N) ? 1 : 0;
         d[r].Gm = d[r].march ? march_blocks(N) : 0;
         d[r].split = split;
+        d[r].res = resident_ok(N, runs[r].F.sigw, split, runs[r].P.numerical) ? 1 : 0;
+        if (d[r].res) {
+            B->has_res = 1;
+            const int key = 100000000 + runs[r].P.numerical * 10000000;
+            size_t g = 0;
+            while (g < keys.size() && keys[g] != key) ++g;
+            if (g == keys.size()) {
+                keys.push_back(key);
+                maps.emplace_back();
+                rmaps.emplace_back();
+                tmaps.emplace_back();
+                lds.push_back(0);
+            }
+            lds[g] = std::max(lds[g], pot_lds_bytes(runs[r].P));
+            maps[g].push_back(make_uint2((unsigned)r, 0u));
+            continue;
+        }
         (d[r].march ? B->has_march : B->has_plain) = 1;
         const int kE = d[r].march ? row_elems(N) : shape_elems(N, runs[r].F.kick_e);
         const int tc = runs[r].F.tensor_e;
@@ -2040,7 +2309,8 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     };
     for (size_t g = 0; g < keys.size() && e == hipSuccess; ++g) {
         BatchGroup G;
-        G.pot = keys[g] / 10000000;
+        G.res = keys[g] / 100000000;
+        G.pot = keys[g] / 10000000 % 10;
         G.pf = keys[g] / 1000000 % 10;
         G.march = keys[g] / 100000 % 10;
         G.E = keys[g] / 10000 % 10;
@@ -2160,17 +2430,43 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n
     else kb_row_tensor<T, 4><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);
 }
 
+template <typename T, bool SPLIT>
+static void batch_group_resident(const BatchDev *B, const BatchGroup &G, int count, hipStream_t s)
+{
+    if (G.pot) kb_resident<T, SPLIT, kPotTab><<<G.nblocks, kResThreads, G.pot_lds, s>>>(B->runs, G.map, count);
+    else kb_resident<T, SPLIT, kPotHill><<<G.nblocks, kResThreads, G.pot_lds, s>>>(B->runs, G.map, count);
+}
+
+hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s);
+
+hipError_t batch_steps(const BatchDev *B, long long n0, int count, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    for (const BatchGroup &G : B->groups) {
+        if (!G.res) continue;
+        if (B->split) batch_group_resident<float, true>(B, G, count, s);
+        else batch_group_resident<double, false>(B, G, count, s);
+    }
+    if (B->has_plain || B->has_march)
+        for (int q = 0; q < count; ++q) {
+            const hipError_t e = batch_step(B, n0 + q, s);
+            if (e != hipSuccess) return e;
+        }
+    return hipGetLastError();
+}
+
 hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s)
 {
     if (B->has_plain) {
         for (const BatchGroup &G : B->groups) {
-            if (G.march) continue;
+            if (G.march || G.res) continue;
             if (B->split) batch_group_reduce<float, true>(B, G, s);
             else batch_group_reduce<double, false>(B, G, s);
         }
         kb_friedmann_finish<<<B->nruns, kBlock, 0, s>>>(B->runs);
     }
     for (const BatchGroup &G : B->groups) {
+        if (G.res) continue;
         if (B->split) batch_group_kick<float, true>(B, G, n, s);
         else batch_group_kick<double, false>(B, G, n, s);
     }

@@ -1,5 +1,6 @@
 """Batched stepping (ie_batch_*): many lattices of uneven N advanced by one launch
-per kernel per step, captured in a HIP graph.
+per kernel per step, captured in a HIP graph; 64^3 lattices without tensors by
+one resident workgroup each, every step of a launch in one kernel.
 
   - a run's result does not depend on the other runs of its batch: alone (batch of
     one) and in a mix of sizes and potentials it is bitwise identical;
@@ -41,7 +42,8 @@ def state(run):
 
 def test_alone_equals_in_mix_bitwise():
     table = potential_table()
-    specs = [(32, 11, 0, 0, 0.97), (16, 12, 1, 0, 0.97), (128, 13, 0, 1, 0.96), (32, 14, 0, 0, 0.95)]
+    specs = [(32, 11, 0, 0, 0.97), (16, 12, 1, 0, 0.97), (128, 13, 0, 1, 0.96), (32, 14, 0, 0, 0.95),
+             (64, 15, 0, 0, 0.96), (64, 16, 1, 0, 0.97), (64, 17, 0, 1, 0.97)]
     mix = [new_run(N, s, numerical=p, sigw=g, table=table, ns=ns) for N, s, p, g, ns in specs]
     alone = [new_run(N, s, numerical=p, sigw=g, table=table, ns=ns) for N, s, p, g, ns in specs]
     E.Batch(mix).steps(60, graph_steps=20)
@@ -119,8 +121,68 @@ def test_batch_stops_each_run_at_its_af():
     short64.initialize(43, 0.0935, 0.000796)
     short64.evolve_fields(0.5 * HILL["dt"])
     long_ = new_run(16, 42)
-    E.Batch([short, long_, short64]).steps(40, graph_steps=10)
+    cfg_res = E.make_config(64, precision=E.IE_SPLIT, af=1.005, **HILL)
+    short_res = E.Run(cfg_res)
+    short_res.initialize(44, 0.0935, 0.000796)
+    short_res.evolve_fields(0.5 * HILL["dt"])
+    E.Batch([short, long_, short64, short_res]).steps(40, graph_steps=10)
     bs, bl, b64 = short.background(), long_.background(), short64.background()
+    br = short_res.background()
+    assert br["done"] == 1 and br["steps"] < 40 and br["a"] > 1.005
     assert bs["done"] == 1 and bs["steps"] < 40 and bs["a"] > 1.005
     assert b64["done"] == 1 and b64["steps"] < 40 and b64["a"] > 1.005
     assert bl["done"] == 0 and bl["steps"] == 40
+
+
+def test_fp64_tabulated_64_takes_the_step_path_beside_resident_runs():
+    """An FP64 tabulated 64^3 lattice (its rows and table exceed a resident
+    workgroup's LDS) steps by the per-step kernels in a batch with resident split
+    lattices, and each run gets the bits it gets alone."""
+    table = potential_table()
+    specs = [(64, 51, 1, E.IE_FP64), (64, 52, 1, E.IE_SPLIT), (64, 53, 0, E.IE_FP64)]
+    for prec in (E.IE_FP64, E.IE_SPLIT):
+        grp = [(N, s, p, q) for N, s, p, q in specs if q == prec]
+        mix = [new_run(N, s, numerical=p, precision=q, table=table) for N, s, p, q in grp]
+        alone = [new_run(N, s, numerical=p, precision=q, table=table) for N, s, p, q in grp]
+        E.Batch(mix).steps(30, graph_steps=10)
+        for r in alone:
+            E.Batch([r]).steps(30, graph_steps=0)
+        for a, b in zip(mix, alone):
+            fa, fda, ba = state(a)
+            fb, fdb, bb = state(b)
+            assert np.array_equal(fa, fb) and np.array_equal(fda, fdb)
+            assert ba["a"] == bb["a"] and ba["steps"] == bb["steps"] == 30
+
+
+@pytest.mark.parametrize("chunks", [(10, 10, 10), (7, 1, 6, 16)])
+def test_resident_launches_in_chunks_equal_one(chunks):
+    """A resident 64^3 lattice starts every launch with the sums of its current f
+    in a step's own partition and order, so launches of any lengths give the bits
+    of one launch."""
+    ra, rb = new_run(64, 61), new_run(64, 61)
+    ba = E.Batch([ra])
+    for n in chunks:
+        ba.steps(n, graph_steps=0)
+    E.Batch([rb]).steps(sum(chunks), graph_steps=0)
+    fa, fda, bga = state(ra)
+    fb, fdb, bgb = state(rb)
+    assert np.array_equal(fa, fb) and np.array_equal(fda, fdb)
+    assert bga["a"] == bgb["a"] and bga["ad"] == bgb["ad"] and bga["steps"] == bgb["steps"] == sum(chunks)
+
+
+def test_resident_tabulated_split_matches_single_run_path():
+    """The resident kernel against the single-run row kernels (another sum order):
+    a to 1e-13 relative, the FP32 fluctuation to 1e-5 of its rms over 200 steps."""
+    table = potential_table()
+    rb = new_run(64, 71, numerical=1, table=table)
+    rs = new_run(64, 71, numerical=1, table=table)
+    E.Batch([rb]).steps(200, graph_steps=50)
+    rs.leapfrog_steps(200)
+    fa, _, ba = state(rb)
+    fb, _, bb = state(rs)
+    assert ba["steps"] == bb["steps"] == 200
+    assert ba["a"] == pytest.approx(bb["a"], rel=1e-13)
+    d = fb - fb.mean()
+    err = np.max(np.abs((fa - fa.mean()) - d)) / np.sqrt(np.mean(d ** 2))
+    print(f"resident tabulated 64 vs single run: fluct err {err:.3e}", flush=True)
+    assert err < 1e-5
