@@ -209,6 +209,13 @@ typedef struct {
     int64_t steps;             /* loop steps taken while tuning */
 } ie_tuning;
 ie_status ie_tune(ie_run *run, int steps_per_candidate, ie_tuning *out);
+/* Pins the step kernels' shapes of one run (no counterpart in the reference):
+ * kick_elems 1, 2 or 4 (0: the default), tensor_code one of ie_tuning's codes
+ * (0: the default row kernel), march_variant 0 or 1.  The run keeps them and
+ * ie_leapfrog_steps no longer tunes it; the process-wide choice is unchanged.  A
+ * shape beyond the run's lattice (E above what a row holds, E = 4 march with FP64
+ * storage) is IE_ERR_ARG. */
+ie_status ie_set_shapes(ie_run *run, int kick_elems, int tensor_code, int march_variant);
 /* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904), FP64
  * storage only (IE_ERR_STATE for split storage) in the inflation phase; in the
  * post-inflation phase both kinds, with the post-inflation right-hand side.  RK45 returns *accepted and the

@@ -43,6 +43,7 @@ struct ie_run {
     double *dn5[3] = {nullptr, nullptr, nullptr};   // deltaN RK45 trial state (f, fd, N)
     int in_deltaN = 0;          // FP64 state (dn_f / F.f): deltaN and post-inflation phases
     int in_post = 0;            // post-inflation phase
+    int pinned = 0;             // kernel shapes set by ie_set_shapes: not tuned
     ie_post_params post{};
     double *post_f2 = nullptr, *post_fd2 = nullptr;   // second Phi, Phi' buffers (split runs)
     RkWork *rk4 = nullptr, *rk45 = nullptr;    // RK stage buffers, made on first use
@@ -651,7 +652,7 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
     const Shapes sh = tuned_shapes(r->N, split);
     const bool untuned = march_ok(r->N) ? (sh.march_pf < 0 || (r->F.sigw && sh.tensor_e == 0))
                                         : (sh.kick_e == 0 || (r->F.sigw && sh.tensor_e == 0));
-    if (max_steps >= 256 && row_elems(r->N) >= 2 && untuned) {
+    if (max_steps >= 256 && row_elems(r->N) >= 2 && untuned && !r->pinned) {
         ie_tuning tun;
         ie_status st = ie_tune(r, 8, &tun);
         if (st != IE_OK) return st;
@@ -892,6 +893,27 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     // timings of a loop that reached af part way are not kept
     if (out->steps == (int64_t)rounds * K)
         store_shapes(r->N, split, out->kick_elems, out->tensor_code_chosen, fused ? bm : -1);
+    return IE_OK;
+}
+
+ie_status ie_set_shapes(ie_run *r, int kick_elems, int tensor_code, int march_variant)
+{
+    if (!r) return fail(r, IE_ERR_ARG, "null run");
+    const int split = r->cfg.precision == IE_SPLIT;
+    const int def = row_elems(r->N);
+    const auto elems_ok = [&](int e) { return e == 0 || ((e == 1 || e == 2 || e == 4) && e <= def); };
+    if (!elems_ok(kick_elems)) return fail(r, IE_ERR_ARG, "kick_elems must be 0 or 1, 2, 4 within a row");
+    const int te = tensor_code & 7, tflags = tensor_code & ~7;
+    bool tensor_ok = tensor_code >= 0 && elems_ok(te);
+    if (tflags == kTensorMarch) tensor_ok = tensor_ok && te > 0 && (te < 4 || split);
+    else if (tflags == (kTensorMarch | kTensorPf)) tensor_ok = tensor_ok && te > 0 && te < 4;
+    else tensor_ok = tensor_ok && tflags == 0;
+    if (!tensor_ok) return fail(r, IE_ERR_ARG, "tensor_code is not one of the tuner's codes for this run");
+    if (march_variant != 0 && march_variant != 1) return fail(r, IE_ERR_ARG, "march_variant must be 0 or 1");
+    r->F.kick_e = kick_elems;
+    r->F.tensor_e = tensor_code;
+    r->F.march_pf = march_variant;
+    r->pinned = 1;
     return IE_OK;
 }
 

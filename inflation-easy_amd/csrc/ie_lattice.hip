@@ -308,8 +308,11 @@ __device__ void close_state(DevBg &b, const double (&tot_in)[3], double gsd, int
 // finds the previous step's record in slot n & 1: its kernel closes it with the
 // partial sums that step wrote to partials slot n & 1 (every workgroup alike, the
 // same bits), marches with the result, writes that record to slot (n + 1) & 1
-// (workgroup 0) and its own partials to slot (n + 1) & 1.  No launch writes what
-// it reads; bg itself is brought up to date once, after the loop.
+// (workgroup 0) and its own partials to slot (n + 1) & 1.  A step whose record
+// says the loop has ended writes that record on to slot (n + 1) & 1 unchanged, so
+// both slots say so and no tensor launch after the end finds an older record.
+// No launch writes what it reads; bg itself is brought up to date once, after
+// the loop.
 // Word-wise copy of a background record by the lanes of one wave.
 static_assert(sizeof(DevBg) % 8 == 0 && sizeof(DevBg) / 8 <= 64, "DevBg is copied by one wave");
 __device__ __forceinline__ void copy_bg_wave(DevBg *dst, const DevBg *src, unsigned lane)
@@ -744,7 +747,13 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
             for (unsigned b = threadIdx.x; b < G; b += kBlock)
 #pragma unroll
                 for (int q = 0; q < 3; ++q) tot[q] += partials[((size_t)sp * G + b) * 3 + q];
-        if (prev->done) return;
+        if (prev->done) {
+            // the ended loop's record goes on into this step's slot, which this
+            // step's tensor launch and the next step read: left alone, it would
+            // hold the record of two steps back, not done
+            if (blk == 0 && w == 0) copy_bg_wave(F.rec + (sp ^ 1), prev, lane);
+            return;
+        }
         pmarched = prev->marched != 0;
         par = prev->parity ^ (pmarched ? 1 : 0);
     }

@@ -171,6 +171,7 @@ def lib() -> C.CDLL:
             "ie_leapfrog_steps": (st, [P, C.c_int, C.POINTER(Background)]),
             "ie_profile_steps": (st, [P, C.c_int, C.POINTER(Profile)]),
             "ie_tune": (st, [P, C.c_int, C.POINTER(Tuning)]),
+            "ie_set_shapes": (st, [P, C.c_int, C.c_int, C.c_int]),
             "ie_rk4_step": (st, [P, C.c_double]),
             "ie_rk45_step": (st, [P, C.c_double, C.c_double, ip, dp]),
             "ie_spectrum_numbins": (C.c_int, [C.c_int]),
@@ -399,6 +400,11 @@ class Run:
         t = Tuning()
         self._check(lib().ie_tune(self.h, steps_per_candidate, C.byref(t)))
         return t.as_dict()
+
+    def set_shapes(self, kick_elems: int = 0, tensor_code: int = 0, march_variant: int = 0) -> None:
+        """Pin this run's step kernel shapes (ie_tuning's codes); the run is no
+        longer tuned by leapfrog_steps."""
+        self._check(lib().ie_set_shapes(self.h, kick_elems, tensor_code, march_variant))
 
     def profile_steps(self, n: int) -> dict:
         p = Profile()
