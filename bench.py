@@ -92,6 +92,13 @@ def tabulated_case(N: int = 128, steps: int = 1000, warmup: int = 50) -> dict:
         out[f"{name}_ms_per_step"] = secs / steps * 1e3
         out[f"{name}_site_updates_per_s"] = N ** 3 * steps / secs
     out["split_over_fp64"] = out["split_site_updates_per_s"] / out["fp64_site_updates_per_s"]
+    # the hilltop split step at the same N and dt: the cost of the table lookups
+    run = make_run(N, E.IE_SPLIT, 0)
+    run.tune(8)
+    secs, _ = time_steps(run, steps, warmup)
+    run.close()
+    out["hilltop_split_ms_per_step"] = secs / steps * 1e3
+    out["split_over_hilltop_time"] = out["split_ms_per_step"] / out["hilltop_split_ms_per_step"]
     return out
 
 
@@ -526,7 +533,7 @@ def main() -> int:
                                                   graph_steps=args.graph_steps)}), flush=True)
         return 0
     if args.only == "tab":
-        print(json.dumps({"tabulated_128": tabulated_case(128)}), flush=True)
+        print(json.dumps({f"tabulated_{N}": tabulated_case(N)}), flush=True)
         return 0
     cases = {"split": (E.IE_SPLIT, 0), "fp64": (E.IE_FP64, 0), "sigw": (E.IE_SPLIT, 1)}
     todo = [args.only] if args.only else ["split", "fp64", "sigw"]
@@ -585,7 +592,7 @@ def main() -> int:
         ms = prof["loop_ms"] if one_launch else prof["kick_drift_ms"]
         achieved = alg / (ms * 1e-3) / 1e9
         variant = head["tuning"].get("march_variant", 0)
-        kname = ("k_march_pf<float,true,4>" if variant == 1 else "k_march_w5<float,true,4>") + \
+        kname = ("k_march_pf<float, true, 4, 0>" if variant == 1 else "k_march<float, true, 4, 0>") + \
             " (previous step's close + kick + drift + next-step sums: the whole loop step)"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
@@ -595,8 +602,9 @@ def main() -> int:
                            "kernel_ms_per_launch_events": prof["kick_drift_ms"],
                            "step_ms_events": prof["step_ms"]}
     if not args.only:
-        out["tabulated_128"] = tabulated_case(128)
-        print(f"# tabulated 128: {out['tabulated_128']}", file=sys.stderr, flush=True)
+        for tn in (128, N):
+            out[f"tabulated_{tn}"] = tabulated_case(tn)
+            print(f"# tabulated {tn}: {out[f'tabulated_{tn}']}", file=sys.stderr, flush=True)
     if args.pzeta_check and not args.only:
         out["c2_tabulated_128"] = c2_tabulated(128)
         print(f"# c2: {out['c2_tabulated_128']}", file=sys.stderr, flush=True)

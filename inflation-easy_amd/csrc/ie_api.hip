@@ -187,6 +187,20 @@ int ie_device_count(void)
 
 const char *ie_last_error(const ie_run *run) { return run ? run->err.c_str() : g_create_error.c_str(); }
 
+// Device bytes of a tabulated potential: x, V, V' as given, then the interval
+// records and the two index levels (ie_tabindex.h), each part 256-byte aligned.
+static size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static size_t table_bytes(const ie_config &c)
+{
+    if (!c.numerical_potential || c.tab_n < 2 || !c.tab_x) return 0;
+    const int n = c.tab_n;
+    long subs = 0;
+    const int k1 = ie::tab_k1(c.tab_x, n, &subs);
+    return align_up(3 * (size_t)n * 8) + align_up((size_t)n * sizeof(ie::TabRec)) +
+           align_up((size_t)k1 * sizeof(ie::TabBucket)) + align_up((size_t)subs * sizeof(ie::TabSub));
+}
+
 size_t ie_footprint_bytes(const ie_config *c)
 {
     if (!c || !is_pow2(c->N)) return 0;
@@ -197,7 +211,7 @@ size_t ie_footprint_bytes(const ie_config *c)
     b += gs * 8;                                   // scratch (downloads, deltaN)
     if (c->precision == IE_SPLIT) b += 2 * gs * 8; // FP64 state of the deltaN phase
     b += ie::fft_workspace_bytes(c->N);            // spectra
-    b += 2048 * 8 * 8 + 3 * sizeof(DevBg) + 7 * (size_t)std::max(c->tab_n, 0) * 8;
+    b += 2048 * 8 * 8 + 3 * sizeof(DevBg) + table_bytes(*c);
     return b;
 }
 
@@ -300,32 +314,30 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->P.n = 0;
     if (c.numerical_potential) {
         const size_t n = (size_t)c.tab_n;
-        // x, V, V' as given; then the lattice kernels' form: V/B^2, V'/B^2 and
-        // the slopes of the intervals [l-1, l]
-        std::vector<double> t(7 * n, 0.0);
-        for (size_t l = 0; l < n; ++l) {
-            t[l] = c.tab_x[l];
-            t[n + l] = c.tab_v[l];
-            t[2 * n + l] = c.tab_dv[l];
-            t[3 * n + l] = c.tab_v[l] * r->P.invB2;
-            t[4 * n + l] = c.tab_dv[l] * r->P.invB2;
-            if (l > 0) {
-                const double dx = c.tab_x[l - 1] - c.tab_x[l];
-                t[5 * n + l] = (c.tab_v[l - 1] - c.tab_v[l]) / dx * r->P.invB2;
-                t[6 * n + l] = (c.tab_dv[l - 1] - c.tab_dv[l]) / dx * r->P.invB2;
-            }
-        }
-        if ((st = dev_alloc(r, (void **)&r->table, 7 * n * 8)) != IE_OK) return bail(st);
-        if (hipMemcpy(r->table, t.data(), 7 * n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        // x, V, V' as given; then the interval records and the O(1) index of
+        // the lattice kernels
+        const ie::TabIndexHost H = ie::tab_index_build(c.tab_x, c.tab_v, c.tab_dv, c.tab_n, r->P.invB2);
+        const size_t o_rec = align_up(3 * n * 8), o_l1 = o_rec + align_up(H.rec.size() * sizeof(ie::TabRec));
+        const size_t o_l2 = o_l1 + align_up(H.l1.size() * sizeof(ie::TabBucket));
+        const size_t total = o_l2 + align_up(H.l2.size() * sizeof(ie::TabSub));
+        if (total > table_bytes(c)) return bail(fail(nullptr, IE_ERR_ARG, "table index larger than its footprint"));
+        std::vector<char> t(total, 0);
+        memcpy(t.data(), c.tab_x, n * 8);
+        memcpy(t.data() + n * 8, c.tab_v, n * 8);
+        memcpy(t.data() + 2 * n * 8, c.tab_dv, n * 8);
+        memcpy(t.data() + o_rec, H.rec.data(), H.rec.size() * sizeof(ie::TabRec));
+        memcpy(t.data() + o_l1, H.l1.data(), H.l1.size() * sizeof(ie::TabBucket));
+        memcpy(t.data() + o_l2, H.l2.data(), H.l2.size() * sizeof(ie::TabSub));
+        if ((st = dev_alloc(r, (void **)&r->table, total)) != IE_OK) return bail(st);
+        if (hipMemcpy(r->table, t.data(), total, hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(nullptr, IE_ERR_HIP, "table upload failed"));
+        char *base = (char *)r->table;
         r->P.n = c.tab_n;
         r->P.x = r->table;
         r->P.v = r->table + n;
         r->P.dv = r->table + 2 * n;
-        r->P.vB = r->table + 3 * n;
-        r->P.dvB = r->table + 4 * n;
-        r->P.svB = r->table + 5 * n;
-        r->P.sdvB = r->table + 6 * n;
+        r->P.T = ie::tab_view(H, (const ie::TabRec *)(base + o_rec), (const ie::TabBucket *)(base + o_l1),
+                              (const ie::TabSub *)(base + o_l2), r->P.x, c.tab_n);
     }
     DevBg &b = *r->bg_host;
     memset(&b, 0, sizeof(b));

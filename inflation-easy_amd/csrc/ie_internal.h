@@ -9,6 +9,7 @@
 #include <string>
 
 #include "ie_api.h"
+#include "ie_tabindex.h"
 
 namespace ie {
 
@@ -60,10 +61,9 @@ struct Pot {
     const double *x, *v, *dv;   // device table, descending x
     // constants of the lattice kernels: V0/B^2, V0 (1-ns)/4 /B^2, V0 (1-ns)/2 /B^2, 1/B^2
     double hV, hV4, hV2, invB2;
-    // the table in the lattice kernels' form: V/B^2, V'/B^2 and the slopes of
-    // their intervals ([l-1, l] at index l), so an interpolation is one
-    // multiply-add instead of a division
-    const double *vB, *dvB, *svB, *sdvB;
+    // the table's O(1) interval index with the lattice kernels' form of V/B^2,
+    // V'/B^2 and their slopes per interval
+    TabIndex T;
 };
 
 // Device pointers of one lattice's fields.  Storage of f, fd is double (FP64) or
@@ -152,30 +152,9 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblocks
 
 // ----------------------------------------------------------------- potential
 
-// A tabulated potential of up to kPotLds rows is staged by the lattice kernels in
-// their workgroup's dynamic LDS (5 n doubles: x, V/B^2, V'/B^2 and their
-// slopes), so the bracket search and interpolation of every site read LDS
-// instead of L2.  Every launch of a staging kernel passes pot_lds_bytes of its
-// potential (the largest of its runs' in a batch).
-constexpr int kPotLds = 1536;
-inline size_t pot_lds_bytes(const Pot &P)
-{
-    return P.numerical && P.n <= kPotLds ? (size_t)5 * P.n * sizeof(double) : 0;
-}
-
 __device__ __forceinline__ int pot_bracket(const Pot &P, double v, int *err)
 {
-    if (v > P.x[0] || v < P.x[P.n - 1]) {
-        *err = 1;
-        return 1;
-    }
-    int lo = 0, hi = P.n - 1;
-    while (hi - lo > 1) {
-        const int mid = lo + (hi - lo) / 2;
-        if (P.x[mid] >= v) lo = mid;
-        else hi = mid;
-    }
-    return hi;
+    return tab_bisect_ref(P.x, P.n, v, err);
 }
 
 __device__ __forceinline__ double pot_interp(const Pot &P, double v, int l, const double *y)
@@ -184,7 +163,8 @@ __device__ __forceinline__ double pot_interp(const Pot &P, double v, int l, cons
     return y1 + (v - x1) * (y0 - y1) / (x0 - x1);
 }
 
-// V/B^2 and V'/B^2 at v; *err set when v leaves the table.
+// V/B^2 and V'/B^2 at v with the reference's expressions; *err set when v
+// leaves the table.
 __device__ __forceinline__ void pot_eval(const Pot &P, double v, double *V, double *dV, int *err)
 {
     if (P.numerical) {
@@ -197,175 +177,178 @@ __device__ __forceinline__ void pot_eval(const Pot &P, double v, double *V, doub
     }
 }
 
-__device__ __forceinline__ double pot_dV(const Pot &P, double v, int *err)
-{
-    if (P.numerical) {
-        const int l = pot_bracket(P, v, err);
-        return pot_interp(P, v, l, P.dv) / P.B2;
-    }
-    return -P.V0 * (1.0 - P.ns) / 2.0 * v / P.B2;
-}
-
-// Lattice-kernel forms: the division by B^2 folded into host constants (one
-// rounding apart from the reference's per-call expression; the lattice sums are
-// order-dependent at that level anyway).  The deltaN kernel keeps the exact
-// expressions.
-__device__ __forceinline__ void pot_eval_fast(const Pot &P, double v, double *V, double *dV, int *err)
-{
-    if (P.numerical) {
-        const int l = pot_bracket(P, v, err);
-        *V = pot_interp(P, v, l, P.v) * P.invB2;
-        *dV = pot_interp(P, v, l, P.dv) * P.invB2;
-    } else {
-        *V = P.hV - P.hV4 * (v * v);
-        *dV = -P.hV2 * v;
-    }
-}
-
-__device__ __forceinline__ double pot_V_fast(const Pot &P, double v, int *err)
-{
-    if (P.numerical) {
-        const int l = pot_bracket(P, v, err);
-        return pot_interp(P, v, l, P.v) * P.invB2;
-    }
-    return P.hV - P.hV4 * (v * v);
-}
-
-__device__ __forceinline__ double pot_dV_fast(const Pot &P, double v, int *err)
-{
-    if (P.numerical) {
-        const int l = pot_bracket(P, v, err);
-        return pot_interp(P, v, l, P.dv) * P.invB2;
-    }
-    return -P.hV2 * v;
-}
-
-// Bracket search on a table x (descending) of n rows: the unique l with
-// x[l-1] >= v > x[l] (n - 1 at v = x[n-1]); *err set when v leaves the table.
-// tab_bisect bisects; tab_walk walks from a hint l (the bracket of a nearby
-// value, as the previous site of the same lane) and ends at the same l (the
-// bracket is unique), in a step or two instead of log2 n dependent reads.
-__device__ __forceinline__ int tab_bisect(const double *x, int n, double v, int *err)
-{
-    if (v > x[0] || v < x[n - 1]) {
-        *err = 1;
-        return 1;
-    }
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) {
-        const int mid = lo + (hi - lo) / 2;
-        if (x[mid] >= v) lo = mid;
-        else hi = mid;
-    }
-    return hi;
-}
-
-__device__ __forceinline__ int tab_walk(const double *x, int n, double xfirst, double xlast, double v, int l,
-                                        int *err)
-{
-    if (v > xfirst || v < xlast) {
-        *err = 1;
-        return 1;
-    }
-    l = l < 1 ? 1 : (l > n - 1 ? n - 1 : l);
-    // v usually lies in the hint's bracket or a neighbour of it (a field whose
-    // sites straddle a grid point moves lanes by one bracket all the time): one
-    // step toward v from the bracket's two ends, without a branch, then the walk
-    // only for the lanes (of a wave with any) that are still outside
-    l += (x[l] >= v && l < n - 1 ? 1 : 0) - (x[l - 1] < v && l > 1 ? 1 : 0);
-    const bool ok = (l == n - 1 || x[l] < v) && x[l - 1] >= v;
-    if (!__all(ok)) {
-        while (l < n - 1 && x[l] >= v) ++l;
-        while (l > 1 && x[l - 1] < v) --l;
-    }
-    return l;
-}
-
-__device__ __forceinline__ double tab_interp(const double *x, const double *y, double v, int l)
-{
-    const double x0 = x[l - 1], x1 = x[l], y0 = y[l - 1], y1 = y[l];
-    return y1 + (v - x1) * (y0 - y1) / (x0 - x1);
-}
-
 // The potential as the lattice kernels evaluate it: the division by B^2 folded
-// into constants and the interpolation through precomputed interval slopes
-// (y_l + (v - x_l) s_l; a rounding or two from the reference's
+// into constants, and for a table the interpolation y_l + (v - x_l) s_l through
+// the interval's precomputed slope (a rounding or two from the reference's
 // y_l + (v - x_l)(y_{l-1} - y_l)/(x_{l-1} - x_l), below the lattice sums' own
-// order dependence).  A table of up to kPotLds rows is read from the workgroup's
-// copy in dynamic LDS (stage_pot), whose pointers the methods take straight from
-// the LDS symbol so that the reads are LDS reads; a longer table from global
-// memory.  Walks start from the lane's previous bracket.
+// order dependence).
+// A lane's K values are looked up together.  The neighbouring sites of a wave
+// lie in a few neighbouring intervals: the first active lane's hint (the
+// interval of a nearby earlier value) names the wave's interval, whose record is
+// read once for the wave through the scalar cache (the tables are never written
+// by a kernel, so they are read as constant memory); a wave across grid points
+// reads its neighbours' too, and then the three intervals beyond them on one
+// side.  The values inside these interpolate from scalar registers: a per-lane
+// copy of a shared record costs the vector memory path as much as a field row,
+// and would bound the step.  A value outside them takes its own hint's record,
+// then the O(1) index of ie_tabindex.h, then (a rounding at an index edge, a
+// table end) the bisection.
+typedef const __attribute__((address_space(4))) double *ConstDoubles;
+
+__device__ __forceinline__ TabRec tab_rec_uniform(const TabRec *rec, int l)
+{
+    const ConstDoubles p = (ConstDoubles)(rec + l);
+    return TabRec{p[0], p[1], p[2], p[3], p[4], p[5]};
+}
+
 struct PotView {
     Pot P;
-    bool lds;
-    double xfirst, xlast;   // the table's ends (numerical potentials)
 
-    __device__ __forceinline__ int bracket(double v, int l, bool walk, int *err) const
+    // The three intervals around c, read through the scalar cache: each value not
+    // yet `ok` that lies in one of them takes that interval's interpolation (mask
+    // arithmetic, no short-circuit, so the checks stay branch-free).
+    template <int K, bool WANT_V>
+    __device__ __forceinline__ void window(int c, const double (&v)[K], bool (&ok)[K], int (&l)[K], double (&V)[K],
+                                           double (&dv)[K]) const
     {
-        extern __shared__ double pot_lds[];
-        if (lds) return walk ? tab_walk(pot_lds, P.n, xfirst, xlast, v, l, err) : tab_bisect(pot_lds, P.n, v, err);
-        return walk ? tab_walk(P.x, P.n, xfirst, xlast, v, l, err) : tab_bisect(P.x, P.n, v, err);
-    }
-    // V/B^2 (q = 1) or V'/B^2 (q = 2) at v in bracket l
-    __device__ __forceinline__ double interp(double v, int l, int q) const
-    {
-        extern __shared__ double pot_lds[];
-        if (lds) {
-            const double *y = pot_lds + q * P.n, *sl = pot_lds + (q + 2) * P.n;
-            return y[l] + (v - pot_lds[l]) * sl[l];
+        const int cm = c > 1 ? c - 1 : c, cp = c < P.n - 1 ? c + 1 : c;
+        const TabRec rm = tab_rec_uniform(P.T.rec, cm), ru = tab_rec_uniform(P.T.rec, c),
+                     rp = tab_rec_uniform(P.T.rec, cp);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const bool le = v[q] <= ru.xhi, gt = v[q] > ru.xlo;
+            const bool im = !le & (v[q] <= rm.xhi) & (cm != c), ip = !gt & (v[q] > rp.xlo) & (cp != c);
+            const bool take = !ok[q] & ((le & gt) | im | ip);
+            const double dm = v[q] - rm.xlo, du = v[q] - ru.xlo, dp = v[q] - rp.xlo;
+            const double gm = rm.ydv + dm * rm.sdv, gu = ru.ydv + du * ru.sdv, gp = rp.ydv + dp * rp.sdv;
+            dv[q] = take ? (im ? gm : (ip ? gp : gu)) : dv[q];
+            if (WANT_V) {
+                const double wm = rm.yv + dm * rm.sv, wu = ru.yv + du * ru.sv, wp = rp.yv + dp * rp.sv;
+                V[q] = take ? (im ? wm : (ip ? wp : wu)) : V[q];
+            }
+            l[q] = take ? (im ? cm : (ip ? cp : c)) : l[q];
+            ok[q] = ok[q] | take;
         }
-        const double *y = q == 1 ? P.vB : P.dvB, *sl = q == 1 ? P.svB : P.sdvB;
-        return y[l] + (v - P.x[l]) * sl[l];
     }
-    __device__ __forceinline__ double dV(double v, int &hint, bool walk, int *err) const
+
+    template <int K, bool WANT_V>
+    __device__ __forceinline__ void lookup(const double (&v)[K], int (&l)[K], double (&V)[K], double (&dv)[K],
+                                           int *err) const
     {
-        if (!P.numerical) return -P.hV2 * v;
-        hint = bracket(v, hint, walk, err);
-        return interp(v, hint, 2);
-    }
-    __device__ __forceinline__ void eval(double v, double *V, double *dV, int &hint, bool walk, int *err) const
-    {
-        if (!P.numerical) {
-            *V = P.hV - P.hV4 * (v * v);
-            *dV = -P.hV2 * v;
+        // the wave's interval lu, in scalar registers: the common case, every
+        // value of the wave inside it
+        const int lu = __builtin_amdgcn_readfirstlane(l[0]);
+        const TabRec ru = tab_rec_uniform(P.T.rec, lu);
+        bool ok[K], all = true;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            ok[q] = (v[q] <= ru.xhi) & (v[q] > ru.xlo);
+            all = all & ok[q];
+        }
+        if (__all(all)) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                dv[q] = ru.ydv + (v[q] - ru.xlo) * ru.sdv;
+                if (WANT_V) V[q] = ru.yv + (v[q] - ru.xlo) * ru.sv;
+                l[q] = lu;
+            }
             return;
         }
-        hint = bracket(v, hint, walk, err);
-        *V = interp(v, hint, 1);
-        *dV = interp(v, hint, 2);
+        // a wave across grid points: lu and its neighbours, then, for the values
+        // still outside, the three intervals two further on the side of the wave's
+        // first such value
+#pragma unroll
+        for (int q = 0; q < K; ++q) ok[q] = false;
+        window<K, WANT_V>(lu, v, ok, l, V, dv);
+        all = true;
+        int side = 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            all = all & ok[q];
+            side = side == 0 && !ok[q] ? (v[q] > ru.xhi ? -1 : 1) : side;
+        }
+        if (__any(!all)) {
+            const unsigned long long miss = __ballot(side != 0);
+            const int s = __builtin_amdgcn_readlane(side, __ffsll((long long)miss) - 1);
+            const int c2 = lu + 2 * s;
+            window<K, WANT_V>(c2 < 1 ? 1 : (c2 > P.n - 1 ? P.n - 1 : c2), v, ok, l, V, dv);
+            all = true;
+#pragma unroll
+            for (int q = 0; q < K; ++q) all = all & ok[q];
+        }
+        // the rest (a wave across more intervals, an index edge, a table end):
+        // the value's own hint, then the O(1) index, then the bisection
+        if (__any(!all)) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                if (!ok[q]) {
+                    int lq = l[q];
+                    TabRec r = P.T.rec[lq];
+                    if (!tab_in(r, v[q])) {
+                        lq = tab_candidate(P.T, v[q]);
+                        r = P.T.rec[lq];
+                        if (!tab_in(r, v[q])) {
+                            lq = tab_bisect_ref(P.x, P.n, v[q], err);
+                            r = P.T.rec[lq];
+                        }
+                    }
+                    l[q] = lq;
+                    dv[q] = r.ydv + (v[q] - r.xlo) * r.sdv;
+                    if (WANT_V) V[q] = r.yv + (v[q] - r.xlo) * r.sv;
+                }
+            }
+        }
     }
-    // a first hint for the lane's walks
-    __device__ __forceinline__ int start(double v) const
+    // V'/B^2 at the K values v (hints l, updated)
+    template <int K>
+    __device__ __forceinline__ void dV(const double (&v)[K], int (&l)[K], double (&dv)[K], int *err) const
     {
-        int unused = 0;
-        return P.numerical ? bracket(v, 0, false, &unused) : 0;
+        if (!P.numerical) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) dv[q] = -P.hV2 * v[q];
+            return;
+        }
+        double unused[K];
+        lookup<K, false>(v, l, unused, dv, err);
     }
+    // V/B^2 and V'/B^2 at the K values v (hints l, updated)
+    template <int K>
+    __device__ __forceinline__ void eval(const double (&v)[K], int (&l)[K], double (&V)[K], double (&dv)[K],
+                                         int *err) const
+    {
+        if (!P.numerical) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                V[q] = P.hV - P.hV4 * (v[q] * v[q]);
+                dv[q] = -P.hV2 * v[q];
+            }
+            return;
+        }
+        lookup<K, true>(v, l, V, dv, err);
+    }
+    // one value from its hint (updated)
+    __device__ __forceinline__ double dV1(double v, int &hint, int *err) const
+    {
+        double x[1] = {v}, d[1];
+        int l[1] = {hint};
+        dV<1>(x, l, d, err);
+        hint = l[0];
+        return d[0];
+    }
+    __device__ __forceinline__ void eval1(double v, double *V, double *dv, int &hint, int *err) const
+    {
+        double x[1] = {v}, a[1], d[1];
+        int l[1] = {hint};
+        eval<1>(x, l, a, d, err);
+        hint = l[0];
+        *V = a[0];
+        *dv = d[0];
+    }
+    // a first hint for the interval of v
+    __device__ __forceinline__ int start(double v) const { return P.numerical ? tab_candidate(P.T, v) : 0; }
 };
 
-// Copies a table of up to kPotLds rows (x, V/B^2, V'/B^2, slopes) into the workgroup's dynamic
-// LDS (a uniform branch: every thread of the workgroup reaches the barrier or
-// none does).
-__device__ __forceinline__ PotView stage_pot(const Pot &P)
-{
-    extern __shared__ double pot_lds[];
-    PotView Q{P, false, 0.0, 0.0};
-    if (P.numerical) {
-        Q.xfirst = P.x[0];
-        Q.xlast = P.x[P.n - 1];
-    }
-    if (P.numerical && P.n <= kPotLds) {
-        for (int e = threadIdx.x; e < P.n; e += blockDim.x) {
-            pot_lds[e] = P.x[e];
-            pot_lds[P.n + e] = P.vB[e];
-            pot_lds[2 * P.n + e] = P.dvB[e];
-            pot_lds[3 * P.n + e] = P.svB[e];
-            pot_lds[4 * P.n + e] = P.sdvB[e];
-        }
-        __syncthreads();
-        Q.lds = true;
-    }
-    return Q;
-}
+__device__ __forceinline__ PotView pot_view(const Pot &P) { return PotView{P}; }
 
 // ---------------------------------------------------------------- launchers
 // (ie_lattice.hip)

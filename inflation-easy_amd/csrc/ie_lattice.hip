@@ -411,7 +411,7 @@ __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P0, 
                                                 unsigned G, unsigned blk, double *smem)
 {
     if (loop && bg->done) return;
-    const PotView Pv = stage_pot(P0);
+    const PotView Pv = pot_view(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
     const double phibar = SPLIT ? bg->phibar : 0.0;
@@ -433,6 +433,10 @@ __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P0, 
         ld_row<T, E>(plane_m + (j << logN), lane, dn);
         T kp_last, km_first;
         k_neighbours<T, E>(cur, plane + (j << logN) - w.k0, w, N, lane, kp_last, km_first);
+        double tv[E], tV[E], tdV[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) tv[e] = phibar + (double)cur[e];
+        Pv.eval<E>(tv, hint, tV, tdV, &err);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const double c = (double)cur[e];
@@ -440,10 +444,8 @@ __device__ __forceinline__ void row_reduce_body(const Fields &F, const Pot &P0, 
             const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
             const double lap = lapl_sites(kp, km, (double)nxt[e], (double)prev[e], (double)up[e], (double)dn[e], c);
             acc[0] -= c * lap;
-            double V, dV;
-            Pv.eval(phibar + c, &V, &dV, hint[e], true, &err);
-            acc[1] += V;
-            if (SPLIT) acc[2] += dV;
+            acc[1] += tV[e];
+            if (SPLIT) acc[2] += tdV[e];
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -462,7 +464,7 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P0, De
                                               unsigned blk)
 {
     if (loop && bg->done) return;
-    const PotView Pv = stage_pot(P0);
+    const PotView Pv = pot_view(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const int par = bg->parity;
     const T *__restrict__ f = (const T *)F.f[par];
@@ -492,6 +494,10 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P0, De
         ld_row<T, E>(fd + roff, lane, vin);
         T kp_last, km_first;
         k_neighbours<T, E>(cur, f + roff - w.k0, w, N, lane, kp_last, km_first);
+        double tv[E], tdV[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) tv[e] = (SPLIT ? phibar : 0.0) + (double)cur[e];
+        Pv.dV<E>(tv, hint, tdV, &err);
         T vout[E], fnew[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -499,7 +505,7 @@ __device__ __forceinline__ void row_kick_body(const Fields &F, const Pot &P0, De
             const double kp = (double)(e < E - 1 ? cur[e + 1 < E ? e + 1 : e] : kp_last);
             const double km = (double)(e > 0 ? cur[e > 0 ? e - 1 : 0] : km_first);
             const double lap = lapl_sites(kp, km, (double)nxt[e], (double)prev[e], (double)up[e], (double)dn[e], c);
-            double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
+            double dV = tdV[e];
             if (SPLIT) dV -= vpmean;
             double v = (double)vin[e];
             v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
@@ -757,7 +763,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         pmarched = prev->marched != 0;
         par = prev->parity ^ (pmarched ? 1 : 0);
     }
-    const PotView Pv = stage_pot(P0);
+    const PotView Pv = pot_view(P0);
     // fd is read from [par] and written to [par ^ 1]: the halo row and the extra
     // plane read fd rows that their owning workgroups update in the same launch
     const unsigned bytes = N * N * N * (unsigned)sizeof(T);
@@ -867,7 +873,9 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
             njm[e] = L.oldf[0][w].at(lane, e);
         }
     }
-    // brackets of the tabulated potential, walked from each lane's previous site
+    // intervals of the tabulated potential: each lane's sites start from the
+    // intervals of the previous plane's sites (the kick's) and of the same site
+    // (the sums')
     int hint[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) hint[e] = POT == kPotTab ? Pv.start((SPLIT ? phibar : 0.0) + dcur[e]) : 0;
@@ -889,6 +897,13 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         } else {
             const double kp_last = __shfl(dcur[0], (int)((lane + 1) & 63));
             const double km_first = __shfl(dcur[E - 1], (int)((lane + 63) & 63));
+            double tdv[E];
+            if constexpr (POT == kPotTab) {
+                double tv[E];
+#pragma unroll
+                for (int e = 0; e < E; ++e) tv[e] = SPLIT ? phibar + dcur[e] : dcur[e];
+                Pv.dV<E>(tv, hint, tdv, &err);
+            }
             T vout[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -901,9 +916,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
                 if constexpr (POT == kPotHill) {
                     pt = SPLIT ? fma(ch, c, ck) : ch * c;
                 } else {
-                    double dV = Pv.dV(SPLIT ? phibar + c : c, hint[e], true, &err);
-                    if (SPLIT) dV -= vpmean;
-                    pt = -dpot * dV;
+                    pt = -dpot * (SPLIT ? tdv[e] - vpmean : tdv[e]);
                 }
                 const double v = fma(dl, lap, fma(cf, (double)cur.vin[e], pt));
                 vout[e] = (T)v;
@@ -937,6 +950,13 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         if (owner) {
             if (t < TI) {
                 const double knext = __shfl(dnew[0], (int)((lane + 1) & 63));
+                double tV[E], tdV[E];
+                if constexpr (POT == kPotTab) {
+                    double tv[E];
+#pragma unroll
+                    for (int e = 0; e < E; ++e) tv[e] = SPLIT ? phibar_next + dnew[e] : dnew[e];
+                    Pv.eval<E>(tv, hint, tV, tdV, &err);
+                }
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const double c = dnew[e];
@@ -947,10 +967,8 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
                         acc1 += c;
                         acc2 += c * c;
                     } else {
-                        double V, dV;
-                        Pv.eval(SPLIT ? phibar_next + c : c, &V, &dV, hint[e], true, &err);
-                        acc1 += V;
-                        acc2 += dV;
+                        acc1 += tV[e];
+                        acc2 += tdV[e];
                     }
                 }
             }
@@ -1121,7 +1139,7 @@ __device__ __forceinline__ void resident_body(const RunDesc &R, int nsteps, ResL
     if (w == 0) copy_bg_wave(&sb, bg, lane);
     __syncthreads();
     if (sb.done) return;
-    const PotView Pv = stage_pot(R.P);
+    const PotView Pv = pot_view(R.P);
     const unsigned bytes = N * N * N * (unsigned)sizeof(T);
     const double gsd = (double)N * N * N;
     int err = 0;
@@ -1155,7 +1173,7 @@ __device__ __forceinline__ void resident_body(const RunDesc &R, int nsteps, ResL
         // in their storage type (each converts exactly)
         double dcur[kResRows];
         T ddn[kResRows], dprev[kResRows], dfirst[kResRows];
-        int hint[kResRows];
+        int hint[kResRows];   // intervals of the tabulated potential, per row
 #pragma unroll
         for (int r = 0; r < kResRows; ++r) {
             T a[1], b[1];
@@ -1216,7 +1234,7 @@ __device__ __forceinline__ void resident_body(const RunDesc &R, int nsteps, ResL
                     if constexpr (POT == kPotHill) {
                         pt = SPLIT ? fma(ch, c, ck) : ch * c;
                     } else {
-                        double dV = Pv.dV(SPLIT ? phibar + c : c, hint[r], true, &err);
+                        double dV = Pv.dV1(SPLIT ? phibar + c : c, hint[r], &err);
                         if (SPLIT) dV -= vpmean;
                         pt = -dpot * dV;
                     }
@@ -1258,7 +1276,7 @@ __device__ __forceinline__ void resident_body(const RunDesc &R, int nsteps, ResL
                     acc2 += c * c;
                 } else {
                     double V, dV;
-                    Pv.eval(SPLIT ? phibar_next + c : c, &V, &dV, hint[r], true, &err);
+                    Pv.eval1(SPLIT ? phibar_next + c : c, &V, &dV, hint[r], &err);
                     acc1 += V;
                     acc2 += dV;
                 }
@@ -1648,7 +1666,7 @@ __device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P0,
                                                  unsigned G, unsigned blk, double *smem)
 {
     if (loop && bg->done) return;
-    const PotView Pv = stage_pot(P0);
+    const PotView Pv = pot_view(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
     const unsigned gs = N * N * N;
     const T *__restrict__ f = (const T *)F.f[bg->parity];
@@ -1666,8 +1684,8 @@ __device__ __forceinline__ void site_reduce_body(const Fields &F, const Pot &P0,
             const double lap = stencil_lapl<T, kUnroll>(s, u);
             acc[0] -= c * lap;
             double V, dV;
-            int h = 0;
-            Pv.eval(phibar + c, &V, &dV, h, false, &err);
+            int h = Pv.start(phibar + c);
+            Pv.eval1(phibar + c, &V, &dV, h, &err);
             acc[1] += V;
             if (SPLIT) acc[2] += dV;
         }
@@ -1683,7 +1701,7 @@ __device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P0, D
                                                unsigned G, unsigned blk)
 {
     if (loop && bg->done) return;
-    const PotView Pv = stage_pot(P0);
+    const PotView Pv = pot_view(P0);
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN;
     const unsigned gs = N * N * N;
     const int par = bg->parity;
@@ -1708,8 +1726,8 @@ __device__ __forceinline__ void site_kick_body(const Fields &F, const Pot &P0, D
                 if (!s.ok[u]) continue;
                 const double c = (double)s.c[u];
                 const double lap = stencil_lapl<T, kUnroll>(s, u);
-                int h = 0;
-                double dV = Pv.dV(SPLIT ? phibar + c : c, h, false, &err);
+                int h = Pv.start(SPLIT ? phibar + c : c);
+                double dV = Pv.dV1(SPLIT ? phibar + c : c, h, &err);
                 if (SPLIT) dV -= vpmean;
                 double v = (double)vin[u];
                 v = v + d * (laplnorm * lap - friction * v - potnorm * dV);
@@ -1923,10 +1941,10 @@ template <typename T, bool SPLIT>
 void reduce_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, int loop, unsigned G, hipStream_t s)
 {
     switch (row_elems(F.N)) {
-    case 1: k_row_reduce<T, SPLIT, 1><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
-    case 2: k_row_reduce<T, SPLIT, 2><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
-    case 4: k_row_reduce<T, SPLIT, 4><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
-    default: k_site_reduce<T, SPLIT><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, partials, loop); break;
+    case 1: k_row_reduce<T, SPLIT, 1><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    case 2: k_row_reduce<T, SPLIT, 2><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    case 4: k_row_reduce<T, SPLIT, 4><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
+    default: k_site_reduce<T, SPLIT><<<G, kBlock, 0, s>>>(F, P, bg, partials, loop); break;
     }
 }
 
@@ -1934,9 +1952,9 @@ template <typename T, bool SPLIT, int MODE>
 void row_kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int loop, unsigned G, hipStream_t s)
 {
     switch (shape_elems(F.N, F.kick_e)) {
-    case 1: k_row_kick<T, SPLIT, 1, MODE><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, loop); break;
-    case 2: k_row_kick<T, SPLIT, 2, MODE><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, loop); break;
-    default: k_row_kick<T, SPLIT, 4, MODE><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, loop); break;
+    case 1: k_row_kick<T, SPLIT, 1, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
+    case 2: k_row_kick<T, SPLIT, 2, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
+    default: k_row_kick<T, SPLIT, 4, MODE><<<G, kBlock, 0, s>>>(F, P, bg, loop); break;
     }
 }
 
@@ -1945,10 +1963,10 @@ void kick_dispatch(const Fields &F, const Pot &P, DevBg *bg, int mode, double d_
                    hipStream_t s)
 {
     if (mode == DRIFT) {
-        k_site_kick<T, SPLIT, DRIFT><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, d_arg, loop);
+        k_site_kick<T, SPLIT, DRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
     } else if (row_elems(F.N) == 0) {
-        if (mode == KICK) k_site_kick<T, SPLIT, KICK><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, d_arg, loop);
-        else k_site_kick<T, SPLIT, KICKDRIFT><<<G, kBlock, pot_lds_bytes(P), s>>>(F, P, bg, d_arg, loop);
+        if (mode == KICK) k_site_kick<T, SPLIT, KICK><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
+        else k_site_kick<T, SPLIT, KICKDRIFT><<<G, kBlock, 0, s>>>(F, P, bg, d_arg, loop);
     } else {
         if (mode == KICK) row_kick_dispatch<T, SPLIT, KICK>(F, P, bg, loop, G, s);
         else row_kick_dispatch<T, SPLIT, KICKDRIFT>(F, P, bg, loop, G, s);
@@ -1998,11 +2016,11 @@ void march_dispatch_pot(const Fields &F, const Pot &P, DevBg *bg, double *partia
 {
     const int E = row_elems(F.N);
     if (F.march_pf) {
-        if (E == 2) k_march_pf<T, SPLIT, 2, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
-        else k_march_pf<T, SPLIT, 4, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march_pf<T, SPLIT, 2, POT><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        else k_march_pf<T, SPLIT, 4, POT><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
     } else {
-        if (E == 2) k_march<T, SPLIT, 2, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
-        else k_march<T, SPLIT, 4, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march<T, SPLIT, 2, POT><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
+        else k_march<T, SPLIT, 4, POT><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials, sp);
     }
 }
 
@@ -2016,8 +2034,8 @@ void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, 
 template <typename T, bool SPLIT, int POT>
 void march_sums_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
 {
-    if (row_elems(F.N) == 2) k_march_sums<T, SPLIT, 2, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
-    else k_march_sums<T, SPLIT, 4, POT><<<G, kMarchThreads, pot_lds_bytes(P), s>>>(F, P, bg, partials);
+    if (row_elems(F.N) == 2) k_march_sums<T, SPLIT, 2, POT><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
+    else k_march_sums<T, SPLIT, 4, POT><<<G, kMarchThreads, 0, s>>>(F, P, bg, partials);
 }
 
 }  // namespace
@@ -2230,7 +2248,6 @@ struct BatchGroup {
     uint2 *rmap = nullptr;       // reduce workgroups (march groups: loop start only)
     uint2 *tmap = nullptr;       // tensor workgroups
     int nblocks = 0, nrblocks = 0, ntblocks = 0;
-    size_t pot_lds = 0;          // dynamic LDS of the potential tables (largest of the group)
 };
 
 struct BatchDev {
@@ -2251,7 +2268,6 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     // groups of runs launched together: same step structure and kernel shapes
     std::vector<int> keys;
     std::vector<std::vector<uint2>> maps, rmaps, tmaps;
-    std::vector<size_t> lds;
     for (int r = 0; r < n; ++r) {
         const int N = runs[r].F.N;
         d[r].F = runs[r].F;
@@ -2273,9 +2289,7 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
                 maps.emplace_back();
                 rmaps.emplace_back();
                 tmaps.emplace_back();
-                lds.push_back(0);
             }
-            lds[g] = std::max(lds[g], pot_lds_bytes(runs[r].P));
             maps[g].push_back(make_uint2((unsigned)r, 0u));
             continue;
         }
@@ -2295,9 +2309,7 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
             maps.emplace_back();
             rmaps.emplace_back();
             tmaps.emplace_back();
-            lds.push_back(0);
         }
-        lds[g] = std::max(lds[g], pot_lds_bytes(runs[r].P));
         for (unsigned b = 0; b < d[r].G; ++b) rmaps[g].push_back(make_uint2((unsigned)r, b));
         if (runs[r].F.sigw) {
             // tensor step: march kernel (N >= 64) or site kernel
@@ -2328,7 +2340,6 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         G.nblocks = (int)maps[g].size();
         G.nrblocks = (int)rmaps[g].size();
         G.ntblocks = (int)tmaps[g].size();
-        G.pot_lds = lds[g];
         upload(maps[g], &G.map);
         upload(rmaps[g], &G.rmap);
         upload(tmaps[g], &G.tmap);
@@ -2360,8 +2371,8 @@ static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream
 template <typename T, bool SPLIT, int POT>
 static void batch_march_sums(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
-    if (G.E == 2) kb_march_sums<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
-    else kb_march_sums<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map);
+    if (G.E == 2) kb_march_sums<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
+    else kb_march_sums<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map);
 }
 
 template <typename T, bool SPLIT, int POT>
@@ -2369,11 +2380,11 @@ static void batch_march(const BatchDev *B, const BatchGroup &G, unsigned sp, hip
 {
     // (march groups have N = 128 or 256: E = 2 or 4)
     if (G.pf) {
-        if (G.E == 2) kb_march_pf<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-        else kb_march_pf<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+        if (G.E == 2) kb_march_pf<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+        else kb_march_pf<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
     } else {
-        if (G.E == 2) kb_march<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
-        else kb_march<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, G.pot_lds, s>>>(B->runs, G.map, sp);
+        if (G.E == 2) kb_march<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
+        else kb_march<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, 0, s>>>(B->runs, G.map, sp);
     }
 }
 
@@ -2398,10 +2409,10 @@ template <typename T, bool SPLIT>
 static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
     switch (G.E) {
-    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
-    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
-    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
-    default: kb_site_reduce<T, SPLIT><<<G.nrblocks, kBlock, G.pot_lds, s>>>(B->runs, G.rmap); break;
+    case 1: kb_row_reduce<T, SPLIT, 1><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    case 2: kb_row_reduce<T, SPLIT, 2><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    case 4: kb_row_reduce<T, SPLIT, 4><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
+    default: kb_site_reduce<T, SPLIT><<<G.nrblocks, kBlock, 0, s>>>(B->runs, G.rmap); break;
     }
 }
 
@@ -2414,10 +2425,10 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n
         if (G.pot) batch_march<T, SPLIT, kPotTab>(B, G, sp, s);
         else batch_march<T, SPLIT, kPotHill>(B, G, sp, s);
     } else switch (G.kE) {
-    case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
-    case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
-    case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
-    default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, G.pot_lds, s>>>(B->runs, G.map); break;
+    case 1: kb_row_kick<T, SPLIT, 1><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 2: kb_row_kick<T, SPLIT, 2><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    case 4: kb_row_kick<T, SPLIT, 4><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
+    default: kb_site_kick<T, SPLIT><<<G.nblocks, kBlock, 0, s>>>(B->runs, G.map); break;
     }
     if (!G.ntblocks) return;
     if (G.E == 0) kb_tensor<T><<<G.ntblocks, kBlock, 0, s>>>(B->runs, G.tmap, rs);
@@ -2442,8 +2453,8 @@ static void batch_group_kick(const BatchDev *B, const BatchGroup &G, long long n
 template <typename T, bool SPLIT>
 static void batch_group_resident(const BatchDev *B, const BatchGroup &G, int count, hipStream_t s)
 {
-    if (G.pot) kb_resident<T, SPLIT, kPotTab><<<G.nblocks, kResThreads, G.pot_lds, s>>>(B->runs, G.map, count);
-    else kb_resident<T, SPLIT, kPotHill><<<G.nblocks, kResThreads, G.pot_lds, s>>>(B->runs, G.map, count);
+    if (G.pot) kb_resident<T, SPLIT, kPotTab><<<G.nblocks, kResThreads, 0, s>>>(B->runs, G.map, count);
+    else kb_resident<T, SPLIT, kPotHill><<<G.nblocks, kResThreads, 0, s>>>(B->runs, G.map, count);
 }
 
 hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s);
