@@ -967,29 +967,27 @@ ie_status ie_spectrum(ie_run *r, int which, double norm1, double *p, int *counts
     ie_status st = pull_bg(r);
     if (st != IE_OK) return st;
     const int split = r->cfg.precision == IE_SPLIT;
-    double dc = 0.0;
+    const int nb = fft_numbins(r->N);
+    std::vector<double> sums(nb);
     if (which == IE_FIELD_DELTAN) {
-        IE_HIP(r, fft_forward_double(r->fft, r->F.dn, 0, r->stream));
+        IE_HIP(r, fft_spectrum_double(r->fft, r->F.dn, sums.data(), r->stream));
     } else if (which == IE_FIELD_F || which == IE_FIELD_FD) {
         const int par = r->bg_host->parity;
         if (r->in_deltaN) {
             const double *src = which == IE_FIELD_F ? (split ? r->dn_f : (const double *)r->F.f[par])
                                                     : (split ? r->dn_fd : (const double *)r->F.fd[par]);
-            IE_HIP(r, fft_forward_double(r->fft, src, 0, r->stream));
+            IE_HIP(r, fft_spectrum_double(r->fft, src, sums.data(), r->stream));
         } else if (split) {
             const float *src = (const float *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd[par]);
-            dc = (double)r->gs * (which == IE_FIELD_F ? r->bg_host->phibar : r->bg_host->pibar);
-            IE_HIP(r, fft_forward_float(r->fft, src, dc, 0, r->stream));
+            const double dc = (double)r->gs * (which == IE_FIELD_F ? r->bg_host->phibar : r->bg_host->pibar);
+            IE_HIP(r, fft_spectrum_float(r->fft, src, dc, sums.data(), r->stream));
         } else {
             const double *src = (const double *)(which == IE_FIELD_F ? r->F.f[par] : r->F.fd[par]);
-            IE_HIP(r, fft_forward_double(r->fft, src, 0, r->stream));
+            IE_HIP(r, fft_spectrum_double(r->fft, src, sums.data(), r->stream));
         }
     } else {
         return fail(r, IE_ERR_ARG, "bad field");
     }
-    const int nb = fft_numbins(r->N);
-    std::vector<double> sums(nb);
-    IE_HIP(r, fft_bin_scalar_dc(r->fft, 0, dc, sums.data(), r->stream));
     fft_counts(r->N, counts);
     const double pi = 2.0 * asin(1.0), dp = 2.0 * pi / r->cfg.L;
     for (int b = 0; b < nb; ++b) {

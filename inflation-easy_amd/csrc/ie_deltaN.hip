@@ -20,12 +20,13 @@
 #include <math.h>
 
 #include "ie_internal.h"
+#include "ie_numeric.h"
 
 namespace ie {
 
 namespace {
 
-constexpr int kMaxTable = 2048;   // table rows staged in LDS (3 x 2048 doubles = 48 KB)
+constexpr int kMaxTable = 2048;   // table rows staged in dynamic LDS (4 x 2048 doubles = 64 KB)
 
 struct DnParams {
     int mono;          // 1 monotonic, -1 antimonotonic, 0 generic
@@ -35,16 +36,56 @@ struct DnParams {
 
 #pragma clang fp contract(off)
 
+// The reference's expressions (src/potential.cpp:77-83), their division by B^2
+// through the run's reciprocal of B^2 (div_by: the same correctly rounded result).
 __device__ __forceinline__ double hill_V(const Pot &P, double v)
 {
-    return P.V0 * (1.0 - (1.0 - P.ns) / 4.0 * (v * v)) / P.B2;
+    return div_by(P.V0 * (1.0 - (1.0 - P.ns) / 4.0 * (v * v)), P.B2, P.invB2);
 }
 __device__ __forceinline__ double hill_dV(const Pot &P, double v)
 {
-    return -P.V0 * (1.0 - P.ns) / 2.0 * v / P.B2;
+    return div_by(-P.V0 * (1.0 - P.ns) / 2.0 * v, P.B2, P.invB2);
 }
 
-// Hinted bracket walk in an LDS table: returns l with x[l-1] >= v > x[l].
+// The table as a kernel reads it: x, V, V' and the reciprocals of the interval
+// widths x[l-1] - x[l] (rdx[l], or null: divide).
+struct DnTable {
+    const double *x, *v, *dv, *rdx;
+    int n;
+};
+
+// A table of up to kMaxTable rows staged in the workgroup's dynamic LDS (4 n
+// doubles, dn_table_lds); a longer one read from global memory.  Every thread of
+// the workgroup calls it (a uniform barrier when staging).
+__device__ __forceinline__ DnTable stage_table(const DnParams &D)
+{
+    extern __shared__ double dn_tab[];
+    const int n = D.P.numerical ? D.P.n : 0;
+    DnTable T{D.P.x, D.P.v, D.P.dv, nullptr, n};
+    if (n > 0 && n <= kMaxTable) {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            dn_tab[e] = D.P.x[e];
+            dn_tab[n + e] = D.P.v[e];
+            dn_tab[2 * n + e] = D.P.dv[e];
+            dn_tab[3 * n + e] = e > 0 ? 1.0 / (D.P.x[e - 1] - D.P.x[e]) : 0.0;
+        }
+        __syncthreads();
+        T = DnTable{dn_tab, dn_tab + n, dn_tab + 2 * n, dn_tab + 3 * n, n};
+    }
+    return T;
+}
+
+__host__ inline size_t dn_table_lds(const Pot &P)
+{
+    return P.numerical && P.n > 0 && P.n <= kMaxTable ? (size_t)4 * P.n * sizeof(double) : 0;
+}
+
+__device__ __forceinline__ DnTable global_table(const DnParams &D)
+{
+    return DnTable{D.P.x, D.P.v, D.P.dv, nullptr, D.P.numerical ? D.P.n : 0};
+}
+
+// Hinted bracket walk: returns l with x[l-1] >= v > x[l].
 __device__ __forceinline__ int walk(const double *x, int n, double v, int l)
 {
     if (l < 1) l = 1;
@@ -54,57 +95,62 @@ __device__ __forceinline__ int walk(const double *x, int n, double v, int l)
     return l;
 }
 
-__device__ __forceinline__ double interp(const double *x, const double *y, double v, int l)
+// interpolate_from_table (src/potential.cpp:52-68): y1 + ((v - x1)(y0 - y1)) / (x0 - x1),
+// the division through the interval's reciprocal when staged.
+__device__ __forceinline__ double interp(const DnTable &T, const double *y, double v, int l)
 {
-    const double x0 = x[l - 1], x1 = x[l], y0 = y[l - 1], y1 = y[l];
-    return y1 + (v - x1) * (y0 - y1) / (x0 - x1);
+    const double x0 = T.x[l - 1], x1 = T.x[l], y0 = y[l - 1], y1 = y[l];
+    const double num = (v - x1) * (y0 - y1), dx = x0 - x1;
+    return y1 + (T.rdx ? div_by(num, dx, T.rdx[l]) : num / dx);
 }
 
-__device__ __forceinline__ bool active(const DnParams &D, double v, const double *x, const double *tv, int n, int *hint)
+// V / B^2 and V' / B^2 of the table at v in interval l
+__device__ __forceinline__ double tab_V(const DnParams &D, const DnTable &T, double v, int l)
+{
+    return div_by(interp(T, T.v, v, l), D.P.B2, D.P.invB2);
+}
+__device__ __forceinline__ double tab_dV(const DnParams &D, const DnTable &T, double v, int l)
+{
+    return div_by(interp(T, T.dv, v, l), D.P.B2, D.P.invB2);
+}
+
+__device__ __forceinline__ bool active(const DnParams &D, const DnTable &T, double v, int *hint)
 {
     if (D.mono > 0) return fabs(v) > fabs(D.phiref);
     if (D.mono < 0) return fabs(v) < fabs(D.phiref);
     double V;
     if (D.P.numerical) {
-        *hint = walk(x, n, v, *hint);
-        V = interp(x, tv, v, *hint) / D.P.B2;
+        *hint = walk(T.x, T.n, v, *hint);
+        V = tab_V(D, T, v, *hint);
     } else {
         V = hill_V(D.P, v);
     }
     return V > D.Vref;
 }
 
-__device__ __forceinline__ bool in_table(const double *x, int n, double v) { return !(v > x[0] || v < x[n - 1]); }
+__device__ __forceinline__ bool in_table(const DnTable &T, double v) { return !(v > T.x[0] || v < T.x[T.n - 1]); }
 
 __global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f, double *__restrict__ fd,
                                                           double *__restrict__ dn, size_t gs, DnParams D,
                                                           double dN, long long nsteps, int *error)
 {
-    __shared__ double tx[kMaxTable], tv[kMaxTable], tdv[kMaxTable];
-    const int n = D.P.numerical ? D.P.n : 0;
-    const bool lds = n > 0 && n <= kMaxTable;
-    for (int e = threadIdx.x; lds && e < n; e += blockDim.x) {
-        tx[e] = D.P.x[e];
-        tv[e] = D.P.v[e];
-        tdv[e] = D.P.dv[e];
-    }
-    __syncthreads();
-    const double *x = lds ? tx : D.P.x, *yv = lds ? tv : D.P.v, *ydv = lds ? tdv : D.P.dv;
+    const DnTable T = stage_table(D);
+    const int n = T.n;
     int err = 0;
     for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
         double phi = f[id], pi = fd[id], N = dn[id];
         int hint = n / 2;
-        if (D.P.numerical && in_table(x, n, phi)) {
+        if (D.P.numerical && in_table(T, phi)) {
             int lo = 0, hi = n - 1;   // initial bracket by bisection
             while (hi - lo > 1) {
                 const int mid = lo + (hi - lo) / 2;
-                if (x[mid] >= phi) lo = mid;
+                if (T.x[mid] >= phi) lo = mid;
                 else hi = mid;
             }
             hint = hi;
         }
         int ghint = hint;
-        bool act = active(D, phi, x, yv, n, &ghint);
+        bool act = active(D, T, phi, &ghint);
         double ratio = 0.0;
         bool frozen = false;   // phi of a finished site no longer moves: its ratio is fixed
         for (long long s = 0; s < nsteps; ++s) {
@@ -112,12 +158,12 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f,
             // or not, with pot_ratio (src/potential.cpp:177-192)
             if (!frozen) {
                 if (D.P.numerical) {
-                    if (!in_table(x, n, phi)) {
+                    if (!in_table(T, phi)) {
                         err = 1;
                         break;
                     }
-                    hint = walk(x, n, phi, hint);
-                    ratio = interp(x, ydv, phi, hint) / interp(x, yv, phi, hint);
+                    hint = walk(T.x, n, phi, hint);
+                    ratio = interp(T, T.dv, phi, hint) / interp(T, T.v, phi, hint);
                 } else {
                     ratio = hill_dV(D.P, phi) / hill_V(D.P, phi);
                 }
@@ -128,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f,
             if (act) {
                 N += dN;
                 phi += dN * pi;
-                act = active(D, phi, x, yv, n, &ghint);
+                act = active(D, T, phi, &ghint);
             }
         }
         f[id] = phi;
@@ -141,20 +187,19 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_steps(double *__restrict__ f,
 // RHS of the deltaN system at one state (compute_deltaN_rhs,
 // src/evolution.cpp:1065-1113): (dphi, dpi, dN)/dN-time, with the activity test
 // at that state.
-__device__ __forceinline__ void dn_rhs(const DnParams &D, const double *x, const double *yv, const double *ydv, int n,
-                                       double phi, double pi, int *hint, int *ghint, double *kf, double *kfd,
-                                       double *kdn, int *err)
+__device__ __forceinline__ void dn_rhs(const DnParams &D, const DnTable &T, double phi, double pi, int *hint,
+                                       int *ghint, double *kf, double *kfd, double *kdn, int *err)
 {
     double pot, dpot;
     if (D.P.numerical) {
-        if (!in_table(x, n, phi)) {
+        if (!in_table(T, phi)) {
             *err = 1;
             pot = 1.0;
             dpot = 0.0;
         } else {
-            *hint = walk(x, n, phi, *hint);
-            pot = interp(x, yv, phi, *hint) / D.P.B2;
-            dpot = interp(x, ydv, phi, *hint) / D.P.B2;
+            *hint = walk(T.x, T.n, phi, *hint);
+            pot = tab_V(D, T, phi, *hint);
+            dpot = tab_dV(D, T, phi, *hint);
         }
     } else {
         pot = hill_V(D.P, phi);
@@ -183,16 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk4(double *__restrict__ f, d
                                                         double *__restrict__ dn, size_t gs, DnParams D, double h,
                                                         long long nsteps, int *error)
 {
-    __shared__ double tx[kMaxTable], tv[kMaxTable], tdv[kMaxTable];
-    const int n = D.P.numerical ? D.P.n : 0;
-    const bool lds = n > 0 && n <= kMaxTable;
-    for (int e = threadIdx.x; lds && e < n; e += blockDim.x) {
-        tx[e] = D.P.x[e];
-        tv[e] = D.P.v[e];
-        tdv[e] = D.P.dv[e];
-    }
-    __syncthreads();
-    const double *x = lds ? tx : D.P.x, *yv = lds ? tv : D.P.v, *ydv = lds ? tdv : D.P.dv;
+    const DnTable T = stage_table(D);
+    const int n = T.n;
     const double A[4][4] = {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1.0, 0}};
     const double B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     int err = 0;
@@ -201,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk4(double *__restrict__ f, d
         int hint = n > 0 ? n / 2 : 0, ghint = hint;
         for (long long s = 0; s < nsteps; ++s) {
             double kf[4], kfd[4], kdn[4];
-            dn_rhs(D, x, yv, ydv, n, phi, pi, &hint, &ghint, &kf[0], &kfd[0], &kdn[0], &err);
+            dn_rhs(D, T, phi, pi, &hint, &ghint, &kf[0], &kfd[0], &kdn[0], &err);
 #pragma unroll
             for (int st = 1; st < 4; ++st) {
                 double sf = 0.0, sfd = 0.0, sdn = 0.0;
@@ -214,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk4(double *__restrict__ f, d
                     sdn += c * kdn[q];
                 }
                 (void)sdn;
-                dn_rhs(D, x, yv, ydv, n, phi + h * sf, pi + h * sfd, &hint, &ghint, &kf[st], &kfd[st], &kdn[st],
+                dn_rhs(D, T, phi + h * sf, pi + h * sfd, &hint, &ghint, &kf[st], &kfd[st], &kdn[st],
                        &err);
             }
             double sf = 0.0, sfd = 0.0, sdn = 0.0;
@@ -245,17 +282,9 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk45_trial(
     double *__restrict__ f5, double *__restrict__ fd5, double *__restrict__ dn5, size_t gs, DnParams D, double h,
     double abs_tol, double rel_tol, double *__restrict__ partials, int *error)
 {
-    __shared__ double tx[kMaxTable], tv[kMaxTable], tdv[kMaxTable];
     __shared__ double smem[kWaves];
-    const int n = D.P.numerical ? D.P.n : 0;
-    const bool lds = n > 0 && n <= kMaxTable;
-    for (int e = threadIdx.x; lds && e < n; e += blockDim.x) {
-        tx[e] = D.P.x[e];
-        tv[e] = D.P.v[e];
-        tdv[e] = D.P.dv[e];
-    }
-    __syncthreads();
-    const double *x = lds ? tx : D.P.x, *yv = lds ? tv : D.P.v, *ydv = lds ? tdv : D.P.dv;
+    const DnTable T = stage_table(D);
+    const int n = T.n;
     const double A[7][7] = {{0, 0, 0, 0, 0, 0, 0},
                             {1.0 / 5.0, 0, 0, 0, 0, 0, 0},
                             {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0, 0},
@@ -274,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk45_trial(
         const double phi = f[id], pi = fd[id], N = dn[id];
         int hint = n > 0 ? n / 2 : 0, ghint = hint;
         double kf[7], kfd[7], kdn[7];
-        dn_rhs(D, x, yv, ydv, n, phi, pi, &hint, &ghint, &kf[0], &kfd[0], &kdn[0], &err);
+        dn_rhs(D, T, phi, pi, &hint, &ghint, &kf[0], &kfd[0], &kdn[0], &err);
 #pragma unroll
         for (int st = 1; st < 7; ++st) {
             double sf = 0.0, sfd = 0.0;
@@ -285,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk45_trial(
                 sf += c * kf[q];
                 sfd += c * kfd[q];
             }
-            dn_rhs(D, x, yv, ydv, n, phi + h * sf, pi + h * sfd, &hint, &ghint, &kf[st], &kfd[st], &kdn[st], &err);
+            dn_rhs(D, T, phi + h * sf, pi + h * sfd, &hint, &ghint, &kf[st], &kfd[st], &kdn[st], &err);
         }
         double y5f = phi, y5fd = pi, y5dn = N, ef = 0.0, efd = 0.0, edn = 0.0;
 #pragma unroll
@@ -319,11 +348,11 @@ __global__ __launch_bounds__(kBlock) void k_deltaN_rk45_trial(
 __global__ __launch_bounds__(kBlock) void k_fieldsN(double *__restrict__ f, const double *__restrict__ fd,
                                                      double *__restrict__ dn, size_t gs, DnParams D, double d)
 {
-    const int n = D.P.numerical ? D.P.n : 0;
+    const DnTable T = global_table(D);
     for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < gs; id += (size_t)gridDim.x * blockDim.x) {
-        int hint = n / 2;
+        int hint = T.n / 2;
         const double v = f[id];
-        if (active(D, v, D.P.x, D.P.v, n, &hint)) {
+        if (active(D, T, v, &hint)) {
             dn[id] += d;
             f[id] = v + d * fd[id];
         }
@@ -350,7 +379,8 @@ __global__ __launch_bounds__(kBlock) void k_initN(const double *__restrict__ f, 
                     if (P.x[mid] >= v) lo = mid;
                     else hi = mid;
                 }
-                V = interp(P.x, P.v, v, hi) / P.B2;
+                const DnTable T{P.x, P.v, P.dv, nullptr, P.n};
+                V = div_by(interp(T, P.v, v, hi), P.B2, P.invB2);
             }
         } else {
             V = hill_V(P, v);
@@ -396,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_phiref(const double *__restrict__ f,
                 if (D.P.x[mid] >= v) lo = mid;
                 else hi = mid;
             }
-            key = interp(D.P.x, D.P.v, v, hi) / D.P.B2;
+            key = tab_V(D, global_table(D), v, hi);
         } else key = hill_V(D.P, v);
         if (bi == ~0ull || better(D.mono, key, id, bk, bi)) {
             bk = key;
@@ -439,14 +469,135 @@ __global__ void k_phiref_final(const double *__restrict__ f, const double *__res
 }
 
 // saveN's mean (src/output.cpp:1527-1537): a float accumulator walked in lattice
-// order over sites with deltaN <= cutoff, then divided by N^3 in float.  The
-// serial order is part of the result: one wave adds in order (serial_walk).
-__global__ __launch_bounds__(64) void k_float_mean(const double *__restrict__ dn, size_t gs, double cutoff,
-                                                    float *out)
+// order over sites with deltaN <= cutoff, acc = (float)((double)acc + v), then
+// divided by N^3 in float.  The order of the roundings is part of the result,
+// and it is kept exactly without a serial walk:
+//   - while acc lies in one binade [B, 2B) it is k u with u = B 2^-23 and k an
+//     integer; a step with 0 <= v < B adds the integer n = rint(r / u), where
+//     r = fl64(B + v) - B is v on the double grid of that binade (the double
+//     rounding of acc + v, whose parity acc's multiple of u cannot change), as
+//     long as k + n < 2^24 and r / u is not a tie (its rounding would depend on
+//     k's parity);
+//   - so the steps of a binade are an integer prefix sum: one workgroup scans the
+//     sites a chunk at a time and stops at the first step that is not of this
+//     kind (it leaves the binade, is a tie, or has v < 0 or v >= B), which it
+//     takes as a float step, then goes on in the new binade;
+//   - while acc is 0, negative or not finite, excluded sites (v > cutoff or NaN)
+//     leave it alone and every included one is a float step.
+// The result is the serial walk's bit for bit.
+constexpr int kMeanThreads = 1024, kMeanPer = 8, kMeanChunk = kMeanThreads * kMeanPer;
+
+__device__ __forceinline__ float mean_step(float acc, double v, double cutoff)
 {
-    auto step = [cutoff](float acc, double v) { return v <= cutoff ? (float)((double)acc + v) : acc; };
-    const float acc = serial_walk(dn, gs, 0.f, step);
-    if (threadIdx.x == 0) *out = acc / (float)gs;
+    return v <= cutoff ? (float)((double)acc + v) : acc;
+}
+
+__global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__restrict__ dn, size_t gs, double cutoff,
+                                                              float *out)
+{
+    constexpr int NW = kMeanThreads / 64;
+    constexpr long long kTop = 1ll << 24;
+    __shared__ long long wtot[NW];
+    __shared__ unsigned long long s_first;
+    __shared__ long long s_kfirst;
+    __shared__ float s_acc;
+    __shared__ unsigned long long s_pos;
+    const unsigned tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        s_acc = 0.f;
+        s_pos = 0;
+    }
+    __syncthreads();
+    for (;;) {
+        const float acc = s_acc;
+        const size_t pos = s_pos;
+        if (pos >= gs) break;
+        // the phase: acc's binade (regular), or a hold (acc 0, negative, not finite)
+        const bool regular = acc >= 1.17549435e-38f && acc <= 3.40282347e38f;
+        int E = 0;
+        if (regular) (void)frexpf(acc, &E);
+        const double B = regular ? ldexp(1.0, E - 1) : 0.0, u = regular ? ldexp(1.0, E - 24) : 0.0;
+        long long k = regular ? (long long)((double)acc / u) : 0;
+        size_t c0 = pos;
+        for (;;) {
+            const size_t base = c0 + (size_t)tid * kMeanPer;
+            long long n[kMeanPer];
+            bool bad[kMeanPer];
+            long long tsum = 0;
+#pragma unroll
+            for (int q = 0; q < kMeanPer; ++q) {
+                const size_t id = base + q;
+                n[q] = 0;
+                bad[q] = id >= gs;
+                if (id < gs) {
+                    const double v = dn[id];
+                    if (v <= cutoff) {
+                        if (!regular || !(v >= 0.0) || !(v < B)) {
+                            bad[q] = true;
+                        } else {
+                            const double qv = ((B + v) - B) / u;
+                            const double fl = floor(qv);
+                            bad[q] = qv - fl == 0.5;
+                            n[q] = (long long)rint(qv);
+                        }
+                    }
+                }
+                tsum += n[q];
+            }
+            // exclusive prefix of the thread sums: in the wave, then over waves
+            long long incl = tsum;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const long long o = __shfl_up(incl, d);
+                if ((int)lane >= d) incl += o;
+            }
+            __syncthreads();   // every thread has read the last chunk's s_first and wtot
+            if (tid == 0) s_first = ~0ull;
+            if (lane == 63) wtot[wave] = incl;
+            __syncthreads();
+            long long before = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                before += w < (int)wave ? wtot[w] : 0;
+                total += wtot[w];
+            }
+            long long K = k + before + incl - tsum;
+            size_t mine = ~(size_t)0;
+            long long kmine = 0;
+#pragma unroll
+            for (int q = 0; q < kMeanPer; ++q) {
+                const bool stop = bad[q] || (regular && K + n[q] >= kTop);
+                if (stop && mine == ~(size_t)0) {
+                    mine = base + q;
+                    kmine = K;
+                }
+                K += n[q];
+            }
+            if (mine != ~(size_t)0) atomicMin(&s_first, (unsigned long long)mine);
+            __syncthreads();
+            const unsigned long long first = s_first;
+            if (first == ~0ull) {
+                k += total;
+                c0 += kMeanChunk;
+                continue;
+            }
+            if (mine == first) s_kfirst = kmine;
+            __syncthreads();
+            if (tid == 0) {
+                const float at = regular ? (float)((double)s_kfirst * u) : acc;
+                if (first < gs) {
+                    s_acc = mean_step(at, dn[first], cutoff);
+                    s_pos = first + 1;
+                } else {
+                    s_acc = at;
+                    s_pos = gs;
+                }
+            }
+            __syncthreads();
+            break;
+        }
+    }
+    if (tid == 0) *out = s_acc / (float)gs;
 }
 
 // zero = 0: deltaN -= Nmean (src/output.cpp:1539-1542); zero = 1: sites above
@@ -565,7 +716,8 @@ hipError_t dn_steps(size_t gs, double *f, double *fd, double *dn, const Pot &P, 
                     double dN, long long nsteps, int *error, hipStream_t s)
 {
     size_t b = (gs + kBlock - 1) / kBlock;
-    k_deltaN_steps<<<(int)b, kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), dN, nsteps, error);
+    k_deltaN_steps<<<(int)b, kBlock, dn_table_lds(P), s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), dN, nsteps,
+                                                           error);
     return hipGetLastError();
 }
 
@@ -573,7 +725,7 @@ hipError_t dn_rk4_steps(size_t gs, double *f, double *fd, double *dn, const Pot 
                         double Vref, double h, long long nsteps, int *error, hipStream_t s)
 {
     size_t b = (gs + kBlock - 1) / kBlock;
-    k_deltaN_rk4<<<(int)b, kBlock, 0, s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), h, nsteps, error);
+    k_deltaN_rk4<<<(int)b, kBlock, dn_table_lds(P), s>>>(f, fd, dn, gs, params(P, mono, phiref, Vref), h, nsteps, error);
     return hipGetLastError();
 }
 
@@ -582,7 +734,7 @@ hipError_t dn_rk45_trial(size_t gs, const double *f, const double *fd, const dou
                          double rel_tol, double *work, double *out, int *error, hipStream_t s)
 {
     const int nb = grid_for(gs) > 2048 ? 2048 : grid_for(gs);
-    k_deltaN_rk45_trial<<<nb, kBlock, 0, s>>>(f, fd, dn, f5, fd5, dn5, gs, params(P, mono, phiref, Vref), h, abs_tol,
+    k_deltaN_rk45_trial<<<nb, kBlock, dn_table_lds(P), s>>>(f, fd, dn, f5, fd5, dn5, gs, params(P, mono, phiref, Vref), h, abs_tol,
                                                rel_tol, work, error);
     k_sum_final<<<1, 64, 0, s>>>(work, nb, out);
     return hipGetLastError();
@@ -600,7 +752,7 @@ hipError_t dn_phiref(size_t gs, const double *f, const Pot &P, int mono, double 
 
 hipError_t dn_float_mean(const double *dn, size_t gs, double cutoff, float *out, hipStream_t s)
 {
-    k_float_mean<<<1, 64, 0, s>>>(dn, gs, cutoff, out);
+    k_float_mean<<<1, kMeanThreads, 0, s>>>(dn, gs, cutoff, out);
     return hipGetLastError();
 }
 

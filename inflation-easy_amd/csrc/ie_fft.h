@@ -33,6 +33,11 @@ int fft_gw_numbins(int N);
 hipError_t fft_forward_double(FftPlan *p, const double *in, int slot, hipStream_t s);
 hipError_t fft_forward_float(FftPlan *p, const float *in, double dc_add, int slot, hipStream_t s);
 hipError_t fft_bin_scalar(FftPlan *p, int slot, double *sums_host, hipStream_t s);
+// Forward transform and shell sums in one: the last pass bins its modes instead
+// of storing them (for 8 <= N <= 256; larger N as forward + bin).  Slot 0 is
+// used as workspace; sums_host receives fft_numbins(N) values.
+hipError_t fft_spectrum_double(FftPlan *p, const double *in, double *sums_host, hipStream_t s);
+hipError_t fft_spectrum_float(FftPlan *p, const float *in, double dc_add, double *sums_host, hipStream_t s);
 // as fft_bin_scalar with dc_add added to the k = 0 amplitude first (split storage:
 // N^3 phibar is the transform of the homogeneous mode)
 hipError_t fft_bin_scalar_dc(FftPlan *p, int slot, double dc_add, double *sums_host, hipStream_t s);

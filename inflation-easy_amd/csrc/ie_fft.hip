@@ -19,6 +19,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "ie_fft.h"
@@ -124,6 +126,19 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_c2r(const double2 *__restric
     }
 }
 
+// The shell of a mode, lround(sqrt(r2)) for the integer r2 = |n|^2, without the
+// FP64 square root: a float estimate, then the exact integer test
+// (2b - 1)^2 <= 4 r2 < (2b + 1)^2, the lower bound for b > 0 (r2 is never a
+// half-integer square, so there is no tie).  The same integer as the reference's shell index (r2 < 2^22).
+__device__ __forceinline__ int shell_of(int r2)
+{
+    int b = (int)(__fsqrt_rn((float)r2) + 0.5f);
+    const int q = 4 * r2;
+    b += q >= 4 * b * b + 4 * b + 1 ? 1 : 0;
+    b -= b > 0 && q < 4 * b * b - 4 * b + 1 ? 1 : 0;
+    return b;
+}
+
 __device__ __forceinline__ void mode_coords(size_t m, int N, int K, int *px, int *py, int *kz)
 {
     const size_t row = m / K;
@@ -146,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_bin_scalar(const double2 *__restri
     for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < modes; m += (size_t)gridDim.x * blockDim.x) {
         int px, py, kz;
         mode_coords(m, N, K, &px, &py, &kz);
-        const int bin = (int)lround(sqrt((double)px * px + (double)py * py + (double)kz * kz));
+        const int bin = shell_of(px * px + py * py + kz * kz);
         if (bin >= nb) continue;
         double2 F = H[m];
         if (m == 0) F.x += dc_add;
@@ -172,19 +187,21 @@ struct SixSpectra {
 __global__ __launch_bounds__(kThreads) void k_bin_tt(SixSpectra S, int N, int nb, int imax, double kpre,
                                                       double to_phys, double *__restrict__ partials)
 {
-    extern __shared__ double wbins[];
+    extern __shared__ double wbins[];   // [waves][nb], then kpre sin(pi p / N) for p = -N/2 .. N/2
     const int K = N / 2 + 1, wave = threadIdx.x >> 6;
+    double *ksin = wbins + (kThreads / 64) * nb + N / 2;
+    const double pin = 3.14159265358979323846 / N;
     for (int e = threadIdx.x; e < (kThreads / 64) * nb; e += blockDim.x) wbins[e] = 0.0;
+    for (int p = (int)threadIdx.x - N / 2; p <= N / 2; p += blockDim.x) ksin[p] = kpre * sin(pin * p);
     __syncthreads();
     const size_t modes = (size_t)N * N * K;
-    const double pin = 3.14159265358979323846 / N;
     for (size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x; m < modes; m += (size_t)gridDim.x * blockDim.x) {
         int px, py, kz;
         mode_coords(m, N, K, &px, &py, &kz);
         if (px == 0 && py == 0 && kz == 0) continue;
-        const int bin = (int)lround(sqrt((double)px * px + (double)py * py + (double)kz * kz));
+        const int bin = shell_of(px * px + py * py + kz * kz);
         if (bin >= imax) continue;
-        double k3[3] = {kpre * sin(pin * px), kpre * sin(pin * py), kpre * sin(pin * kz)};
+        double k3[3] = {ksin[px], ksin[py], ksin[kz]};
         const double inv = 1.0 / sqrt(k3[0] * k3[0] + k3[1] * k3[1] + k3[2] * k3[2]);
         double P[3][3];
 #pragma unroll
@@ -257,14 +274,237 @@ __global__ __launch_bounds__(kThreads) void k_kill_modes(double2 *__restrict__ H
     }
 }
 
-// Fixed-order sum of the per-workgroup shell partials.
-__global__ void k_sum_bins(const double *__restrict__ partials, int nblocks, int nb, double *__restrict__ out)
+
+// ------------------------------------------------ register-resident line FFT
+//
+// A line of N <= 256 points split as N = N1 x N2 (N2 = N1 or 2 N1).  Thread n1
+// of the line holds x[n1 + N1 n2], n2 < N2, in registers:
+//   1. the N2-point transform over n2 in registers;
+//   2. the twiddles w_N^(n1 k2);
+//   3. one exchange through LDS, then the N1-point transforms over n1 in
+//      registers (N2 / N1 of them per thread): X[k2 + N2 k1].
+// Two register passes and one barrier per line, against log2 N LDS stages with a
+// barrier each.  Sub-transform twiddles sit at compile-time offsets of the
+// workgroup's LDS copy of the exact table (every lane reads the same entry).
+template <int N>
+struct LineShape;
+template <> struct LineShape<8> { static constexpr int N1 = 2; };
+template <> struct LineShape<16> { static constexpr int N1 = 4; };
+template <> struct LineShape<32> { static constexpr int N1 = 4; };
+template <> struct LineShape<64> { static constexpr int N1 = 8; };
+template <> struct LineShape<128> { static constexpr int N1 = 8; };
+template <> struct LineShape<256> { static constexpr int N1 = 16; };
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+constexpr int bitrev_c(int x, int bits)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    double s = 0.0;
-    for (int q = 0; q < nblocks; ++q) s += partials[(size_t)q * nb + b];
-    out[b] = s;
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((x >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+// In-register radix-2 DIT transform of R points, natural order in and out,
+// kernel e^{SIGN 2 pi i k n / R}; tw holds e^{+2 pi i m / N}, tstride = N / R.
+template <int R, int SIGN>
+__device__ __forceinline__ void reg_fft(double2 (&a)[R], const double2 *tw, int tstride)
+{
+    constexpr int LR = ilog2c(R);
+    double2 b[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) b[bitrev_c(i, LR)] = a[i];
+#pragma unroll
+    for (int ls = 1; ls <= LR; ++ls) {
+        const int span = 1 << ls, half = span >> 1;
+#pragma unroll
+        for (int pos = 0; pos < half; ++pos) {
+            double2 w = make_double2(1.0, 0.0);
+            if (pos > 0) {
+                w = tw[pos * (R / span) * tstride];
+                if (SIGN < 0) w.y = -w.y;
+            }
+#pragma unroll
+            for (int g = 0; g < R; g += span) {
+                const double2 u = b[g + pos], v = b[g + pos + half];
+                const double2 t = pos == 0 ? v : cmul(w, v);
+                b[g + pos] = make_double2(u.x + t.x, u.y + t.y);
+                b[g + pos + half] = make_double2(u.x - t.x, u.y - t.y);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) a[i] = b[i];
+}
+
+// Exchange area of one line: N2 rows of N1 + 1 points (the pad keeps the reads
+// of step 3, a row per thread, off a common bank).
+template <int N>
+constexpr int kExRow = LineShape<N>::N1 + 1;
+
+// Four-step transform of one line held by its N1 threads: in a[n2] = x[n1 + N1 n2]
+// (thread n1), out a[u N1 + k1] = X[k2 + N2 k1] with k2 = n1 + N1 u.  Every
+// thread of the workgroup calls it (one barrier inside); ex is the line's
+// exchange area, tw the workgroup's twiddle table (written before a barrier).
+template <int N, int SIGN>
+__device__ __forceinline__ void line_fft(double2 (&a)[N / LineShape<N>::N1], int n1, double2 *ex,
+                                         const double2 *tw)
+{
+    constexpr int N1 = LineShape<N>::N1, N2 = N / N1, Q = N2 / N1;
+    reg_fft<N2, SIGN>(a, tw, N / N2);
+#pragma unroll
+    for (int k2 = 1; k2 < N2; ++k2) {
+        double2 w = tw[n1 * k2];
+        if (SIGN < 0) w.y = -w.y;
+        a[k2] = cmul(w, a[k2]);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) ex[k2 * kExRow<N> + n1] = a[k2];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < Q; ++u) {
+        double2 c[N1];
+#pragma unroll
+        for (int m = 0; m < N1; ++m) c[m] = ex[(n1 + N1 * u) * kExRow<N> + m];
+        reg_fft<N1, SIGN>(c, tw, N / N1);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) a[u * N1 + k1] = c[k1];
+    }
+}
+
+constexpr int kRegThreads = 256;   // z passes: 256 / N1 rows per workgroup
+constexpr int kColLines = 16;      // column passes: 16 consecutive kz per workgroup
+
+// z pass, real to half-complex.
+template <typename T, int N>
+__global__ __launch_bounds__(kRegThreads) void k_z_r2c_reg(const T *__restrict__ in, double2 *__restrict__ out,
+                                                            const double2 *__restrict__ twg)
+{
+    constexpr int N1 = LineShape<N>::N1, N2 = N / N1, L = kRegThreads / N1, K = N / 2 + 1;
+    __shared__ double2 tw[N];
+    __shared__ double2 ex[L][N2 * kExRow<N>];
+    for (int e = threadIdx.x; e < N; e += kRegThreads) tw[e] = twg[e];
+    const int l = threadIdx.x / N1, n1 = threadIdx.x % N1;
+    const size_t row = (size_t)blockIdx.x * L + l;
+    double2 a[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) a[n2] = make_double2((double)in[row * N + n1 + N1 * n2], 0.0);
+    __syncthreads();
+    line_fft<N, +1>(a, n1, ex[l], tw);
+#pragma unroll
+    for (int u = 0; u < N2 / N1; ++u)
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+            const int k = n1 + N1 * u + N2 * k1;
+            if (k < K) out[row * K + k] = a[u * N1 + k1];
+        }
+}
+
+// z pass, half-complex to real with Hermitian extension; out = scale Re + add.
+template <int N>
+__global__ __launch_bounds__(kRegThreads) void k_z_c2r_reg(const double2 *__restrict__ H, double *__restrict__ out,
+                                                            const double2 *__restrict__ twg, double scale,
+                                                            double add)
+{
+    constexpr int N1 = LineShape<N>::N1, N2 = N / N1, L = kRegThreads / N1, K = N / 2 + 1;
+    __shared__ double2 tw[N];
+    __shared__ double2 ex[L][N2 * kExRow<N>];
+    for (int e = threadIdx.x; e < N; e += kRegThreads) tw[e] = twg[e];
+    const int l = threadIdx.x / N1, n1 = threadIdx.x % N1;
+    const size_t row = (size_t)blockIdx.x * L + l;
+    double2 a[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+        const int n = n1 + N1 * n2;
+        double2 v = H[row * K + (n < K ? n : N - n)];
+        if (n >= K) v.y = -v.y;
+        a[n2] = v;
+    }
+    __syncthreads();
+    line_fft<N, -1>(a, n1, ex[l], tw);
+#pragma unroll
+    for (int u = 0; u < N2 / N1; ++u)
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) out[row * N + n1 + N1 * u + N2 * k1] = a[u * N1 + k1].x * scale + add;
+}
+
+// y (DIM = 1) or x (DIM = 0) pass over the half spectrum, in place: a workgroup
+// holds 16 consecutive kz columns of one outer index, thread (l, n1) with l the
+// fastest index so that a load instruction reads 16 consecutive modes per row.
+// BIN (x pass of a spectrum): instead of storing, the shell sums of w |F|^2 with
+// the reference weights (as k_bin_scalar), per wave in LDS and per workgroup
+// into partials, dc_add added to the k = 0 amplitude first.
+template <int N, int SIGN, int DIM, bool BIN>
+__global__ __launch_bounds__(kColLines * LineShape<N>::N1) void k_cols_reg(double2 *__restrict__ H,
+                                                                            const double2 *__restrict__ twg,
+                                                                            int nb, double dc_add,
+                                                                            double *__restrict__ partials)
+{
+    constexpr int N1 = LineShape<N>::N1, N2 = N / N1, K = N / 2 + 1, NT = kColLines * N1;
+    constexpr int NW = NT >= 64 ? NT / 64 : 1;
+    __shared__ double2 tw[N];
+    __shared__ double2 ex[kColLines][N2 * kExRow<N>];
+    extern __shared__ double wbins[];   // BIN: [NW][nb]
+    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
+    if (BIN)
+        for (int e = threadIdx.x; e < NW * nb; e += NT) wbins[e] = 0.0;
+    const int l = threadIdx.x % kColLines, n1 = threadIdx.x / kColLines;
+    const int o = blockIdx.y, kz = blockIdx.x * kColLines + l;
+    const size_t stride = DIM == 1 ? (size_t)K : (size_t)N * K;
+    const size_t base = (DIM == 1 ? (size_t)o * N * K : (size_t)o * K) + (size_t)(kz < K ? kz : 0);
+    const bool live = kz < K;
+    double2 a[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2)
+        a[n2] = live ? H[base + (size_t)(n1 + N1 * n2) * stride] : make_double2(0.0, 0.0);
+    __syncthreads();
+    line_fft<N, SIGN>(a, n1, ex[l], tw);
+    if constexpr (!BIN) {
+        if (live)
+#pragma unroll
+            for (int u = 0; u < N2 / N1; ++u)
+#pragma unroll
+                for (int k1 = 0; k1 < N1; ++k1)
+                    H[base + (size_t)(n1 + N1 * u + N2 * k1) * stride] = a[u * N1 + k1];
+    } else {
+        const int wave = NT >= 64 ? (int)(threadIdx.x >> 6) : 0;
+        if (live) {
+            const int py = o <= N / 2 ? o : o - N;
+            const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+#pragma unroll
+            for (int u = 0; u < N2 / N1; ++u)
+#pragma unroll
+                for (int k1 = 0; k1 < N1; ++k1) {
+                    const int i = n1 + N1 * u + N2 * k1;
+                    const int px = i <= N / 2 ? i : i - N;
+                    const int bin = shell_of(px * px + py * py + kz * kz);
+                    double2 F = a[u * N1 + k1];
+                    if (i == 0 && o == 0 && kz == 0) F.x += dc_add;
+                    if (bin < nb) atomicAdd(&wbins[wave * nb + bin], w * (F.x * F.x + F.y * F.y));
+                }
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < nb; b += NT) {
+            double t = 0.0;
+            for (int q = 0; q < NW; ++q) t += wbins[q * nb + b];
+            partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * nb + b] = t;
+        }
+    }
+}
+
+// Fixed-order sum of per-workgroup shell partials, one workgroup per shell.
+__global__ __launch_bounds__(256) void k_sum_bins_tree(const double *__restrict__ partials, int nblocks, int nb,
+                                                        double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int b = blockIdx.x;
+    double t = 0.0;
+    for (int q = threadIdx.x; q < nblocks; q += 256) t += partials[(size_t)q * nb + b];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = red[0];
 }
 
 int ilog2(int n)
@@ -296,13 +536,49 @@ static int bin_blocks_for(size_t modes)
     return (int)(b < 1 ? 1 : b);
 }
 
+// Lines of the register-resident transforms: 8 <= N <= 256.
+static bool reg_path(int N) { return N >= 8 && N <= 256; }
+static int col_groups(int N) { return (N / 2 + 1 + kColLines - 1) / kColLines; }
+
+// Rows of shell partials: the binning kernels' workgroups, or the x pass's of a
+// fused spectrum, whichever is more.
+static int partial_rows(int N)
+{
+    const size_t modes = (size_t)N * N * (N / 2 + 1);
+    return std::max(bin_blocks_for(modes), reg_path(N) ? N * col_groups(N) : 0);
+}
+
 size_t fft_workspace_bytes(int N)
 {
     const size_t K = (size_t)N / 2 + 1;
     const size_t modes = (size_t)N * N * K;
     const int nb = fft_gw_numbins(N);
-    return modes * sizeof(double2) + (size_t)N * sizeof(double2) +
-           (size_t)bin_blocks_for(modes) * nb * 8 + (size_t)nb * 8;
+    return modes * sizeof(double2) + (size_t)N * sizeof(double2) + (size_t)partial_rows(N) * nb * 8 +
+           (size_t)nb * 8;
+}
+
+// Calls f with std::integral_constant<int, N> for a register-path N.
+template <typename F>
+static hipError_t with_line(int N, F &&f)
+{
+    switch (N) {
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 16: return f(std::integral_constant<int, 16>{});
+    case 32: return f(std::integral_constant<int, 32>{});
+    case 64: return f(std::integral_constant<int, 64>{});
+    case 128: return f(std::integral_constant<int, 128>{});
+    case 256: return f(std::integral_constant<int, 256>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int N, int SIGN, int DIM, bool BIN>
+static hipError_t launch_cols(double2 *H, const double2 *tw, int nb, double dc_add, double *partials, hipStream_t s)
+{
+    constexpr int N1 = LineShape<N>::N1, NT = kColLines * N1, NW = NT >= 64 ? NT / 64 : 1;
+    const dim3 grid(col_groups(N), N);
+    k_cols_reg<N, SIGN, DIM, BIN><<<grid, NT, BIN ? (size_t)NW * nb * 8 : 0, s>>>(H, tw, nb, dc_add, partials);
+    return hipGetLastError();
 }
 
 void fft_twiddles(int N, double2 *tw)
@@ -319,7 +595,7 @@ size_t fft_view_bytes(int N)
 {
     const size_t modes = (size_t)N * N * (N / 2 + 1);
     const int nb = fft_gw_numbins(N);
-    return align256(modes * sizeof(double2)) + align256((size_t)bin_blocks_for(modes) * nb * 8) +
+    return align256(modes * sizeof(double2)) + align256((size_t)partial_rows(N) * nb * 8) +
            align256((size_t)nb * 8);
 }
 
@@ -334,7 +610,7 @@ FftPlan *fft_create(int N)
     const int nb = fft_gw_numbins(N);
     bool ok = hipMalloc(&p->H[0], p->modes * sizeof(double2)) == hipSuccess &&
               hipMalloc(&p->tw, (size_t)N * sizeof(double2)) == hipSuccess &&
-              hipMalloc(&p->partials, (size_t)p->bin_blocks * nb * 8) == hipSuccess &&
+              hipMalloc(&p->partials, (size_t)partial_rows(N) * nb * 8) == hipSuccess &&
               hipMalloc(&p->sums, (size_t)nb * 8) == hipSuccess;
     if (!ok) {
         fft_destroy(p);
@@ -379,6 +655,15 @@ static int rows_per_block(int N)
 static hipError_t columns(FftPlan *p, double2 *H, int sign, hipStream_t s)
 {
     const int N = p->N;
+    if (reg_path(N))
+        return with_line(N, [&](auto n) {
+            constexpr int NN = decltype(n)::value;
+            hipError_t e = sign > 0 ? launch_cols<NN, +1, 1, false>(H, p->tw, 0, 0.0, nullptr, s)
+                                    : launch_cols<NN, -1, 1, false>(H, p->tw, 0, 0.0, nullptr, s);
+            if (e != hipSuccess) return e;
+            return sign > 0 ? launch_cols<NN, +1, 0, false>(H, p->tw, 0, 0.0, nullptr, s)
+                            : launch_cols<NN, -1, 0, false>(H, p->tw, 0, 0.0, nullptr, s);
+        });
     int L = kLdsComplex / N;
     if (L > 8) L = 8;
     if (L < 1) L = 1;
@@ -389,17 +674,82 @@ static hipError_t columns(FftPlan *p, double2 *H, int sign, hipStream_t s)
     return hipGetLastError();
 }
 
+// z pass of a forward transform into H.
+template <typename T>
+static hipError_t z_forward(FftPlan *p, const T *in, double2 *H, hipStream_t s)
+{
+    const int N = p->N;
+    if (reg_path(N))
+        return with_line(N, [&](auto n) {
+            constexpr int NN = decltype(n)::value, L = kRegThreads / LineShape<NN>::N1;
+            k_z_r2c_reg<T, NN><<<NN * NN / L, kRegThreads, 0, s>>>(in, H, p->tw);
+            return hipGetLastError();
+        });
+    const int L = rows_per_block(N);
+    k_fft_z_r2c<T><<<N * N / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(in, H, N, p->logN, L, p->tw);
+    return hipGetLastError();
+}
+
+// z pass of an inverse transform: out = scale Re + add.
+static hipError_t z_inverse(FftPlan *p, const double2 *H, double *out, double scale, double add, hipStream_t s)
+{
+    const int N = p->N;
+    if (reg_path(N))
+        return with_line(N, [&](auto n) {
+            constexpr int NN = decltype(n)::value, L = kRegThreads / LineShape<NN>::N1;
+            k_z_c2r_reg<NN><<<NN * NN / L, kRegThreads, 0, s>>>(H, out, p->tw, scale, add);
+            return hipGetLastError();
+        });
+    const int L = rows_per_block(N);
+    k_fft_z_c2r<<<N * N / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(H, out, N, p->logN, L, p->tw, scale,
+                                                                             add);
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t forward(FftPlan *p, const T *in, int slot, hipStream_t s)
 {
     hipError_t e = ensure_slot(p, slot);
     if (e != hipSuccess) return e;
-    const int N = p->N, L = rows_per_block(N);
-    const int rows = N * N;
-    k_fft_z_r2c<T><<<rows / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(in, p->H[slot], N, p->logN, L,
-                                                                                p->tw);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = z_forward<T>(p, in, p->H[slot], s)) != hipSuccess) return e;
     return columns(p, p->H[slot], +1, s);
+}
+
+// Forward transform and shell sums of |F|^2 (dc_add on the k = 0 amplitude),
+// the x pass binning its modes instead of storing them (slot 0 holds the
+// partially transformed field afterwards).
+template <typename T>
+static hipError_t spectrum(FftPlan *p, const T *in, double dc_add, double *sums_host, hipStream_t s)
+{
+    const int N = p->N, nb = fft_numbins(N);
+    hipError_t e = ensure_slot(p, 0);
+    if (e != hipSuccess) return e;
+    if (!reg_path(N)) {
+        if ((e = forward<T>(p, in, 0, s)) != hipSuccess) return e;
+        return fft_bin_scalar_dc(p, 0, dc_add, sums_host, s);
+    }
+    if ((e = z_forward<T>(p, in, p->H[0], s)) != hipSuccess) return e;
+    e = with_line(N, [&](auto n) {
+        constexpr int NN = decltype(n)::value;
+        hipError_t q = launch_cols<NN, +1, 1, false>(p->H[0], p->tw, 0, 0.0, nullptr, s);
+        if (q != hipSuccess) return q;
+        return launch_cols<NN, +1, 0, true>(p->H[0], p->tw, nb, dc_add, p->partials, s);
+    });
+    if (e != hipSuccess) return e;
+    k_sum_bins_tree<<<nb, 256, 0, s>>>(p->partials, N * col_groups(N), nb, p->sums);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(sums_host, p->sums, (size_t)nb * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t fft_spectrum_double(FftPlan *p, const double *in, double *sums_host, hipStream_t s)
+{
+    return spectrum<double>(p, in, 0.0, sums_host, s);
+}
+
+hipError_t fft_spectrum_float(FftPlan *p, const float *in, double dc_add, double *sums_host, hipStream_t s)
+{
+    return spectrum<float>(p, in, dc_add, sums_host, s);
 }
 
 hipError_t fft_forward_double(FftPlan *p, const double *in, int slot, hipStream_t s)
@@ -415,7 +765,7 @@ hipError_t fft_forward_float(FftPlan *p, const float *in, double dc_add, int slo
 
 static hipError_t finish_bins(FftPlan *p, int nb, double *sums_host, hipStream_t s)
 {
-    k_sum_bins<<<(nb + 255) / 256, 256, 0, s>>>(p->partials, p->bin_blocks, nb, p->sums);
+    k_sum_bins_tree<<<nb, 256, 0, s>>>(p->partials, p->bin_blocks, nb, p->sums);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if ((e = hipMemcpyAsync(sums_host, p->sums, (size_t)nb * 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
@@ -450,12 +800,10 @@ hipError_t fft_spectrum_in(int N, void *work, const double2 *tw, const double *i
     p.H[0] = (double2 *)w;
     w += align256(p.modes * sizeof(double2));
     p.partials = (double *)w;
-    w += align256((size_t)p.bin_blocks * fft_gw_numbins(N) * 8);
+    w += align256((size_t)partial_rows(N) * fft_gw_numbins(N) * 8);
     p.sums = (double *)w;
     p.tw = (double2 *)tw;
-    hipError_t e = forward<double>(&p, in, 0, s);
-    if (e != hipSuccess) return e;
-    e = fft_bin_scalar_dc(&p, 0, 0.0, sums_host, s);
+    hipError_t e = spectrum<double>(&p, in, 0.0, sums_host, s);
     p.H[0] = nullptr;   // caller memory: nothing for the destructor-free plan to own
     return e;
 }
@@ -469,8 +817,8 @@ hipError_t fft_bin_tt(FftPlan *p, double dx, double to_phys, double *sums_host, 
         if (!p->H[c]) return hipErrorInvalidValue;
         S.h[c] = p->H[c];
     }
-    k_bin_tt<<<p->bin_blocks, kThreads, (size_t)(kThreads / 64) * nb * 8, s>>>(S, p->N, nb, imax, 2.0 / dx,
-                                                                               to_phys, p->partials);
+    k_bin_tt<<<p->bin_blocks, kThreads, ((size_t)(kThreads / 64) * nb + p->N + 1) * 8, s>>>(S, p->N, nb, imax, 2.0 / dx,
+                                                                                           to_phys, p->partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return finish_bins(p, nb, sums_host, s);
@@ -491,11 +839,7 @@ hipError_t fft_inverse_ffteasy(FftPlan *p, const double *fk, const double *fny, 
     hipError_t e = hipMemcpyAsync(p->H[0], h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     if ((e = columns(p, p->H[0], -1, s)) != hipSuccess) return e;
-    const int L = rows_per_block(N);
-    const double scale = 1.0 / ((double)N * N * N);
-    k_fft_z_c2r<<<N * N / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(p->H[0], out, N, p->logN, L, p->tw,
-                                                                             scale, add);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = z_inverse(p, p->H[0], out, 1.0 / ((double)N * N * N), add, s)) != hipSuccess) return e;
     return hipStreamSynchronize(s);
 }
 
@@ -520,11 +864,7 @@ hipError_t fft_inverse_slot(FftPlan *p, int slot, double add, double *out, hipSt
     const int N = p->N;
     hipError_t e = columns(p, p->H[slot], -1, s);
     if (e != hipSuccess) return e;
-    const int L = rows_per_block(N);
-    const double scale = 1.0 / ((double)N * N * N);
-    k_fft_z_c2r<<<N * N / L, kThreads, (size_t)L * N * sizeof(double2), s>>>(p->H[slot], out, N, p->logN, L, p->tw,
-                                                                             scale, add);
-    return hipGetLastError();
+    return z_inverse(p, p->H[slot], out, 1.0 / ((double)N * N * N), add, s);
 }
 
 void fft_counts(int N, int *counts)

@@ -28,10 +28,12 @@
 
 #include <vector>
 
+#ifndef IE_HD
 #if defined(__HIPCC__)
 #define IE_HD __host__ __device__
 #else
 #define IE_HD
+#endif
 #endif
 
 namespace ie {

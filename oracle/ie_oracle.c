@@ -2138,6 +2138,16 @@ static int program_inflation(ieo_state *st, ieo_out *o)
     return 0;
 }
 
+/* saveN's mean (src/output.cpp:1527-1537): a float accumulator over the sites
+ * with deltaN <= cutoff in lattice order, float += double, then / gridsize. */
+float ieo_float_mean(const double *dn, size_t gs, double cutoff)
+{
+    float Nmean = 0.;
+    for (size_t id = 0; id < gs; ++id)
+        if (dn[id] <= cutoff) Nmean += dn[id];
+    return Nmean / (int)gs;
+}
+
 int ieo_deltaN_zeta_map(ieo_state *st, float *nmean_out)
 {
     const ieo_config *c = &st->cfg;
@@ -2154,11 +2164,8 @@ int ieo_deltaN_zeta_map(ieo_state *st, float *nmean_out)
         ++n;
     }
     ieo_evolve_fieldsN(st, -.5 * c->dN);
-    float Nmean = 0.;
     const double cutoff = c->Nend - c->dN;
-    for (size_t id = 0; id < st->gs; ++id)
-        if (st->dn[id] <= cutoff) Nmean += st->dn[id];
-    Nmean = Nmean / (int)st->gs;
+    const float Nmean = ieo_float_mean(st->dn, st->gs, cutoff);
     for (size_t id = 0; id < st->gs; ++id) st->dn[id] -= Nmean;
     for (size_t id = 0; id < st->gs; ++id)
         if (st->dn[id] > (cutoff - Nmean)) st->dn[id] = 0;
@@ -2177,11 +2184,8 @@ static void program_saveN(ieo_state *st, ieo_out *o)
     const int N = st->N;
     const int staggered = c->deltaN_integrator == IEO_LEAPFROG;
     if (staggered && st->t > 0.) ieo_evolve_fieldsN(st, -.5 * c->dN);
-    float Nmean = 0.;
     const double cutoff = c->Nend - (staggered ? c->dN : 0.0);
-    for (size_t id = 0; id < st->gs; ++id)
-        if (st->dn[id] <= cutoff) Nmean += st->dn[id];
-    Nmean = Nmean / (int)st->gs;
+    const float Nmean = ieo_float_mean(st->dn, st->gs, cutoff);
     for (size_t id = 0; id < st->gs; ++id) st->dn[id] -= Nmean;
     {
         FILE *ht = out_open(o, &o->histtimesN, "histogramtimesN");

@@ -164,6 +164,8 @@ double ieo_get_phiref(ieo_state *st);
  * st->dn holds the map spectraN transforms.  Returns the deltaN steps taken,
  * *nmean the float mean. */
 int ieo_deltaN_zeta_map(ieo_state *st, float *nmean);
+/* saveN's float mean of a deltaN map over the sites <= cutoff (src/output.cpp:1527-1537). */
+float ieo_float_mean(const double *dn, size_t gs, double cutoff);
 void ieo_rk4_stepN(ieo_state *st, double h);
 int ieo_rk45_stepN(ieo_state *st, double h, double hmax, double *h_next);
 

@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
             "ieo_evolve_fieldsN": (None, [P, C.c_double]),
             "ieo_get_phiref": (C.c_double, [P]),
             "ieo_deltaN_zeta_map": (C.c_int, [P, C.POINTER(C.c_float)]),
+            "ieo_float_mean": (C.c_float, [C.POINTER(C.c_double), C.c_size_t, C.c_double]),
             "ieo_rk4_stepN": (None, [P, C.c_double]),
             "ieo_rk45_stepN": (C.c_int, [P, C.c_double, C.c_double, dp]),
             "ieo_post_initialize": (None, [P]),
@@ -317,6 +318,12 @@ def _fp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def float_mean(dn: np.ndarray, cutoff: float) -> np.float32:
+    """saveN's float mean of a deltaN map (ieo_float_mean)."""
+    a = np.ascontiguousarray(dn, dtype=np.float64).ravel()
+    return np.float32(lib().ieo_float_mean(_dp(a), a.size, cutoff))
 
 
 def fftrn(field: np.ndarray, forward: int):
