@@ -6,20 +6,25 @@ reference.  Tolerances: %e/%f text files to 2e-6 relative (the reference prints
 7 significant digits; the GPU's sum order differs at 1e-15); histograms to two
 sites per bin; %.17g snapshots as fluctuations about the mean, to 1e-8 of their
 rms; GW spectra to 5e-4 relative (the reference sums shells in float).  The deltaN
-map can differ for sites that reach the phiref surface within one step of
-round-off, so zeta is compared by the fraction of sites that agree and P_zeta to
-1e-2 per bin.
+map could differ at a site that reaches the phiref surface within a round-off of
+the step: the share of zeta sites that agree and of those one dN apart is printed
+(at N = 16 every site agrees; at most one may differ), and P_zeta is compared per
+shell to the printed 7 digits (2e-6), shell 0 (the k = 0 mode of the
+mean-subtracted map, round-off) against 1e-12 of the spectrum's scale.  Spectra
+are compared with no absolute floor (their values are far below numpy's default
+atol): assert_rel and assert_spectrum print the largest error of each check.
 """
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 from pathlib import Path
 
 import numpy as np
 import pytest
 
-from conftest import REPO, load_golden
+from conftest import REPO, assert_rel, assert_spectrum, load_golden
 from oracle.oracle import read_blocks
 
 pytestmark = pytest.mark.gpu
@@ -89,15 +94,28 @@ def test_driver_matches_reference_files(case, tmp_path):
         for r, q in zip(ref, got):
             d = r - r.mean()
             assert np.max(np.abs(d - (q - q.mean()))) <= 1e-8 * np.sqrt(np.mean(d ** 2)) + 1e-300, key
-    # deltaN outputs
+    # deltaN outputs: the share of zeta sites equal to the reference's, of those
+    # one deltaN step (dN) apart, and the largest P_zeta bin error
     z_ref = g["snapshots_2d_deltaN"].ravel()
     z_got = blocks(res / "snapshots_2d_deltaN.dat", "snapshots_2d_deltaN", g["snapshots_2d_deltaN"]).ravel()
-    assert np.mean(np.abs(z_ref - z_got) < 1e-12) > 0.95
+    dN = float(re.search(r"^\s*dN\s*=\s*([0-9.eE+-]+)", str(g["params_text"]), re.M).group(1))
+    d = np.abs(z_ref - z_got)
+    same = np.mean(d < 1e-12)
+    step = np.abs(d - dN) < 1e-6 * dN
+    one_step = np.mean(step)
+    other = d[(d >= 1e-12) & ~step]
     sN_ref = g["spectraN"]
     sN = blocks(res / "spectraN.dat", "spectraN", sN_ref)
     assert np.array_equal(sN[..., :2], sN_ref[..., :2])
-    live = sN_ref[..., 2] > 0
-    assert np.allclose(sN[..., 2][live], sN_ref[..., 2][live], rtol=1e-2)
+    print(f"{case}: zeta sites equal {same:.6f}, one dN apart {one_step:.6f}, other {other.size} "
+          f"(max {other.max() if other.size else 0.0:.3e})", flush=True)
+    # measured at N = 16: every site of the slice equal (no site reaches the
+    # surface within a round-off of the other run); at most one may differ
+    assert same >= 1.0 - 1.5 / z_ref.size
+    # shells 1.. to the file's 7 printed digits; shell 0 is the k = 0 mode of a
+    # mean-subtracted map (round-off), bounded against the spectrum's scale
+    assert_rel(sN[..., 1:, 2], sN_ref[..., 1:, 2], 2e-6, f"{case} spectraN shells 1..")
+    assert np.all(np.abs(sN[..., 0, 2] - sN_ref[..., 0, 2]) <= 1e-12 * sN_ref[..., 1:, 2].max())
 
 
 def test_driver_split_precision_close_to_fp64(tmp_path):
@@ -109,7 +127,9 @@ def test_driver_split_precision_close_to_fp64(tmp_path):
     run_driver("hilltop_n32", b, "precision = split\npost_inflation = 0\n")
     s64 = np.stack(read_blocks(a / "results" / "spectra.dat"))
     s32 = np.stack(read_blocks(b / "results" / "spectra.dat"))
-    assert np.allclose(s32, s64, rtol=1e-4)
+    assert np.array_equal(s32[..., :2], s64[..., :2])
+    live = s64[..., 2] != 0
+    assert_rel(s32[..., 2][live], s64[..., 2][live], 1e-5, "split vs FP64 spectra.dat")
     m64 = np.concatenate(read_blocks(a / "results" / "sf.dat"))
     m32 = np.concatenate(read_blocks(b / "results" / "sf.dat"))
     assert np.allclose(m32, m64, rtol=1e-6)
@@ -123,10 +143,11 @@ def post_blocks(res: Path, g: dict, key: str) -> tuple[np.ndarray, np.ndarray]:
     return blocks(res / "post_inflation" / f"{key}.dat", key, ref), ref
 
 
-# results/post_inflation/*.dat.  From the deltaN map, Phi inherits the map's
-# one-step differences at a few sites, so the Phi statistics get 1e-2 (spectra,
-# variance, tensor spectra 2e-2) and the means an absolute bound of 1e-2 of the
-# rms of Phi; without deltaN (Phi from f) they follow the inflation tolerances.
+# results/post_inflation/*.dat.  Phi comes from the deltaN map (or from f without
+# deltaN), which agrees site by site at these sizes: the Phi spectra and variance
+# to 2e-6, the tensor spectra to 1e-5 per shell, the means to 1e-4 of the rms of
+# Phi and the velocity to 1e-4 (measured: equal to the printed digits, tensor
+# spectra <= 3e-6, means <= 1e-9 of the rms).
 @pytest.mark.parametrize("case", ["tabulated_n16_post", "tabulated_n16_post_rk4", "tabulated_n16_post_nodn",
                                   "hilltop_n16_post", "tabulated_n16_cutoff_post"])
 def test_driver_post_inflation_files(case, tmp_path):
@@ -135,7 +156,6 @@ def test_driver_post_inflation_files(case, tmp_path):
     run_driver(case, tmp_path, extra)
     res = tmp_path / "results"
     N = int(g["N"])
-    exact = int(g["perform_deltaN"]) == 0
     for key in POST_SERIES:
         got, ref = post_blocks(res, g, key)
         assert got.shape == ref.shape, key
@@ -143,29 +163,29 @@ def test_driver_post_inflation_files(case, tmp_path):
         assert np.allclose(got[:, cols], ref[:, cols], rtol=2e-6, atol=0), key
     var, var_ref = post_blocks(res, g, "variance")
     assert np.allclose(var[:, :2], var_ref[:, :2], rtol=2e-6, atol=0)
-    assert np.allclose(var[:, 2], var_ref[:, 2], rtol=2e-6 if exact else 1e-2, atol=0)
+    assert_rel(var[:, 2], var_ref[:, 2], 2e-6, f"{case} post variance")
     rms = np.sqrt(var_ref[:, 2] / N ** 3)
     for key in ("means", "velocity"):
         got, ref = post_blocks(res, g, key)
         assert np.allclose(got[:, :2], ref[:, :2], rtol=2e-6, atol=0), key
         if key == "means":
-            bound = 1e-4 * rms if exact else 1e-2 * rms
-            assert np.all(np.abs(got[:, 2] - ref[:, 2]) <= bound + 1e-6 * np.abs(ref[:, 2])), key
+            bound = 1e-4 * rms
+            err = np.abs(got[:, 2] - ref[:, 2])
+            print(f"{case} post means: max err / rms {float(np.max(err / rms)):.3e}", flush=True)
+            assert np.all(err <= bound + 1e-6 * np.abs(ref[:, 2])), key
         else:
-            assert np.allclose(got[:, 2], ref[:, 2], rtol=1e-4 if exact else 5e-2, atol=1e-30), key
+            assert_spectrum(got[:, 2], ref[:, 2], 1e-4, f"{case} post velocity", floor=1e-30)
     got, ref = post_blocks(res, g, "spectra")
     assert np.array_equal(got[..., :2], ref[..., :2])
-    live = ref[..., 2] > 0
-    assert np.allclose(got[..., 2][live], ref[..., 2][live], rtol=2e-6 if exact else 1e-2)
+    assert_spectrum(got[..., 2], ref[..., 2], 2e-6, f"{case} post spectra")
     for key in ("spectraGW", "spectraGWdot"):
         got, ref = post_blocks(res, g, key)
         assert np.array_equal(got[..., :2], ref[..., :2])
-        live = ref[..., 2] > 0
-        assert live.sum() > 0
-        assert np.allclose(got[..., 2][live], ref[..., 2][live], rtol=5e-4 if exact else 2e-2), key
+        assert np.any(ref[..., 2] > 0)
+        assert_spectrum(got[..., 2], ref[..., 2], 1e-5, f"{case} post {key}")
     got, ref = post_blocks(res, g, "histogram")
     assert got.shape == ref.shape
-    assert np.max(np.abs(got - ref)) <= (2.0 / N ** 3 + 1e-9 if exact else 0.02)
+    assert np.max(np.abs(got - ref)) <= 2.0 / N ** 3 + 1e-9
 
 
 # the k-space forcing cutoff after each spectraf (inflation): the standard files;

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import ie_engine as E
-from conftest import load_golden, potential_table
+from conftest import assert_rel, load_golden, potential_table
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -200,7 +200,7 @@ def test_post_outputs_match_oracle():
         assert np.array_equal(cg, co) and np.array_equal(p, po)
         live = vo > 0
         assert live.sum() > 5
-        assert np.allclose(vg[live], vo[live], rtol=2e-4)
+        assert_rel(vg[live], vo[live], 1e-5, f"post GW spectrum hdot={hdot}")
 
 
 def test_post_phase_refuses_inflation_steps():
