@@ -351,13 +351,16 @@ def c3_sigw(N: int = 256) -> dict:
     return res
 
 
-def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203, graph_steps: int = 50) -> dict:
+def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 20261203, graph_steps: int = 50,
+             sequential: bool = True, whole: bool = True) -> dict:
     """Batched ensemble: nruns hilltop lattices of N^3, seeds 1000 + r, V0
     log-uniform in [1e-11, 1e-9], ns uniform in [0.95, 0.98], split storage,
     shapes tuned before the batch is built, the deltaN window scaled with V0.  The same number of steps is timed
     once as one batch (one launch per kernel per step, HIP graph replay) and once
     as sequential per-run launches; then the batch runs every lattice to
-    af = 2N and each run's P_zeta is computed (ie_run_outputs)."""
+    af = 2N and each run's P_zeta is computed (ie_run_outputs).  sequential /
+    whole: False leaves out the per-run comparison / the whole runs (profiling
+    runs of the batched step alone)."""
     from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
@@ -390,13 +393,24 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
     t0 = time.perf_counter()
     batch.steps(steps, graph_steps=graph_steps)
     tb = time.perf_counter() - t0
-    for r in runs[:4]:
-        r.leapfrog_steps(5)                          # warm-up of the per-run path
-    t0 = time.perf_counter()
-    for r in runs[4:]:
-        r.leapfrog_steps(steps)
-    ts = (time.perf_counter() - t0) * nruns / (nruns - 4)
+    sites = nruns * N ** 3 * steps
+    res = {"runs": nruns, "N": N, "steps": steps, "tuning": tuning,
+           "batched_site_updates_per_s": sites / tb, "batched_ms_per_step": 1e3 * tb / steps,
+           "graph_steps": graph_steps}
+    if sequential:
+        for r in runs[:4]:
+            r.leapfrog_steps(5)                          # warm-up of the per-run path
+        t0 = time.perf_counter()
+        for r in runs[4:]:
+            r.leapfrog_steps(steps)
+        ts = (time.perf_counter() - t0) * nruns / (nruns - 4)
+        res.update({"sequential_site_updates_per_s": sites / ts, "batched_over_sequential": ts / tb,
+                    "sequential_ms_per_step": 1e3 * ts / steps})
     batch.close()
+    if not whole:
+        for r in runs:
+            r.close()
+        return res
     # whole runs: a fresh batch of the same lattices to af, then per-run P_zeta
     for r in runs:
         r.close()
@@ -419,17 +433,12 @@ def ensemble(nruns: int = 256, N: int = 64, steps: int = 200, seed: int = 202612
     for r in runs:
         r.close()
     pz = np.array([o["pzeta"] for o in outs])
-    sites = nruns * N ** 3 * steps
-    return {"runs": nruns, "N": N, "steps": steps, "tuning": tuning,
-            "batched_site_updates_per_s": sites / tb,
-            "sequential_site_updates_per_s": sites / ts, "batched_over_sequential": ts / tb,
-            "batched_ms_per_step": 1e3 * tb / steps, "sequential_ms_per_step": 1e3 * ts / steps,
-            "graph_steps": graph_steps,
-            "whole_runs": {"steps_per_run": int(outs[0]["steps"]), "loop_s": t_whole, "outputs_s": t_out,
+    res["whole_runs"] = {"steps_per_run": int(outs[0]["steps"]), "loop_s": t_whole, "outputs_s": t_out,
                            "pzeta_returned": int(sum(o["nbins"] > 0 for o in outs)), "pzeta_shells": int(pz.shape[1]),
                            "pzeta_positive_runs": int(sum(np.all(p[1:] > 0) for p in pz)),
                            "pzeta_shell8_min": float(pz[:, 8].min()), "pzeta_shell8_max": float(pz[:, 8].max()),
-                           "pzeta_run0": [float(x) for x in pz[0]]}}
+                           "pzeta_run0": [float(x) for x in pz[0]]}
+    return res
 
 
 def post_inflation(N: int = 256, steps: int = 200, warmup: int = 10, precision: int | None = None) -> dict:
@@ -491,7 +500,8 @@ def main() -> int:
     ap.add_argument("--post-steps", type=int, default=200, help="post-inflation steps timed (0: skip)")
     ap.add_argument("--pzeta-check", type=int, default=1,
                     help="whole runs, split vs FP64: 128^3 tabulated and 256^3 SIGW + deltaN (0: skip)")
-    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "c2", "c3", "serve", "tab", "ensemble"],
+    ap.add_argument("--only", choices=["split", "fp64", "sigw", "post", "c2", "c3", "serve", "tab", "ensemble",
+                                       "batch"],
                     default=None, help="time a single case (for profiling runs)")
     ap.add_argument("--ensemble-runs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=100)
@@ -524,9 +534,12 @@ def main() -> int:
     if args.only == "c3":
         print(json.dumps({"c3_sigw_256": c3_sigw(N)}), flush=True)
         return 0
-    if args.only == "ensemble":
+    if args.only in ("ensemble", "batch"):
+        # batch: the batched steps alone (no per-run comparison, no whole runs)
         gs = 50 if args.graph_steps is None else args.graph_steps
-        print(json.dumps({"ensemble": ensemble(args.ensemble_runs, graph_steps=gs)}), flush=True)
+        full = args.only == "ensemble"
+        print(json.dumps({"ensemble": ensemble(args.ensemble_runs, steps=args.steps if not full else 200,
+                                               graph_steps=gs, sequential=full, whole=full)}), flush=True)
         return 0
     if args.only == "serve":
         print(json.dumps({"serving": serve_stream(args.serve_requests, args.serve_load,
@@ -592,11 +605,14 @@ def main() -> int:
         ms = prof["loop_ms"] if one_launch else prof["kick_drift_ms"]
         achieved = alg / (ms * 1e-3) / 1e9
         variant = head["tuning"].get("march_variant", 0)
-        kname = ("k_march_pf<float, true, 4, 0>" if variant == 1 else "k_march<float, true, 4, 0>") + \
+        split_head = cases[head_name][0] == E.IE_SPLIT
+        targs = "float, true" if split_head else "double, false"
+        kname = f"{'k_march_pf' if variant == 1 else 'k_march'}<{targs}, 4, 0>" + \
             " (previous step's close + kick + drift + next-step sums: the whole loop step)"
+        storage = "split" if split_head else "fp64"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS,
-                           "traffic": read_traffic(f"{'march' if fused else 'kick_drift'}_split_{N}"),
+                           "traffic": read_traffic(f"{'march' if fused else 'kick_drift'}_{storage}_{N}"),
                            "kernel": kname if fused else "k_row_kick (kick + drift)", "kernel_ms": ms,
                            "algorithmic_bytes": alg, "reduce_ms": prof["reduce_ms"],
                            "kernel_ms_per_launch_events": prof["kick_drift_ms"],
