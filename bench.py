@@ -270,12 +270,29 @@ def serve_stream(nreq: int, load: float = 0.8, burst: int = 24, seed: int = 2026
                    "each run: ICs, inflation to af=2N, Omega_GW (SIGW), deltaN, P_zeta"}
 
 
-def compare_spectra(pa, pb, nlog: int, N: int | None = None) -> dict:
+def shell_modes(N: int, n: int):
+    """Weighted mode count per reference shell (bin = lround |k|, weight 2 for
+    0 < kz < N/2, 1 for kz = 0 and N/2; src/output.cpp:63-151)."""
+    import numpy as np
+    i = np.arange(N)
+    p = np.where(i <= N // 2, i, i - N)
+    px, py = np.meshgrid(p, p, indexing="ij")
+    cnt = np.zeros(n, dtype=np.int64)
+    for z in range(N // 2 + 1):
+        b = np.floor(np.sqrt(px * px + py * py + z * z) + 0.5).astype(np.int64).ravel()
+        keep = b < n
+        np.add.at(cnt, b[keep], 1 if z in (0, N // 2) else 2)
+    return cnt
+
+
+def compare_spectra(pa, pb, nlog: int, N: int | None = None, notebook_cut: bool = False) -> dict:
     """Split (pa) against FP64 (pb) spectra on the reference shells: the largest
     relative difference over shells 1 .. n-1 with pb > 0, over the complete
-    spherical shells 1 .. N/2 (when n > N/2) and over nlog log-spaced bins in k
-    (each the mean of the shell values whose index falls in it; bins narrower
-    than a shell stay empty)."""
+    spherical shells 1 .. N/2 (when n > N/2), over the shells the reference's
+    P_zeta plot shows (notebook_cut: notebooks/plot.ipynb cell 9 plots k_list up
+    to len(k_list) - 10) and over nlog log-spaced bins in k (each the mean of the
+    shell values whose index falls in it; bins narrower than a shell stay empty);
+    the three worst shells with their weighted mode counts."""
     import numpy as np
     n = min(len(pa), len(pb))
     pa, pb = np.asarray(pa[:n]), np.asarray(pb[:n])
@@ -291,6 +308,13 @@ def compare_spectra(pa, pb, nlog: int, N: int | None = None) -> dict:
            "log_bins_filled": len(lrel), "max_rel_err_log_bins": float(max(lrel)) if lrel else None}
     if N is not None and n > N // 2:
         out["max_rel_err_shells_1_to_N_over_2"] = float(rel[1:N // 2 + 1].max())
+    if notebook_cut and n > 11:
+        out["max_rel_err_shells_notebook_range"] = float(rel[1:n - 10].max())
+        out["notebook_range"] = f"shells 1..{n - 11}"
+    if N is not None:
+        modes = shell_modes(N, n)
+        worst = np.argsort(rel)[::-1][:3]
+        out["worst_shells"] = [[int(b), float(rel[b]), int(modes[b])] for b in worst]
     return out
 
 
@@ -330,7 +354,7 @@ def c2_tabulated(N: int = 128) -> dict:
     b = whole_run(E, N, E.IE_FP64, 0, 1, T)
     res = {"N": N, "potential": "tabulated SR-USR-SR (inputs/ tables)", "steps": a["steps"],
            "split_loop_s": a["loop_s"], "fp64_loop_s": b["loop_s"], "outputs_s": a["outputs_s"]}
-    res["pzeta"] = compare_spectra(a["pzeta"], b["pzeta"], 64, N)
+    res["pzeta"] = compare_spectra(a["pzeta"], b["pzeta"], 64, N, notebook_cut=True)
     return res
 
 
@@ -345,7 +369,7 @@ def c3_sigw(N: int = 256) -> dict:
     b = whole_run(E, N, E.IE_FP64, 1, 0)
     res = {"N": N, "potential": "hilltop", "sigw": True, "steps": a["steps"], "a_end": a["a_end"],
            "split_loop_s": a["loop_s"], "fp64_loop_s": b["loop_s"], "outputs_s": a["outputs_s"]}
-    res["pzeta"] = compare_spectra(a["pzeta"], b["pzeta"], 128, N)
+    res["pzeta"] = compare_spectra(a["pzeta"], b["pzeta"], 128, N, notebook_cut=True)
     res["omega_gw"] = compare_spectra(a["omega_gw"], b["omega_gw"], 128, N)
     res["omega_gw"]["kept_shells"] = int(a["gw_nbins"])
     return res

@@ -371,6 +371,16 @@ ie_status ie_batch_launch(ie_batch *batch, int nsteps, int graph_steps);
 ie_status ie_batch_wait(ie_batch *batch);
 ie_status ie_batch_destroy(ie_batch *batch);
 const char *ie_batch_last_error(const ie_batch *batch);
+/* Path of the 64^3 lattices without tensors in batches created afterwards: one
+ * resident workgroup per lattice running every step of a launch (throughput:
+ * it fills the GPU with many lattices), or the per-step kernels, which spread
+ * each lattice over the whole GPU (latency: a lone lattice steps several times
+ * faster).  A batch takes the resident path for the 64^3 lattices of a
+ * potential kind when it holds at least min_hilltop (analytic potential) or
+ * min_tabulated (tabulated potential) of them; 0 restores the measured
+ * defaults.  Process-wide; set it before creating batches, not beside another
+ * thread's ie_batch_create.  No reference counterpart. */
+ie_status ie_batch_set_resident_min(int min_hilltop, int min_tabulated);
 
 /* --- serving ---------------------------------------------------------------- */
 /* A request loop over whole runs (initial conditions, inflation loop to af, then

@@ -1800,6 +1800,13 @@ ie_status ie_batch_wait(ie_batch *b)
     return IE_OK;
 }
 
+ie_status ie_batch_set_resident_min(int min_hilltop, int min_tabulated)
+{
+    if (min_hilltop < 0 || min_tabulated < 0) return bfail(nullptr, IE_ERR_ARG, "negative minimum");
+    set_resident_min(min_hilltop, min_tabulated);
+    return IE_OK;
+}
+
 ie_status ie_batch_steps(ie_batch *b, int nsteps, int graph_steps)
 {
     const ie_status st = ie_batch_launch(b, nsteps, graph_steps);

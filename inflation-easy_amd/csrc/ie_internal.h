@@ -438,6 +438,14 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out);
 // launch, the other groups one step at a time
 hipError_t batch_steps(const BatchDev *B, long long n0, int count, hipStream_t s);
 bool resident_ok(int N, int sigw, int split, int numerical);
+// Minimum lattices of a potential kind for the resident path (0 restores these
+// defaults).  Measured per step, 64^3 split storage, graph replay: hilltop
+// resident 0.116 ms from 1 to 16 lattices, 0.134 at 64, 0.147 at 128, 0.221 at
+// 256, against 0.012, 0.035, 0.120, 0.241, 0.470 ms for the per-step kernels
+// (1, 16, 64, 128, 256 lattices); tabulated resident 0.31-0.32 ms from 1 to 256
+// lattices against 0.014, 0.039, 0.123, 0.260, 0.513.
+constexpr int kResMinHill = 72, kResMinTab = 152;
+void set_resident_min(int hill, int tab);
 void batch_free(BatchDev *B);
 hipError_t batch_loop_begin(const BatchDev *B, hipStream_t s);
 hipError_t batch_step(const BatchDev *B, long long n, hipStream_t s);        // step n of the loop

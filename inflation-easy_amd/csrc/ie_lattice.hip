@@ -1101,9 +1101,9 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
 
 // ------------------------------------------------ resident small lattices
 //
-// Batched 64^3 lattices without SIGW: one workgroup of 16 waves owns a whole
+// Batched 64^3 lattices without SIGW: one workgroup of 8 waves owns a whole
 // lattice and runs every loop step of a launch itself.  A plane is its 64 rows,
-// 4 consecutive rows per wave with one site per lane, so a kick finds its k +- 1
+// 8 consecutive rows per wave with one site per lane, so a kick finds its k +- 1
 // neighbours by lane shuffles, its j +- 1 rows in the wave's own registers (the
 // wave's two edge rows through LDS), its i +- 1 rows in the registers of the
 // planes the wave marches through.  Each site is kicked once, with no halo, and
@@ -1114,9 +1114,12 @@ __global__ __launch_bounds__(kBlock) void kb_sums_finish(const RunDesc *__restri
 // (close_state), then starts the next step.  A launch first forms the sums of
 // the current f (the same partition and order), so a loop split into several
 // launches gives the same bits as one.  Reads f, fd and writes them once per
-// step (16 B/site in split storage).
+// step (16 B/site in split storage).  8 waves of 8 rows hold every row in
+// registers without spills (243 VGPRs, 2 waves per SIMD); 16 waves of 4 rows
+// spill at the 128 VGPRs that 4 waves per SIMD allow and step 1 % slower
+// (256 x 64^3, split storage: 0.2255 against 0.2293 ms per step).
 constexpr int kResN = 64;
-constexpr int kResWaves = 16;
+constexpr int kResWaves = 8;
 constexpr int kResRows = kResN / kResWaves;   // rows per wave
 constexpr int kResThreads = 64 * kResWaves;
 
@@ -2140,6 +2143,17 @@ bool resident_ok(int N, int sigw, int split, int numerical)
     return N == kResN && !sigw && (split || !numerical);
 }
 
+// Minimum number of resident-capable lattices of a potential kind for a batch
+// to take the resident path (below it the per-step kernels spread each lattice
+// over the GPU and step faster).
+static int g_res_min[2] = {kResMinHill, kResMinTab};
+
+void set_resident_min(int hill, int tab)
+{
+    g_res_min[0] = hill > 0 ? hill : kResMinHill;
+    g_res_min[1] = tab > 0 ? tab : kResMinTab;
+}
+
 unsigned march_blocks(int N) { return (unsigned)((N / kTileJ) * (N / (int)march_ti((unsigned)N))); }
 
 hipError_t launch_march_begin(const Fields &F, int split, const Pot &P, DevBg *bg, double *partials, hipStream_t s)
@@ -2265,6 +2279,11 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
     // lattices excepted: their rows and table exceed the workgroup's LDS)
     B->nruns = n;
     B->split = split;
+    // resident only with enough lattices of the kind to fill the GPU
+    int res_count[2] = {0, 0};
+    for (int r = 0; r < n; ++r)
+        if (resident_ok(runs[r].F.N, runs[r].F.sigw, split, runs[r].P.numerical))
+            ++res_count[runs[r].P.numerical ? 1 : 0];
     // groups of runs launched together: same step structure and kernel shapes
     std::vector<int> keys;
     std::vector<std::vector<uint2>> maps, rmaps, tmaps;
@@ -2278,7 +2297,9 @@ hipError_t batch_build(const BatchRun *runs, int n, int split, BatchDev **out)
         d[r].march = march_ok(N) ? 1 : 0;
         d[r].Gm = d[r].march ? march_blocks(N) : 0;
         d[r].split = split;
-        d[r].res = resident_ok(N, runs[r].F.sigw, split, runs[r].P.numerical) ? 1 : 0;
+        const int kind = runs[r].P.numerical ? 1 : 0;
+        d[r].res = resident_ok(N, runs[r].F.sigw, split, runs[r].P.numerical) && res_count[kind] >= g_res_min[kind]
+                       ? 1 : 0;
         if (d[r].res) {
             B->has_res = 1;
             const int key = 100000000 + runs[r].P.numerical * 10000000;

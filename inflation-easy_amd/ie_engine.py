@@ -190,6 +190,7 @@ def lib() -> C.CDLL:
             "ie_modes": (st, [C.c_int, C.c_double, C.c_double, dp]),
             "ie_batch_create": (st, [C.POINTER(P), C.c_int, C.POINTER(P)]),
             "ie_batch_steps": (st, [P, C.c_int, C.c_int]),
+            "ie_batch_set_resident_min": (st, [C.c_int, C.c_int]),
             "ie_batch_launch": (st, [P, C.c_int, C.c_int]),
             "ie_batch_wait": (st, [P]),
             "ie_batch_destroy": (st, [P]),
@@ -545,6 +546,14 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+    @staticmethod
+    def set_resident_min(hilltop: int = 0, tabulated: int = 0) -> None:
+        """Minimum 64^3 lattices of a potential kind for the resident path in
+        batches created afterwards (ie_batch_set_resident_min; 0: the default)."""
+        st = lib().ie_batch_set_resident_min(hilltop, tabulated)
+        if st:
+            raise EngineError(st, lib().ie_batch_last_error(None).decode())
 
 
 class Server:

@@ -40,6 +40,15 @@ def state(run):
     return run.field(E.IE_FIELD_F), run.field(E.IE_FIELD_FD), run.background()
 
 
+@pytest.fixture
+def resident_always():
+    """The resident path for every 64^3 lattice without tensors (the default
+    takes it only for batches of many such lattices)."""
+    E.Batch.set_resident_min(1, 1)
+    yield
+    E.Batch.set_resident_min(0, 0)
+
+
 def test_alone_equals_in_mix_bitwise():
     table = potential_table()
     specs = [(32, 11, 0, 0, 0.97), (16, 12, 1, 0, 0.97), (128, 13, 0, 1, 0.96), (32, 14, 0, 0, 0.95),
@@ -111,7 +120,7 @@ def test_batch_matches_single_run_path():
         assert np.max(np.abs((fa - fa.mean()) - d)) < 1e-9 * np.sqrt(np.mean(d ** 2))
 
 
-def test_batch_stops_each_run_at_its_af():
+def test_batch_stops_each_run_at_its_af(resident_always):
     cfg_short = E.make_config(16, precision=E.IE_SPLIT, af=1.005, **HILL)
     short = E.Run(cfg_short)
     short.initialize(41, 0.0935, 0.000796)
@@ -134,7 +143,7 @@ def test_batch_stops_each_run_at_its_af():
     assert bl["done"] == 0 and bl["steps"] == 40
 
 
-def test_fp64_tabulated_64_takes_the_step_path_beside_resident_runs():
+def test_fp64_tabulated_64_takes_the_step_path_beside_resident_runs(resident_always):
     """An FP64 tabulated 64^3 lattice (its rows and table exceed a resident
     workgroup's LDS) steps by the per-step kernels in a batch with resident split
     lattices, and each run gets the bits it gets alone."""
@@ -155,7 +164,7 @@ def test_fp64_tabulated_64_takes_the_step_path_beside_resident_runs():
 
 
 @pytest.mark.parametrize("chunks", [(10, 10, 10), (7, 1, 6, 16)])
-def test_resident_launches_in_chunks_equal_one(chunks):
+def test_resident_launches_in_chunks_equal_one(chunks, resident_always):
     """A resident 64^3 lattice starts every launch with the sums of its current f
     in a step's own partition and order, so launches of any lengths give the bits
     of one launch."""
@@ -170,7 +179,7 @@ def test_resident_launches_in_chunks_equal_one(chunks):
     assert bga["a"] == bgb["a"] and bga["ad"] == bgb["ad"] and bga["steps"] == bgb["steps"] == sum(chunks)
 
 
-def test_resident_tabulated_split_matches_single_run_path():
+def test_resident_tabulated_split_matches_single_run_path(resident_always):
     """The resident kernel against the single-run row kernels (another sum order):
     a to 1e-13 relative, the FP32 fluctuation to 1e-5 of its rms over 200 steps."""
     table = potential_table()
@@ -186,3 +195,34 @@ def test_resident_tabulated_split_matches_single_run_path():
     err = np.max(np.abs((fa - fa.mean()) - d)) / np.sqrt(np.mean(d ** 2))
     print(f"resident tabulated 64 vs single run: fluct err {err:.3e}", flush=True)
     assert err < 1e-5
+
+
+def test_resident_path_taken_by_lattice_count():
+    """ie_batch_set_resident_min: a batch takes the resident path for its 64^3
+    lattices of a potential kind only when it holds at least the minimum number of
+    them; each path gives the bits it gives when forced."""
+    table = potential_table()
+    specs = [(64, 81, 0), (64, 82, 0), (64, 83, 1), (64, 84, 1)]
+
+    def run_batch(min_hill, min_tab, which):
+        E.Batch.set_resident_min(min_hill, min_tab)
+        try:
+            rr = [new_run(N, s, numerical=p, table=table) for N, s, p in (specs[q] for q in which)]
+            E.Batch(rr).steps(30, graph_steps=10)
+            return [state(r)[:2] for r in rr]
+        finally:
+            E.Batch.set_resident_min(0, 0)
+
+    big = 1 << 20
+    forced_res = run_batch(1, 1, range(4))
+    forced_steps = run_batch(big, big, range(4))
+    # two hilltop lattices meet a minimum of 2, two tabulated ones miss 3
+    mixed = run_batch(2, 3, range(4))
+    for q in range(4):
+        want = forced_res[q] if q < 2 else forced_steps[q]
+        assert np.array_equal(mixed[q][0], want[0]) and np.array_equal(mixed[q][1], want[1]), q
+    # the two paths differ only by the sum order: same physics
+    for q in range(4):
+        d = forced_steps[q][0] - forced_steps[q][0].mean()
+        err = np.max(np.abs((forced_res[q][0] - forced_res[q][0].mean()) - d)) / np.sqrt(np.mean(d ** 2))
+        assert err < 1e-5, (q, err)

@@ -276,11 +276,20 @@ def test_ensemble_256x64_per_run_pzeta():
     for r in runs[16:]:
         r.close()
     sample = [0, 37, 64, 101, 128, 170, 211, 255]
-    for q in sample:
-        alone = ensemble_run(q, V0, ns, af)
-        b1 = E.Batch([alone])
-        step_until_done(b1, [alone])
-        b1.close()
+    # alone, a batch of one takes the resident path only when forced (the
+    # default gives small batches the per-step kernels)
+    E.Batch.set_resident_min(1, 1)
+    try:
+        alones = []
+        for q in sample:
+            alone = ensemble_run(q, V0, ns, af)
+            b1 = E.Batch([alone])
+            step_until_done(b1, [alone])
+            b1.close()
+            alones.append(alone)
+    finally:
+        E.Batch.set_resident_min(0, 0)
+    for q, alone in zip(sample, alones):
         o = alone.outputs(1)
         alone.close()
         assert o["steps"] == outs[q]["steps"] and o["a_end"] == outs[q]["a_end"], q
