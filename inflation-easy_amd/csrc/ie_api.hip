@@ -147,6 +147,45 @@ Shapes tuned_shapes(int N, int split)
     return it == g_shapes.end() ? Shapes() : it->second;
 }
 
+// Shapes of a class no run of this process has tuned: the tuner's choices on
+// MI355X (16 steps per candidate, params.analytic.txt and params.numerical.txt
+// presets).  N = 64: the march tensor kernel with early loads, one site per lane
+// (0.017 ms per tensor step against 0.039 for the row kernel); N = 128: the
+// march step with the j +- 1 rows from L2, tensor march with early loads (E = 1
+// split, 2 FP64); N = 256: the march step from L2 in split storage and for the
+// tabulated potential in FP64, through LDS for the hilltop in FP64, tensor march
+// E = 4 in split storage (0.335 ms against 0.451 for the row kernel), E = 2 with
+// early loads in FP64.  Results do not depend on the shapes.
+Shapes default_shapes(int N, int split, int numerical)
+{
+    Shapes s;
+    if (N == 64) {
+        s.kick_e = 1;
+        s.tensor_e = kTensorMarch | kTensorPf | 1;
+    } else if (N == 128) {
+        s.kick_e = 2;
+        s.march_pf = 1;
+        s.tensor_e = kTensorMarch | kTensorPf | (split ? 1 : 2);
+    } else if (N == 256) {
+        s.kick_e = 4;
+        s.march_pf = split || numerical ? 1 : 0;
+        s.tensor_e = split ? kTensorMarch | 4 : kTensorMarch | kTensorPf | 2;
+    }
+    return s;
+}
+
+// A class's tuned shapes, else its defaults; the parts the tuner left unset
+// take the defaults too.
+Shapes class_shapes(int N, int split, int numerical)
+{
+    const Shapes t = tuned_shapes(N, split), d = default_shapes(N, split, numerical);
+    Shapes s;
+    s.kick_e = t.kick_e ? t.kick_e : d.kick_e;
+    s.tensor_e = t.tensor_e ? t.tensor_e : d.tensor_e;
+    s.march_pf = t.march_pf >= 0 ? t.march_pf : d.march_pf;
+    return s;
+}
+
 void apply_shapes(Fields &F, const Shapes &sh)
 {
     F.kick_e = sh.kick_e;
@@ -265,7 +304,7 @@ ie_status ie_create(const ie_config *cfg, ie_run **out)
     r->F.N = c.N;
     r->F.logN = r->logN;
     r->F.sigw = c.calculate_sigw ? 1 : 0;
-    apply_shapes(r->F, tuned_shapes(c.N, c.precision == IE_SPLIT));
+    apply_shapes(r->F, class_shapes(c.N, c.precision == IE_SPLIT, c.numerical_potential ? 1 : 0));
     for (int p = 0; p < 2; ++p)
         if ((st = dev_alloc(r, &r->F.f[p], r->gs * es)) != IE_OK) return bail(st);
     for (int p = 0; p < 2; ++p)
