@@ -590,25 +590,6 @@ __device__ __forceinline__ RowWalk row_walk(unsigned N, unsigned G, unsigned blk
     return w;
 }
 
-// As row_walk, but the 4 waves of a workgroup take the same rows of 4 consecutive
-// planes, so the i +- 1 rows a wave reads are the rows its neighbours load at the
-// same time (kernels whose i +- 1 rows L2 would otherwise evict between uses).
-__device__ __forceinline__ RowWalk row_walk_stacked(unsigned N, unsigned G, unsigned blk, int E)
-{
-    const unsigned segs = N / (64u * (unsigned)E);
-    const unsigned R = (N * N * segs) / (4u * G);
-    const unsigned jblocks = N / R;
-    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
-    const unsigned t = lb / jblocks;
-    RowWalk w;
-    w.j0 = (lb % jblocks) * R;
-    w.k0 = (t % segs) * (64u * (unsigned)E);
-    w.i = xcd * (N / kXcd) + (t / segs) * 4u + (threadIdx.x >> 6);
-    w.R = R;
-    w.segs = segs;
-    return w;
-}
-
 // The value of the first active lane, in scalar registers (the same in every lane).
 __device__ __forceinline__ float first_lane(float x)
 {
@@ -736,7 +717,7 @@ __device__ __forceinline__ void post_source(const SiteStencil &q, const PostStep
 #pragma clang fp contract(fast)
 
 int post_row_elems(int N);
-unsigned post_row_blocks(int N, int E);
+unsigned post_row_blocks(int N, int E, int PW, unsigned Rmax);
 hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s);
 hipError_t post_drift(const PostState &S, size_t gs, double d, hipStream_t s);
 // serial sums of g1 (and g2 when non-null) in lattice order into out[0], out[1]

@@ -70,6 +70,36 @@ __device__ __forceinline__ unsigned sidx(unsigned i, unsigned j, unsigned k, uns
     return ((i & m) << (2 * logN)) | ((j & m) << logN) | (k & m);
 }
 
+// Row walks of the row kernels: a workgroup of PW waves takes as many
+// consecutive planes of its XCD's slab (wave w plane i0 + w), all walking the
+// same R rows j0 .. j0 + R - 1 of one 64E-site segment in step, so the i +- 1
+// rows a wave reads are, inside the group, the rows its neighbouring waves load
+// at about the same time (L2 hits): per plane the group reads its planes plus
+// two, per row walk R + 2 rows.  Measured at 256^3: the kick with 8 planes and
+// walks of 32 rows moves 1.04x its algorithmic bytes (111 us; 1.17x and 123 us
+// with 4 planes of 8 rows); the tensor kernel (158 VGPRs, so at most 12 waves
+// per CU) keeps 4 planes of 8 rows: with 8 planes its traffic fell from 1.59x
+// to 1.28x but a CU then held one 8-wave workgroup and the step took 0.82
+// against 0.70 ms, and walks of 32 rows left too few workgroups for the last
+// round of the grid.
+constexpr int kPostKickPlanes = 8, kPostTensorPlanes = 4;
+constexpr unsigned kPostKickRows = 32, kPostTensorRows = 8;   // longest walks
+
+template <int PW>
+__device__ __forceinline__ RowWalk post_walk(unsigned N, unsigned blk, int E, unsigned R)
+{
+    const unsigned segs = N / (64u * (unsigned)E), jblocks = N / R;
+    const unsigned xcd = blk % kXcd, lb = blk / kXcd;
+    const unsigned t = lb / jblocks;
+    RowWalk w;
+    w.j0 = (lb % jblocks) * R;
+    w.k0 = (t % segs) * (64u * (unsigned)E);
+    w.i = xcd * (N / kXcd) + (t / segs) * (unsigned)PW + (threadIdx.x >> 6);
+    w.R = R;
+    w.segs = segs;
+    return w;
+}
+
 // ---------------------------------------------------------------- site kernels
 
 __global__ __launch_bounds__(kBlock) void k_post_kick_site(const double *__restrict__ f,
@@ -154,13 +184,13 @@ __device__ __forceinline__ T kminus(const T (&c)[E], int e, T km_first)
 }
 
 template <int E>
-__global__ __launch_bounds__(kBlock) void k_post_kick_rows(const double *__restrict__ f,
-                                                            const double *__restrict__ fd,
-                                                            double *__restrict__ fd_out, unsigned N, unsigned logN,
-                                                            PostStep P)
+__global__ __launch_bounds__(64 * kPostKickPlanes) void k_post_kick_rows(const double *__restrict__ f,
+                                                                      const double *__restrict__ fd,
+                                                                      double *__restrict__ fd_out, unsigned N,
+                                                                      unsigned logN, unsigned R, PostStep P)
 {
     const unsigned m = N - 1, lane = threadIdx.x & 63;
-    const RowWalk w = row_walk_stacked(N, gridDim.x, blockIdx.x, E);
+    const RowWalk w = post_walk<kPostKickPlanes>(N, blockIdx.x, E, R);
     const unsigned P0 = w.i << (2 * logN), Pp = ((w.i + 1) & m) << (2 * logN), Pm = ((w.i - 1) & m) << (2 * logN);
     double prev[E], cur[E], nxt[E];
     ld_row<double, E>(f + P0 + (((w.j0 - 1) & m) << logN) + w.k0, lane, prev);
@@ -193,10 +223,11 @@ __global__ __launch_bounds__(kBlock) void k_post_kick_rows(const double *__restr
 // slide along j in registers; the kicked Phi' rows of plane i and the rows of
 // the six h_ij of plane i likewise.
 template <int E>
-__global__ __launch_bounds__(kBlock) void k_post_tensor_rows(PostTensorArgs T, unsigned N, unsigned logN, PostStep P)
+__global__ __launch_bounds__(64 * kPostTensorPlanes) void k_post_tensor_rows(PostTensorArgs T, unsigned N, unsigned logN,
+                                                                        unsigned R, PostStep P)
 {
     const unsigned m = N - 1, lane = threadIdx.x & 63;
-    const RowWalk w = row_walk_stacked(N, gridDim.x, blockIdx.x, E);
+    const RowWalk w = post_walk<kPostTensorPlanes>(N, blockIdx.x, E, R);
     const unsigned pl[3] = {((w.i - 1) & m) << (2 * logN), w.i << (2 * logN), ((w.i + 1) & m) << (2 * logN)};
     const double *f = T.f, *fd = T.fd;
     double F[3][3][E], D[3][E];
@@ -414,11 +445,20 @@ unsigned site_grid(size_t gs)
 // tensor kernel 0.82 ms per step with 2, 1.06 ms with 1)
 int post_row_elems(int N) { return N < 64 ? 0 : (N < 128 ? 1 : 2); }
 
-// N^2 segments / 32 workgroups: 8 rows per wave (R = 8 divides N >= 64)
-unsigned post_row_blocks(int N, int E)
+// Rows per walk: the longest walk (up to 32 rows) that still gives at least 256
+// workgroups, so that short walks' R + 2 row reads matter only at small N.
+unsigned post_row_walk(int N, int E, int PW, unsigned Rmax)
 {
     const size_t segs = (size_t)N / (64 * (size_t)E);
-    return (unsigned)((size_t)N * N * segs / 32);
+    unsigned R = Rmax;
+    while (R > 2 && (size_t)N * N * segs / ((size_t)PW * R) < 256) R >>= 1;
+    return R;
+}
+
+unsigned post_row_blocks(int N, int E, int PW, unsigned Rmax)
+{
+    const size_t segs = (size_t)N / (64 * (size_t)E);
+    return (unsigned)((size_t)N * N * segs / ((size_t)PW * post_row_walk(N, E, PW, Rmax)));
 }
 
 hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s)
@@ -438,10 +478,16 @@ hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hip
         k_post_kick_site<<<G, kBlock, 0, s>>>(S.f, S.fd, S.fd_out, (unsigned)N, (unsigned)logN, P);
         k_post_tensor_site<<<G, kBlock, 0, s>>>(T, (unsigned)N, (unsigned)logN, P);
     } else {
-        const unsigned G1 = post_row_blocks(N, 1);
-        k_post_kick_rows<1><<<G1, kBlock, 0, s>>>(S.f, S.fd, S.fd_out, (unsigned)N, (unsigned)logN, P);
-        if (E == 2) k_post_tensor_rows<2><<<post_row_blocks(N, 2), kBlock, 0, s>>>(T, (unsigned)N, (unsigned)logN, P);
-        else k_post_tensor_rows<1><<<G1, kBlock, 0, s>>>(T, (unsigned)N, (unsigned)logN, P);
+        constexpr int KP = kPostKickPlanes, TP = kPostTensorPlanes;
+        constexpr unsigned KR = kPostKickRows, TR = kPostTensorRows;
+        k_post_kick_rows<1><<<post_row_blocks(N, 1, KP, KR), 64 * KP, 0, s>>>(
+            S.f, S.fd, S.fd_out, (unsigned)N, (unsigned)logN, post_row_walk(N, 1, KP, KR), P);
+        if (E == 2)
+            k_post_tensor_rows<2><<<post_row_blocks(N, 2, TP, TR), 64 * TP, 0, s>>>(
+                T, (unsigned)N, (unsigned)logN, post_row_walk(N, 2, TP, TR), P);
+        else
+            k_post_tensor_rows<1><<<post_row_blocks(N, 1, TP, TR), 64 * TP, 0, s>>>(
+                T, (unsigned)N, (unsigned)logN, post_row_walk(N, 1, TP, TR), P);
     }
     return hipGetLastError();
 }
