@@ -212,8 +212,15 @@ struct PotView {
                                            double (&dv)[K]) const
     {
         const int cm = c > 1 ? c - 1 : c, cp = c < P.n - 1 ? c + 1 : c;
-        const TabRec rm = tab_rec_uniform(P.T.rec, cm), ru = tab_rec_uniform(P.T.rec, c),
-                     rp = tab_rec_uniform(P.T.rec, cp);
+        window_at<K, WANT_V>(c, cm, cp, tab_rec_uniform(P.T.rec, cm), tab_rec_uniform(P.T.rec, c),
+                             tab_rec_uniform(P.T.rec, cp), v, ok, l, V, dv);
+    }
+    // window() with the three records already read
+    template <int K, bool WANT_V>
+    __device__ __forceinline__ void window_at(int c, int cm, int cp, const TabRec &rm, const TabRec &ru,
+                                              const TabRec &rp, const double (&v)[K], bool (&ok)[K], int (&l)[K],
+                                              double (&V)[K], double (&dv)[K]) const
+    {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             const bool le = v[q] <= ru.xhi, gt = v[q] > ru.xlo;
@@ -236,9 +243,13 @@ struct PotView {
                                            int *err) const
     {
         // the wave's interval lu, in scalar registers: the common case, every
-        // value of the wave inside it
+        // value of the wave inside it; its neighbours are read in the same round
+        // (a wave whose values straddle a grid point needs them, and a second
+        // round of scalar loads would add its latency to every such lookup)
         const int lu = __builtin_amdgcn_readfirstlane(l[0]);
-        const TabRec ru = tab_rec_uniform(P.T.rec, lu);
+        const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
+        const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
+                     rp = tab_rec_uniform(P.T.rec, cp);
         bool ok[K], all = true;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
@@ -259,7 +270,7 @@ struct PotView {
         // first such value
 #pragma unroll
         for (int q = 0; q < K; ++q) ok[q] = false;
-        window<K, WANT_V>(lu, v, ok, l, V, dv);
+        window_at<K, WANT_V>(lu, cm, cp, rm, ru, rp, v, ok, l, V, dv);
         all = true;
         int side = 0;
 #pragma unroll
