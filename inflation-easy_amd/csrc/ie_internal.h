@@ -291,24 +291,78 @@ struct PotView {
         // the value's own hint, then the O(1) index, then the bisection
         if (__any(!all)) {
 #pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (!ok[q]) lookup_value<WANT_V>(v[q], l[q], V[q], dv[q], err);
+        }
+    }
+    // Sums of V/B^2 and V'/B^2 over the K values v, added to sV and sdV (hints l,
+    // updated), for lattice sums.  A value in the wave's interval lu or one of its
+    // neighbours adds its count and v - xlo(lu) to that interval's tallies; each
+    // interval's share then follows from them, n y + s (sum d + n (xlo(lu) - xlo)),
+    // the sum of its interpolations y + (v - xlo) s in another rounding order.
+    // Values outside the three intervals take lookup() one by one.
+    template <int K>
+    __device__ __forceinline__ void sums(const double (&v)[K], int (&l)[K], double &sV, double &sdV, int *err) const
+    {
+        const int lu = __builtin_amdgcn_readfirstlane(l[0]);
+        const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
+        const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
+                     rp = tab_rec_uniform(P.T.rec, cp);
+        double dm = 0.0, du = 0.0, dp = 0.0;
+        int nm = 0, nu = 0, np = 0;
+        bool out[K], any = false;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const double d = v[q] - ru.xlo;
+            const bool above = v[q] > ru.xhi, below = v[q] <= ru.xlo;
+            const bool im = above & (v[q] <= rm.xhi) & (cm != lu), ip = below & (v[q] > rp.xlo) & (cp != lu);
+            const bool iu = (v[q] <= ru.xhi) & (v[q] > ru.xlo);   // false for NaN
+            dm += im ? d : 0.0;
+            du += iu ? d : 0.0;
+            dp += ip ? d : 0.0;
+            nm += im ? 1 : 0;
+            nu += iu ? 1 : 0;
+            np += ip ? 1 : 0;
+            l[q] = im ? cm : (ip ? cp : (iu ? lu : l[q]));
+            out[q] = !(im | iu | ip);
+            any = any | out[q];
+        }
+        const double fm = (double)nm, fu = (double)nu, fp = (double)np;
+        const double om = dm + fm * (ru.xlo - rm.xlo), op = dp + fp * (ru.xlo - rp.xlo);
+        sV += (fu * ru.yv + ru.sv * du) + (fm * rm.yv + rm.sv * om) + (fp * rp.yv + rp.sv * op);
+        sdV += (fu * ru.ydv + ru.sdv * du) + (fm * rm.ydv + rm.sdv * om) + (fp * rp.ydv + rp.sdv * op);
+        if (__any(any)) {
+#pragma unroll
             for (int q = 0; q < K; ++q) {
-                if (!ok[q]) {
-                    int lq = l[q];
-                    TabRec r = P.T.rec[lq];
-                    if (!tab_in(r, v[q])) {
-                        lq = tab_candidate(P.T, v[q]);
-                        r = P.T.rec[lq];
-                        if (!tab_in(r, v[q])) {
-                            lq = tab_bisect_ref(P.x, P.n, v[q], err);
-                            r = P.T.rec[lq];
-                        }
-                    }
-                    l[q] = lq;
-                    dv[q] = r.ydv + (v[q] - r.xlo) * r.sdv;
-                    if (WANT_V) V[q] = r.yv + (v[q] - r.xlo) * r.sv;
+                if (out[q]) {
+                    double x[1] = {v[q]}, a[1], b[1];
+                    int h[1] = {l[q]};
+                    lookup_value<true>(x[0], h[0], a[0], b[0], err);
+                    l[q] = h[0];
+                    sV += a[0];
+                    sdV += b[0];
                 }
             }
         }
+    }
+    // One value outside its wave's window: its own hint, the O(1) index, the
+    // bisection.
+    template <bool WANT_V>
+    __device__ __forceinline__ void lookup_value(double v, int &l, double &V, double &dv, int *err) const
+    {
+        int lq = l;
+        TabRec r = P.T.rec[lq];
+        if (!tab_in(r, v)) {
+            lq = tab_candidate(P.T, v);
+            r = P.T.rec[lq];
+            if (!tab_in(r, v)) {
+                lq = tab_bisect_ref(P.x, P.n, v, err);
+                r = P.T.rec[lq];
+            }
+        }
+        l = lq;
+        dv = r.ydv + (v - r.xlo) * r.sdv;
+        if (WANT_V) V = r.yv + (v - r.xlo) * r.sv;
     }
     // V'/B^2 at the K values v (hints l, updated)
     template <int K>

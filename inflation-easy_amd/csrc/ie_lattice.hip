@@ -950,12 +950,12 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         if (owner) {
             if (t < TI) {
                 const double knext = __shfl(dnew[0], (int)((lane + 1) & 63));
-                double tV[E], tdV[E];
                 if constexpr (POT == kPotTab) {
+                    // sum V and sum V' of the lane's sites through per-interval tallies
                     double tv[E];
 #pragma unroll
                     for (int e = 0; e < E; ++e) tv[e] = SPLIT ? phibar_next + dnew[e] : dnew[e];
-                    Pv.eval<E>(tv, hint, tV, tdV, &err);
+                    Pv.sums<E>(tv, hint, acc1, acc2, &err);
                 }
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
@@ -966,9 +966,6 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
                     if constexpr (POT == kPotHill) {
                         acc1 += c;
                         acc2 += c * c;
-                    } else {
-                        acc1 += tV[e];
-                        acc2 += tdV[e];
                     }
                 }
             }
