@@ -204,18 +204,10 @@ __device__ __forceinline__ TabRec tab_rec_uniform(const TabRec *rec, int l)
 struct PotView {
     Pot P;
 
-    // The three intervals around c, read through the scalar cache: each value not
-    // yet `ok` that lies in one of them takes that interval's interpolation (mask
-    // arithmetic, no short-circuit, so the checks stay branch-free).
-    template <int K, bool WANT_V>
-    __device__ __forceinline__ void window(int c, const double (&v)[K], bool (&ok)[K], int (&l)[K], double (&V)[K],
-                                           double (&dv)[K]) const
-    {
-        const int cm = c > 1 ? c - 1 : c, cp = c < P.n - 1 ? c + 1 : c;
-        window_at<K, WANT_V>(c, cm, cp, tab_rec_uniform(P.T.rec, cm), tab_rec_uniform(P.T.rec, c),
-                             tab_rec_uniform(P.T.rec, cp), v, ok, l, V, dv);
-    }
-    // window() with the three records already read
+    // The three intervals around c (records rm, ru, rp, read through the scalar
+    // cache): each value not yet `ok` that lies in one of them takes that
+    // interval's interpolation (mask arithmetic, no short-circuit, so the checks
+    // stay branch-free).
     template <int K, bool WANT_V>
     __device__ __forceinline__ void window_at(int c, int cm, int cp, const TabRec &rm, const TabRec &ru,
                                               const TabRec &rp, const double (&v)[K], bool (&ok)[K], int (&l)[K],
@@ -265,35 +257,21 @@ struct PotView {
             }
             return;
         }
-        // a wave across grid points: lu and its neighbours, then, for the values
-        // still outside, the three intervals two further on the side of the wave's
-        // first such value
+        // a wave across grid points: if every value lies in lu or a neighbour,
+        // the three records in scalar registers; otherwise (a dense stretch of the
+        // grid, where a wave's values spread over many intervals) every value by
+        // its own lookup, without first trying windows that most values miss
+        bool win = true;
 #pragma unroll
-        for (int q = 0; q < K; ++q) ok[q] = false;
-        window_at<K, WANT_V>(lu, cm, cp, rm, ru, rp, v, ok, l, V, dv);
-        all = true;
-        int side = 0;
+        for (int q = 0; q < K; ++q) win = win & (v[q] <= rm.xhi) & (v[q] > rp.xlo);
+        if (__all(win)) {
 #pragma unroll
-        for (int q = 0; q < K; ++q) {
-            all = all & ok[q];
-            side = side == 0 && !ok[q] ? (v[q] > ru.xhi ? -1 : 1) : side;
+            for (int q = 0; q < K; ++q) ok[q] = false;
+            window_at<K, WANT_V>(lu, cm, cp, rm, ru, rp, v, ok, l, V, dv);
+            return;
         }
-        if (__any(!all)) {
-            const unsigned long long miss = __ballot(side != 0);
-            const int s = __builtin_amdgcn_readlane(side, __ffsll((long long)miss) - 1);
-            const int c2 = lu + 2 * s;
-            window<K, WANT_V>(c2 < 1 ? 1 : (c2 > P.n - 1 ? P.n - 1 : c2), v, ok, l, V, dv);
-            all = true;
 #pragma unroll
-            for (int q = 0; q < K; ++q) all = all & ok[q];
-        }
-        // the rest (a wave across more intervals, an index edge, a table end):
-        // the value's own hint, then the O(1) index, then the bisection
-        if (__any(!all)) {
-#pragma unroll
-            for (int q = 0; q < K; ++q)
-                if (!ok[q]) lookup_value<WANT_V>(v[q], l[q], V[q], dv[q], err);
-        }
+        for (int q = 0; q < K; ++q) lookup_value<WANT_V>(v[q], l[q], V[q], dv[q], err);
     }
     // Sums of V/B^2 and V'/B^2 over the K values v, added to sV and sdV (hints l,
     // updated), for lattice sums.  A value in the wave's interval lu or one of its
@@ -345,20 +323,19 @@ struct PotView {
             }
         }
     }
-    // One value outside its wave's window: its own hint, the O(1) index, the
-    // bisection.
+    // One value outside its wave's window: the O(1) index (its two levels in
+    // LDS when the kernel staged them), then, at an index edge or a table end, the
+    // bisection.  The value's own hint is not tried: where values leave their
+    // wave's window the grid is dense and a neighbouring site's interval rarely
+    // holds them, and each try costs a dependent record load.
     template <bool WANT_V>
     __device__ __forceinline__ void lookup_value(double v, int &l, double &V, double &dv, int *err) const
     {
-        int lq = l;
+        int lq = tab_candidate(P.T, v);
         TabRec r = P.T.rec[lq];
         if (!tab_in(r, v)) {
-            lq = tab_candidate(P.T, v);
+            lq = tab_bisect_ref(P.x, P.n, v, err);
             r = P.T.rec[lq];
-            if (!tab_in(r, v)) {
-                lq = tab_bisect_ref(P.x, P.n, v, err);
-                r = P.T.rec[lq];
-            }
         }
         l = lq;
         dv = r.ydv + (v - r.xlo) * r.sdv;
@@ -414,6 +391,36 @@ struct PotView {
 };
 
 __device__ __forceinline__ PotView pot_view(const Pot &P) { return PotView{P}; }
+
+// Dynamic LDS of the tabulated march launches: the two index levels of the run's
+// table (for the inputs/ table: 35 KB).  (A copy of the interval records, 55 KB
+// more, measured no faster than their reads from L2.)
+constexpr unsigned kTabIndexLds = 48 * 1024;
+
+__host__ __device__ __forceinline__ unsigned tab_index_bytes(const TabIndex &T)
+{
+    return (unsigned)(T.k1 * sizeof(TabBucket) + T.nsub * sizeof(TabSub));
+}
+
+// Copies the index levels of Pv's table into `lds` (every thread of the
+// workgroup takes part) when they fit in `budget` bytes, and points Pv at the
+// copy; the caller's next barrier publishes it.
+__device__ __forceinline__ void stage_tab_index(PotView &Pv, char *lds, unsigned nthreads, unsigned budget)
+{
+    static_assert(sizeof(TabBucket) == 8 && sizeof(TabSub) == 16, "index record sizes");
+    const unsigned ib = tab_index_bytes(Pv.P.T);
+    if (!Pv.P.numerical || ib > budget) return;
+    // l1 (k1 a power of two >= 16, so l2 starts 16-byte aligned)
+    const unsigned k1 = (unsigned)Pv.P.T.k1, nsub = (unsigned)Pv.P.T.nsub;
+    const uint2 *s1 = (const uint2 *)Pv.P.T.l1;
+    const uint4 *s2 = (const uint4 *)Pv.P.T.l2;
+    uint2 *d1 = (uint2 *)lds;
+    uint4 *d2 = (uint4 *)(lds + k1 * sizeof(TabBucket));
+    for (unsigned q = threadIdx.x; q < k1; q += nthreads) d1[q] = s1[q];
+    for (unsigned q = threadIdx.x; q < nsub; q += nthreads) d2[q] = s2[q];
+    Pv.P.T.l1 = (const TabBucket *)d1;
+    Pv.P.T.l2 = (const TabSub *)d2;
+}
 
 // ---------------------------------------------------------------- launchers
 // (ie_lattice.hip)

@@ -62,6 +62,7 @@ struct TabIndex {
     const TabSub *l2;       // sub-buckets
     const double *x;        // the grid (bisection)
     int n, k1;
+    int nsub;               // sub-buckets in l2
     double x0, inv_w;       // x[n-1] and k1 / (x[0] - x[n-1])
     double tmax;            // the largest double below k1
 };
@@ -191,7 +192,7 @@ inline TabIndexHost tab_index_build(const double *x, const double *v, const doub
 inline TabIndex tab_view(const TabIndexHost &H, const TabRec *rec, const TabBucket *l1, const TabSub *l2,
                          const double *x, int n)
 {
-    return TabIndex{rec, l1, l2, x, n, H.k1, H.x0, H.inv_w, H.tmax};
+    return TabIndex{rec, l1, l2, x, n, H.k1, (int)H.l2.size(), H.x0, H.inv_w, H.tmax};
 }
 
 }  // namespace ie
