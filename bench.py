@@ -66,10 +66,13 @@ def bench_case(N: int, precision: int, sigw: int, steps: int, warmup: int, profi
     return secs, prof, tuning
 
 
-def tabulated_case(N: int = 128, steps: int = 1000, warmup: int = 50) -> dict:
+def tabulated_case(N: int = 128, steps: int = 1000, warmup: int = 50, dense_from: int = 5500) -> dict:
     """Split against FP64 storage with the tabulated SR -> USR -> SR potential
     (inputs/ tables, linear interpolation), N^3, the USR preset's initial field:
-    loop steps per second of each, after the tuner and an untimed warmup."""
+    loop steps per second of each, after the tuner and an untimed warmup; then
+    `steps` more from loop step `dense_from`, where the field has reached the
+    dense stretch of the grid (phi near 2.683: spacing 3.6e-6 against a spread of
+    2e-5, so a wave's sites lie in tens of intervals)."""
     import numpy as np
 
     import ie_engine as E
@@ -88,10 +91,17 @@ def tabulated_case(N: int = 128, steps: int = 1000, warmup: int = 50) -> dict:
         secs, bg = time_steps(run, steps, warmup)
         if bg["done"]:
             raise RuntimeError(f"unexpected loop state {bg}")
-        run.close()
         out[f"{name}_ms_per_step"] = secs / steps * 1e3
         out[f"{name}_site_updates_per_s"] = N ** 3 * steps / secs
+        run.leapfrog_steps(max(0, dense_from - bg["steps"]))
+        secs, bg = time_steps(run, steps, 0)
+        if bg["done"]:
+            raise RuntimeError(f"unexpected loop state {bg}")
+        out[f"{name}_dense_ms_per_step"] = secs / steps * 1e3
+        run.close()
     out["split_over_fp64"] = out["split_site_updates_per_s"] / out["fp64_site_updates_per_s"]
+    out["dense_from_step"] = dense_from
+    out["dense_split_over_fp64"] = out["fp64_dense_ms_per_step"] / out["split_dense_ms_per_step"]
     # the hilltop split step at the same N and dt: the cost of the table lookups
     run = make_run(N, E.IE_SPLIT, 0)
     run.tune(8)
