@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -973,28 +974,41 @@ int ie_spectrum_numbins(int N) { return (int)(1.73205 * (N / 2)) + 1; }
 ie_status ie_modes(int N, double L, double B, double *out)
 {
     if (!is_pow2(N) || !out) return fail(nullptr, IE_ERR_ARG, "bad argument");
-    // get_modes (src/output.cpp:674-738): closed form per shell, host side
-    const double pi = 2.0 * asin(1.0);
     const int nb = (int)(sqrt(3.0) * (N / 2)) + 1;
-    std::vector<int> np(nb, 0);
-    std::vector<double> pp(nb, 0.0);
-    auto shell = [&](int px, int py, int pz, int w) {
-        const double pm = sqrt((double)px * px + (double)py * py + (double)pz * pz);
-        const double sx = sin((double)px * pi / N), sy = sin((double)py * pi / N), sz = sin((double)pz * pi / N);
-        const double ph = sqrt(4.0 * (((double)N / L) * ((double)N / L)) * (sx * sx + sy * sy + sz * sz));
-        np[(int)pm] += w;
-        pp[(int)pm] += w == 2 ? 2.0 * ph : ph;
-    };
-    for (int i = 0; i < N; ++i) {
-        const int px = i <= N / 2 ? i : i - N;
-        for (int j = 0; j < N; ++j) {
-            const int py = j <= N / 2 ? j : j - N;
-            for (int k = 1; k < N / 2; ++k) shell(px, py, k, 2);   // weight-2 interior modes
-            shell(px, py, 0, 1);                                  // kz = 0
-            shell(px, py, N / 2, 1);                              // kz = N/2
+    // the result depends on (N, L, B) alone: computed once and kept (a pass over
+    // the N^2 (N/2 + 1) modes takes about 0.1 s at N = 256)
+    static std::mutex mu;
+    static std::map<std::tuple<int, double, double>, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    std::vector<double> &res = cache[std::make_tuple(N, L, B)];
+    if (res.empty()) {
+        // get_modes (src/output.cpp:674-738): closed form per shell, host side, in
+        // the reference's mode order; sin(p pi / N) from a table of the same values
+        const double pi = 2.0 * asin(1.0);
+        std::vector<double> sn(N + 1);
+        for (int p = -N / 2; p <= N / 2; ++p) sn[p + N / 2] = sin((double)p * pi / N);
+        std::vector<int> np(nb, 0);
+        std::vector<double> pp(nb, 0.0);
+        auto shell = [&](int px, int py, int pz, int w) {
+            const double pm = sqrt((double)px * px + (double)py * py + (double)pz * pz);
+            const double sx = sn[px + N / 2], sy = sn[py + N / 2], sz = sn[pz + N / 2];
+            const double ph = sqrt(4.0 * (((double)N / L) * ((double)N / L)) * (sx * sx + sy * sy + sz * sz));
+            np[(int)pm] += w;
+            pp[(int)pm] += w == 2 ? 2.0 * ph : ph;
+        };
+        for (int i = 0; i < N; ++i) {
+            const int px = i <= N / 2 ? i : i - N;
+            for (int j = 0; j < N; ++j) {
+                const int py = j <= N / 2 ? j : j - N;
+                for (int k = 1; k < N / 2; ++k) shell(px, py, k, 2);   // weight-2 interior modes
+                shell(px, py, 0, 1);                                  // kz = 0
+                shell(px, py, N / 2, 1);                              // kz = N/2
+            }
         }
+        res.resize(nb);
+        for (int b = 0; b < nb; ++b) res[b] = B * (np[b] > 0 ? pp[b] / np[b] : pp[b]);
     }
-    for (int b = 0; b < nb; ++b) out[b] = B * (np[b] > 0 ? pp[b] / np[b] : pp[b]);
+    memcpy(out, res.data(), sizeof(double) * (size_t)nb);
     return IE_OK;
 }
 
@@ -1027,7 +1041,7 @@ ie_status ie_spectrum(ie_run *r, int which, double norm1, double *p, int *counts
     } else {
         return fail(r, IE_ERR_ARG, "bad field");
     }
-    fft_counts(r->N, counts);
+    memcpy(counts, fft_counts_cached(r->N, 0), sizeof(int) * (size_t)nb);
     const double pi = 2.0 * asin(1.0), dp = 2.0 * pi / r->cfg.L;
     for (int b = 0; b < nb; ++b) {
         p[b] = dp * b;
@@ -1054,7 +1068,7 @@ ie_status ie_gw_spectrum(ie_run *r, int hdot, double *p, int *counts, double *va
     const int nb = fft_gw_numbins(r->N);
     std::vector<double> sums(nb);
     IE_HIP(r, fft_bin_tt(r->fft, r->dx, to_phys, sums.data(), r->stream));
-    fft_gw_counts(r->N, counts);
+    memcpy(counts, fft_counts_cached(r->N, 1), sizeof(int) * (size_t)nb);
     // output units follow src/output.cpp:475,618-621 (float p and norm1)
     const float dp = 2.f * (float)(2.0 * asin(1.0)) / (float)r->cfg.L;
     const float norm1 = 0.5f * pow(r->cfg.L / r->cfg.rescale_B, 3) / pow((float)r->N, 6);

@@ -483,7 +483,8 @@ __global__ void k_phiref_final(const double *__restrict__ f, const double *__res
 //     kind (it leaves the binade, is a tie, or has v < 0 or v >= B), which it
 //     takes as a float step, then goes on in the new binade;
 //   - while acc is 0, negative or not finite, excluded sites (v > cutoff or NaN)
-//     leave it alone and every included one is a float step.
+//     leave it alone, so do zero sites while acc is a zero of their sign or +0,
+//     and every other included one is a float step.
 // The result is the serial walk's bit for bit.
 constexpr int kMeanThreads = 1024, kMeanPer = 8, kMeanChunk = kMeanThreads * kMeanPer;
 
@@ -533,7 +534,11 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
                     const double v = dn[id];
                     if (v <= cutoff) {
                         if (!regular || !(v >= 0.0) || !(v < B)) {
-                            bad[q] = true;
+                            // a zero added to a zero accumulator leaves it as it
+                            // is (+0 + 0 = +0, -0 + -0 = -0): no step, so that a
+                            // long run of zero sites at the head of the map is not
+                            // walked one float step at a time
+                            bad[q] = !(acc == 0.0f && v == 0.0 && !(signbit(acc) && !signbit(v)));
                         } else {
                             const double qv = ((B + v) - B) / u;
                             const double fl = floor(qv);

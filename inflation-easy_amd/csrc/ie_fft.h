@@ -64,5 +64,8 @@ hipError_t fft_inverse_slot(FftPlan *p, int slot, double add, double *out, hipSt
 // Reference mode counts per shell (closed form, host).
 void fft_counts(int N, int *counts);
 void fft_gw_counts(int N, int *counts);
+// The same, computed once per N and kind (gw 0: fft_counts, 1: fft_gw_counts)
+// and kept for the process.
+const int *fft_counts_cached(int N, int gw);
 
 }  // namespace ie

@@ -20,7 +20,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "ie_fft.h"
@@ -900,6 +903,23 @@ void fft_gw_counts(int N, int *counts)
             }
         }
     }
+}
+
+// Shell counts depend on N alone (a triple loop over the N^2 (N/2 + 1) modes, 25 ms
+// at 256^3 on the host): computed once per N and kind and kept, under a lock so
+// that runs stepped from several threads share them.
+const int *fft_counts_cached(int N, int gw)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::vector<int>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    std::vector<int> &v = cache[{N, gw ? 1 : 0}];
+    if (v.empty()) {
+        v.resize((size_t)(gw ? fft_gw_numbins(N) : fft_numbins(N)));
+        if (gw) fft_gw_counts(N, v.data());
+        else fft_counts(N, v.data());
+    }
+    return v.data();
 }
 
 }  // namespace ie

@@ -8,7 +8,8 @@ and every other step as a float step; ie_deltaN_zeta's mean must equal the
 oracle's serial walk (ieo_float_mean) bit for bit, and the map must come back as
 deltaN - mean.  Maps: uniform and lognormal values with some sites above the
 cutoff, values on a power-of-two grid (ties of the rounding) mixed with negative
-values, zeros, NaN and large jumps, and a run of excluded sites ahead of the rest.
+values, zeros, NaN and large jumps, a run of excluded sites ahead of the rest,
+and a run of zero sites (+0 and -0) ahead of the rest.
 """
 from __future__ import annotations
 
@@ -33,6 +34,12 @@ def maps(gs, rng):
     h = rng.uniform(0.0, 1e-3, gs)
     h[: gs // 2] = 5.0
     yield "excluded head", h, 1e-3
+    # sites finished before the first step (deltaN = 0) ahead of the rest: a zero
+    # accumulator takes them without a step, -0 included
+    z = rng.uniform(0.0, 1e-3, gs)
+    z[: gs // 2] = 0.0
+    z[1: gs // 2: 7] = -0.0
+    yield "zero head", z, 1e-3
 
 
 @pytest.mark.parametrize("N", [64, 128, 256])
