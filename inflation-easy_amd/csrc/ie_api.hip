@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -1223,9 +1224,11 @@ ie_status ie_initialize(ie_run *r, const ie_ic_params *ic, ie_background *out)
 {
     if (!r || !ic) return fail(r, IE_ERR_ARG, "null argument");
     const size_t ny = (size_t)r->N * 2 * r->N;
-    std::vector<double> fk(r->gs), fdk(r->gs), fny(ny), fdny(ny);
+    // left uninitialised: ie_fill_modes zeroes them (in parallel)
+    std::unique_ptr<double[]> fk(new double[r->gs]), fdk(new double[r->gs]), fny(new double[ny]),
+        fdny(new double[ny]);
     double H = 0.0;
-    ie_status st = ie_fill_modes(&r->cfg, ic, fk.data(), fny.data(), fdk.data(), fdny.data(), &H);
+    ie_status st = ie_fill_modes(&r->cfg, ic, fk.get(), fny.get(), fdk.get(), fdny.get(), &H);
     if (st == IE_ERR_COURANT) {
         char buf[160];
         snprintf(buf, sizeof(buf), "Time step too large. Courant condition violated: dt/dx = %f", r->cfg.dt / r->dx);
@@ -1246,7 +1249,7 @@ ie_status ie_initialize(ie_run *r, const ie_ic_params *ic, ie_background *out)
     b.done = 0;
     b.error = 0;
     if ((st = push_bg(r)) != IE_OK) return st;
-    st = ie_upload_modes(r, fk.data(), fny.data(), fdk.data(), fdny.data(), ic->initial_field,
+    st = ie_upload_modes(r, fk.get(), fny.get(), fdk.get(), fdny.get(), ic->initial_field,
                          ic->initial_derivative);
     if (st != IE_OK) return st;
     // H0 from the lattice energies (src/initialize.cpp:202-212)
