@@ -555,11 +555,9 @@ ie_status ie_download_tensor(ie_run *r, int comp, float *out)
     return sync(r);
 }
 
-ie_status ie_get_background(ie_run *r, ie_background *out)
+// The host copy of the background as an ie_background (no read-back).
+static void background_out(const ie_run *r, ie_background *out)
 {
-    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
-    ie_status st = pull_bg(r);
-    if (st != IE_OK) return st;
     const DevBg &b = *r->bg_host;
     out->t = b.t;
     out->a = b.a;
@@ -573,6 +571,14 @@ ie_status ie_get_background(ie_run *r, ie_background *out)
     out->steps = b.steps;
     out->done = b.done;
     out->error = b.error ? IE_ERR_TABLE_RANGE : IE_OK;
+}
+
+ie_status ie_get_background(ie_run *r, ie_background *out)
+{
+    if (!r || !out) return fail(r, IE_ERR_ARG, "null argument");
+    ie_status st = pull_bg(r);
+    if (st != IE_OK) return st;
+    background_out(r, out);
     return IE_OK;
 }
 
@@ -716,6 +722,7 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
     // finished loop return at once, but each launch still costs its dispatch);
     // a loop in chunks gives the same bits as one loop
     constexpr int kChunk = 1024;
+    bool fresh = false;   // bg_host read back after the last launch
     for (int done = 0; done < max_steps;) {
         const int n_chunk = std::min(kChunk, max_steps - done);
         k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
@@ -736,15 +743,19 @@ ie_status ie_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
         done += n_chunk;
         ie_status st = check_kernel_error(r);
         if (st != IE_OK) return st;
+        fresh = true;
         if (r->bg_host->done) break;
     }
     if (max_steps == 0) {
         k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
         IE_HIP(r, hipGetLastError());
+        fresh = false;
     }
-    ie_status st = check_kernel_error(r);
-    if (st != IE_OK) return st;
-    if (out) return ie_get_background(r, out);
+    if (!fresh) {
+        ie_status st = check_kernel_error(r);
+        if (st != IE_OK) return st;
+    }
+    if (out) background_out(r, out);
     return IE_OK;
 }
 
@@ -1731,7 +1742,7 @@ ie_status ie_post_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
     }
     b.done = !(b.a <= r->post.af) ? 1 : 0;
     if ((st = push_bg(r)) != IE_OK) return st;
-    if (out) return ie_get_background(r, out);
+    if (out) background_out(r, out);   // the host copy just written
     return IE_OK;
 }
 
