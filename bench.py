@@ -506,7 +506,7 @@ def post_inflation(N: int = 256, steps: int = 200, warmup: int = 10, precision: 
     alg = 144 * N ** 3
     return {"N": N, "steps": steps, "ms_per_step": ms, "site_updates_per_s": N ** 3 * steps / secs,
             "algorithmic_bytes_per_step": alg, "achieved_GBps": alg / (ms * 1e-3) / 1e9,
-            "kernels": "k_post_kick_rows (24 B/site) + k_post_tensor_rows (120 B/site)",
+            "kernels": "k_post_kick_rows (24 B/site) + k_post_tensor_march (120 B/site; k_post_tensor_rows from N = 512)",
             "storage": "FP64 Phi, Phi'; FP32 h, hdot"}
 
 

@@ -214,7 +214,9 @@ ie_status ie_tune(ie_run *run, int steps_per_candidate, ie_tuning *out);
  * (0: the default row kernel), march_variant 0 or 1.  The run keeps them and
  * ie_leapfrog_steps no longer tunes it; the process-wide choice is unchanged.  A
  * shape beyond the run's lattice (E above what a row holds, E = 4 march with FP64
- * storage) is IE_ERR_ARG. */
+ * storage) is IE_ERR_ARG.  In the post-inflation phase a pinned row tensor code
+ * (1, 2, 4) keeps that phase's row tensor kernel, which otherwise serves only
+ * N >= 512 (N = 64 .. 256 take its plane-marching kernel). */
 ie_status ie_set_shapes(ie_run *run, int kick_elems, int tensor_code, int march_variant);
 /* rk4_step_inflation / rk45_step_inflation (src/evolution.cpp:760-904), FP64
  * storage only (IE_ERR_STATE for split storage) in the inflation phase; in the

@@ -1704,6 +1704,15 @@ ie_status ie_post_init(ie_run *r, const ie_post_params *p, ie_background *out)
     return IE_OK;
 }
 
+
+// The post-inflation tensor step runs as the plane-marching kernel where the
+// lattice allows it, unless the run's pinned tensor shape is a row kernel code
+// (1, 2, 4), which keeps the row kernel (the path of N >= 512 lattices).
+static bool post_tensor_rows(const ie_run *r)
+{
+    return r->pinned && r->F.tensor_e > 0 && r->F.tensor_e <= 4;
+}
+
 ie_status ie_post_evolve_derivs(ie_run *r, double d)
 {
     if (!r || !(d != 0.0)) return fail(r, IE_ERR_ARG, "bad step");
@@ -1714,7 +1723,7 @@ ie_status ie_post_evolve_derivs(ie_run *r, double d)
     if ((st = post_state(r, &S)) != IE_OK) return st;
     DevBg &b = *r->bg_host;
     const PostStep P = post_kick_scalars(r, b, d, 0.0);
-    IE_HIP(r, post_step(S, r->N, r->logN, P, r->stream));
+    IE_HIP(r, post_step(S, r->N, r->logN, P, r->stream, post_tensor_rows(r)));
     b.ad += 0.5 * d * b.ad2;
     post_flip(r);
     return push_bg(r);
@@ -1732,7 +1741,7 @@ ie_status ie_post_leapfrog_steps(ie_run *r, int max_steps, ie_background *out)
         PostState S;
         if ((st = post_state(r, &S)) != IE_OK) return st;
         const PostStep P = post_kick_scalars(r, b, d, d);
-        IE_HIP(r, post_step(S, r->N, r->logN, P, r->stream));
+        IE_HIP(r, post_step(S, r->N, r->logN, P, r->stream, post_tensor_rows(r)));
         b.ad += 0.5 * d * b.ad2;
         b.t += d;
         b.a += d * b.ad;

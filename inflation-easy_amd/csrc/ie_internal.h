@@ -675,46 +675,6 @@ __device__ __forceinline__ double first_lane(double x)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// The value of lane l of the wave (l a compile-time constant after unrolling),
-// in every lane.
-__device__ __forceinline__ double lane_value(double x, int l)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Serial sums in lattice order on one wave: the lanes load 8 x 64 consecutive
-// values (the next 512 in flight while these are added) and the running sum,
-// kept the same in every lane, takes them lane by lane: the dependent adds are
-// the only serial part.  step(acc, v) applies one element.
-constexpr int kSerialBatch = 8;
-template <typename Acc, typename Step>
-__device__ __forceinline__ Acc serial_walk(const double *__restrict__ g, size_t gs, Acc acc, Step step)
-{
-    const unsigned lane = threadIdx.x & 63;
-    const size_t chunk = 64 * kSerialBatch, full = gs / chunk * chunk;
-    double a[kSerialBatch];
-    if (full)
-#pragma unroll
-        for (int q = 0; q < kSerialBatch; ++q) a[q] = g[q * 64 + lane];
-    for (size_t base = 0; base < full; base += chunk) {
-        const size_t next = base + chunk < full ? base + chunk : base;
-        double na[kSerialBatch];
-#pragma unroll
-        for (int q = 0; q < kSerialBatch; ++q) na[q] = g[next + q * 64 + lane];
-#pragma unroll
-        for (int q = 0; q < kSerialBatch; ++q)
-#pragma unroll
-            for (int l = 0; l < 64; ++l) acc = step(acc, lane_value(a[q], l));
-#pragma unroll
-        for (int q = 0; q < kSerialBatch; ++q) a[q] = na[q];
-    }
-    for (size_t id = full; id < gs; ++id) acc = step(acc, g[id]);
-    return acc;
-}
-
 // k +- 1 neighbours of the lane's E sites of row `c` (one segment of the row at
 // `row`): in-lane, then across lanes, then across segments for N > 64 E.
 template <typename T, int E>
@@ -790,7 +750,8 @@ __device__ __forceinline__ void post_source(const SiteStencil &q, const PostStep
 
 int post_row_elems(int N);
 unsigned post_row_blocks(int N, int E, int PW, unsigned Rmax);
-hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s);
+// rows: the row tensor kernel even where the tensor march takes the lattice
+hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s, bool rows = false);
 hipError_t post_drift(const PostState &S, size_t gs, double d, hipStream_t s);
 // serial sums of g1 (and g2 when non-null) in lattice order into out[0], out[1]
 hipError_t serial_sums(const double *g1, const double *g2, size_t gs, double *out, hipStream_t s);

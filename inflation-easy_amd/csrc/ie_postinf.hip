@@ -77,11 +77,9 @@ __device__ __forceinline__ unsigned sidx(unsigned i, unsigned j, unsigned k, uns
 // at about the same time (L2 hits): per plane the group reads its planes plus
 // two, per row walk R + 2 rows.  Measured at 256^3: the kick with 8 planes and
 // walks of 32 rows moves 1.04x its algorithmic bytes (111 us; 1.17x and 123 us
-// with 4 planes of 8 rows); the tensor kernel (158 VGPRs, so at most 12 waves
-// per CU) keeps 4 planes of 8 rows: with 8 planes its traffic fell from 1.59x
-// to 1.28x but a CU then held one 8-wave workgroup and the step took 0.82
-// against 0.70 ms, and walks of 32 rows left too few workgroups for the last
-// round of the grid.
+// with 4 planes of 8 rows).  The row tensor kernel (N >= 512; the tensor march
+// below takes N = 64 .. 256) moved 1.59x its bytes at 256^3 with 4 planes of 8
+// rows (680 us, against 470 us and 1.07x for the march).
 constexpr int kPostKickPlanes = 8, kPostTensorPlanes = 4;
 constexpr unsigned kPostKickRows = 32, kPostTensorRows = 8;   // longest walks
 
@@ -366,6 +364,189 @@ __global__ __launch_bounds__(64 * kPostTensorPlanes) void k_post_tensor_rows(Pos
     }
 }
 
+// ------------------------------------------------------ tensor march kernel
+//
+// The tensor step for N = 64, 128, 256 (one 64E-site segment per row, E = N / 64):
+// a workgroup of 8 waves owns rows j0 .. j0 + 7 and marches along i over TI
+// planes of its XCD's slab.  Each wave keeps the i-1, i, i+1 rows of Phi, Phi'
+// and the six h_ij in registers (the x faces, and the k faces by lane shuffles)
+// and puts its rows in LDS, where the neighbouring waves read their j +- 1 rows:
+// Phi for planes i-1, i, i+1 (the xy Hessian entry reads Phi(i +- 1, j +- 1)),
+// Phi' and h_ij for plane i.  Waves 0 and 7 also bring the tile's halo rows
+// j0 - 1 and j0 + 8.  Every row comes from HBM about once (the halo rows are the
+// neighbouring tiles' own rows, read at about the same time on the same XCD), so
+// the step moves about its 120 B/site, against the row walk's re-reads of the
+// i +- 1 planes.  The LDS holds one plane per field (Phi: a ring of three), so a
+// plane step has two barriers: after the writes, and after the reads.  The site
+// arithmetic is the row kernels' (post_source, post_tensor_update), so the values
+// are the same bits.
+constexpr int kPostMarchWaves = 8;
+
+template <int E>
+struct PostTile {
+    double f[3][kPostMarchWaves + 2][64 * E];   // Phi rows of planes i-1, i, i+1 (slot: plane mod 3)
+    double fd[kPostMarchWaves + 2][64 * E];     // Phi' rows of plane i
+    float h[6][kPostMarchWaves + 2][64 * E];    // h_ij rows of plane i
+};
+
+template <typename T, int E>
+__device__ __forceinline__ void lds_row_put(T *row, unsigned lane, const T (&v)[E])
+{
+#pragma unroll
+    for (int e = 0; e < E; ++e) row[lane * E + e] = v[e];
+}
+
+// The values past the lane's ends of a whole row held E per lane: k = 64E wraps
+// to k = 0 (lane 63 to lane 0).
+template <typename T, int E>
+__device__ __forceinline__ void k_ends(const T (&c)[E], unsigned lane, T &nxt, T &prv)
+{
+    nxt = __shfl(c[0], (int)((lane + 1) & 63));
+    prv = __shfl(c[E - 1], (int)((lane + 63) & 63));
+}
+
+template <int E>
+__global__ __launch_bounds__(64 * kPostMarchWaves) void k_post_tensor_march(PostTensorArgs T, unsigned N,
+                                                                             unsigned logN, unsigned TI, PostStep P)
+{
+    constexpr unsigned W = kPostMarchWaves, S = 64u * E;
+    __shared__ PostTile<E> L;
+    const unsigned m = N - 1, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned jblocks = N / W, xcd = blockIdx.x % kXcd, lb = blockIdx.x / kXcd;
+    const unsigned j0 = (lb % jblocks) * W, i0 = xcd * (N / kXcd) + (lb / jblocks) * TI;
+    const unsigned rj = (j0 + w) << logN, P2 = 2 * logN;
+    // waves 0 and 7 also carry the halo rows j0 - 1 and j0 + 8 (LDS rows 0 and 9)
+    const bool halo = w == 0 || w == W - 1;
+    const unsigned rh = ((w == 0 ? j0 - 1 : j0 + W) & m) << logN;
+    const unsigned sh = w == 0 ? 0u : W + 1;
+    const double *__restrict__ f = T.f;
+    const double *__restrict__ fd = T.fd;
+    auto pl = [&](unsigned i) { return (i & m) << P2; };
+
+    double fdn[E], fcur[E];   // Phi rows i-1, i
+    double ddn[E], dcur[E];   // Phi' rows i-1, i
+    float hdn[6][E], hcur[6][E];
+    {
+        const unsigned pm = pl(i0 - 1), p0 = pl(i0);
+        ld_row<double, E>(f + pm + rj, lane, fdn);
+        ld_row<double, E>(f + p0 + rj, lane, fcur);
+        ld_row<double, E>(fd + pm + rj, lane, ddn);
+        ld_row<double, E>(fd + p0 + rj, lane, dcur);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            ld_row<float, E>(T.h[c] + pm + rj, lane, hdn[c]);
+            ld_row<float, E>(T.h[c] + p0 + rj, lane, hcur[c]);
+        }
+        // Phi planes i0 - 1 and i0 into their ring slots (0 and 1)
+        lds_row_put<double, E>(L.f[0][w + 1], lane, fdn);
+        lds_row_put<double, E>(L.f[1][w + 1], lane, fcur);
+        if (halo) {
+            double t0[E], t1[E];
+            ld_row<double, E>(f + pm + rh, lane, t0);
+            ld_row<double, E>(f + p0 + rh, lane, t1);
+            lds_row_put<double, E>(L.f[0][sh], lane, t0);
+            lds_row_put<double, E>(L.f[1][sh], lane, t1);
+        }
+    }
+    for (unsigned t = 0; t < TI; ++t) {
+        const unsigned p = i0 + t, pi = pl(p), pu = pl(p + 1);
+        const unsigned sm = t % 3, s0 = (t + 1) % 3, sp = (t + 2) % 3;
+        // this plane's loads: the wave's Phi and Phi' rows of plane i+1; the
+        // halo waves' rows of Phi (plane i+1), Phi' and h (plane i) for the LDS
+        double fup[E], dup[E];
+        ld_row<double, E>(f + pu + rj, lane, fup);
+        ld_row<double, E>(fd + pu + rj, lane, dup);
+        lds_row_put<double, E>(L.fd[w + 1], lane, dcur);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lds_row_put<float, E>(L.h[c][w + 1], lane, hcur[c]);
+        if (halo) {
+            double fh[E], dh[E];
+            float hh[6][E];
+            ld_row<double, E>(f + pu + rh, lane, fh);
+            ld_row<double, E>(fd + pi + rh, lane, dh);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ld_row<float, E>(T.h[c] + pi + rh, lane, hh[c]);
+            lds_row_put<double, E>(L.f[sp][sh], lane, fh);
+            lds_row_put<double, E>(L.fd[sh], lane, dh);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) lds_row_put<float, E>(L.h[c][sh], lane, hh[c]);
+        }
+        lds_row_put<double, E>(L.f[sp][w + 1], lane, fup);
+        __syncthreads();
+        double Sv[E][6], fnew[E];
+        double c_n, c_p, u_n, u_p, d_n, d_p, v_n, v_p;
+        k_ends<double, E>(fcur, lane, c_n, c_p);
+        k_ends<double, E>(fup, lane, u_n, u_p);
+        k_ends<double, E>(fdn, lane, d_n, d_p);
+        k_ends<double, E>(dcur, lane, v_n, v_p);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const unsigned x = lane * E + e, xp = (x + 1) & (S - 1), xm = (x - 1) & (S - 1);
+            SiteStencil q;
+            q.c = fcur[e];
+            q.fxp = fup[e];
+            q.fxm = fdn[e];
+            q.fyp = L.f[s0][w + 2][x];
+            q.fym = L.f[s0][w][x];
+            q.fzp = kplus<double, E>(fcur, e, c_n);
+            q.fzm = kminus<double, E>(fcur, e, c_p);
+            q.fxpyp = L.f[sp][w + 2][x];
+            q.fxpym = L.f[sp][w][x];
+            q.fxmyp = L.f[sm][w + 2][x];
+            q.fxmym = L.f[sm][w][x];
+            q.fxpzp = kplus<double, E>(fup, e, u_n);
+            q.fxpzm = kminus<double, E>(fup, e, u_p);
+            q.fxmzp = kplus<double, E>(fdn, e, d_n);
+            q.fxmzm = kminus<double, E>(fdn, e, d_p);
+            q.fypzp = L.f[s0][w + 2][xp];
+            q.fymzm = L.f[s0][w][xm];
+            q.dc = dcur[e];
+            q.dxp = dup[e];
+            q.dxm = ddn[e];
+            q.dyp = L.fd[w + 2][x];
+            q.dym = L.fd[w][x];
+            q.dzp = kplus<double, E>(dcur, e, v_n);
+            q.dzm = kminus<double, E>(dcur, e, v_p);
+            post_source(q, P, Sv[e]);
+            fnew[e] = q.c + P.dd * q.dc;
+        }
+        st_row<double, E>(T.f_out + pi + rj, lane, fnew);
+        float hup[6][E];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            float hv[E], hdnew[E], hn[E];
+            ld_row<float, E>(T.h[c] + pu + rj, lane, hup[c]);
+            ld_row<float, E>(T.hd[c] + pi + rj, lane, hv);
+            float h_n, h_p;
+            k_ends<float, E>(hcur[c], lane, h_n, h_p);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const unsigned x = lane * E + e;
+                const float hl = kplus<float, E>(hcur[c], e, h_n) + kminus<float, E>(hcur[c], e, h_p) +
+                                 L.h[c][w + 2][x] + L.h[c][w][x] + hup[c][e] + hdn[c][e] - 6.0f * hcur[c][e];
+                post_tensor_update(hl, hcur[c][e], hv[e], Sv[e][c], P, &hdnew[e], &hn[e]);
+            }
+            st_row<float, E>(T.hd[c] + pi + rj, lane, hdnew);
+            st_row<float, E>(T.h_out[c] + pi + rj, lane, hn);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            fdn[e] = fcur[e];
+            fcur[e] = fup[e];
+            ddn[e] = dcur[e];
+            dcur[e] = dup[e];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                hdn[c][e] = hcur[c][e];
+                hcur[c][e] = hup[c][e];
+            }
+    }
+}
+
 // --------------------------------------------------------- drift and set-up
 
 // evolve_fields on the post-inflation state (src/evolution.cpp:260-284)
@@ -381,14 +562,62 @@ __global__ __launch_bounds__(kBlock) void k_post_drift(double *__restrict__ f, c
 
 // The reference's set-up sums walk the lattice in order with one accumulator
 // each (src/initialize.cpp:261-263, 270-276); the order is part of the result, so
-// one wave adds in order (serial_walk).
+// one wave adds in order.  The lanes load blocks of 512 consecutive values (the
+// next block in flight while one is added) and put them in LDS, from where every
+// lane reads the same two values at a time (an LDS broadcast); the running sums,
+// the same in every lane, take them in order, so the dependent FP64 adds are the
+// only serial part (no lane moves between them).  With g2 the two sums' chains
+// are interleaved.
+constexpr int kSerialBlock = 512;
+
+template <bool TWO>
 __global__ __launch_bounds__(64) void k_serial_sums(const double *__restrict__ g1, const double *__restrict__ g2,
                                                      size_t gs, double *out)
 {
-    auto add = [](double acc, double v) { return acc + v; };
-    const double s1 = serial_walk(g1, gs, 0.0, add);
-    const double s2 = g2 ? serial_walk(g2, gs, 0.0, add) : 0.0;
-    if (threadIdx.x == 0) {
+    constexpr int B = kSerialBlock / 64;
+    __shared__ __align__(16) double L1[kSerialBlock];
+    __shared__ __align__(16) double L2[TWO ? kSerialBlock : 2];
+    const unsigned lane = threadIdx.x;
+    const size_t full = gs / kSerialBlock * kSerialBlock;
+    double s1 = 0.0, s2 = 0.0, a[B], b[B];
+    if (full) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            a[q] = g1[q * 64 + lane];
+            if constexpr (TWO) b[q] = g2[q * 64 + lane];
+        }
+    }
+    for (size_t base = 0; base < full; base += kSerialBlock) {
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            L1[q * 64 + lane] = a[q];
+            if constexpr (TWO) L2[q * 64 + lane] = b[q];
+        }
+        const size_t nx = base + kSerialBlock < full ? base + kSerialBlock : base;
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            a[q] = g1[nx + q * 64 + lane];
+            if constexpr (TWO) b[q] = g2[nx + q * 64 + lane];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSerialBlock; k += 2) {
+            const double2 v = *reinterpret_cast<const double2 *>(&L1[k]);
+            s1 = s1 + v.x;
+            s1 = s1 + v.y;
+            if constexpr (TWO) {
+                const double2 u = *reinterpret_cast<const double2 *>(&L2[k]);
+                s2 = s2 + u.x;
+                s2 = s2 + u.y;
+            }
+        }
+        __syncthreads();
+    }
+    for (size_t id = full; id < gs; ++id) {
+        s1 = s1 + g1[id];
+        if constexpr (TWO) s2 = s2 + g2[id];
+    }
+    if (lane == 0) {
         out[0] = s1;
         out[1] = s2;
     }
@@ -461,7 +690,24 @@ unsigned post_row_blocks(int N, int E, int PW, unsigned Rmax)
     return (unsigned)((size_t)N * N * segs / ((size_t)PW * post_row_walk(N, E, PW, Rmax)));
 }
 
-hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s)
+// The tensor march takes lattices whose rows are one segment of E = N / 64 sites
+// per lane (N = 64, 128, 256).  Planes per tile: the XCD's whole slab, halved
+// until the grid has 256 workgroups (one per CU at N = 256).
+int post_march_elems(int N) { return (N == 64 || N == 128 || N == 256) ? N / 64 : 0; }
+
+unsigned post_march_blocks(int N, unsigned TI)
+{
+    return (unsigned)(kXcd * (N / kPostMarchWaves) * ((N / kXcd) / TI));
+}
+
+unsigned post_march_planes(int N)
+{
+    unsigned TI = (unsigned)(N / kXcd);
+    while (TI > 2 && post_march_blocks(N, TI) < 256) TI >>= 1;
+    return TI;
+}
+
+hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hipStream_t s, bool rows)
 {
     PostTensorArgs T;
     T.f = S.f;
@@ -482,7 +728,13 @@ hipError_t post_step(const PostState &S, int N, int logN, const PostStep &P, hip
         constexpr unsigned KR = kPostKickRows, TR = kPostTensorRows;
         k_post_kick_rows<1><<<post_row_blocks(N, 1, KP, KR), 64 * KP, 0, s>>>(
             S.f, S.fd, S.fd_out, (unsigned)N, (unsigned)logN, post_row_walk(N, 1, KP, KR), P);
-        if (E == 2)
+        const int EM = rows ? 0 : post_march_elems(N);
+        if (EM) {
+            const unsigned TI = post_march_planes(N), G = post_march_blocks(N, TI), nt = 64 * kPostMarchWaves;
+            if (EM == 1) k_post_tensor_march<1><<<G, nt, 0, s>>>(T, (unsigned)N, (unsigned)logN, TI, P);
+            else if (EM == 2) k_post_tensor_march<2><<<G, nt, 0, s>>>(T, (unsigned)N, (unsigned)logN, TI, P);
+            else k_post_tensor_march<4><<<G, nt, 0, s>>>(T, (unsigned)N, (unsigned)logN, TI, P);
+        } else if (E == 2)
             k_post_tensor_rows<2><<<post_row_blocks(N, 2, TP, TR), 64 * TP, 0, s>>>(
                 T, (unsigned)N, (unsigned)logN, post_row_walk(N, 2, TP, TR), P);
         else
@@ -505,7 +757,8 @@ hipError_t post_drift(const PostState &S, size_t gs, double d, hipStream_t s)
 
 hipError_t serial_sums(const double *g1, const double *g2, size_t gs, double *out, hipStream_t s)
 {
-    k_serial_sums<<<1, 64, 0, s>>>(g1, g2, gs, out);
+    if (g2) k_serial_sums<true><<<1, 64, 0, s>>>(g1, g2, gs, out);
+    else k_serial_sums<false><<<1, 64, 0, s>>>(g1, nullptr, gs, out);
     return hipGetLastError();
 }
 

@@ -84,12 +84,18 @@ def start_post(N, precision, from_deltaN=1):
     return st, run
 
 
-# N = 16, 32: site kernels; 64: rows of one segment, 1 site per lane; 128: 2 per
-# lane; 256: 2 per lane over two segments per row
-@pytest.mark.parametrize("N,precision", [(16, E.IE_FP64), (32, E.IE_FP64), (64, E.IE_FP64), (128, E.IE_FP64),
-                                         (256, E.IE_FP64), (16, E.IE_SPLIT), (128, E.IE_SPLIT)])
-def test_post_leapfrog_bitwise(N, precision):
+# N = 16, 32: site kernels; 64, 128, 256: the tensor march (1, 2, 4 sites per
+# lane); rows=True pins a row tensor code, which keeps the row tensor kernel (the
+# path of N >= 512: 2 sites per lane, two segments per row at N = 256)
+@pytest.mark.parametrize("N,precision,rows", [(16, E.IE_FP64, False), (32, E.IE_FP64, False),
+                                              (64, E.IE_FP64, False), (128, E.IE_FP64, False),
+                                              (256, E.IE_FP64, False), (16, E.IE_SPLIT, False),
+                                              (128, E.IE_SPLIT, False), (64, E.IE_FP64, True),
+                                              (256, E.IE_FP64, True)])
+def test_post_leapfrog_bitwise(N, precision, rows):
     st, run = start_post(N, precision)
+    if rows:
+        run.set_shapes(tensor_code=1)
     assert_same_state(st, run, "set-up")
     assert np.abs(st.field(0)).max() > 0
     c = st.cfg
