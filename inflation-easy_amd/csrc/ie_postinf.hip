@@ -560,64 +560,188 @@ __global__ __launch_bounds__(kBlock) void k_post_drift(double *__restrict__ f, c
     }
 }
 
-// The reference's set-up sums walk the lattice in order with one accumulator
-// each (src/initialize.cpp:261-263, 270-276); the order is part of the result, so
-// one wave adds in order.  The lanes load blocks of 512 consecutive values (the
-// next block in flight while one is added) and put them in LDS, from where every
-// lane reads the same two values at a time (an LDS broadcast); the running sums,
-// the same in every lane, take them in order, so the dependent FP64 adds are the
-// only serial part (no lane moves between them).  With g2 the two sums' chains
-// are interleaved.
-constexpr int kSerialBlock = 512;
+// The reference's set-up sums walk the lattice in order with one FP64
+// accumulator each (src/initialize.cpp:261-263, 270-276); the order of the
+// roundings is part of the result, and one workgroup keeps it exactly:
+//   - while acc lies in one binade [2^e, 2^(e+1)) (either sign) it is A u with
+//     u = 2^(e-52) and A an integer, 2^52 <= |A| < 2^53; a step adds v, and with
+//     x = v / u (exact: u is a power of two) the rounded sum is (A + rint(x)) u
+//     as long as x is not a half-integer (a tie, whose rounding depends on the
+//     parity of A) and the exact sum stays in the binade, which holds when
+//     2^52 < |A + rint(x)| < 2^53 - 1;
+//   - so a round of 4096 values (8 waves x 512, 8 consecutive per lane, staged
+//     in LDS) is an integer prefix sum: every wave scans its block, the waves'
+//     totals give each block its offset, and every partial sum is checked;
+//   - at the round's first step that is not of this kind (a tie, out of the
+//     binade, |x| >= 2^51, NaN; or acc 0, subnormal or not finite) wave 0 takes
+//     FP64 steps from the LDS copy (every lane reads the same value: a
+//     broadcast), in runs of 64 to the end of that step's block, and the round
+//     is scanned again from there in the new binade;
+//   - each wave loads its block of the next round while one is summed.
+// The result is the serial walk's bit for bit.  Where the sum seldom leaves its
+// binade a round costs one scan instead of 4096 dependent adds; a map centred
+// on zero changes binade often near the sum's zero crossings and takes about
+// one step in eight serially (256^3).
+constexpr int kOrdWaves = 8, kOrdPer = 8, kOrdBlock = 64 * kOrdPer, kOrdRound = kOrdWaves * kOrdBlock;
+constexpr int kOrdRun = 64;
+
+struct OrdLds {
+    double data[kOrdRound];
+    long long tot[kOrdWaves], kb[kOrdWaves];
+    unsigned first[kOrdWaves];
+    double acc;
+    unsigned pos;
+};
+
+__device__ __forceinline__ double pow2(int k)
+{
+    return __longlong_as_double((long long)(1023 + k) << 52);
+}
+
+// FP64 steps over data[p ..] in runs of 64 until p >= target (target <= count);
+// a run that would pass count (the round's last values) goes one step at a time.
+__device__ __forceinline__ double ordered_steps(double acc, const double *data, unsigned &p, unsigned target,
+                                                unsigned count)
+{
+    while (p < target) {
+        if (p + kOrdRun <= count) {
+#pragma unroll 16
+            for (int k = 0; k < kOrdRun; ++k) acc = acc + data[p + k];
+            p += kOrdRun;
+        } else {
+            for (; p < count; ++p) acc = acc + data[p];
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ double ordered_sum(const double *__restrict__ g, size_t gs, OrdLds &S)
+{
+    constexpr double kMagic = 0x1.8p52;   // t = x + kMagic rounds x to an integer (|x| < 2^51)
+    constexpr long long kLo = 1ll << 52, kHi = (1ll << 53) - 1;
+    constexpr unsigned kNone = ~0u;
+    const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const unsigned mine = w * kOrdBlock + lane * kOrdPer;   // this lane's first value in a round
+    double nxt[kOrdPer];
+    auto fetch = [&](size_t r0) {
+#pragma unroll
+        for (int q = 0; q < kOrdPer; ++q) {
+            const size_t id = r0 + mine + q;
+            nxt[q] = id < gs ? g[id] : 0.0;
+        }
+    };
+    fetch(0);
+    double acc = 0.0;   // the same in every thread
+    for (size_t r0 = 0; r0 < gs; r0 += kOrdRound) {
+        const unsigned count = (unsigned)(gs - r0 < (size_t)kOrdRound ? gs - r0 : (size_t)kOrdRound);
+        double v[kOrdPer];
+#pragma unroll
+        for (int q = 0; q < kOrdPer; ++q) {
+            v[q] = nxt[q];
+            S.data[mine + q] = v[q];
+        }
+        fetch(r0 + kOrdRound);
+        __syncthreads();
+        unsigned pos = 0;
+        while (pos < count) {
+            const int ef = (int)((__double_as_longlong(acc) >> 52) & 0x7ff);
+            unsigned from = pos;
+            if (ef >= 63 && ef < 2047) {   // normal, |acc| >= 2^-960, finite
+                const int e = ef - 1023;
+                const double iu = pow2(52 - e), u = pow2(e - 52);
+                const long long A = (long long)(acc * iu);
+                long long n[kOrdPer], tsum = 0;
+                bool bad[kOrdPer];
+#pragma unroll
+                for (int q = 0; q < kOrdPer; ++q) {
+                    const unsigned id = mine + q;
+                    const bool in = id >= pos && id < count;
+                    const double x = v[q] * iu;
+                    const bool ok = fabs(x) < 0x1p51;
+                    const double t = x + kMagic;
+                    const double r = t - kMagic;
+                    n[q] = in && ok ? __double_as_longlong(t) - __double_as_longlong(kMagic) : 0;
+                    bad[q] = in && (!ok || fabs(r - x) == 0.5);
+                    tsum += n[q];
+                }
+                long long incl = tsum;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const long long o = __shfl_up(incl, d);
+                    if ((int)lane >= d) incl += o;
+                }
+                if (lane == 63) S.tot[w] = incl;
+                __syncthreads();
+                long long off = 0, total = 0;
+#pragma unroll
+                for (int q = 0; q < kOrdWaves; ++q) {
+                    const long long t = S.tot[q];
+                    off += q < (int)w ? t : 0;
+                    total += t;
+                }
+                long long K = A + off + incl - tsum, Kb = 0;
+                unsigned first = kNone;
+#pragma unroll
+                for (int q = 0; q < kOrdPer; ++q) {
+                    const unsigned id = mine + q;
+                    if (id >= pos && id < count) {
+                        const long long Kq = K + n[q], aK = Kq < 0 ? -Kq : Kq;
+                        if ((bad[q] || aK <= kLo || aK >= kHi) && first == kNone) {
+                            first = id;
+                            Kb = K;
+                        }
+                        K = Kq;
+                    }
+                }
+                const unsigned long long m = __ballot(first != kNone);
+                const int l = m ? __builtin_ctzll(m) : 0;
+                const unsigned fw = (unsigned)__shfl((int)first, l);
+                const long long kw = __shfl(Kb, l);
+                if (lane == 0) {
+                    S.first[w] = m ? fw : kNone;
+                    S.kb[w] = kw;
+                }
+                __syncthreads();
+                int ws = -1;
+#pragma unroll
+                for (int q = kOrdWaves - 1; q >= 0; --q)
+                    if (S.first[q] != kNone) ws = q;
+                if (ws < 0) {
+                    acc = (double)(A + total) * u;
+                    break;
+                }
+                from = S.first[ws];
+                acc = (double)S.kb[ws] * u;
+            }
+            // FP64 steps by wave 0 to the end of the block holding `from`
+            if (w == 0) {
+                unsigned p = from;
+                const unsigned target = (from / kOrdBlock + 1) * kOrdBlock;
+                acc = ordered_steps(acc, S.data, p, target < count ? target : count, count);
+                if (lane == 0) {
+                    S.acc = acc;
+                    S.pos = p;
+                }
+            }
+            __syncthreads();
+            acc = S.acc;
+            pos = S.pos;
+            __syncthreads();
+        }
+    }
+    return acc;
+}
 
 template <bool TWO>
-__global__ __launch_bounds__(64) void k_serial_sums(const double *__restrict__ g1, const double *__restrict__ g2,
-                                                     size_t gs, double *out)
+__global__ __launch_bounds__(64 * kOrdWaves) void k_serial_sums(const double *__restrict__ g1,
+                                                                 const double *__restrict__ g2, size_t gs,
+                                                                 double *out)
 {
-    constexpr int B = kSerialBlock / 64;
-    __shared__ __align__(16) double L1[kSerialBlock];
-    __shared__ __align__(16) double L2[TWO ? kSerialBlock : 2];
-    const unsigned lane = threadIdx.x;
-    const size_t full = gs / kSerialBlock * kSerialBlock;
-    double s1 = 0.0, s2 = 0.0, a[B], b[B];
-    if (full) {
-#pragma unroll
-        for (int q = 0; q < B; ++q) {
-            a[q] = g1[q * 64 + lane];
-            if constexpr (TWO) b[q] = g2[q * 64 + lane];
-        }
-    }
-    for (size_t base = 0; base < full; base += kSerialBlock) {
-#pragma unroll
-        for (int q = 0; q < B; ++q) {
-            L1[q * 64 + lane] = a[q];
-            if constexpr (TWO) L2[q * 64 + lane] = b[q];
-        }
-        const size_t nx = base + kSerialBlock < full ? base + kSerialBlock : base;
-#pragma unroll
-        for (int q = 0; q < B; ++q) {
-            a[q] = g1[nx + q * 64 + lane];
-            if constexpr (TWO) b[q] = g2[nx + q * 64 + lane];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kSerialBlock; k += 2) {
-            const double2 v = *reinterpret_cast<const double2 *>(&L1[k]);
-            s1 = s1 + v.x;
-            s1 = s1 + v.y;
-            if constexpr (TWO) {
-                const double2 u = *reinterpret_cast<const double2 *>(&L2[k]);
-                s2 = s2 + u.x;
-                s2 = s2 + u.y;
-            }
-        }
-        __syncthreads();
-    }
-    for (size_t id = full; id < gs; ++id) {
-        s1 = s1 + g1[id];
-        if constexpr (TWO) s2 = s2 + g2[id];
-    }
-    if (lane == 0) {
+    __shared__ OrdLds S;
+    const double s1 = ordered_sum(g1, gs, S);
+    __syncthreads();
+    const double s2 = TWO ? ordered_sum(g2, gs, S) : 0.0;
+    if (threadIdx.x == 0) {
         out[0] = s1;
         out[1] = s2;
     }
@@ -757,8 +881,8 @@ hipError_t post_drift(const PostState &S, size_t gs, double d, hipStream_t s)
 
 hipError_t serial_sums(const double *g1, const double *g2, size_t gs, double *out, hipStream_t s)
 {
-    if (g2) k_serial_sums<true><<<1, 64, 0, s>>>(g1, g2, gs, out);
-    else k_serial_sums<false><<<1, 64, 0, s>>>(g1, nullptr, gs, out);
+    if (g2) k_serial_sums<true><<<1, 64 * kOrdWaves, 0, s>>>(g1, g2, gs, out);
+    else k_serial_sums<false><<<1, 64 * kOrdWaves, 0, s>>>(g1, nullptr, gs, out);
     return hipGetLastError();
 }
 

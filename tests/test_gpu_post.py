@@ -123,6 +123,76 @@ def test_post_setup_without_deltaN_bitwise():
     assert np.abs(st.field(0)).max() > 0
 
 
+# Maps for the set-up's lattice-order mean (k_serial_sums), which takes runs of
+# steps inside one binade of the accumulator as integer prefix sums: many binade
+# changes (centered maps), rounding ties (values on a 2^-10 grid once the sum
+# reaches 2^48), sign flips, zero and non-finite accumulators, huge and tiny values.
+def serial_map(name: str, gs: int, rng) -> np.ndarray:
+    if name == "centered":
+        return rng.standard_normal(gs) * 1e-5
+    if name == "centered drift":
+        return rng.standard_normal(gs) * 1e-5 + 3e-12
+    if name == "ties":
+        return rng.integers(0, 2 ** 40, gs) / 1024.0
+    if name == "ties signed":
+        return rng.integers(-2 ** 40, 2 ** 40, gs) / 1024.0
+    if name == "grid halves":
+        return rng.integers(0, 64, gs) * 2.0 ** -20 - 2.0 ** -15
+    if name == "alternating":
+        return np.where(np.arange(gs) % 2 == 0, 1.0, -1.0)
+    if name == "zeros then values":
+        return np.concatenate([np.zeros(gs // 2), rng.uniform(0, 1e-3, gs - gs // 2)])
+    if name == "huge and tiny":
+        return np.where(rng.random(gs) < 0.001, 1e300, rng.uniform(-1, 1, gs) * 1e-310)
+    if name == "lognormal":
+        return rng.lognormal(-7, 3, gs)
+    m = rng.standard_normal(gs)
+    m[gs - 1000] = np.nan if name == "nan late" else np.inf
+    return m
+
+
+SERIAL_MAPS = ["centered", "centered drift", "ties", "ties signed", "grid halves", "alternating",
+               "zeros then values", "huge and tiny", "lognormal", "nan late", "inf late"]
+
+
+@pytest.mark.parametrize("name", SERIAL_MAPS)
+def test_post_setup_mean_bitwise_on_crafted_maps(name):
+    N = 64
+    st, table = pre_post_state(N)
+    st.field(2)[:] = serial_map(name, N ** 3, np.random.default_rng(11))
+    print(f"set-up mean N={N} map {name}", flush=True)
+    run = engine_twin(st, table, E.IE_FP64)
+    c = st.cfg
+    LIB.ieo_post_initialize(st.ptr)
+    run.post_init(omega=c.omega, horizon_factor=c.horizon_factor, dt=c.dt_post, af=c.af_post, integrator=0,
+                  from_deltaN=1)
+    assert np.array_equal(run.field(E.IE_FIELD_F), st.field(0), equal_nan=True), name
+    run.close()
+
+
+def test_post_setup_two_means_bitwise_on_crafted_maps():
+    # Phi from f without deltaN: lattice-order means of f and f' (two sums)
+    N = 64
+    st, table = pre_post_state(N)
+    rng = np.random.default_rng(12)
+    cfg = st.cfg
+    cfg.perform_deltaN = 0
+    st2 = O.State(cfg, table)
+    # f stays inside the potential table's range (the deltaN set-up evaluates
+    # it); f' takes the crafted values
+    st2.field(0)[:] = st.field(0) + serial_map("centered drift", N ** 3, rng)
+    st2.field(1)[:] = serial_map("ties signed", N ** 3, rng)
+    st2.field(2)[:] = st.field(2)
+    st2.set_background(**st.background())
+    run = engine_twin(st2, table, E.IE_FP64)
+    c = st2.cfg
+    LIB.ieo_post_initialize(st2.ptr)
+    run.post_init(omega=c.omega, horizon_factor=c.horizon_factor, dt=c.dt_post, af=c.af_post, integrator=0,
+                  from_deltaN=0)
+    assert np.array_equal(run.field(E.IE_FIELD_F), st2.field(0))
+    run.close()
+
+
 def test_post_loop_stops_at_af():
     st, table = pre_post_state(16)
     run = engine_twin(st, table, E.IE_FP64)
