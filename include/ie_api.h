@@ -185,9 +185,12 @@ ie_status ie_profile_steps(ie_run *run, int nsteps, ie_profile *out);
  * fused step kernel can load the next plane before or after the current one's
  * arithmetic (variants 0, 1; the same sums in the same order).
  * None of these changes a result (no cross-site sums), so the tuner times real
- * loop steps: round r uses kick candidate r mod ncand and tensor candidate
- * r mod ntensor for steps_per_candidate steps of the device-resident loop, with
- * events around the two launches, then keeps the fastest of each.  The choice is
+ * loop steps: with R = max(candidates) rounds, loop step s takes kick (or fused
+ * step) candidate (s mod R) mod ncand and tensor candidate (s mod R) mod ntensor,
+ * for R x steps_per_candidate steps of the device-resident loop (the candidates
+ * in turn, so each samples the same stretch of the run), with events around the
+ * two launches; each candidate's first launch is not counted, and the fastest
+ * mean of each kind is kept.  The choice is
  * stored process-wide per (N, storage) and taken by later runs and batches of
  * that kind; ie_leapfrog_steps tunes once per kind by itself on a call of >= 256
  * steps.  The reduce and fused step kernels keep one shape, since theirs fixes the

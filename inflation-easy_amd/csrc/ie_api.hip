@@ -892,12 +892,15 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     for (auto &e : ev) IE_HIP(r, hipEventCreate(&e));
     k_loop_begin<<<1, 64, 0, r->stream>>>(r->bg);
     if (fused) IE_HIP(r, launch_march_begin(r->F, split, r->P, r->bg, r->partials, r->stream));
-    for (int c = 0; c < rounds; ++c) {
-        r->F.kick_e = fused ? def : out->elems[c % nk];
-        r->F.march_pf = fused ? c % nk : 0;
-        r->F.tensor_e = nt ? out->tensor_code[c % nt] : 0;
-        for (int q = 0; q < K; ++q) {
-            const long long n = (long long)c * K + q;
+    // candidates in turn, one step each, so that every candidate samples the
+    // same stretch of the run (the cost of a step drifts in the first steps
+    // after the initial conditions)
+    for (int q = 0; q < K; ++q) {
+        for (int c = 0; c < rounds; ++c) {
+            r->F.kick_e = fused ? def : out->elems[c % nk];
+            r->F.march_pf = fused ? c % nk : 0;
+            r->F.tensor_e = nt ? out->tensor_code[c % nt] : 0;
+            const long long n = (long long)q * rounds + c;
             hipEvent_t *e = &ev[(size_t)n * 4];
             if (!fused) IE_HIP(r, launch_friedmann_reduce(r->F, split, r->P, r->bg, r->partials, 0.0, 1, r->stream));
             IE_HIP(r, hipEventRecord(e[0], r->stream));
@@ -916,9 +919,10 @@ ie_status ie_tune(ie_run *r, int K, ie_tuning *out)
     if ((st = sync(r)) != IE_OK) return st;
     std::vector<double> ksum(3, 0.0), tsum(8, 0.0);
     std::vector<int> kcnt(3, 0), tcnt(8, 0);
+    // each candidate's first launch is not counted (its kernel's first use)
     for (int c = 0; c < rounds; ++c)
-        for (int q = 0; q < K; ++q) {
-            hipEvent_t *e = &ev[((size_t)c * K + q) * 4];
+        for (int q = K > 1 ? 1 : 0; q < K; ++q) {
+            hipEvent_t *e = &ev[((size_t)q * rounds + c) * 4];
             float ms = 0.f;
             IE_HIP(r, hipEventElapsedTime(&ms, e[0], e[1]));
             ksum[c % nk] += ms;

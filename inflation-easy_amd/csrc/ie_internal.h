@@ -204,123 +204,112 @@ __device__ __forceinline__ TabRec tab_rec_uniform(const TabRec *rec, int l)
 struct PotView {
     Pot P;
 
-    // The three intervals around c (records rm, ru, rp, read through the scalar
-    // cache): each value not yet `ok` that lies in one of them takes that
-    // interval's interpolation (mask arithmetic, no short-circuit, so the checks
-    // stay branch-free).
+    // Values in the three intervals around lu (records rm above, ru, rp below,
+    // read through the scalar cache).  V and V' are continuous and linear on each
+    // interval, so a value's interpolation is ru's line plus a hinge at each of
+    // ru's grid points:
+    //   y(v) = y_u + d s_u + max(v - xhi_u, 0) (s_m - s_u) + min(d, 0) (s_p - s_u),
+    //   d = v - xlo_u,
+    // which is the line of the value's own interval in another rounding order,
+    // with no per-value compare or select.  At a table end the missing
+    // neighbour is ru itself (no hinge: the window then ends at ru's edge).
     template <int K, bool WANT_V>
-    __device__ __forceinline__ void window_at(int c, int cm, int cp, const TabRec &rm, const TabRec &ru,
-                                              const TabRec &rp, const double (&v)[K], bool (&ok)[K], int (&l)[K],
-                                              double (&V)[K], double (&dv)[K]) const
+    __device__ __forceinline__ void hinges(const TabRec &rm, const TabRec &ru, const TabRec &rp, double base,
+                                           const double (&x)[K], double (&V)[K], double (&dv)[K]) const
     {
+        const double gm = rm.sdv - ru.sdv, gp = rp.sdv - ru.sdv;
+        const double hm = rm.sv - ru.sv, hp = rp.sv - ru.sv;
+        const double ol = base - ru.xlo, oh = base - ru.xhi;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const bool le = v[q] <= ru.xhi, gt = v[q] > ru.xlo;
-            const bool im = !le & (v[q] <= rm.xhi) & (cm != c), ip = !gt & (v[q] > rp.xlo) & (cp != c);
-            const bool take = !ok[q] & ((le & gt) | im | ip);
-            const double dm = v[q] - rm.xlo, du = v[q] - ru.xlo, dp = v[q] - rp.xlo;
-            const double gm = rm.ydv + dm * rm.sdv, gu = ru.ydv + du * ru.sdv, gp = rp.ydv + dp * rp.sdv;
-            dv[q] = take ? (im ? gm : (ip ? gp : gu)) : dv[q];
-            if (WANT_V) {
-                const double wm = rm.yv + dm * rm.sv, wu = ru.yv + du * ru.sv, wp = rp.yv + dp * rp.sv;
-                V[q] = take ? (im ? wm : (ip ? wp : wu)) : V[q];
-            }
-            l[q] = take ? (im ? cm : (ip ? cp : c)) : l[q];
-            ok[q] = ok[q] | take;
+            const double d = x[q] + ol, up = fmax(x[q] + oh, 0.0), dn = fmin(d, 0.0);
+            dv[q] = ru.ydv + d * ru.sdv + up * gm + dn * gp;
+            if (WANT_V) V[q] = ru.yv + d * ru.sv + up * hm + dn * hp;
         }
     }
 
-    template <int K, bool WANT_V>
-    __device__ __forceinline__ void lookup(const double (&v)[K], int (&l)[K], double (&V)[K], double (&dv)[K],
-                                           int *err) const
+    // The wave's interval for its next lookup: that of lane 0's first value
+    // (base + x0), one of the window's three.
+    __device__ __forceinline__ int follow(double base, double x0, int lu, const TabRec &ru) const
     {
-        // the wave's interval lu, in scalar registers: the common case, every
-        // value of the wave inside it; its neighbours are read in the same round
-        // (a wave whose values straddle a grid point needs them, and a second
-        // round of scalar loads would add its latency to every such lookup)
-        const int lu = __builtin_amdgcn_readfirstlane(l[0]);
-        const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
-        const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
-                     rp = tab_rec_uniform(P.T.rec, cp);
-        bool ok[K], all = true;
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            ok[q] = (v[q] <= ru.xhi) & (v[q] > ru.xlo);
-            all = all & ok[q];
-        }
-        if (__all(all)) {
-#pragma unroll
-            for (int q = 0; q < K; ++q) {
-                dv[q] = ru.ydv + (v[q] - ru.xlo) * ru.sdv;
-                if (WANT_V) V[q] = ru.yv + (v[q] - ru.xlo) * ru.sv;
-                l[q] = lu;
-            }
-            return;
-        }
-        // a wave across grid points: if every value lies in lu or a neighbour,
-        // the three records in scalar registers; otherwise (a dense stretch of the
-        // grid, where a wave's values spread over many intervals) every value by
-        // its own lookup, without first trying windows that most values miss
+        return x0 > ru.xhi - base ? lu - 1 : (x0 <= ru.xlo - base ? lu + 1 : lu);
+    }
+
+    // Whether every value base + x of the wave lies in the window rm .. rp.
+    template <int K>
+    __device__ __forceinline__ bool in_window(const TabRec &rm, const TabRec &rp, double base,
+                                              const double (&x)[K]) const
+    {
+        const double hi = rm.xhi - base, lo = rp.xlo - base;
         bool win = true;
 #pragma unroll
-        for (int q = 0; q < K; ++q) win = win & (v[q] <= rm.xhi) & (v[q] > rp.xlo);
-        if (__all(win)) {
+        for (int q = 0; q < K; ++q) win = win & (x[q] <= hi) & (x[q] > lo);   // false for NaN
+        return __all(win);
+    }
+
+    // V/B^2 (WANT_V) and V'/B^2 at the K values base + x (base uniform: the
+    // homogeneous mode of split storage, else 0), hints l updated.
+    template <int K, bool WANT_V>
+    __device__ __forceinline__ void lookup(double base, const double (&x)[K], int (&l)[K], double (&V)[K],
+                                           double (&dv)[K], int *err) const
+    {
+        // the wave's interval lu and its two neighbours, in scalar registers: the
+        // common case, every value of the wave in one of them (read in one round:
+        // a second round of scalar loads would add its latency to the lookup)
+        const int lu = __builtin_amdgcn_readfirstlane(l[0]);
+        const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
+        const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
+                     rp = tab_rec_uniform(P.T.rec, cp);
+        if (in_window<K>(rm, rp, base, x)) {
+            hinges<K, WANT_V>(rm, ru, rp, base, x, V, dv);
+            const int nl = follow(base, x[0], lu, ru);
 #pragma unroll
-            for (int q = 0; q < K; ++q) ok[q] = false;
-            window_at<K, WANT_V>(lu, cm, cp, rm, ru, rp, v, ok, l, V, dv);
+            for (int q = 0; q < K; ++q) l[q] = nl;
             return;
         }
+        // otherwise (a dense stretch of the grid, where a wave's values spread
+        // over many intervals) every value by its own lookup
 #pragma unroll
-        for (int q = 0; q < K; ++q) lookup_value<WANT_V>(v[q], l[q], V[q], dv[q], err);
+        for (int q = 0; q < K; ++q) lookup_value<WANT_V>(base + x[q], l[q], V[q], dv[q], err);
     }
     // Sums of V/B^2 and V'/B^2 over the K values v, added to sV and sdV (hints l,
-    // updated), for lattice sums.  A value in the wave's interval lu or one of its
-    // neighbours adds its count and v - xlo(lu) to that interval's tallies; each
-    // interval's share then follows from them, n y + s (sum d + n (xlo(lu) - xlo)),
-    // the sum of its interpolations y + (v - xlo) s in another rounding order.
-    // Values outside the three intervals take lookup() one by one.
+    // updated), for lattice sums.  Values in the window of three intervals
+    // around lu take the hinge form summed: K y_u + s_u sum d + (s_m - s_u) sum
+    // max(v - xhi_u, 0) + (s_p - s_u) sum min(d, 0) (the sum of their
+    // interpolations in another rounding order).  A wave with values outside
+    // the window takes lookup() one value at a time.
     template <int K>
-    __device__ __forceinline__ void sums(const double (&v)[K], int (&l)[K], double &sV, double &sdV, int *err) const
+    __device__ __forceinline__ void sums(double base, const double (&x)[K], int (&l)[K], double &sV, double &sdV,
+                                         int *err) const
     {
         const int lu = __builtin_amdgcn_readfirstlane(l[0]);
         const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
         const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
                      rp = tab_rec_uniform(P.T.rec, cp);
-        double dm = 0.0, du = 0.0, dp = 0.0;
-        int nm = 0, nu = 0, np = 0;
-        bool out[K], any = false;
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            const double d = v[q] - ru.xlo;
-            const bool above = v[q] > ru.xhi, below = v[q] <= ru.xlo;
-            const bool im = above & (v[q] <= rm.xhi) & (cm != lu), ip = below & (v[q] > rp.xlo) & (cp != lu);
-            const bool iu = (v[q] <= ru.xhi) & (v[q] > ru.xlo);   // false for NaN
-            dm += im ? d : 0.0;
-            du += iu ? d : 0.0;
-            dp += ip ? d : 0.0;
-            nm += im ? 1 : 0;
-            nu += iu ? 1 : 0;
-            np += ip ? 1 : 0;
-            l[q] = im ? cm : (ip ? cp : (iu ? lu : l[q]));
-            out[q] = !(im | iu | ip);
-            any = any | out[q];
-        }
-        const double fm = (double)nm, fu = (double)nu, fp = (double)np;
-        const double om = dm + fm * (ru.xlo - rm.xlo), op = dp + fp * (ru.xlo - rp.xlo);
-        sV += (fu * ru.yv + ru.sv * du) + (fm * rm.yv + rm.sv * om) + (fp * rp.yv + rp.sv * op);
-        sdV += (fu * ru.ydv + ru.sdv * du) + (fm * rm.ydv + rm.sdv * om) + (fp * rp.ydv + rp.sdv * op);
-        if (__any(any)) {
+        if (in_window<K>(rm, rp, base, x)) {
+            const double ol = base - ru.xlo, oh = base - ru.xhi;
+            double sd = 0.0, su = 0.0, sn = 0.0;
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                if (out[q]) {
-                    double x[1] = {v[q]}, a[1], b[1];
-                    int h[1] = {l[q]};
-                    lookup_value<true>(x[0], h[0], a[0], b[0], err);
-                    l[q] = h[0];
-                    sV += a[0];
-                    sdV += b[0];
-                }
+                const double d = x[q] + ol;
+                sd += d;
+                su += fmax(x[q] + oh, 0.0);
+                sn += fmin(d, 0.0);
             }
+            const double f = (double)K;
+            sV += f * ru.yv + ru.sv * sd + (rm.sv - ru.sv) * su + (rp.sv - ru.sv) * sn;
+            sdV += f * ru.ydv + ru.sdv * sd + (rm.sdv - ru.sdv) * su + (rp.sdv - ru.sdv) * sn;
+            const int nl = follow(base, x[0], lu, ru);
+#pragma unroll
+            for (int q = 0; q < K; ++q) l[q] = nl;
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            double a, b;
+            lookup_value<true>(base + x[q], l[q], a, b, err);
+            sV += a;
+            sdV += b;
         }
     }
     // One value outside its wave's window: the O(1) index (its two levels in
@@ -351,7 +340,20 @@ struct PotView {
             return;
         }
         double unused[K];
-        lookup<K, false>(v, l, unused, dv, err);
+        lookup<K, false>(0.0, v, l, unused, dv, err);
+    }
+    // V'/B^2 at base + x (base uniform)
+    template <int K>
+    __device__ __forceinline__ void dV(double base, const double (&x)[K], int (&l)[K], double (&dv)[K],
+                                       int *err) const
+    {
+        if (!P.numerical) {
+#pragma unroll
+            for (int q = 0; q < K; ++q) dv[q] = -P.hV2 * (base + x[q]);
+            return;
+        }
+        double unused[K];
+        lookup<K, false>(base, x, l, unused, dv, err);
     }
     // V/B^2 and V'/B^2 at the K values v (hints l, updated)
     template <int K>
@@ -366,7 +368,7 @@ struct PotView {
             }
             return;
         }
-        lookup<K, true>(v, l, V, dv, err);
+        lookup<K, true>(0.0, v, l, V, dv, err);
     }
     // one value from its hint (updated)
     __device__ __forceinline__ double dV1(double v, int &hint, int *err) const

@@ -964,12 +964,12 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         if (owner) {
             if (t < TI) {
                 const double knext = __shfl(dnew[0], (int)((lane + 1) & 63));
+                // sum V and sum V' of the lane's sites (the window's hinge form)
                 if constexpr (POT == kPotTab) {
-                    // sum V and sum V' of the lane's sites through per-interval tallies
                     double tv[E];
 #pragma unroll
                     for (int e = 0; e < E; ++e) tv[e] = SPLIT ? phibar_next + dnew[e] : dnew[e];
-                    Pv.sums<E>(tv, hint, acc1, acc2, &err);
+                    Pv.sums<E>(0.0, tv, hint, acc1, acc2, &err);
                 }
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
