@@ -214,44 +214,40 @@ struct PotView {
     // with no per-value compare or select.  At a table end the missing
     // neighbour is ru itself (no hinge: the window then ends at ru's edge).
     template <int K, bool WANT_V>
-    __device__ __forceinline__ void hinges(const TabRec &rm, const TabRec &ru, const TabRec &rp, double base,
-                                           const double (&x)[K], double (&V)[K], double (&dv)[K]) const
+    __device__ __forceinline__ void hinges(const TabRec &rm, const TabRec &ru, const TabRec &rp, const double (&v)[K],
+                                           double (&V)[K], double (&dv)[K]) const
     {
         const double gm = rm.sdv - ru.sdv, gp = rp.sdv - ru.sdv;
         const double hm = rm.sv - ru.sv, hp = rp.sv - ru.sv;
-        const double ol = base - ru.xlo, oh = base - ru.xhi;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-            const double d = x[q] + ol, up = fmax(x[q] + oh, 0.0), dn = fmin(d, 0.0);
+            const double d = v[q] - ru.xlo, up = fmax(v[q] - ru.xhi, 0.0), dn = fmin(d, 0.0);
             dv[q] = ru.ydv + d * ru.sdv + up * gm + dn * gp;
             if (WANT_V) V[q] = ru.yv + d * ru.sv + up * hm + dn * hp;
         }
     }
 
-    // The wave's interval for its next lookup: that of lane 0's first value
-    // (base + x0), one of the window's three.
-    __device__ __forceinline__ int follow(double base, double x0, int lu, const TabRec &ru) const
+    // The wave's interval for its next lookup: that of lane 0's first value, one
+    // of the window's three.
+    __device__ __forceinline__ int follow(double v0, int lu, const TabRec &ru) const
     {
-        return x0 > ru.xhi - base ? lu - 1 : (x0 <= ru.xlo - base ? lu + 1 : lu);
+        return v0 > ru.xhi ? lu - 1 : (v0 <= ru.xlo ? lu + 1 : lu);
     }
 
-    // Whether every value base + x of the wave lies in the window rm .. rp.
+    // Whether every value of the wave lies in the window rm .. rp.
     template <int K>
-    __device__ __forceinline__ bool in_window(const TabRec &rm, const TabRec &rp, double base,
-                                              const double (&x)[K]) const
+    __device__ __forceinline__ bool in_window(const TabRec &rm, const TabRec &rp, const double (&v)[K]) const
     {
-        const double hi = rm.xhi - base, lo = rp.xlo - base;
         bool win = true;
 #pragma unroll
-        for (int q = 0; q < K; ++q) win = win & (x[q] <= hi) & (x[q] > lo);   // false for NaN
+        for (int q = 0; q < K; ++q) win = win & (v[q] <= rm.xhi) & (v[q] > rp.xlo);   // false for NaN
         return __all(win);
     }
 
-    // V/B^2 (WANT_V) and V'/B^2 at the K values base + x (base uniform: the
-    // homogeneous mode of split storage, else 0), hints l updated.
+    // V/B^2 (WANT_V) and V'/B^2 at the K values v, hints l updated.
     template <int K, bool WANT_V>
-    __device__ __forceinline__ void lookup(double base, const double (&x)[K], int (&l)[K], double (&V)[K],
-                                           double (&dv)[K], int *err) const
+    __device__ __forceinline__ void lookup(const double (&v)[K], int (&l)[K], double (&V)[K], double (&dv)[K],
+                                           int *err) const
     {
         // the wave's interval lu and its two neighbours, in scalar registers: the
         // common case, every value of the wave in one of them (read in one round:
@@ -260,9 +256,9 @@ struct PotView {
         const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
         const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
                      rp = tab_rec_uniform(P.T.rec, cp);
-        if (in_window<K>(rm, rp, base, x)) {
-            hinges<K, WANT_V>(rm, ru, rp, base, x, V, dv);
-            const int nl = follow(base, x[0], lu, ru);
+        if (in_window<K>(rm, rp, v)) {
+            hinges<K, WANT_V>(rm, ru, rp, v, V, dv);
+            const int nl = follow(v[0], lu, ru);
 #pragma unroll
             for (int q = 0; q < K; ++q) l[q] = nl;
             return;
@@ -270,7 +266,7 @@ struct PotView {
         // otherwise (a dense stretch of the grid, where a wave's values spread
         // over many intervals) every value by its own lookup
 #pragma unroll
-        for (int q = 0; q < K; ++q) lookup_value<WANT_V>(base + x[q], l[q], V[q], dv[q], err);
+        for (int q = 0; q < K; ++q) lookup_value<WANT_V>(v[q], l[q], V[q], dv[q], err);
     }
     // Sums of V/B^2 and V'/B^2 over the K values v, added to sV and sdV (hints l,
     // updated), for lattice sums.  Values in the window of three intervals
@@ -279,27 +275,25 @@ struct PotView {
     // interpolations in another rounding order).  A wave with values outside
     // the window takes lookup() one value at a time.
     template <int K>
-    __device__ __forceinline__ void sums(double base, const double (&x)[K], int (&l)[K], double &sV, double &sdV,
-                                         int *err) const
+    __device__ __forceinline__ void sums(const double (&v)[K], int (&l)[K], double &sV, double &sdV, int *err) const
     {
         const int lu = __builtin_amdgcn_readfirstlane(l[0]);
         const int cm = lu > 1 ? lu - 1 : lu, cp = lu < P.n - 1 ? lu + 1 : lu;
         const TabRec ru = tab_rec_uniform(P.T.rec, lu), rm = tab_rec_uniform(P.T.rec, cm),
                      rp = tab_rec_uniform(P.T.rec, cp);
-        if (in_window<K>(rm, rp, base, x)) {
-            const double ol = base - ru.xlo, oh = base - ru.xhi;
+        if (in_window<K>(rm, rp, v)) {
             double sd = 0.0, su = 0.0, sn = 0.0;
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const double d = x[q] + ol;
+                const double d = v[q] - ru.xlo;
                 sd += d;
-                su += fmax(x[q] + oh, 0.0);
+                su += fmax(v[q] - ru.xhi, 0.0);
                 sn += fmin(d, 0.0);
             }
             const double f = (double)K;
             sV += f * ru.yv + ru.sv * sd + (rm.sv - ru.sv) * su + (rp.sv - ru.sv) * sn;
             sdV += f * ru.ydv + ru.sdv * sd + (rm.sdv - ru.sdv) * su + (rp.sdv - ru.sdv) * sn;
-            const int nl = follow(base, x[0], lu, ru);
+            const int nl = follow(v[0], lu, ru);
 #pragma unroll
             for (int q = 0; q < K; ++q) l[q] = nl;
             return;
@@ -307,7 +301,7 @@ struct PotView {
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             double a, b;
-            lookup_value<true>(base + x[q], l[q], a, b, err);
+            lookup_value<true>(v[q], l[q], a, b, err);
             sV += a;
             sdV += b;
         }
@@ -340,20 +334,7 @@ struct PotView {
             return;
         }
         double unused[K];
-        lookup<K, false>(0.0, v, l, unused, dv, err);
-    }
-    // V'/B^2 at base + x (base uniform)
-    template <int K>
-    __device__ __forceinline__ void dV(double base, const double (&x)[K], int (&l)[K], double (&dv)[K],
-                                       int *err) const
-    {
-        if (!P.numerical) {
-#pragma unroll
-            for (int q = 0; q < K; ++q) dv[q] = -P.hV2 * (base + x[q]);
-            return;
-        }
-        double unused[K];
-        lookup<K, false>(base, x, l, unused, dv, err);
+        lookup<K, false>(v, l, unused, dv, err);
     }
     // V/B^2 and V'/B^2 at the K values v (hints l, updated)
     template <int K>
@@ -368,7 +349,7 @@ struct PotView {
             }
             return;
         }
-        lookup<K, true>(0.0, v, l, V, dv, err);
+        lookup<K, true>(v, l, V, dv, err);
     }
     // one value from its hint (updated)
     __device__ __forceinline__ double dV1(double v, int &hint, int *err) const

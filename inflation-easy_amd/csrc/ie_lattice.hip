@@ -969,7 +969,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
                     double tv[E];
 #pragma unroll
                     for (int e = 0; e < E; ++e) tv[e] = SPLIT ? phibar_next + dnew[e] : dnew[e];
-                    Pv.sums<E>(0.0, tv, hint, acc1, acc2, &err);
+                    Pv.sums<E>(tv, hint, acc1, acc2, &err);
                 }
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
