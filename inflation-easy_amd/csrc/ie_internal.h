@@ -315,10 +315,10 @@ struct PotView {
     __device__ __forceinline__ void lookup_value(double v, int &l, double &V, double &dv, int *err) const
     {
         int lq = tab_candidate(P.T, v);
-        TabRec r = P.T.rec[lq];
+        TabRec r = P.T.vrec[lq];
         if (!tab_in(r, v)) {
             lq = tab_bisect_ref(P.x, P.n, v, err);
-            r = P.T.rec[lq];
+            r = P.T.vrec[lq];
         }
         l = lq;
         dv = r.ydv + (v - r.xlo) * r.sdv;
@@ -376,9 +376,15 @@ struct PotView {
 __device__ __forceinline__ PotView pot_view(const Pot &P) { return PotView{P}; }
 
 // Dynamic LDS of the tabulated march launches: the two index levels of the run's
-// table (for the inputs/ table: 35 KB).  (A copy of the interval records, 55 KB
-// more, measured no faster than their reads from L2.)
-constexpr unsigned kTabIndexLds = 48 * 1024;
+// table (for the inputs/ table: 35 KB; an index above kTabIndexMax stays in
+// global memory), and, with four sites per lane (N = 256: one workgroup per CU
+// either way), the interval records after it (55 KB more), which the per-value
+// lookups of a dense stretch then read from LDS instead of L2 (256^3 split step
+// there 0.133 -> 0.108 ms).  With two sites per lane (N = 128) the records would
+// leave one workgroup per CU instead of two: measured slower at the start of a
+// run (0.0179 -> 0.0194 ms) and faster in the dense stretch (0.0272 -> 0.0257),
+// no gain over a whole run, so they stay in L2 there.
+constexpr unsigned kTabIndexMax = 48 * 1024, kTabIndexLds = kTabIndexMax, kTabRecordsLds = 112 * 1024;
 
 __host__ __device__ __forceinline__ unsigned tab_index_bytes(const TabIndex &T)
 {
@@ -386,13 +392,14 @@ __host__ __device__ __forceinline__ unsigned tab_index_bytes(const TabIndex &T)
 }
 
 // Copies the index levels of Pv's table into `lds` (every thread of the
-// workgroup takes part) when they fit in `budget` bytes, and points Pv at the
-// copy; the caller's next barrier publishes it.
+// workgroup takes part) when they fit in `budget` bytes, then the interval
+// records when they fit too, and points Pv at the copies; the caller's next
+// barrier publishes them.
 __device__ __forceinline__ void stage_tab_index(PotView &Pv, char *lds, unsigned nthreads, unsigned budget)
 {
     static_assert(sizeof(TabBucket) == 8 && sizeof(TabSub) == 16, "index record sizes");
     const unsigned ib = tab_index_bytes(Pv.P.T);
-    if (!Pv.P.numerical || ib > budget) return;
+    if (!Pv.P.numerical || ib > kTabIndexMax || ib > budget) return;
     // l1 (k1 a power of two >= 16, so l2 starts 16-byte aligned)
     const unsigned k1 = (unsigned)Pv.P.T.k1, nsub = (unsigned)Pv.P.T.nsub;
     const uint2 *s1 = (const uint2 *)Pv.P.T.l1;
@@ -403,6 +410,14 @@ __device__ __forceinline__ void stage_tab_index(PotView &Pv, char *lds, unsigned
     for (unsigned q = threadIdx.x; q < nsub; q += nthreads) d2[q] = s2[q];
     Pv.P.T.l1 = (const TabBucket *)d1;
     Pv.P.T.l2 = (const TabSub *)d2;
+    // the records too where the budget holds them (16-byte units after the index)
+    const unsigned rb = (unsigned)Pv.P.T.n * (unsigned)sizeof(TabRec);
+    if (ib + rb <= budget) {
+        const uint4 *s3 = (const uint4 *)Pv.P.T.rec;
+        uint4 *d3 = (uint4 *)(lds + ib);
+        for (unsigned q = threadIdx.x; q < rb / 16; q += nthreads) d3[q] = s3[q];
+        Pv.P.T.vrec = (const TabRec *)d3;
+    }
 }
 
 // ---------------------------------------------------------------- launchers

@@ -728,11 +728,12 @@ struct MarchLds {
     double smem[3 * kMarchWaves];
 };
 
-// Dynamic LDS of a march launch: the index of the tabulated potential's table.
-template <int POT, bool JL>
+// Dynamic LDS of a march launch: the index of the tabulated potential's table
+// (and its records with four sites per lane).
+template <int POT, bool JL, int E>
 __host__ __device__ constexpr unsigned march_lds()
 {
-    return POT == kPotTab ? kTabIndexLds : 0u;
+    return POT == kPotTab ? (E == 4 ? kTabRecordsLds : kTabIndexLds) : 0u;
 }
 
 template <typename T, bool SPLIT, int E, int POT, bool JL, bool SUMS = false>
@@ -775,7 +776,7 @@ __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg
         // the table's index levels into this launch's dynamic LDS (published by
         // the barriers below: the close's, or the one here for the sums pass)
         extern __shared__ __align__(16) char tab_lds[];
-        stage_tab_index(Pv, tab_lds, kMarchThreads, march_lds<POT, JL>());
+        stage_tab_index(Pv, tab_lds, kMarchThreads, march_lds<POT, JL, E>());
         if constexpr (SUMS) __syncthreads();
     }
     // fd is read from [par] and written to [par ^ 1]: the halo row and the extra
@@ -2031,11 +2032,11 @@ void march_dispatch_pot(const Fields &F, const Pot &P, DevBg *bg, double *partia
 {
     const int E = row_elems(F.N);
     if (F.march_pf) {
-        if (E == 2) k_march_pf<T, SPLIT, 2, POT><<<G, kMarchThreads, march_lds<POT, false>(), s>>>(F, P, bg, partials, sp);
-        else k_march_pf<T, SPLIT, 4, POT><<<G, kMarchThreads, march_lds<POT, false>(), s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march_pf<T, SPLIT, 2, POT><<<G, kMarchThreads, march_lds<POT, false, 2>(), s>>>(F, P, bg, partials, sp);
+        else k_march_pf<T, SPLIT, 4, POT><<<G, kMarchThreads, march_lds<POT, false, 4>(), s>>>(F, P, bg, partials, sp);
     } else {
-        if (E == 2) k_march<T, SPLIT, 2, POT><<<G, kMarchThreads, march_lds<POT, true>(), s>>>(F, P, bg, partials, sp);
-        else k_march<T, SPLIT, 4, POT><<<G, kMarchThreads, march_lds<POT, true>(), s>>>(F, P, bg, partials, sp);
+        if (E == 2) k_march<T, SPLIT, 2, POT><<<G, kMarchThreads, march_lds<POT, true, 2>(), s>>>(F, P, bg, partials, sp);
+        else k_march<T, SPLIT, 4, POT><<<G, kMarchThreads, march_lds<POT, true, 4>(), s>>>(F, P, bg, partials, sp);
     }
 }
 
@@ -2049,8 +2050,8 @@ void march_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, 
 template <typename T, bool SPLIT, int POT>
 void march_sums_dispatch(const Fields &F, const Pot &P, DevBg *bg, double *partials, unsigned G, hipStream_t s)
 {
-    if (row_elems(F.N) == 2) k_march_sums<T, SPLIT, 2, POT><<<G, kMarchThreads, march_lds<POT, false>(), s>>>(F, P, bg, partials);
-    else k_march_sums<T, SPLIT, 4, POT><<<G, kMarchThreads, march_lds<POT, false>(), s>>>(F, P, bg, partials);
+    if (row_elems(F.N) == 2) k_march_sums<T, SPLIT, 2, POT><<<G, kMarchThreads, march_lds<POT, false, 2>(), s>>>(F, P, bg, partials);
+    else k_march_sums<T, SPLIT, 4, POT><<<G, kMarchThreads, march_lds<POT, false, 4>(), s>>>(F, P, bg, partials);
 }
 
 }  // namespace
@@ -2404,8 +2405,8 @@ static void batch_group_reduce(const BatchDev *B, const BatchGroup &G, hipStream
 template <typename T, bool SPLIT, int POT>
 static void batch_march_sums(const BatchDev *B, const BatchGroup &G, hipStream_t s)
 {
-    if (G.E == 2) kb_march_sums<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false>(), s>>>(B->runs, G.map);
-    else kb_march_sums<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false>(), s>>>(B->runs, G.map);
+    if (G.E == 2) kb_march_sums<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false, 2>(), s>>>(B->runs, G.map);
+    else kb_march_sums<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false, 4>(), s>>>(B->runs, G.map);
 }
 
 template <typename T, bool SPLIT, int POT>
@@ -2413,11 +2414,11 @@ static void batch_march(const BatchDev *B, const BatchGroup &G, unsigned sp, hip
 {
     // (march groups have N = 128 or 256: E = 2 or 4)
     if (G.pf) {
-        if (G.E == 2) kb_march_pf<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false>(), s>>>(B->runs, G.map, sp);
-        else kb_march_pf<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false>(), s>>>(B->runs, G.map, sp);
+        if (G.E == 2) kb_march_pf<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false, 2>(), s>>>(B->runs, G.map, sp);
+        else kb_march_pf<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, march_lds<POT, false, 4>(), s>>>(B->runs, G.map, sp);
     } else {
-        if (G.E == 2) kb_march<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, march_lds<POT, true>(), s>>>(B->runs, G.map, sp);
-        else kb_march<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, march_lds<POT, true>(), s>>>(B->runs, G.map, sp);
+        if (G.E == 2) kb_march<T, SPLIT, 2, POT><<<G.nblocks, kMarchThreads, march_lds<POT, true, 2>(), s>>>(B->runs, G.map, sp);
+        else kb_march<T, SPLIT, 4, POT><<<G.nblocks, kMarchThreads, march_lds<POT, true, 4>(), s>>>(B->runs, G.map, sp);
     }
 }
 

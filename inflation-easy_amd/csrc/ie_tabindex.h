@@ -58,6 +58,7 @@ struct TabSub {
 // Device (or host) view of one table's index.
 struct TabIndex {
     const TabRec *rec;      // n records (record 0 unused: its check always fails)
+    const TabRec *vrec;     // where per-value lookups read the records: an LDS copy, or rec
     const TabBucket *l1;    // k1 buckets
     const TabSub *l2;       // sub-buckets
     const double *x;        // the grid (bisection)
@@ -192,7 +193,7 @@ inline TabIndexHost tab_index_build(const double *x, const double *v, const doub
 inline TabIndex tab_view(const TabIndexHost &H, const TabRec *rec, const TabBucket *l1, const TabSub *l2,
                          const double *x, int n)
 {
-    return TabIndex{rec, l1, l2, x, n, H.k1, (int)H.l2.size(), H.x0, H.inv_w, H.tmax};
+    return TabIndex{rec, rec, l1, l2, x, n, H.k1, (int)H.l2.size(), H.x0, H.inv_w, H.tmax};
 }
 
 }  // namespace ie
