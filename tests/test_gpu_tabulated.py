@@ -1,10 +1,11 @@
-"""Tabulated potential on the fused-step path (N = 128) against the oracle, where
-the lookups leave the common case:
+"""Tabulated potential on the fused-step path against the oracle, where the
+lookups leave the common case:
 
   - in the dense stretch of the inputs/ grid (phi near 2.683, spacing 3.6e-6
     against a field spread of about 2e-5): a wave's sites lie in tens of
     intervals, so most values take the per-value path through the index the step
-    kernel stages in LDS;
+    kernel stages in LDS (N = 128: the records read from L2; N = 256: the records
+    staged in LDS too);
   - with a table too fine for that LDS copy (the inputs/ table resampled on a
     uniform grid of 40001 points: its index exceeds the 48 KB the kernel reserves),
     so the index is read from global memory.
@@ -59,15 +60,17 @@ def check(st, run, b, n_o, nsteps, label):
     assert ef < 1e-9 and ev < 1e-9
 
 
-def test_dense_stretch_of_the_table_matches_oracle():
+@pytest.mark.parametrize("N", [128, 256])
+def test_dense_stretch_of_the_table_matches_oracle(N):
     table = potential_table()
     x = table.x
     spacing = -np.diff(x)[np.searchsorted(-x, -2.6827)]
     assert spacing < 5e-6          # the dense stretch this test is about
-    st, run, b, n_o = run_pair(table, 2.6827)
+    print(f"dense stretch N={N}", flush=True)
+    st, run, b, n_o = run_pair(table, 2.6827, N=N, nsteps=20 if N <= 128 else 8)
     f = st.field(0)
     print(f"phi0 2.6827: field spread {f.std():.2e}, grid spacing {spacing:.2e}", flush=True)
-    check(st, run, b, n_o, 20, "dense stretch")
+    check(st, run, b, n_o, 20 if N <= 128 else 8, f"dense stretch N={N}")
     run.close()
 
 
