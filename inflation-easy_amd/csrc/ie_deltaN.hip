@@ -478,15 +478,43 @@ __global__ void k_phiref_final(const double *__restrict__ f, const double *__res
 //     rounding of acc + v, whose parity acc's multiple of u cannot change), as
 //     long as k + n < 2^24 and r / u is not a tie (its rounding would depend on
 //     k's parity);
-//   - so the steps of a binade are an integer prefix sum: one workgroup scans the
-//     sites a chunk at a time and stops at the first step that is not of this
-//     kind (it leaves the binade, is a tie, or has v < 0 or v >= B), which it
-//     takes as a float step, then goes on in the new binade;
+//   - a tie (r / u = f + 1/2) rounds k + f + 1/2 to the even one of k + f and
+//     k + f + 1: k -> k + f + ((k + f) & 1), which depends on k only through its
+//     parity;
+//   - so the steps of a binade compose: a run of steps maps k to
+//     k + A + (t ? (k + c) & 1 : 0), with A an integer, t whether the run holds
+//     a tie and c a parity bit, and two runs compose into a run of the same
+//     form (after a run's first tie every later tie's parity is known); one
+//     workgroup scans the sites a chunk at a time with this composition, checks
+//     every partial sum, and stops at the first step that is not of this kind
+//     (it leaves the binade, or has v < 0 or v >= B), which it takes as a float
+//     step, then goes on in the new binade (deltaN maps hold values on the dN
+//     grid many times over, and a tie used to stop the scan: 1706 stops on a
+//     128^3 tabulated map, 20 now);
 //   - while acc is 0, negative or not finite, excluded sites (v > cutoff or NaN)
 //     leave it alone, so do zero sites while acc is a zero of their sign or +0,
 //     and every other included one is a float step.
 // The result is the serial walk's bit for bit.
 constexpr int kMeanThreads = 1024, kMeanPer = 8, kMeanChunk = kMeanThreads * kMeanPer;
+
+// A run of steps in one binade: k -> k + A + (t ? (k + c) & 1 : 0).
+struct MeanRun {
+    long long A;
+    int f;   // bit 0: t, bit 1: c
+};
+__device__ __forceinline__ MeanRun mean_compose(MeanRun a, MeanRun b)
+{
+    const int ta = a.f & 1, tb = b.f & 1, ca = (a.f >> 1) & 1, cb = (b.f >> 1) & 1;
+    MeanRun r;
+    r.A = a.A + b.A + ((ta & tb) ? ((int)((a.A + ca + cb) & 1)) : 0);
+    const int c = ta ? ca : (tb ? (int)((a.A + cb) & 1) : 0);
+    r.f = (ta | tb) | (c << 1);
+    return r;
+}
+__device__ __forceinline__ long long mean_apply(MeanRun r, long long k)
+{
+    return k + r.A + ((r.f & 1) ? ((k + ((r.f >> 1) & 1)) & 1) : 0);
+}
 
 __device__ __forceinline__ float mean_step(float acc, double v, double cutoff)
 {
@@ -498,7 +526,7 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
 {
     constexpr int NW = kMeanThreads / 64;
     constexpr long long kTop = 1ll << 24;
-    __shared__ long long wtot[NW];
+    __shared__ MeanRun wtot[NW];
     __shared__ unsigned long long s_first;
     __shared__ long long s_kfirst;
     __shared__ float s_acc;
@@ -522,13 +550,13 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
         size_t c0 = pos;
         for (;;) {
             const size_t base = c0 + (size_t)tid * kMeanPer;
-            long long n[kMeanPer];
+            MeanRun e[kMeanPer];
             bool bad[kMeanPer];
-            long long tsum = 0;
+            MeanRun seg{0, 0};
 #pragma unroll
             for (int q = 0; q < kMeanPer; ++q) {
                 const size_t id = base + q;
-                n[q] = 0;
+                e[q] = MeanRun{0, 0};
                 bad[q] = id >= gs;
                 if (id < gs) {
                     const double v = dn[id];
@@ -542,47 +570,55 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
                         } else {
                             const double qv = ((B + v) - B) / u;
                             const double fl = floor(qv);
-                            bad[q] = qv - fl == 0.5;
-                            n[q] = (long long)rint(qv);
+                            const long long fi = (long long)fl;
+                            e[q] = qv - fl == 0.5 ? MeanRun{fi, 1 | (int)((fi & 1) << 1)}
+                                                 : MeanRun{(long long)rint(qv), 0};
                         }
                     }
                 }
-                tsum += n[q];
+                seg = mean_compose(seg, e[q]);
             }
-            // exclusive prefix of the thread sums: in the wave, then over waves
-            long long incl = tsum;
+            // inclusive scan of the threads' runs: in the wave, then over waves
+            MeanRun incl = seg;
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
-                const long long o = __shfl_up(incl, d);
-                if ((int)lane >= d) incl += o;
+                MeanRun o;
+                o.A = __shfl_up(incl.A, d);
+                o.f = __shfl_up(incl.f, d);
+                if ((int)lane >= d) incl = mean_compose(o, incl);
             }
+            MeanRun excl;
+            excl.A = __shfl_up(incl.A, 1);
+            excl.f = __shfl_up(incl.f, 1);
+            if (lane == 0) excl = MeanRun{0, 0};
             __syncthreads();   // every thread has read the last chunk's s_first and wtot
             if (tid == 0) s_first = ~0ull;
             if (lane == 63) wtot[wave] = incl;
             __syncthreads();
-            long long before = 0, total = 0;
+            MeanRun before{0, 0}, total{0, 0};
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
-                before += w < (int)wave ? wtot[w] : 0;
-                total += wtot[w];
+                if (w < (int)wave) before = mean_compose(before, wtot[w]);
+                total = mean_compose(total, wtot[w]);
             }
-            long long K = k + before + incl - tsum;
+            long long K = mean_apply(mean_compose(before, excl), k);
             size_t mine = ~(size_t)0;
             long long kmine = 0;
 #pragma unroll
             for (int q = 0; q < kMeanPer; ++q) {
-                const bool stop = bad[q] || (regular && K + n[q] >= kTop);
+                const long long Kq = mean_apply(e[q], K);
+                const bool stop = bad[q] || (regular && Kq >= kTop);
                 if (stop && mine == ~(size_t)0) {
                     mine = base + q;
                     kmine = K;
                 }
-                K += n[q];
+                K = Kq;
             }
             if (mine != ~(size_t)0) atomicMin(&s_first, (unsigned long long)mine);
             __syncthreads();
             const unsigned long long first = s_first;
             if (first == ~0ull) {
-                k += total;
+                k = mean_apply(total, k);
                 c0 += kMeanChunk;
                 continue;
             }
