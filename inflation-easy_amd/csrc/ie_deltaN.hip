@@ -497,25 +497,6 @@ __global__ void k_phiref_final(const double *__restrict__ f, const double *__res
 // The result is the serial walk's bit for bit.
 constexpr int kMeanThreads = 1024, kMeanPer = 8, kMeanChunk = kMeanThreads * kMeanPer;
 
-// A run of steps in one binade: k -> k + A + (t ? (k + c) & 1 : 0).
-struct MeanRun {
-    long long A;
-    int f;   // bit 0: t, bit 1: c
-};
-__device__ __forceinline__ MeanRun mean_compose(MeanRun a, MeanRun b)
-{
-    const int ta = a.f & 1, tb = b.f & 1, ca = (a.f >> 1) & 1, cb = (b.f >> 1) & 1;
-    MeanRun r;
-    r.A = a.A + b.A + ((ta & tb) ? ((int)((a.A + ca + cb) & 1)) : 0);
-    const int c = ta ? ca : (tb ? (int)((a.A + cb) & 1) : 0);
-    r.f = (ta | tb) | (c << 1);
-    return r;
-}
-__device__ __forceinline__ long long mean_apply(MeanRun r, long long k)
-{
-    return k + r.A + ((r.f & 1) ? ((k + ((r.f >> 1) & 1)) & 1) : 0);
-}
-
 __device__ __forceinline__ float mean_step(float acc, double v, double cutoff)
 {
     return v <= cutoff ? (float)((double)acc + v) : acc;
@@ -526,7 +507,7 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
 {
     constexpr int NW = kMeanThreads / 64;
     constexpr long long kTop = 1ll << 24;
-    __shared__ MeanRun wtot[NW];
+    __shared__ GridRun wtot[NW];
     __shared__ unsigned long long s_first;
     __shared__ long long s_kfirst;
     __shared__ float s_acc;
@@ -550,13 +531,13 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
         size_t c0 = pos;
         for (;;) {
             const size_t base = c0 + (size_t)tid * kMeanPer;
-            MeanRun e[kMeanPer];
+            GridRun e[kMeanPer];
             bool bad[kMeanPer];
-            MeanRun seg{0, 0};
+            GridRun seg{0, 0};
 #pragma unroll
             for (int q = 0; q < kMeanPer; ++q) {
                 const size_t id = base + q;
-                e[q] = MeanRun{0, 0};
+                e[q] = GridRun{0, 0};
                 bad[q] = id >= gs;
                 if (id < gs) {
                     const double v = dn[id];
@@ -571,42 +552,41 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
                             const double qv = ((B + v) - B) / u;
                             const double fl = floor(qv);
                             const long long fi = (long long)fl;
-                            e[q] = qv - fl == 0.5 ? MeanRun{fi, 1 | (int)((fi & 1) << 1)}
-                                                 : MeanRun{(long long)rint(qv), 0};
+                            e[q] = qv - fl == 0.5 ? grid_tie(fi) : grid_step((long long)rint(qv));
                         }
                     }
                 }
-                seg = mean_compose(seg, e[q]);
+                seg = grid_compose(seg, e[q]);
             }
             // inclusive scan of the threads' runs: in the wave, then over waves
-            MeanRun incl = seg;
+            GridRun incl = seg;
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
-                MeanRun o;
+                GridRun o;
                 o.A = __shfl_up(incl.A, d);
                 o.f = __shfl_up(incl.f, d);
-                if ((int)lane >= d) incl = mean_compose(o, incl);
+                if ((int)lane >= d) incl = grid_compose(o, incl);
             }
-            MeanRun excl;
+            GridRun excl;
             excl.A = __shfl_up(incl.A, 1);
             excl.f = __shfl_up(incl.f, 1);
-            if (lane == 0) excl = MeanRun{0, 0};
+            if (lane == 0) excl = GridRun{0, 0};
             __syncthreads();   // every thread has read the last chunk's s_first and wtot
             if (tid == 0) s_first = ~0ull;
             if (lane == 63) wtot[wave] = incl;
             __syncthreads();
-            MeanRun before{0, 0}, total{0, 0};
+            GridRun before{0, 0}, total{0, 0};
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
-                if (w < (int)wave) before = mean_compose(before, wtot[w]);
-                total = mean_compose(total, wtot[w]);
+                if (w < (int)wave) before = grid_compose(before, wtot[w]);
+                total = grid_compose(total, wtot[w]);
             }
-            long long K = mean_apply(mean_compose(before, excl), k);
+            long long K = grid_apply(grid_compose(before, excl), k);
             size_t mine = ~(size_t)0;
             long long kmine = 0;
 #pragma unroll
             for (int q = 0; q < kMeanPer; ++q) {
-                const long long Kq = mean_apply(e[q], K);
+                const long long Kq = grid_apply(e[q], K);
                 const bool stop = bad[q] || (regular && Kq >= kTop);
                 if (stop && mine == ~(size_t)0) {
                     mine = base + q;
@@ -618,7 +598,7 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
             __syncthreads();
             const unsigned long long first = s_first;
             if (first == ~0ull) {
-                k = mean_apply(total, k);
+                k = grid_apply(total, k);
                 c0 += kMeanChunk;
                 continue;
             }

@@ -420,6 +420,32 @@ __device__ __forceinline__ void stage_tab_index(PotView &Pv, char *lds, unsigned
     }
 }
 
+// Exact sums on the grid of one binade (the float mean of saveN, the lattice-
+// order FP64 sums of the post-inflation set-up): an accumulator that is k
+// units of the binade's grid takes a step to k + n, or at a rounding tie
+// (k + f + 1/2) to the even one of k + f and k + f + 1; a run of such steps is
+// k -> k + A + (t ? (k + c) mod 2 : 0) (t: the run holds a tie; after its first
+// tie every later tie's parity is known), and runs compose into runs.
+struct GridRun {
+    long long A;
+    int f;   // bit 0: t, bit 1: c
+};
+__device__ __forceinline__ GridRun grid_step(long long n) { return GridRun{n, 0}; }
+__device__ __forceinline__ GridRun grid_tie(long long f) { return GridRun{f, 1 | (int)((f & 1) << 1)}; }
+__device__ __forceinline__ GridRun grid_compose(GridRun a, GridRun b)
+{
+    const int ta = a.f & 1, tb = b.f & 1, ca = (a.f >> 1) & 1, cb = (b.f >> 1) & 1;
+    GridRun r;
+    r.A = a.A + b.A + ((ta & tb) ? (int)((a.A + ca + cb) & 1) : 0);
+    const int c = ta ? ca : (tb ? (int)((a.A + cb) & 1) : 0);
+    r.f = (ta | tb) | (c << 1);
+    return r;
+}
+__device__ __forceinline__ long long grid_apply(GridRun r, long long k)
+{
+    return k + r.A + ((r.f & 1) ? ((k + ((r.f >> 1) & 1)) & 1) : 0);
+}
+
 // ---------------------------------------------------------------- launchers
 // (ie_lattice.hip)
 

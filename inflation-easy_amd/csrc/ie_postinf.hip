@@ -569,11 +569,15 @@ __global__ __launch_bounds__(kBlock) void k_post_drift(double *__restrict__ f, c
 //     as long as x is not a half-integer (a tie, whose rounding depends on the
 //     parity of A) and the exact sum stays in the binade, which holds when
 //     2^52 < |A + rint(x)| < 2^53 - 1;
-//   - so a round of 4096 values (8 waves x 512, 8 consecutive per lane, staged
-//     in LDS) is an integer prefix sum: every wave scans its block, the waves'
-//     totals give each block its offset, and every partial sum is checked;
-//   - at the round's first step that is not of this kind (a tie, out of the
-//     binade, |x| >= 2^51, NaN; or acc 0, subnormal or not finite) wave 0 takes
+//   - a tie (x a half-integer) rounds to the even one of A + floor(x) and
+//     A + floor(x) + 1, a map of A through its parity; runs of steps compose
+//     (GridRun), so a round of 4096 values (8 waves x 512, 8 consecutive per
+//     lane, staged in LDS) is a prefix scan under that composition: every wave
+//     scans its block, the waves' totals give each block its start, and every
+//     partial sum is checked (a centred deltaN map, whose values lie on the dN
+//     grid, has a tie in every 30 steps or so);
+//   - at the round's first step that is not of this kind (out of the binade,
+//     |x| >= 2^51, NaN; or acc 0, subnormal or not finite) wave 0 takes
 //     FP64 steps from the LDS copy (every lane reads the same value: a
 //     broadcast), in runs of 64 to the end of that step's block, and the round
 //     is scanned again from there in the new binade;
@@ -587,7 +591,8 @@ constexpr int kOrdRun = 64;
 
 struct OrdLds {
     double data[kOrdRound];
-    long long tot[kOrdWaves], kb[kOrdWaves];
+    GridRun tot[kOrdWaves];
+    long long kb[kOrdWaves];
     unsigned first[kOrdWaves];
     double acc;
     unsigned pos;
@@ -650,8 +655,8 @@ __device__ __forceinline__ double ordered_sum(const double *__restrict__ g, size
                 const int e = ef - 1023;
                 const double iu = pow2(52 - e), u = pow2(e - 52);
                 const long long A = (long long)(acc * iu);
-                long long n[kOrdPer], tsum = 0;
-                bool bad[kOrdPer];
+                GridRun st[kOrdPer];
+                bool bad[kOrdPer], lt = false;
 #pragma unroll
                 for (int q = 0; q < kOrdPer; ++q) {
                     const unsigned id = mine + q;
@@ -660,32 +665,77 @@ __device__ __forceinline__ double ordered_sum(const double *__restrict__ g, size
                     const bool ok = fabs(x) < 0x1p51;
                     const double t = x + kMagic;
                     const double r = t - kMagic;
-                    n[q] = in && ok ? __double_as_longlong(t) - __double_as_longlong(kMagic) : 0;
-                    bad[q] = in && (!ok || fabs(r - x) == 0.5);
-                    tsum += n[q];
+                    const bool tie = in && ok && fabs(r - x) == 0.5;
+                    // n = rint(x); at a tie the lower neighbour is n - (n > x)
+                    const long long n = __double_as_longlong(t) - __double_as_longlong(kMagic);
+                    st[q] = !(in && ok) ? GridRun{0, 0} : tie ? grid_tie(n - (r > x ? 1 : 0)) : grid_step(n);
+                    bad[q] = in && !ok;
+                    lt = lt | tie;
                 }
-                long long incl = tsum;
+                // a wave without ties takes plain integer sums; with ties, the
+                // runs' composition
+                const bool wt = __any(lt);
+                GridRun seg{0, 0}, incl, excl{0, 0};
+                if (wt) {
 #pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const long long o = __shfl_up(incl, d);
-                    if ((int)lane >= d) incl += o;
+                    for (int q = 0; q < kOrdPer; ++q) seg = grid_compose(seg, st[q]);
+                    incl = seg;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        GridRun o;
+                        o.A = __shfl_up(incl.A, d);
+                        o.f = __shfl_up(incl.f, d);
+                        if ((int)lane >= d) incl = grid_compose(o, incl);
+                    }
+                    excl.A = __shfl_up(incl.A, 1);
+                    excl.f = __shfl_up(incl.f, 1);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < kOrdPer; ++q) seg.A += st[q].A;
+                    incl = seg;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const long long o = __shfl_up(incl.A, d);
+                        if ((int)lane >= d) incl.A += o;
+                    }
+                    excl.A = incl.A - seg.A;
                 }
+                if (lane == 0) excl = GridRun{0, 0};
                 if (lane == 63) S.tot[w] = incl;
                 __syncthreads();
-                long long off = 0, total = 0;
+                // the waves' totals: plain sums of the integers when no wave holds a
+                // tie, else composed across lanes 0 .. 7 (one wave's total each)
+                int tied = 0;
 #pragma unroll
-                for (int q = 0; q < kOrdWaves; ++q) {
-                    const long long t = S.tot[q];
-                    off += q < (int)w ? t : 0;
-                    total += t;
+                for (int q = 0; q < kOrdWaves; ++q) tied |= S.tot[q].f;
+                GridRun before{0, 0}, total{0, 0};
+                if (tied & 1) {
+                    GridRun tq = lane < (unsigned)kOrdWaves ? S.tot[lane] : GridRun{0, 0};
+#pragma unroll
+                    for (int d = 1; d < kOrdWaves; d <<= 1) {
+                        GridRun o;
+                        o.A = __shfl_up(tq.A, d);
+                        o.f = __shfl_up(tq.f, d);
+                        if ((int)lane >= d) tq = grid_compose(o, tq);
+                    }
+                    total = GridRun{__shfl(tq.A, kOrdWaves - 1), __shfl(tq.f, kOrdWaves - 1)};
+                    before = GridRun{__shfl(tq.A, (int)w - 1), __shfl(tq.f, (int)w - 1)};
+                    if (w == 0) before = GridRun{0, 0};
+                } else {
+#pragma unroll
+                    for (int q = 0; q < kOrdWaves; ++q) {
+                        const long long t = S.tot[q].A;
+                        before.A += q < (int)w ? t : 0;
+                        total.A += t;
+                    }
                 }
-                long long K = A + off + incl - tsum, Kb = 0;
+                long long K = grid_apply(grid_compose(before, excl), A), Kb = 0;
                 unsigned first = kNone;
 #pragma unroll
                 for (int q = 0; q < kOrdPer; ++q) {
                     const unsigned id = mine + q;
                     if (id >= pos && id < count) {
-                        const long long Kq = K + n[q], aK = Kq < 0 ? -Kq : Kq;
+                        const long long Kq = wt ? grid_apply(st[q], K) : K + st[q].A, aK = Kq < 0 ? -Kq : Kq;
                         if ((bad[q] || aK <= kLo || aK >= kHi) && first == kNone) {
                             first = id;
                             Kb = K;
@@ -707,7 +757,7 @@ __device__ __forceinline__ double ordered_sum(const double *__restrict__ g, size
                 for (int q = kOrdWaves - 1; q >= 0; --q)
                     if (S.first[q] != kNone) ws = q;
                 if (ws < 0) {
-                    acc = (double)(A + total) * u;
+                    acc = (double)grid_apply(total, A) * u;
                     break;
                 }
                 from = S.first[ws];
