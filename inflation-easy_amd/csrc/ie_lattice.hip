@@ -729,18 +729,24 @@ struct MarchLds {
 };
 
 // Dynamic LDS of a march launch: the index of the tabulated potential's table
-// (and its records with four sites per lane).
+// (and its records with four sites per lane, for the variant whose old rows stay
+// out of LDS: with them, 64-82 KB of its own, the records would pass the 160 KB
+// a workgroup can hold).
 template <int POT, bool JL, int E>
 __host__ __device__ constexpr unsigned march_lds()
 {
-    return POT == kPotTab ? (E == 4 ? kTabRecordsLds : kTabIndexLds) : 0u;
+    return POT == kPotTab ? ((E == 4 && !JL) ? kTabRecordsLds : kTabIndexLds) : 0u;
 }
+constexpr unsigned kLdsPerWorkgroup = 160 * 1024;
 
 template <typename T, bool SPLIT, int E, int POT, bool JL, bool SUMS = false>
 __device__ __forceinline__ void march_body(const Fields &F, const Pot &P0, DevBg *bg, double *partials, unsigned blk,
                                            MarchLds<T, E, JL> &L, unsigned sp = 0)
 {
     static_assert(E == 2 || E == 4, "fused steps hold 2 or 4 sites per lane");
+    // static LDS (the rows, this step's record, slack for the sums) and dynamic
+    static_assert(sizeof(MarchLds<T, E, JL>) + sizeof(DevBg) + 1024 + march_lds<POT, JL, E>() <= kLdsPerWorkgroup,
+                  "a march launch's LDS must fit one workgroup");
     const unsigned N = (unsigned)F.N, logN = (unsigned)F.logN, m = N - 1, lane = threadIdx.x & 63;
     const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned G = (N / kTileJ) * (N / march_ti(N));

@@ -124,9 +124,10 @@ def test_post_setup_without_deltaN_bitwise():
 
 
 # Maps for the set-up's lattice-order mean (k_serial_sums), which takes runs of
-# steps inside one binade of the accumulator as integer prefix sums: many binade
+# steps inside one binade of the accumulator as a prefix scan: many binade
 # changes (centered maps), rounding ties (values on a 2^-10 grid once the sum
-# reaches 2^48), sign flips, zero and non-finite accumulators, huge and tiny values.
+# reaches 2^48; a centred map of dN-grid values, like saveN's zeta), sign flips,
+# zero and non-finite accumulators, huge and tiny values.
 def serial_map(name: str, gs: int, rng) -> np.ndarray:
     if name == "centered":
         return rng.standard_normal(gs) * 1e-5
@@ -146,13 +147,18 @@ def serial_map(name: str, gs: int, rng) -> np.ndarray:
         return np.where(rng.random(gs) < 0.001, 1e300, rng.uniform(-1, 1, gs) * 1e-310)
     if name == "lognormal":
         return rng.lognormal(-7, 3, gs)
+    if name == "dN grid centred":
+        # like saveN's zeta: deltaN values taken from an accumulated dN = 1e-4
+        # grid (plus the half step), each many times, less a float mean
+        grid = np.add.accumulate(np.full(14000, 1e-4)) + 5e-5
+        return grid[rng.integers(0, 14000, gs)] - np.float64(np.float32(0.5352471470832825))
     m = rng.standard_normal(gs)
     m[gs - 1000] = np.nan if name == "nan late" else np.inf
     return m
 
 
 SERIAL_MAPS = ["centered", "centered drift", "ties", "ties signed", "grid halves", "alternating",
-               "zeros then values", "huge and tiny", "lognormal", "nan late", "inf late"]
+               "zeros then values", "huge and tiny", "lognormal", "dN grid centred", "nan late", "inf late"]
 
 
 @pytest.mark.parametrize("name", SERIAL_MAPS)

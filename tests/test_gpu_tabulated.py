@@ -30,7 +30,7 @@ def fluct_err(a, b):
     return np.max(np.abs(da - (b - b.mean()))) / np.sqrt(np.mean(da ** 2))
 
 
-def run_pair(table, phi0: float, N: int = 128, nsteps: int = 20, precision=E.IE_FP64):
+def run_pair(table, phi0: float, N: int = 128, nsteps: int = 20, precision=E.IE_FP64, variant=None):
     _, params, _ = golden_config("tabulated_n16")
     params = dict(params, initial_field=phi0)
     cfg = O.make_config(N, 1, params, table, sigw=0)
@@ -40,6 +40,8 @@ def run_pair(table, phi0: float, N: int = 128, nsteps: int = 20, precision=E.IE_
     ecfg = E.make_config(N, precision=precision, numerical=1, sigw=0, table=table, V0=c.V0, ns=c.ns, L=c.L,
                          dt=c.dt, af=c.af, rescale_s=c.rescale_s, rescale_B=c.rescale_B, dN=c.dN, Nend=c.Nend)
     run = E.Run(ecfg, table)
+    if variant is not None:
+        run.set_shapes(0, 0, variant)
     run.upload_fields(st.field(0), st.field(1))
     bg = st.background()
     run.set_background(t=bg["t"], a=bg["a"], ad=bg["ad"], astep=bg["astep"], hubble_init=bg["hubble_init"])
@@ -50,27 +52,35 @@ def run_pair(table, phi0: float, N: int = 128, nsteps: int = 20, precision=E.IE_
     return st, run, b, n_o
 
 
-def check(st, run, b, n_o, nsteps, label):
+def check(st, run, b, n_o, nsteps, label, split=False):
+    # split storage keeps the fluctuations in FP32: their bound, and the
+    # background's through the lattice sums
     bo = st.background()
     assert b["steps"] == n_o == nsteps
     for key in ("a", "ad", "t"):
-        assert b[key] == pytest.approx(bo[key], rel=1e-13), key
+        assert b[key] == pytest.approx(bo[key], rel=1e-12 if split else 1e-13), key
     ef, ev = fluct_err(st.field(0), run.field(E.IE_FIELD_F)), fluct_err(st.field(1), run.field(E.IE_FIELD_FD))
     print(f"{label}: a rel {abs(b['a'] - bo['a']) / bo['a']:.2e}, f {ef:.2e}, fd {ev:.2e}", flush=True)
-    assert ef < 1e-9 and ev < 1e-9
+    tol = 1e-5 if split else 1e-9
+    assert ef < tol and ev < tol
 
 
-@pytest.mark.parametrize("N", [128, 256])
-def test_dense_stretch_of_the_table_matches_oracle(N):
+# N = 256 with each fused-step variant pinned: rows of old f from L2 (1, the
+# records in LDS beside the index) or through LDS (0, the index alone)
+@pytest.mark.parametrize("N,variant,precision", [(128, None, E.IE_FP64), (256, None, E.IE_FP64),
+                                                 (256, 0, E.IE_FP64), (256, 0, E.IE_SPLIT),
+                                                 (256, 1, E.IE_SPLIT)])
+def test_dense_stretch_of_the_table_matches_oracle(N, variant, precision):
     table = potential_table()
     x = table.x
     spacing = -np.diff(x)[np.searchsorted(-x, -2.6827)]
     assert spacing < 5e-6          # the dense stretch this test is about
-    print(f"dense stretch N={N}", flush=True)
-    st, run, b, n_o = run_pair(table, 2.6827, N=N, nsteps=20 if N <= 128 else 8)
+    print(f"dense stretch N={N} variant={variant} precision={precision}", flush=True)
+    st, run, b, n_o = run_pair(table, 2.6827, N=N, nsteps=20 if N <= 128 else 8, precision=precision,
+                               variant=variant)
     f = st.field(0)
     print(f"phi0 2.6827: field spread {f.std():.2e}, grid spacing {spacing:.2e}", flush=True)
-    check(st, run, b, n_o, 20 if N <= 128 else 8, f"dense stretch N={N}")
+    check(st, run, b, n_o, 20 if N <= 128 else 8, f"dense stretch N={N}", split=precision == E.IE_SPLIT)
     run.close()
 
 
