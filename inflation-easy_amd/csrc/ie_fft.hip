@@ -375,6 +375,7 @@ __device__ __forceinline__ void line_fft(double2 (&a)[N / LineShape<N>::N1], int
 
 constexpr int kRegThreads = 256;   // z passes: 256 / N1 rows per workgroup
 constexpr int kColLines = 16;      // column passes: 16 consecutive kz per workgroup
+constexpr unsigned kColXcd = 8;     // XCDs of the MI355X (workgroup b runs on XCD b mod 8)
 
 // z pass, real to half-complex.
 template <typename T, int N>
@@ -449,8 +450,14 @@ __global__ __launch_bounds__(kColLines * LineShape<N>::N1) void k_cols_reg(doubl
     for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
     if (BIN)
         for (int e = threadIdx.x; e < NW * nb; e += NT) wbins[e] = 0.0;
+    // Workgroup b runs on XCD b mod 8: XCD x takes the x-th eighth of the
+    // (outer index, column group) pairs in order, so the column groups of one
+    // outer index share an L2 and the cache line two neighbouring groups
+    // straddle (rows of K modes are not line-aligned) is fetched once.
+    const unsigned nblk = gridDim.x * gridDim.y, blk = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned lid = nblk % kColXcd == 0 ? (blk % kColXcd) * (nblk / kColXcd) + blk / kColXcd : blk;
     const int l = threadIdx.x % kColLines, n1 = threadIdx.x / kColLines;
-    const int o = blockIdx.y, kz = blockIdx.x * kColLines + l;
+    const int o = (int)(lid / gridDim.x), kz = (int)(lid % gridDim.x) * kColLines + l;
     const size_t stride = DIM == 1 ? (size_t)K : (size_t)N * K;
     const size_t base = (DIM == 1 ? (size_t)o * N * K : (size_t)o * K) + (size_t)(kz < K ? kz : 0);
     const bool live = kz < K;
@@ -488,7 +495,7 @@ __global__ __launch_bounds__(kColLines * LineShape<N>::N1) void k_cols_reg(doubl
         for (int b = threadIdx.x; b < nb; b += NT) {
             double t = 0.0;
             for (int q = 0; q < NW; ++q) t += wbins[q * nb + b];
-            partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * nb + b] = t;
+            partials[(size_t)lid * nb + b] = t;
         }
     }
 }
