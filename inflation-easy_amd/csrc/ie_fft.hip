@@ -377,6 +377,11 @@ constexpr int kRegThreads = 256;   // z passes: 256 / N1 rows per workgroup
 constexpr int kColLines = 16;      // column passes: 16 consecutive kz per workgroup
 constexpr unsigned kColXcd = 8;     // XCDs of the MI355X (workgroup b runs on XCD b mod 8)
 
+// Doubles of a column workgroup's exchange area, which holds the binning pass's
+// per-wave shell sums after the transform.
+template <int N>
+constexpr size_t col_bin_capacity() { return 2u * kColLines * (N / LineShape<N>::N1) * kExRow<N>; }
+
 // z pass, real to half-complex.
 template <typename T, int N>
 __global__ __launch_bounds__(kRegThreads) void k_z_r2c_reg(const T *__restrict__ in, double2 *__restrict__ out,
@@ -435,7 +440,9 @@ __global__ __launch_bounds__(kRegThreads) void k_z_c2r_reg(const double2 *__rest
 // fastest index so that a load instruction reads 16 consecutive modes per row.
 // BIN (x pass of a spectrum): instead of storing, the shell sums of w |F|^2 with
 // the reference weights (as k_bin_scalar), per wave in LDS and per workgroup
-// into partials, dc_add added to the k = 0 amplitude first.
+// into partials, dc_add added to the k = 0 amplitude first.  The per-wave sums
+// reuse the exchange area once the transform has read it, so the binning pass
+// needs no LDS beyond the plain pass (two workgroups per CU at N = 256).
 template <int N, int SIGN, int DIM, bool BIN>
 __global__ __launch_bounds__(kColLines * LineShape<N>::N1) void k_cols_reg(double2 *__restrict__ H,
                                                                             const double2 *__restrict__ twg,
@@ -446,10 +453,7 @@ __global__ __launch_bounds__(kColLines * LineShape<N>::N1) void k_cols_reg(doubl
     constexpr int NW = NT >= 64 ? NT / 64 : 1;
     __shared__ double2 tw[N];
     __shared__ double2 ex[kColLines][N2 * kExRow<N>];
-    extern __shared__ double wbins[];   // BIN: [NW][nb]
     for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
-    if (BIN)
-        for (int e = threadIdx.x; e < NW * nb; e += NT) wbins[e] = 0.0;
     // Workgroup b runs on XCD b mod 8: XCD x takes the x-th eighth of the
     // (outer index, column group) pairs in order, so the column groups of one
     // outer index share an L2 and the cache line two neighbouring groups
@@ -475,7 +479,11 @@ __global__ __launch_bounds__(kColLines * LineShape<N>::N1) void k_cols_reg(doubl
                 for (int k1 = 0; k1 < N1; ++k1)
                     H[base + (size_t)(n1 + N1 * u + N2 * k1) * stride] = a[u * N1 + k1];
     } else {
+        double *wbins = &ex[0][0].x;   // [NW][nb], within col_bin_capacity<N>()
         const int wave = NT >= 64 ? (int)(threadIdx.x >> 6) : 0;
+        __syncthreads();   // every thread's last read of ex is done
+        for (int e = threadIdx.x; e < NW * nb; e += NT) wbins[e] = 0.0;
+        __syncthreads();
         if (live) {
             const int py = o <= N / 2 ? o : o - N;
             const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
@@ -586,8 +594,9 @@ template <int N, int SIGN, int DIM, bool BIN>
 static hipError_t launch_cols(double2 *H, const double2 *tw, int nb, double dc_add, double *partials, hipStream_t s)
 {
     constexpr int N1 = LineShape<N>::N1, NT = kColLines * N1, NW = NT >= 64 ? NT / 64 : 1;
+    if (BIN && (size_t)NW * nb > col_bin_capacity<N>()) return hipErrorInvalidValue;
     const dim3 grid(col_groups(N), N);
-    k_cols_reg<N, SIGN, DIM, BIN><<<grid, NT, BIN ? (size_t)NW * nb * 8 : 0, s>>>(H, tw, nb, dc_add, partials);
+    k_cols_reg<N, SIGN, DIM, BIN><<<grid, NT, 0, s>>>(H, tw, nb, dc_add, partials);
     return hipGetLastError();
 }
 
