@@ -507,8 +507,11 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
 {
     constexpr int NW = kMeanThreads / 64;
     constexpr long long kTop = 1ll << 24;
-    __shared__ GridRun wtot[NW];
-    __shared__ unsigned long long s_first;
+    constexpr int kBadWave = 4;   // flag bit of a wave total (grid_compose reads bits 0-1 only)
+    // Chunks alternate between two copies of the wave totals, so a copy is
+    // free again one barrier after its last read.
+    __shared__ GridRun wtot[2][NW];
+    __shared__ unsigned long long s_first;   // ~0 outside a stop's handling
     __shared__ long long s_kfirst;
     __shared__ float s_acc;
     __shared__ unsigned long long s_pos;
@@ -516,8 +519,14 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
     if (tid == 0) {
         s_acc = 0.f;
         s_pos = 0;
+        s_first = ~0ull;
     }
     __syncthreads();
+    auto load = [&](double (&d)[kMeanPer], size_t b) {
+#pragma unroll
+        for (int q = 0; q < kMeanPer; ++q) d[q] = b + q < gs ? dn[b + q] : 0.0;
+    };
+    unsigned p = 0;
     for (;;) {
         const float acc = s_acc;
         const size_t pos = s_pos;
@@ -526,68 +535,108 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
         const bool regular = acc >= 1.17549435e-38f && acc <= 3.40282347e38f;
         int E = 0;
         if (regular) (void)frexpf(acc, &E);
+        // u = 2^(E-24) and 1/u are powers of two in the normal range, so r / u = r * iu
         const double B = regular ? ldexp(1.0, E - 1) : 0.0, u = regular ? ldexp(1.0, E - 24) : 0.0;
+        const double iu = regular ? ldexp(1.0, 24 - E) : 0.0;
         long long k = regular ? (long long)((double)acc / u) : 0;
         size_t c0 = pos;
+        double cur[kMeanPer];
+        load(cur, c0 + (size_t)tid * kMeanPer);
         for (;;) {
             const size_t base = c0 + (size_t)tid * kMeanPer;
-            GridRun e[kMeanPer];
-            bool bad[kMeanPer];
+            double nxt[kMeanPer];   // the next chunk's values, in flight during this chunk's scan
+            load(nxt, base + kMeanChunk);
+            // site q's run (a step, a tie or nothing) and whether it stops the
+            // scan; computed again where a chunk holds a stop rather than kept
+            auto elem = [&](int q, bool &bad) -> GridRun {
+                const size_t id = base + q;
+                bad = id >= gs;
+                if (id >= gs) return GridRun{0, 0};
+                const double v = cur[q];
+                if (!(v <= cutoff)) return GridRun{0, 0};
+                if (!regular || !(v >= 0.0) || !(v < B)) {
+                    // a zero added to a zero accumulator leaves it as it is
+                    // (+0 + 0 = +0, -0 + -0 = -0): no step, so that a long run
+                    // of zero sites at the head of the map is not walked one
+                    // float step at a time
+                    bad = !(acc == 0.0f && v == 0.0 && !(signbit(acc) && !signbit(v)));
+                    return GridRun{0, 0};
+                }
+                const double qv = ((B + v) - B) * iu;   // in [0, 2^23]: int conversions
+                const double fl = floor(qv);
+                return qv - fl == 0.5 ? grid_tie((int)fl) : grid_step((int)rint(qv));
+            };
             GridRun seg{0, 0};
+            bool anybad = false;
 #pragma unroll
             for (int q = 0; q < kMeanPer; ++q) {
-                const size_t id = base + q;
-                e[q] = GridRun{0, 0};
-                bad[q] = id >= gs;
-                if (id < gs) {
-                    const double v = dn[id];
-                    if (v <= cutoff) {
-                        if (!regular || !(v >= 0.0) || !(v < B)) {
-                            // a zero added to a zero accumulator leaves it as it
-                            // is (+0 + 0 = +0, -0 + -0 = -0): no step, so that a
-                            // long run of zero sites at the head of the map is not
-                            // walked one float step at a time
-                            bad[q] = !(acc == 0.0f && v == 0.0 && !(signbit(acc) && !signbit(v)));
-                        } else {
-                            const double qv = ((B + v) - B) / u;
-                            const double fl = floor(qv);
-                            const long long fi = (long long)fl;
-                            e[q] = qv - fl == 0.5 ? grid_tie(fi) : grid_step((long long)rint(qv));
-                        }
-                    }
-                }
-                seg = grid_compose(seg, e[q]);
+                bool bad;
+                seg = grid_compose(seg, elem(q, bad));
+                anybad |= bad;
             }
-            // inclusive scan of the threads' runs: in the wave, then over waves
+            // inclusive scan of the threads' runs: in the wave (plain integer
+            // sums when none of its runs holds a tie), then over waves
             GridRun incl = seg;
+            if (__ballot(seg.f & 1) == 0) {
+                long long a = seg.A;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                GridRun o;
-                o.A = __shfl_up(incl.A, d);
-                o.f = __shfl_up(incl.f, d);
-                if ((int)lane >= d) incl = grid_compose(o, incl);
+                for (int d = 1; d < 64; d <<= 1) {
+                    const long long o = __shfl_up(a, d);
+                    if ((int)lane >= d) a += o;
+                }
+                incl.A = a;
+            } else {
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    GridRun o;
+                    o.A = __shfl_up(incl.A, d);
+                    o.f = __shfl_up(incl.f, d);
+                    if ((int)lane >= d) incl = grid_compose(o, incl);
+                }
             }
-            GridRun excl;
+            const bool wave_bad = __ballot(anybad) != 0;
+            if (lane == 63) wtot[p][wave] = GridRun{incl.A, incl.f | (wave_bad ? kBadWave : 0)};
+            __syncthreads();
+            // the waves' totals scanned once per wave, in its first NW lanes
+            GridRun wt = lane < (unsigned)NW ? wtot[p][lane] : GridRun{0, 0};
+            const bool chunk_bad = __ballot(lane < (unsigned)NW && (wt.f & kBadWave)) != 0;
+#pragma unroll
+            for (int d = 1; d < NW; d <<= 1) {
+                GridRun o;
+                o.A = __shfl_up(wt.A, d);
+                o.f = __shfl_up(wt.f, d);
+                if ((int)lane >= d) wt = grid_compose(o, wt);
+            }
+            GridRun total;
+            total.A = __shfl(wt.A, NW - 1);
+            total.f = __shfl(wt.f, NW - 1);
+            const long long kend = grid_apply(total, k);
+            p ^= 1;
+            // every step of a binade adds n >= 0, so k only grows: a chunk with
+            // no bad site that ends below 2^24 holds no stop
+            if (!chunk_bad && !(regular && kend >= kTop)) {
+                k = kend;
+                c0 += kMeanChunk;
+#pragma unroll
+                for (int q = 0; q < kMeanPer; ++q) cur[q] = nxt[q];
+                continue;
+            }
+            // the chunk holds a stop: find the first one and its k
+            GridRun before, excl;
+            before.A = __shfl(wt.A, wave == 0 ? 0 : wave - 1);
+            before.f = __shfl(wt.f, wave == 0 ? 0 : wave - 1);
+            if (wave == 0) before = GridRun{0, 0};
             excl.A = __shfl_up(incl.A, 1);
             excl.f = __shfl_up(incl.f, 1);
             if (lane == 0) excl = GridRun{0, 0};
-            __syncthreads();   // every thread has read the last chunk's s_first and wtot
-            if (tid == 0) s_first = ~0ull;
-            if (lane == 63) wtot[wave] = incl;
-            __syncthreads();
-            GridRun before{0, 0}, total{0, 0};
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                if (w < (int)wave) before = grid_compose(before, wtot[w]);
-                total = grid_compose(total, wtot[w]);
-            }
             long long K = grid_apply(grid_compose(before, excl), k);
             size_t mine = ~(size_t)0;
             long long kmine = 0;
 #pragma unroll
             for (int q = 0; q < kMeanPer; ++q) {
-                const long long Kq = grid_apply(e[q], K);
-                const bool stop = bad[q] || (regular && Kq >= kTop);
+                bool bad;
+                const long long Kq = grid_apply(elem(q, bad), K);
+                const bool stop = bad || (regular && Kq >= kTop);
                 if (stop && mine == ~(size_t)0) {
                     mine = base + q;
                     kmine = K;
@@ -597,11 +646,6 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
             if (mine != ~(size_t)0) atomicMin(&s_first, (unsigned long long)mine);
             __syncthreads();
             const unsigned long long first = s_first;
-            if (first == ~0ull) {
-                k = grid_apply(total, k);
-                c0 += kMeanChunk;
-                continue;
-            }
             if (mine == first) s_kfirst = kmine;
             __syncthreads();
             if (tid == 0) {
@@ -613,6 +657,7 @@ __global__ __launch_bounds__(kMeanThreads) void k_float_mean(const double *__res
                     s_acc = at;
                     s_pos = gs;
                 }
+                s_first = ~0ull;
             }
             __syncthreads();
             break;
