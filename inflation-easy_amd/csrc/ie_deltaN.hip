@@ -485,12 +485,14 @@ __global__ void k_phiref_final(const double *__restrict__ f, const double *__res
 //     k + A + (t ? (k + c) & 1 : 0), with A an integer, t whether the run holds
 //     a tie and c a parity bit, and two runs compose into a run of the same
 //     form (after a run's first tie every later tie's parity is known); one
-//     workgroup scans the sites a chunk at a time with this composition, checks
-//     every partial sum, and stops at the first step that is not of this kind
-//     (it leaves the binade, or has v < 0 or v >= B), which it takes as a float
-//     step, then goes on in the new binade (deltaN maps hold values on the dN
-//     grid many times over, and a tie used to stop the scan: 1706 stops on a
-//     128^3 tabulated map, 20 now);
+//     workgroup scans the sites a chunk at a time with this composition and
+//     stops at the first step that is not of this kind (it leaves the binade,
+//     or has v < 0 or v >= B), which it takes as a float step, then goes on in
+//     the new binade (deltaN maps hold values on the dN grid many times over,
+//     and a tie used to stop the scan: 1706 stops on a 128^3 tabulated map, 20
+//     now); every step of a binade adds n >= 0, so k only grows, and only a
+//     chunk with a bad site or whose last k reaches 2^24 is searched for the
+//     first stop, partial sum by partial sum;
 //   - while acc is 0, negative or not finite, excluded sites (v > cutoff or NaN)
 //     leave it alone, so do zero sites while acc is a zero of their sign or +0,
 //     and every other included one is a float step.
